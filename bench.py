@@ -1,0 +1,163 @@
+#!/usr/bin/env python3
+"""Benchmark: the stdPFC ETDRK2 step on a 1024^2 grid (one MI355X).
+
+Workload (the standard-PFC single-instance configuration): 1024 x 1024,
+dx = dy = pi/4, r = epsilon = -0.3, psibar = -0.3, g = 0, ETDRK2 with dt = 0.5;
+initial field psibar + N(0, 0.01^2) holding 8 one-mode crystal seeds of radius
+20 grid points (seeded, synthetic).  The field and tables are resident in HBM
+before the timed region.  A "step" is one ETDRK2 time step (two nonlinear
+evaluations, four kernels).
+
+Prints ONE JSON line.  `value` is Mpts*steps/s of the plain FP64 step;
+`fp32` carries the FP32-bulk / FP64-reduction variant of the same workload.
+`roofline` is measured for the dominant kernel with HIP events on the stream
+the kernels run on; `cpu_baseline` times the FP64 NumPy/SciPy CPU restatement
+of the same step on this host's cores over a bounded sample.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "pfc-models_amd"))
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+N = 1024
+R = PSIBAR = -0.3
+DT = 0.5
+
+
+def _workload():
+    import numpy as np
+    from pfc_hip import fields as F
+    from pfc_hip.discretization import Build2DGeometry
+    L = N * math.pi / 4
+    geom = Build2DGeometry(N, N, L, L)
+    psi0 = F.seeded_crystals(geom, R, PSIBAR, n_seeds=8, radius_pts=20, noise_std=0.01, seed=2)
+    return geom, np.ascontiguousarray(psi0)
+
+
+def _algorithmic_bytes(kind, precision):
+    """Bytes a kernel launch must move at minimum: half spectra of Nc complex
+    read and written once each (row pass: S~ in, T out; column pass ETD/stage 1:
+    S~ and T in, S~ and the k-space carry out; stage 2: T and carry in, S~ out)."""
+    c = 16 if precision == 64 else 8
+    nc = N * (N // 2 + 1)
+    return {"row": 2, "col": 4, "col2": 3}[kind] * nc * c
+
+
+def run_gpu(precision, steps, warmup):
+    import torch
+    from pfc_hip.plan import Plan
+    geom, psi0 = _workload()
+    pl = Plan(Nx=N, Ny=N, Lx=geom.Lx, Ly=geom.Ly, Gamma=1.0, beta=1.0, q0=1.0, epsilon=R, g=0.0, dt=DT,
+              precision=precision, scheme="etdrk2", pin_mean=True)
+    pl.set_field(psi0)
+    pl.step(warmup)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    pl.step(steps)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    fe, mean, _, _ = pl.reduce()
+    # per-kernel device time with HIP events (separate pass, same workload)
+    prof = pl.step_profile(min(steps, 200))
+    kinds = {k: (ms / calls, calls) for k, (ms, calls) in prof.items() if calls}
+    dom = max(kinds, key=lambda k: kinds[k][0] * kinds[k][1])
+    avg_ms = kinds[dom][0]
+    ab = _algorithmic_bytes(dom, precision)
+    achieved = ab / (avg_ms * 1e-3) / 1e9
+    names = {"row": "k_row_nonlin", "col": "k_col_etd<COL_RK2_STAGE1>", "col2": "k_col_etd<COL_RK2_STAGE2>"}
+    step_bytes = 2 * 4 * N * (N // 2 + 1) * (16 if precision == 64 else 8)
+    return {
+        "value": N * N * steps / wall / 1e6,
+        "ms_per_step": wall / steps * 1e3,
+        "free_energy": fe,
+        "mean": mean,
+        "kernels_us": {k: round(v[0] * 1e3, 3) for k, v in kinds.items()},
+        "roofline": {
+            "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": names[dom],
+            "algorithmic_bytes_per_launch": ab, "avg_launch_us": avg_ms * 1e3,
+        },
+        "step_roofline": {
+            "bytes_per_step": step_bytes,
+            "achieved": step_bytes / (wall / steps) / 1e9,
+            "frac": step_bytes / (wall / steps) / 1e9 / HBM_PEAK_GBS,
+        },
+        "device_bytes": pl.device_bytes(),
+    }
+
+
+def run_cpu(max_seconds=20.0):
+    """FP64 CPU restatement of the same step (scipy.fft, all host cores)."""
+    sys.path.insert(0, ROOT)
+    from oracle import pfc_oracle as O
+    geom, psi0 = _workload()
+    workers = min(len(os.sched_getaffinity(0)), 16)
+    og = O.build_geometry(N, N, geom.Lx, geom.Ly)
+    sim = O.Simulation(O.Params(model="std", epsilon=R, g=0.0, dt=DT, scheme="etdrk2"), og,
+                       fft="scipy", workers=workers)
+    sim.set_field(psi0)
+    sim.step(2)
+    done, t0 = 0, time.perf_counter()
+    while done < 200:
+        sim.step(5)
+        done += 5
+        if time.perf_counter() - t0 > max_seconds:
+            break
+    wall = time.perf_counter() - t0
+    return {"value": N * N * done / wall / 1e6, "unit": "Mpts*steps/s", "cores": workers, "kind": "port",
+            "sample": f"{done} ETDRK2 FP64 steps of the 1024^2 seeded-crystal workload "
+                      f"(NumPy/SciPy restatement of the reference CPU step, scipy.fft workers={workers})"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=100)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print("bench.py: this build measures one GPU; --gpus above 1 is not supported "
+              "(instances shard trivially, one per GPU; see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+    f64 = run_gpu(64, a.steps, a.warmup)
+    f32 = run_gpu(32, a.steps, a.warmup)
+    cpu = None if a.no_cpu else run_cpu(a.cpu_seconds)
+    line = {
+        "metric": "Mpts*steps/s at 1024^2 (stdPFC ETDRK2 step, FP64; FP32 in 'fp32')",
+        "value": round(f64["value"], 1),
+        "unit": "Mpts*steps/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(f64["ms_per_step"], 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded 1024^2 liquid + 8 one-mode crystal seeds)",
+        "config": {"workload": "stdPFC 1024^2, dx=pi/4, r=-0.3, psibar=-0.3, ETDRK2 dt=0.5, plain FP64 step",
+                   "grid": [N, N], "scheme": "etdrk2", "parallelism": "single GPU"},
+        "roofline": f64["roofline"],
+        "step_roofline": f64["step_roofline"],
+        "kernels_us": f64["kernels_us"],
+        "fp32": {"value": round(f32["value"], 1), "ms_per_step": round(f32["ms_per_step"], 5),
+                 "dtype": "f32 bulk, f64 tables and reductions", "roofline": f32["roofline"],
+                 "kernels_us": f32["kernels_us"],
+                 "rel_fe_vs_fp64": abs(f32["free_energy"] - f64["free_energy"]) / abs(f64["free_energy"])},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,112 @@
+/* C ABI of libpfc_hip.so: the MI355X (gfx950) pseudo-spectral ETD step of the
+ * standard and log phase-field-crystal models.
+ *
+ * Conventions
+ *   - Every entry point returns 0 on success or a negative PFC_E* code; the
+ *     message of the last failure on the calling thread is pfc_last_error().
+ *     No C++ exception crosses this boundary.
+ *   - Fields are real, C-order (Ny, Nx), x fastest, in DEVICE memory owned by
+ *     the caller (e.g. a torch tensor's data_ptr()).  `dtype` is 64 (double)
+ *     or 32 (float).  Spectra are (Ny, Nx/2+1) interleaved complex, ky in
+ *     numpy.fft.fftfreq order, forward transforms unnormalised, inverses 1/N.
+ *   - `stream` is a hipStream_t (NULL = the default stream).  Stepping is
+ *     asynchronous on it; pfc_reduce synchronises it.
+ *   - A plan owns its device state, tables and scratch; it never frees caller
+ *     memory.  A plan is not thread safe: one host thread per plan at a time.
+ *
+ * Reference interfaces replaced (paths under EricStimpsonWSU/PFC-models):
+ *   pfc_create      Build*Simulation + _SimulationState tables
+ *                   (src/pfc_models/models/stdpfc/simulation.py:99-164, :270-286;
+ *                    src/pfc_models/discretization/spectral/kernels.py:38-71)
+ *   pfc_set_field   `sim.State.n[...] = x` (models/stdpfc/simulation.py:148-153)
+ *   pfc_get_field   reading `sim.State.n`
+ *   pfc_step        Simulation.TimeStep (models/stdpfc/simulation.py:225-235;
+ *                   models/logpfc/simulation.py:220-230), nsteps in a row
+ *   pfc_reduce      Simulation.CalculateFreeEnergy (models/stdpfc/simulation.py:249-268;
+ *                   models/logpfc/simulation.py:244-263) and `State.n.mean()`
+ *                   (tests/test_smoke_models.py:245-249)
+ *   pfc_set_shift   logpfc_extv OptimizeM table rebuild
+ *                   (models/logpfc_extv/simulation.py:332-373)
+ *   pfc_rfft2 / pfc_irfft2
+ *                   the backend FFT contract `rfftn`/`irfftn`
+ *                   (src/pfc_models/backends/torch_mps.py:3-14) as used by
+ *                   cp.fft.rfft2 / irfft2 (models/stdpfc/simulation.py:215-223)
+ */
+#ifndef PFC_HIP_H
+#define PFC_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PFC_OK 0
+#define PFC_EINVAL (-1)       /* bad argument */
+#define PFC_EUNSUPPORTED (-2) /* grid size not factorable / too large */
+#define PFC_EHIP (-3)         /* HIP runtime error */
+#define PFC_ENOMEM (-4)       /* device allocation failed */
+
+#define PFC_MODEL_STD 0 /* field n:   N = g n^2 + n^3 */
+#define PFC_MODEL_LOG 1 /* field phi: N = ln phi + g phi^2 + phi^3 - M phi */
+
+#define PFC_SCHEME_ETD1 1
+#define PFC_SCHEME_ETDRK2 2
+
+#define PFC_STATUS_NONFINITE 1 /* a non-finite field value was seen by a step */
+
+typedef struct pfc_plan pfc_plan;
+
+/* Model, grid and numerics of one instance.  Field names follow
+ * Build2DGeometry / Build{Std,Log}PFCModelParameters. */
+typedef struct pfc_params {
+  int model;     /* PFC_MODEL_* */
+  int Nx, Ny;    /* Nx even; both factorable into 2,3,5,7,11,13 */
+  double Lx, Ly; /* box lengths: dx = Lx/Nx, dy = Ly/Ny */
+  double Gamma, beta, q0, epsilon, g, dt;
+  double minPhi;    /* logPFC clamp floor (used when clamp_log) */
+  double smallZTol; /* ETD1 Taylor switch |L dt| <= smallZTol (kernels.py:55-71) */
+  double M;         /* logPFC stabilising shift (0 = none) */
+  int precision;    /* 64: FP64 bulk; 32: FP32 bulk with FP64-built tables and FP64 reductions */
+  int scheme;       /* PFC_SCHEME_* */
+  int clamp_log;    /* logPFC: ln(phi) -> ln(minPhi) where phi <= minPhi */
+  int pin_mean;     /* pin the k=0 mode of every update to the mean captured by pfc_set_field */
+} pfc_params;
+
+const char* pfc_last_error(void);
+int pfc_version(void);
+
+int pfc_create(const pfc_params* params, pfc_plan** out);
+int pfc_destroy(pfc_plan* plan);
+
+/* Load a real (Ny, Nx) device field; captures its FP64 mean as the k=0 pin. */
+int pfc_set_field(pfc_plan* plan, const void* field, int dtype, void* stream);
+/* Write the current real (Ny, Nx) field to device memory. */
+int pfc_get_field(pfc_plan* plan, void* field, int dtype, void* stream);
+/* Advance nsteps time steps (asynchronous). */
+int pfc_step(pfc_plan* plan, long long nsteps, void* stream);
+/* FP64 compensated reductions of the current field; synchronises `stream`.
+ * out[0] = free energy per site (the box mean of f), out[1] = mean,
+ * out[2] = min, out[3] = max. */
+int pfc_reduce(pfc_plan* plan, double out[4], void* stream);
+/* Rebuild the ETD tables with the stabilising shift M (logPFC). */
+int pfc_set_shift(pfc_plan* plan, double M);
+/* Read and clear the PFC_STATUS_* flags (synchronises `stream`). */
+int pfc_get_status(pfc_plan* plan, int* flags, void* stream);
+/* pfc_step with a HIP event pair around every kernel launch, for roofline
+ * figures: ms[k] accumulates the device time of kernel kind k and calls[k]
+ * its launch count; k = 0 row pass, 1 first column pass, 2 ETDRK2 second
+ * column pass.  Synchronises `stream`.  (No reference counterpart: profiling
+ * instrumentation lives outside the model code, README.md:59-60.) */
+int pfc_step_profile(pfc_plan* plan, long long nsteps, void* stream, double ms[3], long long calls[3]);
+/* Device bytes held by a plan (state, tables, scratch). */
+int pfc_plan_bytes(const pfc_plan* plan, long long* bytes);
+
+/* Batched 2D real transforms over (batch, Ny, Nx) device arrays.
+ * pfc_rfft2:  out[b] = rfft2(in[b])             (unnormalised)
+ * pfc_irfft2: out[b] = irfft2(in[b], s=(Ny,Nx)) (scaled 1/(Nx Ny); `in` is not modified) */
+int pfc_rfft2(const void* in, void* out, int batch, int Ny, int Nx, int dtype, void* stream);
+int pfc_irfft2(const void* in, void* out, int batch, int Ny, int Nx, int dtype, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PFC_HIP_H */

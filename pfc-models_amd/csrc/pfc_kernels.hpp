@@ -1,0 +1,480 @@
+// Kernels of the fused ETD step (generic mixed-radix path).
+//
+// State layout in HBM.  An instance of Ny x Nx real points is held as the
+// row spectrum S~[y][kx] = (1/Nx) * sum_x n[y][x] exp(-2 pi i kx x / Nx),
+// kx = 0..Nx/2 (row-major, Nc = Nx/2+1 complex per row).  Its column FFT is
+// n_hat/Nx, with n_hat = numpy.fft.rfft2(n) (ky in fftfreq order).
+//
+// One ETD1 step = two kernels:
+//   row kernel : S~ row -> C2R (length-Nx/2 complex IFFT + pack) -> n
+//                -> N(n) pointwise -> R2C -> T row      (reads S~, writes T)
+//   col kernel : column tile of S~ and T -> forward column FFTs -> A = n_hat/Nx,
+//                B = N_hat -> out = E1 A + Q1 B (k=0 pinned) -> inverse column FFT
+//                -> S~ (in place)                        (reads S~ and T, writes S~)
+// with E1 = E/Ny and Q1 = P Gamma lap / (Nx Ny) precomputed in FP64 on the host.
+// ETDRK2 adds a second row pass and a second column pass; the first column
+// pass also writes D = out1 - Q2 B1 (column-major), the second forms
+// out2 = D + Q2 B2.
+#pragma once
+#include "pfc_fft.hpp"
+
+namespace pfc {
+
+enum { MODEL_STD = 0, MODEL_LOG = 1 };
+
+struct Nonlin {
+  int model;
+  int clamp;      // logPFC: ln(phi) -> ln(minPhi) where phi <= minPhi
+  double g;
+  double minPhi;
+  double logMin;  // ln(minPhi)
+  double M;       // stabilising shift: N -= M phi
+};
+
+template <typename T>
+__device__ __forceinline__ T nonlin_mu(T f, const Nonlin& q) {
+  // std: g n^2 + n^3 ; log: ln(phi) + g phi^2 + phi^3 - M phi
+  T g = T(q.g);
+  T poly = g * (f * f) + (f * f * f);
+  if (q.model == MODEL_STD) return poly;
+  T lg;
+  if (q.clamp) {
+    lg = (f > T(q.minPhi)) ? log(f) : T(q.logMin);
+  } else {
+    lg = log(f);
+  }
+  return lg + poly - T(q.M) * f;
+}
+
+struct RowGeom {
+  int Nx, M, Nc, Ny;
+  int nl;   // rows per block
+  int ld;   // LDS line stride (complex elements), >= Nc
+  Radices plM;
+};
+
+struct ColGeom {
+  int Nc, Ny;
+  int nc;   // columns per block
+  int ld;   // LDS line stride, >= Ny
+  Radices plY;
+};
+
+// ---------------------------------------------------------------------------
+// row-side helpers
+// ---------------------------------------------------------------------------
+// Load rows of the half spectrum (Nc per row) into LDS and form the packed
+// length-M spectrum Z[k] = (X[k] + conj X[M-k]) + i w^-k (X[k] - conj X[M-k]),
+// whose unnormalised inverse is x[2m] + i x[2m+1] when X = rfft(x)/Nx.
+// The imaginary parts of X[0] and X[M] are dropped, as C2R transforms do.
+template <typename T, typename Tin>
+__device__ __forceinline__ void row_load_pack(const V2<Tin>* __restrict__ S, V2<T>* stage, V2<T>* out,
+                                              const RowGeom& g, int row0, int nrows,
+                                              const V2<T>* __restrict__ twN) {
+  using V = V2<T>;
+  const int Nc = g.Nc, M = g.M, ld = g.ld;
+  for (int e = threadIdx.x; e < nrows * Nc; e += blockDim.x) {
+    const int line = e / Nc;
+    const int k = e - line * Nc;
+    V2<Tin> v = S[(size_t)(row0 + line) * Nc + k];
+    stage[line * ld + k] = V{T(v.x), T(v.y)};
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < nrows * M; e += blockDim.x) {
+    const int line = e / M;
+    const int k = e - line * M;
+    V xk = stage[line * ld + k];
+    V xm = stage[line * ld + (M - k)];
+    if (k == 0) {
+      xk.y = T(0);
+      xm.y = T(0);
+    }
+    V s = V{xk.x + xm.x, xk.y - xm.y};  // X[k] + conj X[M-k]
+    V d = V{xk.x - xm.x, xk.y + xm.y};  // X[k] - conj X[M-k]
+    V w = twN[k];                        // exp(-2 pi i k / Nx)
+    V wd = cmul(cconj(w), d);            // w^-k d
+    out[line * ld + k] = V{s.x - wd.y, s.y + wd.x};  // s + i wd
+  }
+  __syncthreads();
+}
+
+// Unpack the length-M forward FFT Z of packed real data into the half
+// spectrum Y[k] = 1/2 [(Z[k] + conj Z[M-k]) - i w^k (Z[k] - conj Z[M-k])], k=0..M.
+template <typename T>
+__device__ __forceinline__ V2<T> row_unpack(const V2<T>* Z, int k, int M, const V2<T>* __restrict__ twN) {
+  using V = V2<T>;
+  V zk = Z[k == M ? 0 : k];
+  V zm = Z[k == 0 ? 0 : M - k];
+  V s = V{zk.x + zm.x, zk.y - zm.y};
+  V d = V{zk.x - zm.x, zk.y + zm.y};
+  V wd = cmul(twN[k], d);
+  // s - i wd
+  return V{T(0.5) * (s.x + wd.y), T(0.5) * (s.y - wd.x)};
+}
+
+// ---------------------------------------------------------------------------
+// row kernel: S~ -> n -> N(n) -> T
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void __launch_bounds__(256) k_row_nonlin(const V2<T>* __restrict__ S, V2<T>* __restrict__ Tout,
+                                                    RowGeom g, Nonlin q, const V2<T>* __restrict__ twN,
+                                                    int* __restrict__ status) {
+  using V = V2<T>;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  V* b0 = reinterpret_cast<V*>(smem_raw);
+  V* b1 = b0 + g.nl * g.ld;
+  const int row0 = blockIdx.x * g.nl;
+  const int nrows = min(g.nl, g.Ny - row0);
+  row_load_pack<T, T>(S, b1, b0, g, row0, nrows, twN);
+  V* z = fft_lines<T, true>(b0, b1, g.plM, nrows, g.ld, twN, 2);
+  bool bad = false;
+  for (int e = threadIdx.x; e < nrows * g.M; e += blockDim.x) {
+    const int line = e / g.M;
+    const int m = e - line * g.M;
+    V v = z[line * g.ld + m];
+    bad |= !(isfinite(v.x) && isfinite(v.y));
+    z[line * g.ld + m] = V{nonlin_mu<T>(v.x, q), nonlin_mu<T>(v.y, q)};
+  }
+  if (bad) atomicOr(status, 1);
+  __syncthreads();
+  V* other = (z == b0) ? b1 : b0;
+  V* Z = fft_lines<T, false>(z, other, g.plM, nrows, g.ld, twN, 2);
+  for (int e = threadIdx.x; e < nrows * g.Nc; e += blockDim.x) {
+    const int line = e / g.Nc;
+    const int k = e - line * g.Nc;
+    Tout[(size_t)(row0 + line) * g.Nc + k] = row_unpack<T>(Z + line * g.ld, k, g.M, twN);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// column kernel: the ETD combine between forward and inverse column FFTs
+// ---------------------------------------------------------------------------
+enum { COL_ETD1 = 0, COL_RK2_STAGE1 = 1, COL_RK2_STAGE2 = 2 };
+
+struct ColTables {
+  const void* E1;  // column-major [Nc][Ny], storage type
+  const void* Q1;
+  const void* Q2;
+  const double* pin;  // device scalar: pinned k=0 value (the mean), or null
+};
+
+constexpr int kColRegs = 16;  // A values held per thread between the two forward FFTs
+
+__device__ __forceinline__ int xcd_remap(int b, int nb) {
+  // Workgroups are dealt round-robin to the 8 XCDs; give each XCD a contiguous
+  // run of tiles so neighbouring columns share that XCD's L2.
+  const int q = nb >> 3, r = nb & 7;
+  const int x = b & 7, l = b >> 3;
+  return x < r ? x * (q + 1) + l : r * (q + 1) + (x - r) * q + l;
+}
+
+template <typename T>
+__device__ __forceinline__ void col_load(const V2<T>* __restrict__ src, V2<T>* buf, const ColGeom& g, int c0,
+                                         int ncols) {
+  for (int e = threadIdx.x; e < ncols * g.Ny; e += blockDim.x) {
+    const int y = e / ncols;
+    const int line = e - y * ncols;
+    buf[line * g.ld + y] = src[(size_t)y * g.Nc + c0 + line];
+  }
+}
+
+template <typename T, int MODE>
+__global__ void __launch_bounds__(256) k_col_etd(V2<T>* __restrict__ S, const V2<T>* __restrict__ Tin,
+                                                 V2<T>* __restrict__ D, ColGeom g, ColTables tb,
+                                                 const V2<T>* __restrict__ twY) {
+  using V = V2<T>;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  V* b0 = reinterpret_cast<V*>(smem_raw);
+  V* b1 = b0 + g.nc * g.ld;
+  const int ntile = (g.Nc + g.nc - 1) / g.nc;
+  const int c0 = xcd_remap(blockIdx.x, ntile) * g.nc;
+  const int ncols = min(g.nc, g.Nc - c0);
+  const int nel = ncols * g.Ny;
+  const T* E1 = reinterpret_cast<const T*>(tb.E1);
+  const T* Q1 = reinterpret_cast<const T*>(tb.Q1);
+  const T* Q2 = reinterpret_cast<const T*>(tb.Q2);
+
+  V a[kColRegs];
+  if (MODE != COL_RK2_STAGE2) {
+    col_load<T>(S, b0, g, c0, ncols);
+    __syncthreads();
+    V* A = fft_lines<T, false>(b0, b1, g.plY, ncols, g.ld, twY, 1);
+#pragma unroll
+    for (int i = 0; i < kColRegs; ++i) {
+      const int e = threadIdx.x + i * blockDim.x;
+      if (e < nel) {
+        const int line = e / g.Ny;
+        const int ky = e - line * g.Ny;
+        a[i] = A[line * g.ld + ky];
+      }
+    }
+    __syncthreads();
+  }
+  col_load<T>(Tin, b0, g, c0, ncols);
+  __syncthreads();
+  V* B = fft_lines<T, false>(b0, b1, g.plY, ncols, g.ld, twY, 1);
+  const double pin = tb.pin ? *tb.pin : 0.0;
+#pragma unroll
+  for (int i = 0; i < kColRegs; ++i) {
+    const int e = threadIdx.x + i * blockDim.x;
+    if (e < nel) {
+      const int line = e / g.Ny;
+      const int ky = e - line * g.Ny;
+      const size_t ti = (size_t)(c0 + line) * g.Ny + ky;
+      V b = B[line * g.ld + ky];
+      V out;
+      if (MODE == COL_RK2_STAGE2) {
+        const T q2 = Q2[ti];
+        V d = D[ti];
+        out = V{d.x + q2 * b.x, d.y + q2 * b.y};
+      } else {
+        const T e1 = E1[ti], q1 = Q1[ti];
+        out = V{e1 * a[i].x + q1 * b.x, e1 * a[i].y + q1 * b.y};
+        if (MODE == COL_RK2_STAGE1) {
+          const T q2 = Q2[ti];
+          D[ti] = V{out.x - q2 * b.x, out.y - q2 * b.y};
+        }
+      }
+      if (tb.pin && c0 + line == 0 && ky == 0) out = V{T(pin), T(0)};
+      B[line * g.ld + ky] = out;
+    }
+  }
+  __syncthreads();
+  V* other = (B == b0) ? b1 : b0;
+  V* R = fft_lines<T, true>(B, other, g.plY, ncols, g.ld, twY, 1);
+  for (int e = threadIdx.x; e < nel; e += blockDim.x) {
+    const int y = e / ncols;
+    const int line = e - y * ncols;
+    V v = R[line * g.ld + y];
+    // The kx = 0 and kx = Nx/2 columns of a real field's row spectrum are
+    // real.  Rounding leaves ~1e-17 imaginary parts there; the C2R of the next
+    // row pass never sees them, so nothing damps them and unstable modes would
+    // grow them as exp(L dt)^n.  Storing only the real part is the projection
+    // onto real fields that irfft2 performs every step in the reference.
+    if (c0 + line == 0 || c0 + line == g.Nc - 1) v.y = T(0);
+    S[(size_t)y * g.Nc + c0 + line] = v;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// plain transforms (field upload/download and the backend rfft2/irfft2)
+// ---------------------------------------------------------------------------
+// real rows (Nx) -> half spectrum rows, times `scale`.  blockIdx.y = batch.
+template <typename T, typename Tin>
+__global__ void __launch_bounds__(256) k_row_r2c(const Tin* __restrict__ x, V2<T>* __restrict__ S, RowGeom g,
+                                                 const V2<T>* __restrict__ twN, T scale, size_t in_bstride,
+                                                 size_t out_bstride) {
+  using V = V2<T>;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  V* b0 = reinterpret_cast<V*>(smem_raw);
+  V* b1 = b0 + g.nl * g.ld;
+  x += blockIdx.y * in_bstride;
+  S += blockIdx.y * out_bstride;
+  const int row0 = blockIdx.x * g.nl;
+  const int nrows = min(g.nl, g.Ny - row0);
+  for (int e = threadIdx.x; e < nrows * g.M; e += blockDim.x) {
+    const int line = e / g.M;
+    const int m = e - line * g.M;
+    const Tin* xr = x + (size_t)(row0 + line) * g.Nx;
+    b0[line * g.ld + m] = V{T(xr[2 * m]), T(xr[2 * m + 1])};
+  }
+  __syncthreads();
+  V* Z = fft_lines<T, false>(b0, b1, g.plM, nrows, g.ld, twN, 2);
+  for (int e = threadIdx.x; e < nrows * g.Nc; e += blockDim.x) {
+    const int line = e / g.Nc;
+    const int k = e - line * g.Nc;
+    V y = row_unpack<T>(Z + line * g.ld, k, g.M, twN);
+    S[(size_t)(row0 + line) * g.Nc + k] = V{y.x * scale, y.y * scale};
+  }
+}
+
+// half spectrum rows -> real rows, unnormalised inverse times `scale`.
+template <typename T, typename Tout>
+__global__ void __launch_bounds__(256) k_row_c2r(const V2<T>* __restrict__ S, Tout* __restrict__ x, RowGeom g,
+                                                 const V2<T>* __restrict__ twN, T scale, size_t in_bstride,
+                                                 size_t out_bstride) {
+  using V = V2<T>;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  V* b0 = reinterpret_cast<V*>(smem_raw);
+  V* b1 = b0 + g.nl * g.ld;
+  S += blockIdx.y * in_bstride;
+  x += blockIdx.y * out_bstride;
+  const int row0 = blockIdx.x * g.nl;
+  const int nrows = min(g.nl, g.Ny - row0);
+  row_load_pack<T, T>(S, b1, b0, g, row0, nrows, twN);
+  V* z = fft_lines<T, true>(b0, b1, g.plM, nrows, g.ld, twN, 2);
+  for (int e = threadIdx.x; e < nrows * g.M; e += blockDim.x) {
+    const int line = e / g.M;
+    const int m = e - line * g.M;
+    V v = z[line * g.ld + m];
+    Tout* xr = x + (size_t)(row0 + line) * g.Nx;
+    xr[2 * m] = Tout(v.x * scale);
+    xr[2 * m + 1] = Tout(v.y * scale);
+  }
+}
+
+// column FFT of a row-major (Ny, Nc) complex array, in place, times `scale`.
+template <typename T, bool INV>
+__global__ void __launch_bounds__(256) k_col_fft(V2<T>* __restrict__ S, ColGeom g, const V2<T>* __restrict__ twY,
+                                                 T scale, size_t bstride) {
+  using V = V2<T>;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  V* b0 = reinterpret_cast<V*>(smem_raw);
+  V* b1 = b0 + g.nc * g.ld;
+  S += blockIdx.y * bstride;
+  const int c0 = blockIdx.x * g.nc;
+  const int ncols = min(g.nc, g.Nc - c0);
+  col_load<T>(S, b0, g, c0, ncols);
+  __syncthreads();
+  V* R = fft_lines<T, INV>(b0, b1, g.plY, ncols, g.ld, twY, 1);
+  for (int e = threadIdx.x; e < ncols * g.Ny; e += blockDim.x) {
+    const int y = e / ncols;
+    const int line = e - y * ncols;
+    V v = R[line * g.ld + y];
+    S[(size_t)y * g.Nc + c0 + line] = V{v.x * scale, v.y * scale};
+  }
+}
+
+// ---------------------------------------------------------------------------
+// FP64 reductions: free energy per site, mean, min, max
+// ---------------------------------------------------------------------------
+struct FeParams {
+  int model;
+  double beta, q0, epsilon, g;
+  const double* kx2;  // [Nc]  kx^2
+  const double* ky2;  // [Ny]  ky^2
+};
+
+// per-block partials: [0] sum n, [1] sum f_local, [2] sum w |A|^2 Lf  (each DD),
+// plus min and max of n.
+struct Partial {
+  DD s[3];
+  double mn, mx;
+};
+
+__device__ __forceinline__ double local_fe(double f, const FeParams& p) {
+  double poly = (p.epsilon / 2.0) * (f * f) + (p.g / 3.0) * (f * f * f) + 0.25 * (f * f * f * f);
+  if (p.model == MODEL_STD) return poly;
+  return f * log(f) + poly;
+}
+
+// `red` is LDS scratch of blockDim.x entries (blockDim.x a power of two).
+__device__ __forceinline__ void block_reduce_partial(Partial& mine, Partial* red, Partial* out) {
+  // fixed-order tree over the block: deterministic for a given launch shape
+  red[threadIdx.x] = mine;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) {
+      Partial o = red[threadIdx.x + s];
+      Partial m = red[threadIdx.x];
+      for (int i = 0; i < 3; ++i) m.s[i] = dd_add(m.s[i], o.s[i]);
+      m.mn = fmin(m.mn, o.mn);
+      m.mx = fmax(m.mx, o.mx);
+      red[threadIdx.x] = m;
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *out = red[0];
+}
+
+// rows of S~ (storage Tin) -> n in FP64 -> sums of n and of the local FE density
+template <typename Tin>
+__global__ void __launch_bounds__(256) k_fe_row(const V2<Tin>* __restrict__ S, RowGeom g, FeParams p,
+                                                const V2<double>* __restrict__ twN, Partial* __restrict__ part) {
+  using V = V2<double>;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  V* b0 = reinterpret_cast<V*>(smem_raw);
+  V* b1 = b0 + g.nl * g.ld;
+  const int row0 = blockIdx.x * g.nl;
+  const int nrows = min(g.nl, g.Ny - row0);
+  row_load_pack<double, Tin>(S, b1, b0, g, row0, nrows, twN);
+  V* z = fft_lines<double, true>(b0, b1, g.plM, nrows, g.ld, twN, 2);
+  Partial mine;
+  for (int i = 0; i < 3; ++i) mine.s[i] = DD{0.0, 0.0};
+  mine.mn = INFINITY;
+  mine.mx = -INFINITY;
+  for (int e = threadIdx.x; e < nrows * g.M; e += blockDim.x) {
+    const int line = e / g.M;
+    const int m = e - line * g.M;
+    V v = z[line * g.ld + m];
+    mine.s[0] = dd_add(mine.s[0], v.x);
+    mine.s[0] = dd_add(mine.s[0], v.y);
+    mine.s[1] = dd_add(mine.s[1], local_fe(v.x, p));
+    mine.s[1] = dd_add(mine.s[1], local_fe(v.y, p));
+    mine.mn = fmin(mine.mn, fmin(v.x, v.y));
+    mine.mx = fmax(mine.mx, fmax(v.x, v.y));
+  }
+  __syncthreads();
+  block_reduce_partial(mine, reinterpret_cast<Partial*>(smem_raw), part + blockIdx.x);
+}
+
+// columns of S~ -> A = n_hat/Nx in FP64 -> sum over the half spectrum of
+// w(kx) |A|^2 (beta/2)(q0^2 - k^2)^2, w = 1 on kx = 0 and kx = Nx/2, else 2.
+template <typename Tin>
+__global__ void __launch_bounds__(256) k_fe_col(const V2<Tin>* __restrict__ S, ColGeom g, FeParams p, int Nx,
+                                                const V2<double>* __restrict__ twY, Partial* __restrict__ part) {
+  using V = V2<double>;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  V* b0 = reinterpret_cast<V*>(smem_raw);
+  V* b1 = b0 + g.nc * g.ld;
+  const int c0 = blockIdx.x * g.nc;
+  const int ncols = min(g.nc, g.Nc - c0);
+  for (int e = threadIdx.x; e < ncols * g.Ny; e += blockDim.x) {
+    const int y = e / ncols;
+    const int line = e - y * ncols;
+    V2<Tin> v = S[(size_t)y * g.Nc + c0 + line];
+    b0[line * g.ld + y] = V{double(v.x), double(v.y)};
+  }
+  __syncthreads();
+  V* A = fft_lines<double, false>(b0, b1, g.plY, ncols, g.ld, twY, 1);
+  Partial mine;
+  for (int i = 0; i < 3; ++i) mine.s[i] = DD{0.0, 0.0};
+  mine.mn = INFINITY;
+  mine.mx = -INFINITY;
+  for (int e = threadIdx.x; e < ncols * g.Ny; e += blockDim.x) {
+    const int line = e / g.Ny;
+    const int ky = e - line * g.Ny;
+    const int kx = c0 + line;
+    V a = A[line * g.ld + ky];
+    const double lap = -(p.kx2[kx] + p.ky2[ky]);
+    const double t = p.q0 * p.q0 + lap;
+    const double lf = (p.beta / 2.0) * (t * t);
+    const double w = (kx == 0 || 2 * kx == Nx) ? 1.0 : 2.0;
+    mine.s[2] = dd_add(mine.s[2], w * lf * (a.x * a.x + a.y * a.y));
+  }
+  __syncthreads();
+  block_reduce_partial(mine, reinterpret_cast<Partial*>(smem_raw), part + blockIdx.x);
+}
+
+// one block: fixed-order sum of the row and column partials.
+// out[0] = free energy per site, out[1] = mean, out[2] = min, out[3] = max
+__global__ void __launch_bounds__(256) k_fe_final(const Partial* __restrict__ part, int nrow, int ncol, int Nx,
+                                                  int Ny, double* __restrict__ out, double* __restrict__ pin_out) {
+  Partial mine;
+  for (int i = 0; i < 3; ++i) mine.s[i] = DD{0.0, 0.0};
+  mine.mn = INFINITY;
+  mine.mx = -INFINITY;
+  for (int b = threadIdx.x; b < nrow + ncol; b += blockDim.x) {
+    Partial o = part[b];
+    for (int i = 0; i < 3; ++i) mine.s[i] = dd_add(mine.s[i], o.s[i]);
+    mine.mn = fmin(mine.mn, o.mn);
+    mine.mx = fmax(mine.mx, o.mx);
+  }
+  __shared__ Partial red[256];
+  __shared__ Partial res;
+  block_reduce_partial(mine, red, &res);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const double N = double(Nx) * double(Ny);
+    const double grad = (res.s[2].hi + res.s[2].lo) / (double(Ny) * double(Ny));
+    const double loc = (res.s[1].hi + res.s[1].lo) / N;
+    const double mean = (res.s[0].hi + res.s[0].lo) / N;
+    out[0] = grad + loc;
+    out[1] = mean;
+    out[2] = res.mn;
+    out[3] = res.mx;
+    if (pin_out) *pin_out = mean;
+  }
+}
+
+}  // namespace pfc

@@ -1,0 +1,126 @@
+"""HIP path against the FP64 oracle on identical seeded inputs (needs a GPU).
+
+Tolerances (stated per test):
+  FFT round trips            rel L2 <= 1e-14 (FP64), 3e-6 (FP32)
+  FP64 step, 100 steps        max|d psi| <= 1e-10          (north-star check (a))
+  recorded FEs (FP64)         rtol 1e-10                    (the reference's own tolerance)
+  free energy / mean (FP64)   rel 1e-12 against the oracle on the same field
+"""
+import numpy as np
+import pytest
+
+from oracle import pfc_oracle as O
+from pfc_hip import fields as F
+
+pytestmark = pytest.mark.gpu
+
+
+def _plan(geom, precision=64, **kw):
+    from pfc_hip.plan import Plan
+    return Plan(Nx=geom.Nx, Ny=geom.Ny, Lx=geom.Lx, Ly=geom.Ly, precision=precision, **kw)
+
+
+@pytest.mark.parametrize("shape", [(24, 28), (64, 64), (132, 132), (48, 96), (1024, 1024), (40, 90)])
+def test_rfft2_irfft2_match_numpy(hip_device, shape):
+    import torch
+    from pfc_hip.plan import rfft2, irfft2
+    rng = np.random.default_rng(11)
+    x = rng.normal(size=(3, *shape))
+    X = rfft2(torch.as_tensor(x, device="cuda")).cpu().numpy()
+    ref = np.fft.rfft2(x)
+    assert np.linalg.norm(X - ref) / np.linalg.norm(ref) < 1e-14
+    # inverse of a non-Hermitian-consistent spectrum: numpy drops the imaginary
+    # parts of the kx=0 / Nyquist columns after the column inverse; so must we
+    Z = ref + 1e-3 * (rng.normal(size=ref.shape) + 1j * rng.normal(size=ref.shape))
+    z = irfft2(torch.as_tensor(Z, device="cuda"), shape).cpu().numpy()
+    zr = np.fft.irfft2(Z, s=shape)
+    assert np.linalg.norm(z - zr) / np.linalg.norm(zr) < 1e-14
+
+
+def test_rfft2_fp32(hip_device):
+    import torch
+    from pfc_hip.plan import rfft2, irfft2
+    rng = np.random.default_rng(12)
+    x = rng.normal(size=(2, 256, 384)).astype(np.float32)
+    X = rfft2(torch.as_tensor(x, device="cuda")).cpu().numpy()
+    ref = np.fft.rfft2(x.astype(np.float64))
+    assert np.linalg.norm(X - ref) / np.linalg.norm(ref) < 3e-6
+    back = irfft2(torch.as_tensor(X, device="cuda"), (256, 384)).cpu().numpy()
+    assert np.linalg.norm(back - x) / np.linalg.norm(x) < 3e-6
+
+
+def _config1(scheme="etd1"):
+    geom = O.build_geometry(128, 128, 128 * np.pi / 4, 128 * np.pi / 4)
+    p = O.Params(model="std", Gamma=1.0, beta=1.0, q0=1.0, epsilon=-0.25, g=0.0, dt=0.5, scheme=scheme)
+    psi0 = F.uniform_noise((128, 128), -0.25, 0.05, seed=1)
+    return geom, p, psi0
+
+
+@pytest.mark.parametrize("scheme", ["etd1", "etdrk2"])
+def test_fp64_step_matches_oracle_100_steps(hip_device, scheme):
+    """North-star check (a): max|d psi| <= 1e-10 after 100 FP64 steps (128^2, r = psibar = -0.25, dt = 0.5)."""
+    geom, p, psi0 = _config1(scheme)
+    sim = O.Simulation(p, geom)
+    sim.set_field(psi0)
+    sim.step(100)
+    pl = _plan(geom, epsilon=-0.25, g=0.0, dt=0.5, scheme=scheme)
+    pl.set_field(psi0)
+    pl.step(100)
+    got = pl.get_field().cpu().numpy()
+    err = np.abs(got - sim.field).max()
+    assert err <= 1e-10, err
+    fe, mean, mn, mx = pl.reduce()
+    assert fe == pytest.approx(sim.free_energy(), rel=1e-12)
+    assert mean == pytest.approx(sim.mean(), rel=1e-13, abs=1e-15)
+    assert mn == pytest.approx(sim.field.min(), abs=1e-10)
+    assert mx == pytest.approx(sim.field.max(), abs=1e-10)
+
+
+def test_recorded_stdpfc_free_energy(hip_device):
+    """models/stdpfc_cpu/test_regression.py:64-115 on the HIP FP64 path (132^2 = (2^2 3 11)^2)."""
+    a, h = 4 * np.pi / 3**0.5, 4 * np.pi
+    c12, c34, c14 = (3/4)**0.5, (3/4)**0.75, (3/4)**0.25
+    geom = O.build_geometry(132, 132, 14 * a, 8 * h)
+    pl = _plan(geom, beta=c12 * 0.5, epsilon=c12 * (-0.9 + 2*(-2.0) + 4), g=c34 * (-2.0 + 2.5), dt=0.1)
+    pl.set_field(F.GeneratePerfectCrystalInitialWithCentralVacancy(geom, c14*(0.55-1), c14*(0.30-1), A0=c14*0.19))
+    m0 = pl.mean()
+    pl.step(200)
+    fe, mean, _, _ = pl.reduce()
+    assert abs(mean - m0) < 1e-12
+    assert fe == pytest.approx(-0.11237092135766821, rel=1e-10)
+
+
+def test_recorded_logpfc_free_energies(hip_device):
+    """models/logpfc_cpu/test_regression.py:28-66 on the HIP FP64 path (28 x 24 = 2^2 7 x 2^3 3)."""
+    Q, MEAN = 0.9648639799649456, 0.6326
+    a, h = 4 * np.pi / (np.sqrt(3) * Q), 4 * np.pi / Q
+    geom = O.build_geometry(28, 24, 2 * a, h)
+    X, Y = geom.X, geom.Y
+    phi = MEAN * (1 + 0.15 * (2 * np.cos(np.sqrt(3) / 2 * Q * X) * np.cos(Q * Y / 2) + np.cos(Q * Y)))
+    phi += 1e-3 * np.random.default_rng(7).standard_normal(phi.shape)
+    phi += MEAN - phi.mean()
+    pl = _plan(geom, model="log", beta=0.5, epsilon=-0.9, g=-2.0, dt=0.1)
+    pl.set_field(phi)
+    ref = {10: -0.5162767800170736, 100: -0.554845641895177, 1000: -0.5548456430520675}
+    done = 0
+    for k in sorted(ref):
+        pl.step(k - done)
+        done = k
+        assert pl.free_energy() == pytest.approx(ref[k], rel=1e-10), k
+    assert pl.mean() == pytest.approx(MEAN, abs=1e-12)
+
+
+def test_reductions_match_oracle_on_random_field(hip_device):
+    geom = O.build_geometry(96, 80, 60.0, 50.0)
+    rng = np.random.default_rng(3)
+    f = 0.4 + 0.2 * rng.normal(size=(80, 96))
+    for model in ("std", "log"):
+        p = O.Params(model=model, beta=0.7, q0=1.1, epsilon=-0.3, g=0.4)
+        sim = O.Simulation(p, geom)
+        sim.set_field(np.abs(f) if model == "log" else f)
+        pl = _plan(geom, model=model, beta=0.7, q0=1.1, epsilon=-0.3, g=0.4)
+        pl.set_field(sim.field)
+        fe, mean, mn, mx = pl.reduce()
+        assert fe == pytest.approx(sim.free_energy(), rel=1e-12)
+        assert mean == pytest.approx(O.exact_mean(sim.field), rel=1e-14)
+        assert (mn, mx) == (pytest.approx(sim.field.min(), abs=1e-13), pytest.approx(sim.field.max(), abs=1e-13))
