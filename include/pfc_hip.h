@@ -91,6 +91,14 @@ int pfc_reduce(pfc_plan* plan, double out[4], void* stream);
 int pfc_set_shift(pfc_plan* plan, double M);
 /* Read and clear the PFC_STATUS_* flags (synchronises `stream`). */
 int pfc_get_status(pfc_plan* plan, int* flags, void* stream);
+/* Chemical potential of the current field, real (Ny, Nx) device outputs:
+ * mu = irfft2(L_mu n_hat + N_hat) and local_mu = irfft2(G (L_mu n_hat + N_hat)),
+ * G = exp(-k^2 w^2/2), w = 2 pi.  Replaces Batch2 (_Batch2_CalcMu_Terms ->
+ * State.mu / State.local_mu, models/stdpfc/simulation.py:217-219, :299-313). */
+int pfc_chemical_potential(pfc_plan* plan, void* mu, void* local_mu, int dtype, void* stream);
+/* Gaussian-smoothed field irfft2(G n_hat) into a real (Ny, Nx) device output.
+ * Replaces Get_local_n / Get_local_phi (models/stdpfc/simulation.py:237-247). */
+int pfc_local_field(pfc_plan* plan, void* out, int dtype, void* stream);
 /* pfc_step with a HIP event pair around every kernel launch, for roofline
  * figures: ms[k] accumulates the device time of kernel kind k and calls[k]
  * its launch count; k = 0 row pass, 1 first column pass, 2 ETDRK2 second

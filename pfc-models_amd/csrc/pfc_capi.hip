@@ -10,10 +10,12 @@
 #include <mutex>
 #include <string>
 #include <tuple>
+#include <type_traits>
+#include <cstdlib>
 #include <vector>
 
 #include "../../include/pfc_hip.h"
-#include "pfc_kernels.hpp"
+#include "pfc_fast.hpp"
 
 using namespace pfc;
 
@@ -184,6 +186,99 @@ int row_blocks(const RowGeom& g) { return (g.Ny + g.nl - 1) / g.nl; }
 int col_blocks(const ColGeom& g) { return (g.Nc + g.nc - 1) / g.nc; }
 
 // ---------------------------------------------------------------------------
+// fast (compile-time length) kernels: supported lengths and launch shapes
+// ---------------------------------------------------------------------------
+#define PFC_FAST_LENGTHS(X) X(32) X(48) X(64) X(96) X(128) X(192) X(256) X(384) X(512) X(768) X(1024) X(2048)
+
+int fast_points(int L) {
+  switch (L) {
+#define PFC_CASE(LEN) \
+  case LEN:           \
+    return FastSched<LEN>::P;
+    PFC_FAST_LENGTHS(PFC_CASE)
+#undef PFC_CASE
+    default:
+      return 0;
+  }
+}
+
+struct FastShape {
+  bool on = false;
+  int tpl = 0;      // lanes per line
+  int lines = 0;    // lines per block
+  int blocks = 0;
+  size_t lds = 0;
+};
+
+// rows: lines = rows per block; aim for >= 512 blocks and 64..256 threads
+FastShape fast_row_shape(int Nx, int Ny, size_t csize) {
+  FastShape f;
+  const int M = Nx / 2, P = fast_points(M);
+  if (!P) return f;
+  f.tpl = M / P;
+  if (f.tpl > 256) return f;
+  const int lo = std::max(1, 64 / f.tpl), hi = std::max(1, 256 / f.tpl);
+  int w = std::max(lo, std::min(hi, Ny / 512));
+  while (w > lo && (size_t)w * (M + 1) * csize > (size_t)lds_limit()) --w;
+  if ((size_t)w * (M + 1) * csize > (size_t)lds_limit()) return f;
+  f.lines = w;
+  f.blocks = (Ny + w - 1) / w;
+  f.lds = (size_t)w * (M + 1) * csize;
+  f.on = true;
+  return f;
+}
+
+// columns: lines = columns per block; aim for >= 256 blocks
+FastShape fast_col_shape(int Nx, int Ny, size_t csize) {
+  FastShape f;
+  const int Nc = Nx / 2 + 1, P = fast_points(Ny);
+  if (!P) return f;
+  f.tpl = Ny / P;
+  if (f.tpl > 256) return f;
+  const int lo = std::max(1, 64 / f.tpl), hi = std::max(1, 256 / f.tpl);
+  int c = std::max(lo, std::min(hi, Nc / 256));
+  while (c > lo && (size_t)c * Ny * csize > (size_t)lds_limit()) --c;
+  if ((size_t)c * Ny * csize > (size_t)lds_limit()) return f;
+  f.lines = c;
+  f.blocks = (Nc + c - 1) / c;
+  f.lds = (size_t)c * Ny * csize;
+  f.on = true;
+  return f;
+}
+
+template <typename T>
+void launch_row_fast(int M, const FastShape& f, hipStream_t st, const V2<T>* S, V2<T>* Tb, int Ny, const Nonlin& q,
+                     const V2<T>* twN, int* status) {
+  switch (M) {
+#define PFC_CASE(LEN)                                                                                           \
+  case LEN:                                                                                                     \
+    hipLaunchKernelGGL((k_row_fast<T, LEN>), dim3(f.blocks), dim3(f.lines * f.tpl), f.lds, st, S, Tb, Ny, q, \
+                       twN, status);                                                                            \
+    return;
+    PFC_FAST_LENGTHS(PFC_CASE)
+#undef PFC_CASE
+    default:
+      fail(PFC_EUNSUPPORTED, "no fast row kernel for this length");
+  }
+}
+
+template <typename T, int MODE>
+void launch_col_fast(int Ny, const FastShape& f, hipStream_t st, V2<T>* S, const V2<T>* Tb, V2<T>* D, int Nc,
+                     const ColTables& tb, const V2<T>* twY) {
+  switch (Ny) {
+#define PFC_CASE(LEN)                                                                                          \
+  case LEN:                                                                                                    \
+    hipLaunchKernelGGL((k_col_fast<T, LEN, MODE>), dim3(f.blocks), dim3(f.lines * f.tpl), f.lds, st, S, Tb, D, \
+                       Nc, tb, twY);                                                                           \
+    return;
+    PFC_FAST_LENGTHS(PFC_CASE)
+#undef PFC_CASE
+    default:
+      fail(PFC_EUNSUPPORTED, "no fast column kernel for this length");
+  }
+}
+
+// ---------------------------------------------------------------------------
 // FP64 k-grids and ETD tables (the reference's host arithmetic, restated)
 // ---------------------------------------------------------------------------
 // kx = 2 pi * rfftfreq(Nx, dx), ky = 2 pi * fftfreq(Ny, dy)  (geometry_2d.py:154-155)
@@ -270,12 +365,14 @@ struct pfc_plan {
   size_t csize = 0;  // bytes per complex of the bulk precision
   RowGeom rg{}, rg64{};
   ColGeom cg{}, cg64{};
+  FastShape frow{}, fcol{};      // compile-time-length kernels, when the sizes have one
   DevBuf S, Tb, D;               // state, row spectrum of N, RK2 k-space carry
   DevBuf E1, Q1, Q2;             // ETD tables (bulk precision)
   DevBuf twN, twY;               // twiddles (bulk precision)
   DevBuf twN64, twY64;           // FP64 twiddles for the reductions
   DevBuf kx2, ky2;               // FP64 k^2 grids for the free energy
   DevBuf part;                   // reduction partials
+  DevBuf diag;                   // two row spectra for the chemical-potential diagnostics
   DevBuf scal;                   // [0..3] reduce out, [4] pin, int status at [5]
   std::vector<double> kx, ky;
   double M = 0.0;
@@ -346,24 +443,32 @@ void launch_step(pfc_plan* pl, long long nsteps, hipStream_t st, Prof* prof = nu
   const int rb = row_blocks(pl->rg), cb = col_blocks(pl->cg);
   auto B = [&]() { if (prof) prof->begin(); };
   auto E = [&](int k) { if (prof) prof->end(k); };
-  for (long long s = 0; s < nsteps; ++s) {
+  const int M = pl->p.Nx / 2, Ny = pl->p.Ny, Nc = pl->Nc;
+  auto row = [&]() {
     B();
-    hipLaunchKernelGGL((k_row_nonlin<T>), dim3(rb), dim3(256), rl, st, S, Tb, pl->rg, q, twN, pl->status());
-    E(0);
-    if (pl->p.scheme == PFC_SCHEME_ETD1) {
-      B();
-      hipLaunchKernelGGL((k_col_etd<T, COL_ETD1>), dim3(cb), dim3(256), cl, st, S, Tb, D, pl->cg, tb, twY);
-      E(1);
-    } else {
-      B();
-      hipLaunchKernelGGL((k_col_etd<T, COL_RK2_STAGE1>), dim3(cb), dim3(256), cl, st, S, Tb, D, pl->cg, tb, twY);
-      E(1);
-      B();
+    if (pl->frow.on)
+      launch_row_fast<T>(M, pl->frow, st, S, Tb, Ny, q, twN, pl->status());
+    else
       hipLaunchKernelGGL((k_row_nonlin<T>), dim3(rb), dim3(256), rl, st, S, Tb, pl->rg, q, twN, pl->status());
-      E(0);
-      B();
-      hipLaunchKernelGGL((k_col_etd<T, COL_RK2_STAGE2>), dim3(cb), dim3(256), cl, st, S, Tb, D, pl->cg, tb, twY);
-      E(2);
+    E(0);
+  };
+  auto col = [&](auto mode_tag, int kind) {
+    constexpr int MODE = decltype(mode_tag)::value;
+    B();
+    if (pl->fcol.on)
+      launch_col_fast<T, MODE>(Ny, pl->fcol, st, S, Tb, D, Nc, tb, twY);
+    else
+      hipLaunchKernelGGL((k_col_etd<T, MODE>), dim3(cb), dim3(256), cl, st, S, Tb, D, pl->cg, tb, twY);
+    E(kind);
+  };
+  for (long long s = 0; s < nsteps; ++s) {
+    row();
+    if (pl->p.scheme == PFC_SCHEME_ETD1) {
+      col(std::integral_constant<int, COL_ETD1>{}, 1);
+    } else {
+      col(std::integral_constant<int, COL_RK2_STAGE1>{}, 1);
+      row();
+      col(std::integral_constant<int, COL_RK2_STAGE2>{}, 2);
     }
   }
   hip_check(hipGetLastError(), "step launch");
@@ -435,6 +540,48 @@ FftEntry* fft_entry(int Ny, int Nx, int dtype) {
 
 }  // namespace
 
+namespace {
+
+template <typename T>
+void launch_filter(pfc_plan* pl, int mode, void* out0, void* out1, int dtype, hipStream_t st) {
+  using V = V2<T>;
+  const size_t nc = (size_t)pl->Nc * pl->p.Ny;
+  if (pl->diag.n < 2 * nc * sizeof(V)) pl->diag.alloc(2 * nc * sizeof(V));
+  V* d0 = reinterpret_cast<V*>(pl->diag.p);
+  V* d1 = d0 + nc;
+  const V* S = reinterpret_cast<const V*>(pl->S.p);
+  V* Tb = reinterpret_cast<V*>(pl->Tb.p);
+  const V* twN = reinterpret_cast<const V*>(pl->twN.p);
+  const V* twY = reinterpret_cast<const V*>(pl->twY.p);
+  FilterParams fp{pl->p.beta, pl->p.q0, pl->p.epsilon, pl->M, 6.283185307179586, pl->p.Nx,
+                  reinterpret_cast<const double*>(pl->kx2.p), reinterpret_cast<const double*>(pl->ky2.p)};
+  const int rb = row_blocks(pl->rg), cb = col_blocks(pl->cg);
+  const size_t rl = row_lds(pl->rg, sizeof(V)), cl = col_lds(pl->cg, sizeof(V));
+  if (mode == FILTER_MU) {
+    hipLaunchKernelGGL((k_row_nonlin<T>), dim3(rb), dim3(256), rl, st, S, Tb, pl->rg, nonlin_of(pl), twN,
+                       pl->status());
+    hipLaunchKernelGGL((k_col_filter<T, FILTER_MU>), dim3(cb), dim3(256), cl, st, S, Tb, d0, d1, pl->cg, fp, twY);
+  } else {
+    hipLaunchKernelGGL((k_col_filter<T, FILTER_LOCAL>), dim3(cb), dim3(256), cl, st, S, Tb, d0, d1, pl->cg, fp,
+                       twY);
+  }
+  const T scale = T(1.0 / ((double)pl->p.Nx * (double)pl->p.Ny));
+  const int nout = mode == FILTER_MU ? 2 : 1;
+  for (int o = 0; o < nout; ++o) {
+    const V* src = o == 0 ? d0 : d1;
+    void* dst = o == 0 ? out0 : out1;
+    if (dtype == 64)
+      hipLaunchKernelGGL((k_row_c2r<T, double>), dim3(rb), dim3(256), rl, st, src, (double*)dst, pl->rg, twN, scale,
+                         (size_t)0, (size_t)0);
+    else
+      hipLaunchKernelGGL((k_row_c2r<T, float>), dim3(rb), dim3(256), rl, st, src, (float*)dst, pl->rg, twN, scale,
+                         (size_t)0, (size_t)0);
+  }
+  hip_check(hipGetLastError(), "diagnostic launch");
+}
+
+}  // namespace
+
 // ---------------------------------------------------------------------------
 // C ABI
 // ---------------------------------------------------------------------------
@@ -459,6 +606,10 @@ int pfc_create(const pfc_params* params, pfc_plan** out) {
     pl->cg = col_geom(Nx, Ny, pl->csize);
     pl->rg64 = row_geom(Nx, Ny, sizeof(double2));
     pl->cg64 = col_geom(Nx, Ny, sizeof(double2));
+    if (!std::getenv("PFC_HIP_GENERIC")) {
+      pl->frow = fast_row_shape(Nx, Ny, pl->csize);
+      pl->fcol = fast_col_shape(Nx, Ny, pl->csize);
+    }
     const size_t nc = (size_t)pl->Nc * Ny;
     pl->S.alloc(nc * pl->csize);
     pl->Tb.alloc(nc * pl->csize);
@@ -601,6 +752,32 @@ int pfc_get_status(pfc_plan* pl, int* flags, void* stream) {
   });
 }
 
+int pfc_chemical_potential(pfc_plan* pl, void* mu, void* local_mu, int dtype, void* stream) {
+  return guarded([&] {
+    if (!pl || !mu || !local_mu) fail(PFC_EINVAL, "NULL argument");
+    if (dtype != 64 && dtype != 32) fail(PFC_EINVAL, "dtype must be 64 or 32");
+    if (!pl->field_set) fail(PFC_EINVAL, "pfc_set_field has not been called");
+    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    if (pl->f64())
+      launch_filter<double>(pl, FILTER_MU, mu, local_mu, dtype, st);
+    else
+      launch_filter<float>(pl, FILTER_MU, mu, local_mu, dtype, st);
+  });
+}
+
+int pfc_local_field(pfc_plan* pl, void* out, int dtype, void* stream) {
+  return guarded([&] {
+    if (!pl || !out) fail(PFC_EINVAL, "NULL argument");
+    if (dtype != 64 && dtype != 32) fail(PFC_EINVAL, "dtype must be 64 or 32");
+    if (!pl->field_set) fail(PFC_EINVAL, "pfc_set_field has not been called");
+    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    if (pl->f64())
+      launch_filter<double>(pl, FILTER_LOCAL, out, nullptr, dtype, st);
+    else
+      launch_filter<float>(pl, FILTER_LOCAL, out, nullptr, dtype, st);
+  });
+}
+
 int pfc_step_profile(pfc_plan* pl, long long nsteps, void* stream, double ms[3], long long calls[3]) {
   Prof prof;
   int rc = guarded([&] {
@@ -629,7 +806,7 @@ int pfc_plan_bytes(const pfc_plan* pl, long long* bytes) {
     if (!pl || !bytes) fail(PFC_EINVAL, "NULL plan or bytes");
     size_t b = 0;
     for (const DevBuf* d : {&pl->S, &pl->Tb, &pl->D, &pl->E1, &pl->Q1, &pl->Q2, &pl->twN, &pl->twY, &pl->twN64,
-                            &pl->twY64, &pl->kx2, &pl->ky2, &pl->part, &pl->scal})
+                            &pl->twY64, &pl->kx2, &pl->ky2, &pl->part, &pl->scal, &pl->diag})
       b += d->n;
     *bytes = (long long)b;
   });

@@ -336,6 +336,110 @@ __global__ void __launch_bounds__(256) k_col_fft(V2<T>* __restrict__ S, ColGeom 
 }
 
 // ---------------------------------------------------------------------------
+// diagnostics: chemical potential and Gaussian-smoothed fields
+// ---------------------------------------------------------------------------
+// Column pass producing row spectra ready for k_row_c2r (scale 1/(Nx Ny)):
+//   FILTER_MU    out0 = IFFT_y(mu_hat), out1 = IFFT_y(G mu_hat),
+//                mu_hat = L_mu n_hat + N_hat, n_hat = Nx FFT_y(S~), N_hat = FFT_y(T)
+//   FILTER_LOCAL out0 = IFFT_y(G n_hat)
+// with L_mu = beta (q0^2 + lap)^2 + eps + M and G = exp(-k^2 w^2 / 2) evaluated in FP64.
+enum { FILTER_MU = 0, FILTER_LOCAL = 1 };
+
+struct FilterParams {
+  double beta, q0, epsilon, M, w;
+  int Nx;
+  const double* kx2;
+  const double* ky2;
+};
+
+template <typename T, int MODE>
+__global__ void __launch_bounds__(256) k_col_filter(const V2<T>* __restrict__ S, const V2<T>* __restrict__ Tin,
+                                                    V2<T>* __restrict__ out0, V2<T>* __restrict__ out1, ColGeom g,
+                                                    FilterParams fp, const V2<T>* __restrict__ twY) {
+  using V = V2<T>;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  V* b0 = reinterpret_cast<V*>(smem_raw);
+  V* b1 = b0 + g.nc * g.ld;
+  const int c0 = blockIdx.x * g.nc;
+  const int ncols = min(g.nc, g.Nc - c0);
+  const int nel = ncols * g.Ny;
+  V a[kColRegs];
+  col_load<T>(S, b0, g, c0, ncols);
+  __syncthreads();
+  V* A = fft_lines<T, false>(b0, b1, g.plY, ncols, g.ld, twY, 1);
+#pragma unroll
+  for (int i = 0; i < kColRegs; ++i) {
+    const int e = threadIdx.x + i * blockDim.x;
+    if (e < nel) {
+      const int line = e / g.Ny;
+      const int ky = e - line * g.Ny;
+      a[i] = A[line * g.ld + ky];
+    }
+  }
+  __syncthreads();
+  V* B = nullptr;
+  if (MODE == FILTER_MU) {
+    col_load<T>(Tin, b0, g, c0, ncols);
+    __syncthreads();
+    B = fft_lines<T, false>(b0, b1, g.plY, ncols, g.ld, twY, 1);
+  } else {
+    B = b0;
+  }
+  // mu_hat (or G n_hat) goes to B; the G-filtered copy of mu_hat stays in
+  // registers for the second output
+  V gm[kColRegs];
+#pragma unroll
+  for (int i = 0; i < kColRegs; ++i) {
+    const int e = threadIdx.x + i * blockDim.x;
+    if (e < nel) {
+      const int line = e / g.Ny;
+      const int ky = e - line * g.Ny;
+      const int kx = c0 + line;
+      const double k2 = fp.kx2[kx] + fp.ky2[ky];
+      const double t = fp.q0 * fp.q0 - k2;
+      const double lmu = fp.beta * (t * t) + fp.epsilon + fp.M;
+      const double gg = exp(-k2 * fp.w * fp.w / 2);
+      const double nx = double(fp.Nx);
+      if (MODE == FILTER_MU) {
+        V b = B[line * g.ld + ky];
+        V mu = V{T(lmu * nx * a[i].x + b.x), T(lmu * nx * a[i].y + b.y)};
+        B[line * g.ld + ky] = mu;
+        gm[i] = V{T(gg * mu.x), T(gg * mu.y)};
+      } else {
+        B[line * g.ld + ky] = V{T(gg * nx * a[i].x), T(gg * nx * a[i].y)};
+      }
+    }
+  }
+  __syncthreads();
+  V* other = (B == b0) ? b1 : b0;
+  V* R = fft_lines<T, true>(B, other, g.plY, ncols, g.ld, twY, 1);
+  for (int e = threadIdx.x; e < nel; e += blockDim.x) {
+    const int y = e / ncols;
+    const int line = e - y * ncols;
+    out0[(size_t)y * g.Nc + c0 + line] = R[line * g.ld + y];
+  }
+  if (MODE == FILTER_MU) {
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < kColRegs; ++i) {
+      const int e = threadIdx.x + i * blockDim.x;
+      if (e < nel) {
+        const int line = e / g.Ny;
+        const int ky = e - line * g.Ny;
+        b0[line * g.ld + ky] = gm[i];
+      }
+    }
+    __syncthreads();
+    V* R2 = fft_lines<T, true>(b0, b1, g.plY, ncols, g.ld, twY, 1);
+    for (int e = threadIdx.x; e < nel; e += blockDim.x) {
+      const int y = e / ncols;
+      const int line = e - y * ncols;
+      out1[(size_t)y * g.Nc + c0 + line] = R2[line * g.ld + y];
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
 // FP64 reductions: free energy per site, mean, min, max
 // ---------------------------------------------------------------------------
 struct FeParams {

@@ -58,6 +58,8 @@ _SIGNATURES = [
     ("pfc_reduce", ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_double), _P]),
     ("pfc_set_shift", ctypes.c_int, [_P, ctypes.c_double]),
     ("pfc_get_status", ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_int), _P]),
+    ("pfc_chemical_potential", ctypes.c_int, [_P, _P, _P, ctypes.c_int, _P]),
+    ("pfc_local_field", ctypes.c_int, [_P, _P, ctypes.c_int, _P]),
     ("pfc_step_profile", ctypes.c_int, [_P, ctypes.c_longlong, _P, ctypes.POINTER(ctypes.c_double),
                                         ctypes.POINTER(ctypes.c_longlong)]),
     ("pfc_plan_bytes", ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_longlong)]),

@@ -1,0 +1,1 @@
+"""log PFC on the hip backend (libpfc_hip.so on MI355X)."""

@@ -124,6 +124,27 @@ class Plan:
     def mean(self):
         return self.reduce()[1]
 
+    def chemical_potential(self, dtype=None):
+        """(mu, local_mu) of the current field as (Ny, Nx) device tensors."""
+        torch = self.torch
+        dt = dtype or torch.float64
+        mu = torch.empty((self.Ny, self.Nx), device="cuda", dtype=dt)
+        lmu = torch.empty_like(mu)
+        code = 64 if dt == torch.float64 else 32
+        _lib.check(self._lib.pfc_chemical_potential(self._h, ctypes.c_void_p(mu.data_ptr()),
+                                                    ctypes.c_void_p(lmu.data_ptr()), code, current_stream_handle()))
+        return mu, lmu
+
+    def local_field(self, dtype=None):
+        """Gaussian-smoothed field irfft2(exp(-k^2 (2 pi)^2 / 2) n_hat) as a device tensor."""
+        torch = self.torch
+        dt = dtype or torch.float64
+        out = torch.empty((self.Ny, self.Nx), device="cuda", dtype=dt)
+        code = 64 if dt == torch.float64 else 32
+        _lib.check(self._lib.pfc_local_field(self._h, ctypes.c_void_p(out.data_ptr()), code,
+                                             current_stream_handle()))
+        return out
+
     def set_shift(self, M):
         _lib.check(self._lib.pfc_set_shift(self._h, float(M)))
 

@@ -124,3 +124,59 @@ def test_reductions_match_oracle_on_random_field(hip_device):
         assert fe == pytest.approx(sim.free_energy(), rel=1e-12)
         assert mean == pytest.approx(O.exact_mean(sim.field), rel=1e-14)
         assert (mn, mx) == (pytest.approx(sim.field.min(), abs=1e-13), pytest.approx(sim.field.max(), abs=1e-13))
+
+
+def test_fp64_1024_etdrk2_matches_oracle(hip_device):
+    """The bench workload itself (1024^2 seeded crystals, ETDRK2 dt = 0.5): 10 FP64 steps
+    against the oracle, max|d psi| <= 1e-10."""
+    from pfc_hip.plan import Plan
+    n = 1024
+    L = n * np.pi / 4
+    geom = O.build_geometry(n, n, L, L)
+    psi0 = F.seeded_crystals(geom, -0.3, -0.3)
+    sim = O.Simulation(O.Params(model="std", epsilon=-0.3, g=0.0, dt=0.5, scheme="etdrk2"), geom)
+    sim.set_field(psi0)
+    sim.step(10)
+    pl = Plan(Nx=n, Ny=n, Lx=L, Ly=L, epsilon=-0.3, g=0.0, dt=0.5, scheme="etdrk2")
+    pl.set_field(psi0)
+    pl.step(10)
+    err = np.abs(pl.get_field().cpu().numpy() - sim.field).max()
+    assert err <= 1e-10, err
+    assert pl.free_energy() == pytest.approx(sim.free_energy(), rel=1e-12)
+
+
+@pytest.mark.parametrize("shape", [(128, 128), (96, 192), (256, 384), (64, 2048)])
+@pytest.mark.parametrize("precision", [64, 32])
+def test_fast_kernels_match_generic(hip_device, shape, precision, monkeypatch):
+    """The compile-time-length kernels and the generic mixed-radix kernels compute the
+    same step (FP64: <= 1e-13 after 20 ETDRK2 steps; FP32: <= 2e-5)."""
+    from pfc_hip.plan import Plan
+    ny, nx = shape
+    geom = O.build_geometry(nx, ny, nx * np.pi / 4, ny * np.pi / 4)
+    psi0 = F.uniform_noise(shape, -0.3, 0.05, seed=5)
+    kw = dict(Nx=nx, Ny=ny, Lx=geom.Lx, Ly=geom.Ly, epsilon=-0.3, g=0.2, dt=0.5, scheme="etdrk2",
+              precision=precision)
+    fast = Plan(**kw).set_field(psi0).step(20)
+    monkeypatch.setenv("PFC_HIP_GENERIC", "1")
+    gen = Plan(**kw).set_field(psi0).step(20)
+    d = np.abs(fast.get_field().cpu().numpy() - gen.get_field().cpu().numpy()).max()
+    assert d <= (1e-13 if precision == 64 else 2e-5), d
+
+
+def test_fp32_step_tolerance(hip_device):
+    """FP32 bulk against the FP64 oracle, 128^2 ETD1 x 100 (r = psibar = -0.25, dt = 0.5):
+    max|d psi| <= 1e-5, relative FE <= 1e-6, mean held exactly by the k = 0 pin (1e-7)."""
+    from pfc_hip.plan import Plan
+    geom, p, psi0 = _config1("etd1")
+    sim = O.Simulation(p, geom)
+    sim.set_field(psi0)
+    sim.step(100)
+    pl = Plan(Nx=128, Ny=128, Lx=geom.Lx, Ly=geom.Ly, epsilon=-0.25, g=0.0, dt=0.5, precision=32)
+    pl.set_field(psi0)
+    m0 = pl.mean()
+    pl.step(100)
+    err = np.abs(pl.get_field().cpu().numpy() - sim.field).max()
+    fe, mean, _, _ = pl.reduce()
+    assert err <= 1e-5, err
+    assert abs(fe - sim.free_energy()) <= 1e-6 * abs(sim.free_energy())
+    assert abs(mean - m0) <= 1e-7
