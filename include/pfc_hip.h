@@ -105,6 +105,11 @@ int pfc_local_field(pfc_plan* plan, void* out, int dtype, void* stream);
  * column pass.  Synchronises `stream`.  (No reference counterpart: profiling
  * instrumentation lives outside the model code, README.md:59-60.) */
 int pfc_step_profile(pfc_plan* plan, long long nsteps, void* stream, double ms[3], long long calls[3]);
+/* Kernel schedule in use (after the first step has run the autotuner):
+ * info[0..3] = row kernel  {compile-time schedule in use, variant, points per lane, rows per block},
+ * info[4..7] = column kernel {same, columns per block}.  A zero in info[0] or
+ * info[4] means the generic mixed-radix kernel runs.  (No reference counterpart.) */
+int pfc_plan_schedule(const pfc_plan* plan, int info[8]);
 /* Device bytes held by a plan (state, tables, scratch). */
 int pfc_plan_bytes(const pfc_plan* plan, long long* bytes);
 

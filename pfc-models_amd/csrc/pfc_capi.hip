@@ -2,8 +2,10 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdint>
+#include <cstdio>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -80,25 +82,25 @@ bool factor(int L, Radices& pl) {
 // exp(-2 pi i m / n), m = 0..n-1, computed in long double and rounded once;
 // exact at the quarter points (1, -i, -1, i), where the real-FFT pack/unpack
 // relies on exact zeros to keep the DC and Nyquist terms real.
-std::vector<double2> twiddles(int n) {
-  std::vector<double2> t(n);
+std::vector<V2<double>> twiddles(int n) {
+  std::vector<V2<double>> t(n);
   const long double twopi = 6.283185307179586476925286766559005768L;
   for (int m = 0; m < n; ++m) {
     if ((4LL * m) % n == 0) {
       const int q = (int)((4LL * m) / n);
       const double c[4] = {1.0, 0.0, -1.0, 0.0}, s[4] = {0.0, -1.0, 0.0, 1.0};
-      t[m] = double2{c[q], s[q]};
+      t[m] = V2<double>{c[q], s[q]};
       continue;
     }
     long double a = twopi * (long double)m / (long double)n;
-    t[m] = double2{(double)cosl(a), -(double)sinl(a)};
+    t[m] = V2<double>{(double)cosl(a), -(double)sinl(a)};
   }
   return t;
 }
 
-std::vector<float2> to_f32(const std::vector<double2>& v) {
-  std::vector<float2> o(v.size());
-  for (size_t i = 0; i < v.size(); ++i) o[i] = float2{(float)v[i].x, (float)v[i].y};
+std::vector<V2<float>> to_f32(const std::vector<V2<double>>& v) {
+  std::vector<V2<float>> o(v.size());
+  for (size_t i = 0; i < v.size(); ++i) o[i] = V2<float>{(float)v[i].x, (float)v[i].y};
   return o;
 }
 
@@ -139,7 +141,7 @@ int lds_limit() {
     int dev = 0, v = 0;
     if (hipGetDevice(&dev) != hipSuccess) return 65536;
     if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess) return 65536;
-    return std::min(v, 65536);
+    return v;
   }();
   return lim;
 }
@@ -186,97 +188,118 @@ int row_blocks(const RowGeom& g) { return (g.Ny + g.nl - 1) / g.nl; }
 int col_blocks(const ColGeom& g) { return (g.Nc + g.nc - 1) / g.nc; }
 
 // ---------------------------------------------------------------------------
-// fast (compile-time length) kernels: supported lengths and launch shapes
+// fast (compile-time schedule) kernels: variants, launch shapes, launchers
 // ---------------------------------------------------------------------------
-#define PFC_FAST_LENGTHS(X) X(32) X(48) X(64) X(96) X(128) X(192) X(256) X(384) X(512) X(768) X(1024) X(2048)
-
-int fast_points(int L) {
-  switch (L) {
-#define PFC_CASE(LEN) \
-  case LEN:           \
-    return FastSched<LEN>::P;
-    PFC_FAST_LENGTHS(PFC_CASE)
-#undef PFC_CASE
-    default:
-      return 0;
-  }
+std::vector<SchedInfo> schedules_for(int L) {
+  std::vector<SchedInfo> v;
+#define PFC_X(LEN, VAR, PP, ...) \
+  if (L == LEN) v.push_back(SchedInfo{LEN, VAR, PP});
+  PFC_SCHEDULES(PFC_X)
+#undef PFC_X
+  return v;
 }
 
 struct FastShape {
   bool on = false;
+  int var = 0;      // schedule variant
+  int P = 0;        // points per lane
   int tpl = 0;      // lanes per line
   int lines = 0;    // lines per block
   int blocks = 0;
   size_t lds = 0;
 };
 
-// rows: lines = rows per block; aim for >= 512 blocks and 64..256 threads
-FastShape fast_row_shape(int Nx, int Ny, size_t csize) {
+// Row kernel shape: LDS = twiddles (Nx) + lines * (M + 1).
+FastShape row_shape(int Nx, int Ny, size_t csize, const SchedInfo& si, int lines) {
   FastShape f;
-  const int M = Nx / 2, P = fast_points(M);
-  if (!P) return f;
-  f.tpl = M / P;
-  if (f.tpl > 256) return f;
-  const int lo = std::max(1, 64 / f.tpl), hi = std::max(1, 256 / f.tpl);
-  int w = std::max(lo, std::min(hi, Ny / 512));
-  while (w > lo && (size_t)w * (M + 1) * csize > (size_t)lds_limit()) --w;
-  if ((size_t)w * (M + 1) * csize > (size_t)lds_limit()) return f;
-  f.lines = w;
-  f.blocks = (Ny + w - 1) / w;
-  f.lds = (size_t)w * (M + 1) * csize;
-  f.on = true;
+  const int M = Nx / 2;
+  f.var = si.variant;
+  f.P = si.P;
+  f.tpl = M / si.P;
+  f.lines = lines;
+  f.blocks = (Ny + lines - 1) / lines;
+  f.lds = ((size_t)Nx + (size_t)lines * row_ld(M)) * csize;
+  f.on = f.tpl * lines <= 256 && f.tpl * lines >= 64 && f.lds <= (size_t)lds_limit();
   return f;
 }
 
-// columns: lines = columns per block; aim for >= 256 blocks
-FastShape fast_col_shape(int Nx, int Ny, size_t csize) {
+// Column kernel shape: LDS = twiddles (Ny) + lines * Ny.
+FastShape col_shape(int Nx, int Ny, size_t csize, const SchedInfo& si, int lines) {
   FastShape f;
-  const int Nc = Nx / 2 + 1, P = fast_points(Ny);
-  if (!P) return f;
-  f.tpl = Ny / P;
-  if (f.tpl > 256) return f;
-  const int lo = std::max(1, 64 / f.tpl), hi = std::max(1, 256 / f.tpl);
-  int c = std::max(lo, std::min(hi, Nc / 256));
-  while (c > lo && (size_t)c * Ny * csize > (size_t)lds_limit()) --c;
-  if ((size_t)c * Ny * csize > (size_t)lds_limit()) return f;
-  f.lines = c;
-  f.blocks = (Nc + c - 1) / c;
-  f.lds = (size_t)c * Ny * csize;
-  f.on = true;
+  const int Nc = Nx / 2 + 1;
+  f.var = si.variant;
+  f.P = si.P;
+  f.tpl = Ny / si.P;
+  f.lines = lines;
+  f.blocks = (Nc + lines - 1) / lines;
+  f.lds = ((size_t)Ny + (size_t)lines * col_ld(Ny)) * csize;
+  f.on = f.tpl * lines <= 256 && f.tpl * lines >= 64 && f.lds <= (size_t)lds_limit();
   return f;
 }
 
-template <typename T>
-void launch_row_fast(int M, const FastShape& f, hipStream_t st, const V2<T>* S, V2<T>* Tb, int Ny, const Nonlin& q,
-                     const V2<T>* twN, int* status) {
-  switch (M) {
-#define PFC_CASE(LEN)                                                                                           \
-  case LEN:                                                                                                     \
-    hipLaunchKernelGGL((k_row_fast<T, LEN>), dim3(f.blocks), dim3(f.lines * f.tpl), f.lds, st, S, Tb, Ny, q, \
-                       twN, status);                                                                            \
-    return;
-    PFC_FAST_LENGTHS(PFC_CASE)
-#undef PFC_CASE
-    default:
-      fail(PFC_EUNSUPPORTED, "no fast row kernel for this length");
-  }
+// Candidate (variant, lines) shapes; the default is variant 0 with enough
+// blocks to cover the 256 CUs.
+std::vector<FastShape> row_candidates(int Nx, int Ny, size_t csize) {
+  std::vector<FastShape> out;
+  for (const SchedInfo& si : schedules_for(Nx / 2))
+    for (int w = 1; w <= 256; w *= 2) {
+      FastShape f = row_shape(Nx, Ny, csize, si, w);
+      if (f.on) out.push_back(f);
+    }
+  return out;
 }
 
-template <typename T, int MODE>
-void launch_col_fast(int Ny, const FastShape& f, hipStream_t st, V2<T>* S, const V2<T>* Tb, V2<T>* D, int Nc,
-                     const ColTables& tb, const V2<T>* twY) {
-  switch (Ny) {
-#define PFC_CASE(LEN)                                                                                          \
-  case LEN:                                                                                                    \
-    hipLaunchKernelGGL((k_col_fast<T, LEN, MODE>), dim3(f.blocks), dim3(f.lines * f.tpl), f.lds, st, S, Tb, D, \
-                       Nc, tb, twY);                                                                           \
-    return;
-    PFC_FAST_LENGTHS(PFC_CASE)
-#undef PFC_CASE
-    default:
-      fail(PFC_EUNSUPPORTED, "no fast column kernel for this length");
-  }
+std::vector<FastShape> col_candidates(int Nx, int Ny, size_t csize) {
+  std::vector<FastShape> out;
+  for (const SchedInfo& si : schedules_for(Ny))
+    for (int c = 1; c <= 256; c *= 2) {
+      FastShape f = col_shape(Nx, Ny, csize, si, c);
+      if (f.on) out.push_back(f);
+    }
+  return out;
 }
+
+FastShape pick_default(const std::vector<FastShape>& cands, int target_blocks) {
+  FastShape best;
+  for (const FastShape& f : cands) {
+    if (f.var != 0) continue;
+    // most lines per block that still gives >= target_blocks blocks, else the fewest lines
+    if (!best.on) {
+      best = f;
+      continue;
+    }
+    if (f.blocks >= target_blocks && f.lines > best.lines) best = f;
+  }
+  return best;
+}
+
+template <typename T> struct FastLaunch;
+template <> struct FastLaunch<double> {
+  static void row(int M, const FastShape& f, hipStream_t st, const V2<double>* S, V2<double>* T, int Ny, const Nonlin& q,
+                  const V2<double>* tw, int* status) {
+    if (!row_fast_launch_f64(M, f.var, dim3(f.blocks), dim3(f.lines * f.tpl), f.lds, st, S, T, Ny, q, tw, status))
+      fail(PFC_EUNSUPPORTED, "no fast row schedule");
+  }
+  static void col(int Ny, int mode, const FastShape& f, hipStream_t st, V2<double>* S, const V2<double>* T, V2<double>* D,
+                  int Nc, const ColTables& tb, const V2<double>* tw) {
+    if (!col_fast_launch_f64(Ny, f.var, mode, dim3(f.blocks), dim3(f.lines * f.tpl), f.lds, st, S, T, D, Nc, tb,
+                             tw))
+      fail(PFC_EUNSUPPORTED, "no fast column schedule");
+  }
+};
+template <> struct FastLaunch<float> {
+  static void row(int M, const FastShape& f, hipStream_t st, const V2<float>* S, V2<float>* T, int Ny, const Nonlin& q,
+                  const V2<float>* tw, int* status) {
+    if (!row_fast_launch_f32(M, f.var, dim3(f.blocks), dim3(f.lines * f.tpl), f.lds, st, S, T, Ny, q, tw, status))
+      fail(PFC_EUNSUPPORTED, "no fast row schedule");
+  }
+  static void col(int Ny, int mode, const FastShape& f, hipStream_t st, V2<float>* S, const V2<float>* T, V2<float>* D,
+                  int Nc, const ColTables& tb, const V2<float>* tw) {
+    if (!col_fast_launch_f32(Ny, f.var, mode, dim3(f.blocks), dim3(f.lines * f.tpl), f.lds, st, S, T, D, Nc, tb,
+                             tw))
+      fail(PFC_EUNSUPPORTED, "no fast column schedule");
+  }
+};
 
 // ---------------------------------------------------------------------------
 // FP64 k-grids and ETD tables (the reference's host arithmetic, restated)
@@ -365,7 +388,8 @@ struct pfc_plan {
   size_t csize = 0;  // bytes per complex of the bulk precision
   RowGeom rg{}, rg64{};
   ColGeom cg{}, cg64{};
-  FastShape frow{}, fcol{};      // compile-time-length kernels, when the sizes have one
+  FastShape frow{}, fcol{};      // compile-time-schedule kernels, when the sizes have one
+  bool tuned = false;            // autotuner has run (or was skipped) for this plan
   DevBuf S, Tb, D;               // state, row spectrum of N, RK2 k-space carry
   DevBuf E1, Q1, Q2;             // ETD tables (bulk precision)
   DevBuf twN, twY;               // twiddles (bulk precision)
@@ -447,7 +471,7 @@ void launch_step(pfc_plan* pl, long long nsteps, hipStream_t st, Prof* prof = nu
   auto row = [&]() {
     B();
     if (pl->frow.on)
-      launch_row_fast<T>(M, pl->frow, st, S, Tb, Ny, q, twN, pl->status());
+      FastLaunch<T>::row(M, pl->frow, st, S, Tb, Ny, q, twN, pl->status());
     else
       hipLaunchKernelGGL((k_row_nonlin<T>), dim3(rb), dim3(256), rl, st, S, Tb, pl->rg, q, twN, pl->status());
     E(0);
@@ -456,7 +480,7 @@ void launch_step(pfc_plan* pl, long long nsteps, hipStream_t st, Prof* prof = nu
     constexpr int MODE = decltype(mode_tag)::value;
     B();
     if (pl->fcol.on)
-      launch_col_fast<T, MODE>(Ny, pl->fcol, st, S, Tb, D, Nc, tb, twY);
+      FastLaunch<T>::col(Ny, MODE, pl->fcol, st, S, Tb, D, Nc, tb, twY);
     else
       hipLaunchKernelGGL((k_col_etd<T, MODE>), dim3(cb), dim3(256), cl, st, S, Tb, D, pl->cg, tb, twY);
     E(kind);
@@ -474,6 +498,102 @@ void launch_step(pfc_plan* pl, long long nsteps, hipStream_t st, Prof* prof = nu
   hip_check(hipGetLastError(), "step launch");
 }
 
+// ---------------------------------------------------------------------------
+// autotuner: at a plan's first step, time every (schedule variant, lines per
+// block) candidate of the fast row and column kernels on scratch copies of the
+// live state and keep the fastest.  Choices are cached per (device, kernel
+// kind, lengths, precision, mode) for the life of the process.  PFC_HIP_TUNE=0
+// keeps the defaults.
+// ---------------------------------------------------------------------------
+std::mutex g_tune_mu;
+std::map<std::array<int, 6>, std::pair<int, int>> g_tune;  // -> (variant, lines)
+
+template <typename F>
+float time_launches(hipStream_t st, int reps, F&& launch) {
+  hipEvent_t a = nullptr, b = nullptr;
+  hip_check(hipEventCreate(&a), "event create");
+  hip_check(hipEventCreate(&b), "event create");
+  launch();
+  hip_check(hipEventRecord(a, st), "event record");
+  for (int i = 0; i < reps; ++i) launch();
+  hip_check(hipEventRecord(b, st), "event record");
+  hip_check(hipEventSynchronize(b), "event sync");
+  float ms = 0.f;
+  hip_check(hipEventElapsedTime(&ms, a, b), "event elapsed");
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  hip_check(hipGetLastError(), "tuning launch");
+  return ms / reps;
+}
+
+template <typename T>
+void autotune(pfc_plan* pl, hipStream_t st) {
+  pl->tuned = true;
+  const char* env = std::getenv("PFC_HIP_TUNE");
+  if (env && env[0] == '0') return;
+  using V = V2<T>;
+  int dev = 0;
+  hip_check(hipGetDevice(&dev), "hipGetDevice");
+  const int Nx = pl->p.Nx, Ny = pl->p.Ny, M = Nx / 2, Nc = pl->Nc;
+  const size_t nc = (size_t)Nc * Ny;
+  const int reps = 5;
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  auto apply = [](FastShape& cur, const std::vector<FastShape>& cands, std::pair<int, int> vl) {
+    for (const FastShape& f : cands)
+      if (f.var == vl.first && f.lines == vl.second) cur = f;
+  };
+  if (pl->frow.on) {
+    const std::array<int, 6> key{dev, 0, M, Ny, pl->p.precision, pl->p.model * 2 + pl->p.clamp_log};
+    auto cands = row_candidates(Nx, Ny, pl->csize);
+    auto it = g_tune.find(key);
+    if (it != g_tune.end()) {
+      apply(pl->frow, cands, it->second);
+    } else if (cands.size() > 1) {
+      DevBuf T2;
+      T2.alloc(nc * sizeof(V));
+      const Nonlin q = nonlin_of(pl);
+      float best = 1e30f;
+      FastShape pick = pl->frow;
+      for (const FastShape& f : cands) {
+        float ms = time_launches(st, reps, [&] {
+          FastLaunch<T>::row(M, f, st, (const V*)pl->S.p, (V*)T2.p, Ny, q, (const V*)pl->twN.p, pl->status());
+        });
+        if (ms < best) best = ms, pick = f;
+      }
+      pl->frow = pick;
+      g_tune[key] = {pick.var, pick.lines};
+    }
+  }
+  if (pl->fcol.on) {
+    const int mode = pl->p.scheme == PFC_SCHEME_ETD1 ? COL_ETD1 : COL_RK2_STAGE1;
+    const std::array<int, 6> key{dev, 1, Ny, Nc, pl->p.precision, mode};
+    auto cands = col_candidates(Nx, Ny, pl->csize);
+    auto it = g_tune.find(key);
+    if (it != g_tune.end()) {
+      apply(pl->fcol, cands, it->second);
+    } else if (cands.size() > 1) {
+      DevBuf S2, T2, D2;
+      S2.alloc(nc * sizeof(V));
+      T2.alloc(nc * sizeof(V));
+      D2.alloc(nc * sizeof(V));
+      hip_check(hipMemcpyAsync(S2.p, pl->S.p, nc * sizeof(V), hipMemcpyDeviceToDevice, st), "copy");
+      hip_check(hipMemcpyAsync(T2.p, pl->S.p, nc * sizeof(V), hipMemcpyDeviceToDevice, st), "copy");
+      ColTables tb{pl->E1.p, pl->Q1.p, pl->Q2.p, pl->p.pin_mean ? pl->pin() : nullptr};
+      float best = 1e30f;
+      FastShape pick = pl->fcol;
+      for (const FastShape& f : cands) {
+        float ms = time_launches(st, reps, [&] {
+          FastLaunch<T>::col(Ny, mode, f, st, (V*)S2.p, (const V*)T2.p, (V*)D2.p, Nc, tb, (const V*)pl->twY.p);
+        });
+        if (ms < best) best = ms, pick = f;
+      }
+      pl->fcol = pick;
+      g_tune[key] = {pick.var, pick.lines};
+      hip_check(hipStreamSynchronize(st), "sync after tuning");
+    }
+  }
+}
+
 // reductions of the current state; pin_out (device) receives the mean if non-null
 template <typename T>
 void launch_reduce(pfc_plan* pl, hipStream_t st, double* pin_out) {
@@ -484,12 +604,12 @@ void launch_reduce(pfc_plan* pl, hipStream_t st, double* pin_out) {
   Partial* part = reinterpret_cast<Partial*>(pl->part.p);
   const int rb = row_blocks(pl->rg64), cb = col_blocks(pl->cg64);
   const size_t red = 256 * sizeof(Partial);
-  const size_t rl = std::max(row_lds(pl->rg64, sizeof(double2)), red);
-  const size_t cl = std::max(col_lds(pl->cg64, sizeof(double2)), red);
+  const size_t rl = std::max(row_lds(pl->rg64, sizeof(V2<double>)), red);
+  const size_t cl = std::max(col_lds(pl->cg64, sizeof(V2<double>)), red);
   hipLaunchKernelGGL((k_fe_row<T>), dim3(rb), dim3(256), rl, st, S, pl->rg64, fp,
-                     reinterpret_cast<const double2*>(pl->twN64.p), part);
+                     reinterpret_cast<const V2<double>*>(pl->twN64.p), part);
   hipLaunchKernelGGL((k_fe_col<T>), dim3(cb), dim3(256), cl, st, S, pl->cg64, fp, pl->p.Nx,
-                     reinterpret_cast<const double2*>(pl->twY64.p), part + rb);
+                     reinterpret_cast<const V2<double>*>(pl->twY64.p), part + rb);
   hipLaunchKernelGGL(k_fe_final, dim3(1), dim3(256), 0, st, part, rb, cb, pl->p.Nx, pl->p.Ny,
                      reinterpret_cast<double*>(pl->scal.p), pin_out);
   hip_check(hipGetLastError(), "reduce launch");
@@ -522,7 +642,7 @@ FftEntry* fft_entry(int Ny, int Nx, int dtype) {
   auto it = g_fft.find(key);
   if (it != g_fft.end()) return it->second.get();
   auto e = std::make_unique<FftEntry>();
-  const size_t cs = dtype == 64 ? sizeof(double2) : sizeof(float2);
+  const size_t cs = dtype == 64 ? sizeof(V2<double>) : sizeof(V2<float>);
   e->rg = row_geom(Nx, Ny, cs);
   e->cg = col_geom(Nx, Ny, cs);
   auto tn = twiddles(Nx), ty = twiddles(Ny);
@@ -601,14 +721,31 @@ int pfc_create(const pfc_params* params, pfc_plan** out) {
     pl->M = params->M;
     const int Nx = params->Nx, Ny = params->Ny;
     pl->Nc = Nx / 2 + 1;
-    pl->csize = pl->f64() ? sizeof(double2) : sizeof(float2);
+    pl->csize = pl->f64() ? sizeof(V2<double>) : sizeof(V2<float>);
     pl->rg = row_geom(Nx, Ny, pl->csize);
     pl->cg = col_geom(Nx, Ny, pl->csize);
-    pl->rg64 = row_geom(Nx, Ny, sizeof(double2));
-    pl->cg64 = col_geom(Nx, Ny, sizeof(double2));
+    pl->rg64 = row_geom(Nx, Ny, sizeof(V2<double>));
+    pl->cg64 = col_geom(Nx, Ny, sizeof(V2<double>));
     if (!std::getenv("PFC_HIP_GENERIC")) {
-      pl->frow = fast_row_shape(Nx, Ny, pl->csize);
-      pl->fcol = fast_col_shape(Nx, Ny, pl->csize);
+      auto rc = row_candidates(Nx, Ny, pl->csize);
+      auto cc = col_candidates(Nx, Ny, pl->csize);
+      pl->frow = pick_default(rc, 512);
+      pl->fcol = pick_default(cc, 256);
+      // PFC_HIP_SCHEDULE="rowvar:rowlines:colvar:collines" pins a schedule
+      // (tests and tuning studies); the autotuner is then skipped.
+      if (const char* fx = std::getenv("PFC_HIP_SCHEDULE")) {
+        int v[4] = {-1, -1, -1, -1};
+        if (std::sscanf(fx, "%d:%d:%d:%d", &v[0], &v[1], &v[2], &v[3]) == 4) {
+          bool okr = false, okc = false;
+          for (const FastShape& f : rc)
+            if (f.var == v[0] && f.lines == v[1]) pl->frow = f, okr = true;
+          for (const FastShape& f : cc)
+            if (f.var == v[2] && f.lines == v[3]) pl->fcol = f, okc = true;
+          if (!okr && pl->frow.on) fail(PFC_EINVAL, "PFC_HIP_SCHEDULE: no such row schedule");
+          if (!okc && pl->fcol.on) fail(PFC_EINVAL, "PFC_HIP_SCHEDULE: no such column schedule");
+          pl->tuned = true;
+        }
+      }
     }
     const size_t nc = (size_t)pl->Nc * Ny;
     pl->S.alloc(nc * pl->csize);
@@ -656,20 +793,20 @@ int pfc_set_field(pfc_plan* pl, const void* field, int dtype, void* stream) {
       const double inv = 1.0 / pl->p.Nx;
       if (dtype == 64)
         hipLaunchKernelGGL((k_row_r2c<double, double>), dim3(rb), dim3(256), rl, st, (const double*)field,
-                           (double2*)pl->S.p, pl->rg, (const double2*)pl->twN.p, inv, (size_t)0, (size_t)0);
+                           (V2<double>*)pl->S.p, pl->rg, (const V2<double>*)pl->twN.p, inv, (size_t)0, (size_t)0);
       else
         hipLaunchKernelGGL((k_row_r2c<double, float>), dim3(rb), dim3(256), rl, st, (const float*)field,
-                           (double2*)pl->S.p, pl->rg, (const double2*)pl->twN.p, inv, (size_t)0, (size_t)0);
+                           (V2<double>*)pl->S.p, pl->rg, (const V2<double>*)pl->twN.p, inv, (size_t)0, (size_t)0);
       hip_check(hipGetLastError(), "set_field launch");
       launch_reduce<double>(pl, st, pl->pin());
     } else {
       const float inv = 1.0f / pl->p.Nx;
       if (dtype == 64)
         hipLaunchKernelGGL((k_row_r2c<float, double>), dim3(rb), dim3(256), rl, st, (const double*)field,
-                           (float2*)pl->S.p, pl->rg, (const float2*)pl->twN.p, inv, (size_t)0, (size_t)0);
+                           (V2<float>*)pl->S.p, pl->rg, (const V2<float>*)pl->twN.p, inv, (size_t)0, (size_t)0);
       else
         hipLaunchKernelGGL((k_row_r2c<float, float>), dim3(rb), dim3(256), rl, st, (const float*)field,
-                           (float2*)pl->S.p, pl->rg, (const float2*)pl->twN.p, inv, (size_t)0, (size_t)0);
+                           (V2<float>*)pl->S.p, pl->rg, (const V2<float>*)pl->twN.p, inv, (size_t)0, (size_t)0);
       hip_check(hipGetLastError(), "set_field launch");
       launch_reduce<float>(pl, st, pl->pin());
     }
@@ -686,18 +823,18 @@ int pfc_get_field(pfc_plan* pl, void* field, int dtype, void* stream) {
     const size_t rl = row_lds(pl->rg, pl->csize);
     if (pl->f64()) {
       if (dtype == 64)
-        hipLaunchKernelGGL((k_row_c2r<double, double>), dim3(rb), dim3(256), rl, st, (const double2*)pl->S.p,
-                           (double*)field, pl->rg, (const double2*)pl->twN.p, 1.0, (size_t)0, (size_t)0);
+        hipLaunchKernelGGL((k_row_c2r<double, double>), dim3(rb), dim3(256), rl, st, (const V2<double>*)pl->S.p,
+                           (double*)field, pl->rg, (const V2<double>*)pl->twN.p, 1.0, (size_t)0, (size_t)0);
       else
-        hipLaunchKernelGGL((k_row_c2r<double, float>), dim3(rb), dim3(256), rl, st, (const double2*)pl->S.p,
-                           (float*)field, pl->rg, (const double2*)pl->twN.p, 1.0, (size_t)0, (size_t)0);
+        hipLaunchKernelGGL((k_row_c2r<double, float>), dim3(rb), dim3(256), rl, st, (const V2<double>*)pl->S.p,
+                           (float*)field, pl->rg, (const V2<double>*)pl->twN.p, 1.0, (size_t)0, (size_t)0);
     } else {
       if (dtype == 64)
-        hipLaunchKernelGGL((k_row_c2r<float, double>), dim3(rb), dim3(256), rl, st, (const float2*)pl->S.p,
-                           (double*)field, pl->rg, (const float2*)pl->twN.p, 1.0f, (size_t)0, (size_t)0);
+        hipLaunchKernelGGL((k_row_c2r<float, double>), dim3(rb), dim3(256), rl, st, (const V2<float>*)pl->S.p,
+                           (double*)field, pl->rg, (const V2<float>*)pl->twN.p, 1.0f, (size_t)0, (size_t)0);
       else
-        hipLaunchKernelGGL((k_row_c2r<float, float>), dim3(rb), dim3(256), rl, st, (const float2*)pl->S.p,
-                           (float*)field, pl->rg, (const float2*)pl->twN.p, 1.0f, (size_t)0, (size_t)0);
+        hipLaunchKernelGGL((k_row_c2r<float, float>), dim3(rb), dim3(256), rl, st, (const V2<float>*)pl->S.p,
+                           (float*)field, pl->rg, (const V2<float>*)pl->twN.p, 1.0f, (size_t)0, (size_t)0);
     }
     hip_check(hipGetLastError(), "get_field launch");
   });
@@ -709,6 +846,12 @@ int pfc_step(pfc_plan* pl, long long nsteps, void* stream) {
     if (nsteps < 0) fail(PFC_EINVAL, "nsteps < 0");
     if (!pl->field_set) fail(PFC_EINVAL, "pfc_set_field has not been called");
     hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    if (!pl->tuned) {
+      if (pl->f64())
+        autotune<double>(pl, st);
+      else
+        autotune<float>(pl, st);
+    }
     if (pl->f64())
       launch_step<double>(pl, nsteps, st);
     else
@@ -785,6 +928,12 @@ int pfc_step_profile(pfc_plan* pl, long long nsteps, void* stream, double ms[3],
     if (nsteps < 0) fail(PFC_EINVAL, "nsteps < 0");
     if (!pl->field_set) fail(PFC_EINVAL, "pfc_set_field has not been called");
     prof.st = reinterpret_cast<hipStream_t>(stream);
+    if (!pl->tuned) {
+      if (pl->f64())
+        autotune<double>(pl, prof.st);
+      else
+        autotune<float>(pl, prof.st);
+    }
     hip_check(hipEventCreate(&prof.ev[0]), "event create");
     hip_check(hipEventCreate(&prof.ev[1]), "event create");
     if (pl->f64())
@@ -799,6 +948,19 @@ int pfc_step_profile(pfc_plan* pl, long long nsteps, void* stream, double ms[3],
   for (hipEvent_t e : prof.ev)
     if (e) (void)hipEventDestroy(e);
   return rc;
+}
+
+int pfc_plan_schedule(const pfc_plan* pl, int info[8]) {
+  return guarded([&] {
+    if (!pl || !info) fail(PFC_EINVAL, "NULL argument");
+    const FastShape* fs[2] = {&pl->frow, &pl->fcol};
+    for (int k = 0; k < 2; ++k) {
+      info[4 * k + 0] = fs[k]->on ? 1 : 0;
+      info[4 * k + 1] = fs[k]->var;
+      info[4 * k + 2] = fs[k]->P;
+      info[4 * k + 3] = fs[k]->lines;
+    }
+  });
 }
 
 int pfc_plan_bytes(const pfc_plan* pl, long long* bytes) {
@@ -824,14 +986,14 @@ int pfc_rfft2(const void* in, void* out, int batch, int Ny, int Nx, int dtype, v
     dim3 rgrid(row_blocks(e->rg), batch), cgrid(col_blocks(e->cg), batch);
     if (dtype == 64) {
       hipLaunchKernelGGL((k_row_r2c<double, double>), rgrid, dim3(256), row_lds(e->rg, 16), st, (const double*)in,
-                         (double2*)out, e->rg, (const double2*)e->twN.p, 1.0, nreal, ncplx);
-      hipLaunchKernelGGL((k_col_fft<double, false>), cgrid, dim3(256), col_lds(e->cg, 16), st, (double2*)out, e->cg,
-                         (const double2*)e->twY.p, 1.0, ncplx);
+                         (V2<double>*)out, e->rg, (const V2<double>*)e->twN.p, 1.0, nreal, ncplx);
+      hipLaunchKernelGGL((k_col_fft<double, false>), cgrid, dim3(256), col_lds(e->cg, 16), st, (V2<double>*)out, e->cg,
+                         (const V2<double>*)e->twY.p, 1.0, ncplx);
     } else {
       hipLaunchKernelGGL((k_row_r2c<float, float>), rgrid, dim3(256), row_lds(e->rg, 8), st, (const float*)in,
-                         (float2*)out, e->rg, (const float2*)e->twN.p, 1.0f, nreal, ncplx);
-      hipLaunchKernelGGL((k_col_fft<float, false>), cgrid, dim3(256), col_lds(e->cg, 8), st, (float2*)out, e->cg,
-                         (const float2*)e->twY.p, 1.0f, ncplx);
+                         (V2<float>*)out, e->rg, (const V2<float>*)e->twN.p, 1.0f, nreal, ncplx);
+      hipLaunchKernelGGL((k_col_fft<float, false>), cgrid, dim3(256), col_lds(e->cg, 8), st, (V2<float>*)out, e->cg,
+                         (const V2<float>*)e->twY.p, 1.0f, ncplx);
     }
     hip_check(hipGetLastError(), "rfft2 launch");
   });
@@ -856,16 +1018,16 @@ int pfc_irfft2(const void* in, void* out, int batch, int Ny, int Nx, int dtype, 
     dim3 rgrid(row_blocks(e->rg), batch), cgrid(col_blocks(e->cg), batch);
     const double scale = 1.0 / ((double)Nx * (double)Ny);
     if (dtype == 64) {
-      hipLaunchKernelGGL((k_col_fft<double, true>), cgrid, dim3(256), col_lds(e->cg, 16), st, (double2*)e->scratch.p,
-                         e->cg, (const double2*)e->twY.p, 1.0, ncplx);
+      hipLaunchKernelGGL((k_col_fft<double, true>), cgrid, dim3(256), col_lds(e->cg, 16), st, (V2<double>*)e->scratch.p,
+                         e->cg, (const V2<double>*)e->twY.p, 1.0, ncplx);
       hipLaunchKernelGGL((k_row_c2r<double, double>), rgrid, dim3(256), row_lds(e->rg, 16), st,
-                         (const double2*)e->scratch.p, (double*)out, e->rg, (const double2*)e->twN.p, scale, ncplx,
+                         (const V2<double>*)e->scratch.p, (double*)out, e->rg, (const V2<double>*)e->twN.p, scale, ncplx,
                          nreal);
     } else {
-      hipLaunchKernelGGL((k_col_fft<float, true>), cgrid, dim3(256), col_lds(e->cg, 8), st, (float2*)e->scratch.p,
-                         e->cg, (const float2*)e->twY.p, 1.0f, ncplx);
+      hipLaunchKernelGGL((k_col_fft<float, true>), cgrid, dim3(256), col_lds(e->cg, 8), st, (V2<float>*)e->scratch.p,
+                         e->cg, (const V2<float>*)e->twY.p, 1.0f, ncplx);
       hipLaunchKernelGGL((k_row_c2r<float, float>), rgrid, dim3(256), row_lds(e->rg, 8), st,
-                         (const float2*)e->scratch.p, (float*)out, e->rg, (const float2*)e->twN.p, (float)scale,
+                         (const V2<float>*)e->scratch.p, (float*)out, e->rg, (const V2<float>*)e->twN.p, (float)scale,
                          ncplx, nreal);
     }
     hip_check(hipGetLastError(), "irfft2 launch");

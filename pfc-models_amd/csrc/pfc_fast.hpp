@@ -1,15 +1,18 @@
-// Register-resident Stockham FFT kernels for compile-time line lengths.
+// Register-resident Stockham FFT kernels for compile-time schedules.
 //
-// A line of L complex points is owned by TPL = L/P lanes; lane l holds the P
-// points l + TPL*q (q = 0..P-1) in registers ("strided layout").  A radix-R
-// Stockham pass gives each lane the butterflies j = l + TPL*i (i < P/R); its
-// inputs j + r*L/R are exactly registers i + r*P/R of the strided layout, and
-// in the last pass (Ns = L/R) its outputs j + r*Ns land on the same
-// registers.  So the first pass reads registers, the last pass writes
-// registers, and only intermediate passes exchange through LDS.  The forward
-// transform therefore ends in the layout the inverse starts from, which is
-// what lets the column kernel apply the ETD combine in registers between the
-// forward and the inverse column FFTs.
+// A schedule Sch<L, P, R...> transforms lines of L complex points with
+// TPL = L/P lanes per line; lane l holds the P points l + TPL*q (q < P) in
+// registers ("strided layout").  A radix-R Stockham pass gives each lane the
+// butterflies j = l + TPL*i (i < P/R); their inputs j + r*L/R are registers
+// i + r*P/R of the strided layout, and in the last pass (Ns = L/R) the outputs
+// j + r*Ns land on the same registers.  So the first pass reads registers, the
+// last pass writes registers, and only intermediate passes exchange through
+// LDS.  The forward transform therefore ends in the layout the inverse starts
+// from, which lets the column kernel apply the ETD combine in registers
+// between its forward and inverse column FFTs.
+//
+// When a line fits in one wavefront (TPL <= 64, TPL | 64) its LDS exchanges
+// need only wave-level ordering; otherwise they use block barriers.
 #pragma once
 #include "pfc_kernels.hpp"
 
@@ -33,21 +36,17 @@ template <typename T, bool INV> struct Dft<T, 16, INV> {
 #pragma unroll
       for (int k1 = 0; k1 < 4; ++k1) a[n2][k1] = t[k1];
     }
-    // twiddle w16^(n2 k1): exponents 0..9
     auto tw = [&](V x, int e) -> V {
-      // w16^e = exp(-+ i pi e / 8)
+      // w16^e = exp(-+ i pi e / 8), e in {1, 2, 3, 4, 6, 9}
       T c, s;
       switch (e) {
-        case 0: return x;
         case 1: c = c1; s = s1; break;
         case 2: c = h; s = h; break;
         case 3: c = s1; s = c1; break;
         case 4: c = T(0); s = T(1); break;
         case 6: c = -h; s = h; break;
-        case 9: c = -c1; s = -s1; break;
-        default: c = T(0); s = T(0); break;
+        default: c = -c1; s = -s1; break;  // 9
       }
-      // forward: multiply by (c - i s); inverse: (c + i s)
       if (INV) return V{x.x * c - x.y * s, x.x * s + x.y * c};
       return V{x.x * c + x.y * s, x.y * c - x.x * s};
     };
@@ -66,152 +65,159 @@ template <typename T, bool INV> struct Dft<T, 16, INV> {
 };
 
 // ---------------------------------------------------------------------------
-// compile-time schedules
+// schedules
 // ---------------------------------------------------------------------------
-template <int L> struct FastSched;  // P points per lane and the radix sequence
-#define PFC_SCHED(LEN, PP, ...)                                   \
-  template <> struct FastSched<LEN> {                             \
-    static constexpr int P = PP;                                  \
-    static constexpr int R[] = {__VA_ARGS__};                     \
-    static constexpr int NP = sizeof(R) / sizeof(int);            \
-  };
-PFC_SCHED(32, 8, 8, 4)
-PFC_SCHED(48, 12, 4, 4, 3)
-PFC_SCHED(64, 8, 8, 8)
-PFC_SCHED(96, 12, 4, 4, 2, 3)
-PFC_SCHED(128, 8, 8, 8, 2)
-PFC_SCHED(192, 12, 4, 4, 4, 3)
-PFC_SCHED(256, 8, 8, 8, 4)
-PFC_SCHED(384, 12, 4, 4, 4, 2, 3)
-PFC_SCHED(512, 8, 8, 8, 8)
-PFC_SCHED(768, 12, 4, 4, 4, 4, 3)
-PFC_SCHED(1024, 16, 16, 8, 8)
-PFC_SCHED(2048, 16, 16, 16, 8)
-#undef PFC_SCHED
+template <int L_, int P_, int... Rs> struct Sch {
+  static constexpr int L = L_;
+  static constexpr int P = P_;
+  static constexpr int TPL = L_ / P_;
+  static constexpr int NP = sizeof...(Rs);
+  static constexpr int R[] = {Rs...};
+  static constexpr bool WAVE = (TPL <= 64) && (64 % TPL == 0);
+  static constexpr int ns(int p) {
+    int n = 1;
+    for (int i = 0; i < p; ++i) n *= R[i];
+    return n;
+  }
+  static constexpr bool valid() {
+    int prod = 1;
+    for (int i = 0; i < NP; ++i) {
+      if (P % R[i] != 0) return false;
+      prod *= R[i];
+    }
+    return prod == L && L % P == 0;
+  }
+};
 
-// product of the first p radices
-template <int L> __host__ __device__ constexpr int sched_ns(int p) {
-  int ns = 1;
-  for (int i = 0; i < p; ++i) ns *= FastSched<L>::R[i];
-  return ns;
+// ordering of LDS traffic between the lanes of one line
+template <class S> __device__ __forceinline__ void line_sync() {
+  if constexpr (S::WAVE) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+  } else {
+    __syncthreads();
+  }
 }
 
-// One pass: butterflies from registers, results back to registers (in
-// butterfly-local order: register i + r*NB holds output r of butterfly i).
-template <typename T, int L, int PI, bool INV>
-__device__ __forceinline__ void reg_pass(V2<T>* v, int lane, const V2<T>* __restrict__ tw, int tws) {
+// LDS line index with one pad slot per 16 elements: Stockham scatters write
+// at strides of R (or R*Ns) elements, which without padding put every lane of
+// a wavefront on the same bank.
+__host__ __device__ constexpr int lpad(int i) { return i + (i >> 4); }
+// padded LDS line lengths of the row kernel (M + 1 points) and column kernel (Ny)
+__host__ __device__ constexpr int row_ld(int M) { return lpad(M) + 2; }
+__host__ __device__ constexpr int col_ld(int Ny) { return lpad(Ny - 1) + 2; }
+
+template <typename T, class S, int PI, bool INV>
+__device__ __forceinline__ void reg_pass(V2<T>* v, int lane, const V2<T>* tw, int tws) {
   using V = V2<T>;
-  constexpr int P = FastSched<L>::P;
-  constexpr int R = FastSched<L>::R[PI];
-  constexpr int NS = sched_ns<L>(PI);
-  constexpr int TPL = L / P;
-  constexpr int NB = P / R;
+  constexpr int R = S::R[PI];
+  constexpr int NS = S::ns(PI);
+  constexpr int NB = S::P / R;
 #pragma unroll
   for (int i = 0; i < NB; ++i) {
     V x[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) x[r] = v[i + r * NB];
-    if (NS > 1) {
-      const int j = lane + TPL * i;
+    if constexpr (NS > 1) {
+      const int j = lane + S::TPL * i;
       const int k = j % NS;
 #pragma unroll
       for (int r = 1; r < R; ++r) {
-        V t = tw[(k * r * (L / (NS * R))) * tws];
+        V t = tw[(k * r * (S::L / (NS * R))) * tws];
         if (INV) t.y = -t.y;
         x[r] = cmul(x[r], t);
       }
     }
-    Dft<T, R, INV>::run(x, tw, (L / R) * tws);
+    Dft<T, R, INV>::run(x, tw, (S::L / R) * tws);
 #pragma unroll
     for (int r = 0; r < R; ++r) v[i + r * NB] = x[r];
   }
 }
 
-// Write pass PI's outputs to their Stockham positions in `line` (LDS).
-template <typename T, int L, int PI>
+template <typename T, class S, int PI>
 __device__ __forceinline__ void reg_scatter(const V2<T>* v, V2<T>* line, int lane) {
-  constexpr int P = FastSched<L>::P;
-  constexpr int R = FastSched<L>::R[PI];
-  constexpr int NS = sched_ns<L>(PI);
-  constexpr int TPL = L / P;
-  constexpr int NB = P / R;
+  constexpr int R = S::R[PI];
+  constexpr int NS = S::ns(PI);
+  constexpr int NB = S::P / R;
 #pragma unroll
   for (int i = 0; i < NB; ++i) {
-    const int j = lane + TPL * i;
+    const int j = lane + S::TPL * i;
     const int k = j % NS;
     const int base = (j - k) * R + k;
 #pragma unroll
-    for (int r = 0; r < R; ++r) line[base + r * NS] = v[i + r * NB];
+    for (int r = 0; r < R; ++r) line[lpad(base + r * NS)] = v[i + r * NB];
   }
 }
 
-template <typename T, int L>
+template <typename T, class S>
 __device__ __forceinline__ void reg_gather(V2<T>* v, const V2<T>* line, int lane) {
-  constexpr int P = FastSched<L>::P;
-  constexpr int TPL = L / P;
 #pragma unroll
-  for (int q = 0; q < P; ++q) v[q] = line[lane + TPL * q];
+  for (int q = 0; q < S::P; ++q) v[q] = line[lpad(lane + S::TPL * q)];
 }
 
-// Full FFT of the lane's strided registers, exchanging through `line`
-// (L elements of LDS private to this line).  All threads of the block must
-// call it (it contains block barriers).
-template <typename T, int L, bool INV, int PI = 0>
-__device__ __forceinline__ void reg_fft(V2<T>* v, V2<T>* line, int lane, const V2<T>* __restrict__ tw, int tws) {
-  reg_pass<T, L, PI, INV>(v, lane, tw, tws);
-  if constexpr (PI + 1 < FastSched<L>::NP) {
-    __syncthreads();  // earlier readers of `line` are done
-    reg_scatter<T, L, PI>(v, line, lane);
-    __syncthreads();
-    reg_gather<T, L>(v, line, lane);
-    reg_fft<T, L, INV, PI + 1>(v, line, lane, tw, tws);
+// Full FFT of the lane's strided registers, exchanging through `line` (S::L
+// LDS elements owned by this line).  Every lane of the block must call it.
+template <typename T, class S, bool INV, int PI = 0>
+__device__ __forceinline__ void reg_fft(V2<T>* v, V2<T>* line, int lane, const V2<T>* tw, int tws) {
+  reg_pass<T, S, PI, INV>(v, lane, tw, tws);
+  if constexpr (PI + 1 < S::NP) {
+    line_sync<S>();  // earlier readers of `line` are done
+    reg_scatter<T, S, PI>(v, line, lane);
+    line_sync<S>();
+    reg_gather<T, S>(v, line, lane);
+    reg_fft<T, S, INV, PI + 1>(v, line, lane, tw, tws);
   }
 }
 
 // ---------------------------------------------------------------------------
-// fast row kernel: S~ -> C2R -> N(n) -> R2C -> T, one row per TPL lanes
+// fast row kernel: S~ -> C2R -> N(n) -> R2C -> T; W rows per block
+// LDS: twiddles exp(-2 pi i m / Nx), m < Nx, then W lines of M + 1.
 // ---------------------------------------------------------------------------
-template <typename T, int M>
-__global__ void __launch_bounds__(256) k_row_fast(const V2<T>* __restrict__ S, V2<T>* __restrict__ Tout, int Ny,
+template <typename T, class S>
+__global__ void __launch_bounds__(256) k_row_fast(const V2<T>* __restrict__ Sg, V2<T>* __restrict__ Tout, int Ny,
                                                   Nonlin q, const V2<T>* __restrict__ twN,
                                                   int* __restrict__ status) {
   using V = V2<T>;
-  constexpr int P = FastSched<M>::P;
-  constexpr int TPL = M / P;
-  constexpr int Nc = M + 1;
-  constexpr int LD = M + 1;
+  constexpr int M = S::L, P = S::P, TPL = S::TPL, Nc = M + 1, LD = row_ld(M);
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  const int W = blockDim.x / TPL;
+  V* tw = reinterpret_cast<V*>(smem_raw);
   const int wl = threadIdx.x / TPL;
   const int lane = threadIdx.x - wl * TPL;
-  const int row = blockIdx.x * W + wl;
+  const int row = blockIdx.x * (blockDim.x / TPL) + wl;
   const bool live = row < Ny;
-  V* line = reinterpret_cast<V*>(smem_raw) + wl * LD;
+  V* line = tw + 2 * M + wl * LD;
 
+  V x[P];
+  V xM = V{T(0), T(0)};
   if (live) {
-    const V* src = S + (size_t)row * Nc;
+    const V* src = Sg + (size_t)row * Nc;
 #pragma unroll
-    for (int qq = 0; qq < P; ++qq) line[lane + TPL * qq] = src[lane + TPL * qq];
-    if (lane == 0) line[M] = src[M];
+    for (int qq = 0; qq < P; ++qq) x[qq] = src[lane + TPL * qq];
+    if (lane == 0) xM = src[M];
   }
+  for (int i = threadIdx.x; i < 2 * M; i += blockDim.x) tw[i] = twN[i];
+#pragma unroll
+  for (int qq = 0; qq < P; ++qq) line[lpad(lane + TPL * qq)] = x[qq];
+  if (lane == 0) line[lpad(M)] = xM;
   __syncthreads();
   V v[P];
 #pragma unroll
   for (int qq = 0; qq < P; ++qq) {
     const int k = lane + TPL * qq;
-    V xk = line[k];
-    V xm = line[M - k];
+    V xk = x[qq];
+    V xm = line[lpad(M - k)];
     if (k == 0) {
       xk.y = T(0);
       xm.y = T(0);
     }
     V s = V{xk.x + xm.x, xk.y - xm.y};
     V d = V{xk.x - xm.x, xk.y + xm.y};
-    V w = twN[k];
+    V w = tw[k];
     V wd = cmul(cconj(w), d);
     v[qq] = V{s.x - wd.y, s.y + wd.x};
   }
-  reg_fft<T, M, true>(v, line, lane, twN, 2);
+  reg_fft<T, S, true>(v, line, lane, tw, 2);
   bool bad = false;
 #pragma unroll
   for (int qq = 0; qq < P; ++qq) {
@@ -219,37 +225,41 @@ __global__ void __launch_bounds__(256) k_row_fast(const V2<T>* __restrict__ S, V
     v[qq] = V{nonlin_mu<T>(v[qq].x, q), nonlin_mu<T>(v[qq].y, q)};
   }
   if (bad && live) atomicOr(status, 1);
-  reg_fft<T, M, false>(v, line, lane, twN, 2);
-  __syncthreads();
+  reg_fft<T, S, false>(v, line, lane, tw, 2);
+  line_sync<S>();
 #pragma unroll
-  for (int qq = 0; qq < P; ++qq) line[lane + TPL * qq] = v[qq];
-  __syncthreads();
+  for (int qq = 0; qq < P; ++qq) line[lpad(lane + TPL * qq)] = v[qq];
+  line_sync<S>();
   if (live) {
     V* dst = Tout + (size_t)row * Nc;
 #pragma unroll
     for (int qq = 0; qq < P; ++qq) {
+      // unpack Y[k] from Z[k] (in registers) and Z[M-k] (from LDS)
       const int k = lane + TPL * qq;
-      dst[k] = row_unpack<T>(line, k, M, twN);
+      const V zk = v[qq];
+      const V zm = line[lpad(k == 0 ? 0 : M - k)];
+      dst[k] = row_unpack_pair<T>(zk, zm, tw[k]);
     }
-    if (lane == 0) dst[M] = row_unpack<T>(line, M, M, twN);
+    if (lane == 0) dst[M] = row_unpack_pair<T>(line[0], line[0], tw[M]);
   }
 }
 
 // ---------------------------------------------------------------------------
 // fast column kernel: forward column FFTs of S~ and T, ETD combine in
 // registers, inverse column FFT, S~ written in place.  C columns per block.
+// LDS: twiddles exp(-2 pi i m / Ny), then C lines of Ny.
 // ---------------------------------------------------------------------------
-template <typename T, int NY, int MODE>
-__global__ void __launch_bounds__(256) k_col_fast(V2<T>* __restrict__ S, const V2<T>* __restrict__ Tin,
+template <typename T, class S, int MODE>
+__global__ void __launch_bounds__(256) k_col_fast(V2<T>* __restrict__ Sg, const V2<T>* __restrict__ Tin,
                                                   V2<T>* __restrict__ D, int Nc, ColTables tb,
                                                   const V2<T>* __restrict__ twY) {
   using V = V2<T>;
-  constexpr int P = FastSched<NY>::P;
-  constexpr int TPL = NY / P;
-  constexpr int LD = NY;
+  constexpr int NY = S::L, P = S::P, TPL = S::TPL, LD = col_ld(NY);
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  V* lds = reinterpret_cast<V*>(smem_raw);
-  const int C = blockDim.x / TPL;
+  V* tw = reinterpret_cast<V*>(smem_raw);
+  V* lds = tw + NY;
+  const int C = blockDim.x / TPL;  // columns per block, a power of two
+  const int lgC = __ffs(C) - 1;
   const int wl = threadIdx.x / TPL;
   const int lane = threadIdx.x - wl * TPL;
   const int ntile = (Nc + C - 1) / C;
@@ -262,68 +272,143 @@ __global__ void __launch_bounds__(256) k_col_fast(V2<T>* __restrict__ S, const V
   const T* Q1 = reinterpret_cast<const T*>(tb.Q1);
   const T* Q2 = reinterpret_cast<const T*>(tb.Q2);
 
-  // coalesced tile load: consecutive threads take consecutive columns of a row
-  auto tile_load = [&](const V* __restrict__ src) {
-    for (int e = threadIdx.x; e < ncols * NY; e += blockDim.x) {
-      const int y = e / ncols;
-      const int c = e - y * ncols;
-      lds[c * LD + y] = src[(size_t)y * Nc + c0 + c];
+  // Tile I/O: thread t moves column tc = t % C of rows ty + TPL*q, so
+  // consecutive threads touch consecutive columns of a row (coalesced).
+  const int tc = threadIdx.x & (C - 1);
+  const int ty = threadIdx.x >> lgC;
+  const bool tlive = tc < ncols;
+  // threads past the last column of a partial tile read the last column again
+  // (in bounds, discarded) so the loads stay branch-free
+  const int tcc = tlive ? tc : ncols - 1;
+  V ra[P], rb[P];
+  {
+    const V* ps = Sg + c0 + tcc;
+    const V* pt = Tin + c0 + tcc;
+#pragma unroll
+    for (int qq = 0; qq < P; ++qq) {
+      const size_t off = (size_t)(ty + TPL * qq) * Nc;
+      if constexpr (MODE != COL_RK2_STAGE2) ra[qq] = ps[off];
+      rb[qq] = pt[off];  // in flight while the first transform runs
     }
-  };
-
-  V a[P], b[P];
-  if (MODE != COL_RK2_STAGE2) {
-    tile_load(S);
-    __syncthreads();
-    reg_gather<T, NY>(a, line, lane);
-    reg_fft<T, NY, false>(a, line, lane, twY, 1);
-    __syncthreads();
   }
-  tile_load(Tin);
-  __syncthreads();
-  reg_gather<T, NY>(b, line, lane);
-  reg_fft<T, NY, false>(b, line, lane, twY, 1);
+  for (int i = threadIdx.x; i < NY; i += blockDim.x) tw[i] = twY[i];
 
+  V a[P];
+  if constexpr (MODE != COL_RK2_STAGE2) {
+#pragma unroll
+    for (int qq = 0; qq < P; ++qq) lds[tcc * LD + lpad(ty + TPL * qq)] = ra[qq];
+    __syncthreads();
+    reg_gather<T, S>(a, line, lane);
+    reg_fft<T, S, false>(a, line, lane, tw, 1);
+    __syncthreads();
+  } else {
+    __syncthreads();  // twiddles visible
+  }
+  // (threads past a partial tile write the same value as the owner of tcc)
+#pragma unroll
+  for (int qq = 0; qq < P; ++qq) lds[tcc * LD + lpad(ty + TPL * qq)] = rb[qq];
+  __syncthreads();
+  // ETD tables (and the stage-2 carry) for this lane's ky, loaded now so they
+  // arrive while the second forward transform runs; dead columns of a
+  // partial tile read column c0 (in bounds, unused)
+  const size_t tbase = (size_t)(live ? col : c0) * NY;
+  T e1r[P], q1r[P], q2r[P];
+  V dr[P];
+#pragma unroll
+  for (int qq = 0; qq < P; ++qq) {
+    const size_t ti = tbase + lane + TPL * qq;
+    if constexpr (MODE == COL_RK2_STAGE2) {
+      q2r[qq] = Q2[ti];
+      dr[qq] = D[ti];
+    } else {
+      e1r[qq] = E1[ti];
+      q1r[qq] = Q1[ti];
+      if constexpr (MODE == COL_RK2_STAGE1) q2r[qq] = Q2[ti];
+    }
+  }
   const double pin = tb.pin ? *tb.pin : 0.0;
+  V b[P];
+  reg_gather<T, S>(b, line, lane);
+  reg_fft<T, S, false>(b, line, lane, tw, 1);
+
   if (live) {
-    const size_t tbase = (size_t)col * NY;
 #pragma unroll
     for (int qq = 0; qq < P; ++qq) {
       const int ky = lane + TPL * qq;
-      const size_t ti = tbase + ky;
       V out;
-      if (MODE == COL_RK2_STAGE2) {
-        const T q2 = Q2[ti];
-        V d = D[ti];
-        out = V{d.x + q2 * b[qq].x, d.y + q2 * b[qq].y};
+      if constexpr (MODE == COL_RK2_STAGE2) {
+        out = V{dr[qq].x + q2r[qq] * b[qq].x, dr[qq].y + q2r[qq] * b[qq].y};
       } else {
-        const T e1 = E1[ti], q1 = Q1[ti];
-        out = V{e1 * a[qq].x + q1 * b[qq].x, e1 * a[qq].y + q1 * b[qq].y};
-        if (MODE == COL_RK2_STAGE1) {
-          const T q2 = Q2[ti];
-          D[ti] = V{out.x - q2 * b[qq].x, out.y - q2 * b[qq].y};
-        }
+        out = V{e1r[qq] * a[qq].x + q1r[qq] * b[qq].x, e1r[qq] * a[qq].y + q1r[qq] * b[qq].y};
+        if constexpr (MODE == COL_RK2_STAGE1)
+          D[tbase + ky] = V{out.x - q2r[qq] * b[qq].x, out.y - q2r[qq] * b[qq].y};
       }
       if (tb.pin && col == 0 && ky == 0) out = V{T(pin), T(0)};
       b[qq] = out;
     }
   }
-  reg_fft<T, NY, true>(b, line, lane, twY, 1);
-  __syncthreads();
+  reg_fft<T, S, true>(b, line, lane, tw, 1);
+  line_sync<S>();
   // real projection of the kx = 0 and kx = Nx/2 columns (see k_col_etd)
   const bool realcol = (col == 0 || col == Nc - 1);
 #pragma unroll
   for (int qq = 0; qq < P; ++qq) {
-    V x = b[qq];
-    if (realcol) x.y = T(0);
-    line[lane + TPL * qq] = x;
+    V xv = b[qq];
+    if (realcol) xv.y = T(0);
+    line[lpad(lane + TPL * qq)] = xv;
   }
   __syncthreads();
-  for (int e = threadIdx.x; e < ncols * NY; e += blockDim.x) {
-    const int y = e / ncols;
-    const int c = e - y * ncols;
-    S[(size_t)y * Nc + c0 + c] = lds[c * LD + y];
+  if (tlive) {
+    V* po = Sg + c0 + tc;
+#pragma unroll
+    for (int qq = 0; qq < P; ++qq) {
+      const int y = ty + TPL * qq;
+      po[(size_t)y * Nc] = lds[tc * LD + lpad(y)];
+    }
   }
 }
+
+// ---------------------------------------------------------------------------
+// schedule variants per length (the autotuner chooses among them)
+// X(L, VARIANT, P, radices...)
+// ---------------------------------------------------------------------------
+#define PFC_SCHEDULES(X)              \
+  X(32, 0, 8, 8, 4)                   \
+  X(48, 0, 12, 4, 4, 3)               \
+  X(64, 0, 8, 8, 8)                   \
+  X(64, 1, 4, 4, 4, 4)                \
+  X(96, 0, 12, 4, 4, 2, 3)            \
+  X(128, 0, 8, 8, 8, 2)               \
+  X(128, 1, 4, 4, 4, 4, 2)            \
+  X(192, 0, 12, 4, 4, 4, 3)           \
+  X(256, 0, 8, 8, 8, 4)               \
+  X(256, 1, 4, 4, 4, 4, 4)            \
+  X(256, 2, 16, 16, 16)               \
+  X(384, 0, 12, 4, 4, 4, 2, 3)        \
+  X(512, 0, 8, 8, 8, 8)               \
+  X(512, 1, 4, 4, 4, 4, 4, 2)         \
+  X(512, 2, 16, 16, 16, 2)            \
+  X(768, 0, 12, 4, 4, 4, 4, 3)        \
+  X(1024, 0, 16, 16, 8, 8)            \
+  X(1024, 1, 8, 8, 8, 8, 2)           \
+  X(1024, 2, 4, 4, 4, 4, 4, 4)        \
+  X(2048, 0, 16, 16, 16, 8)           \
+  X(2048, 1, 8, 8, 8, 8, 4)
+
+// host-visible description of a schedule
+struct SchedInfo {
+  int L, variant, P;
+};
+
+// Launchers, instantiated in the pfc_inst_*.hip translation units.  Each
+// returns false if (L, variant) has no schedule.
+bool row_fast_launch_f64(int M, int var, dim3 grid, dim3 block, size_t lds, hipStream_t st, const V2<double>* S,
+                         V2<double>* T, int Ny, const Nonlin& q, const V2<double>* tw, int* status);
+bool row_fast_launch_f32(int M, int var, dim3 grid, dim3 block, size_t lds, hipStream_t st, const V2<float>* S,
+                         V2<float>* T, int Ny, const Nonlin& q, const V2<float>* tw, int* status);
+bool col_fast_launch_f64(int Ny, int var, int mode, dim3 grid, dim3 block, size_t lds, hipStream_t st, V2<double>* S,
+                         const V2<double>* T, V2<double>* D, int Nc, const ColTables& tb, const V2<double>* tw);
+bool col_fast_launch_f32(int Ny, int var, int mode, dim3 grid, dim3 block, size_t lds, hipStream_t st, V2<float>* S,
+                         const V2<float>* T, V2<float>* D, int Nc, const ColTables& tb, const V2<float>* tw);
 
 }  // namespace pfc

@@ -12,9 +12,12 @@
 
 namespace pfc {
 
-template <typename T> struct Vec2;
-template <> struct Vec2<double> { using type = double2; };
-template <> struct Vec2<float> { using type = float2; };
+// Complex values are 2-element clang vectors (x = real, y = imaginary):
+// copies are single 8/16-byte vector loads and stores of a first-class value.
+// Struct types (HIP's double2/float2 included) are copied with memcpy across
+// address spaces, which keeps register arrays from being promoted out of
+// scratch memory.
+template <typename T> struct Vec2 { using type = T __attribute__((ext_vector_type(2))); };
 template <typename T> using V2 = typename Vec2<T>::type;
 
 template <typename V> __device__ __forceinline__ V cadd(V a, V b) { return V{a.x + b.x, a.y + b.y}; }

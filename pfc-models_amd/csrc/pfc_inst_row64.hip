@@ -1,0 +1,24 @@
+// Fast row-kernel instantiations, double bulk (one translation unit per kind and
+// precision so the library builds in parallel).
+#include "pfc_fast.hpp"
+
+namespace pfc {
+
+#define PFC_X(LEN, VAR, PP, ...) static_assert(Sch<LEN, PP, __VA_ARGS__>::valid(), "bad schedule");
+PFC_SCHEDULES(PFC_X)
+#undef PFC_X
+
+bool row_fast_launch_f64(int M, int var, dim3 grid, dim3 block, size_t lds, hipStream_t st, const V2<double>* S,
+                         V2<double>* T, int Ny, const Nonlin& q, const V2<double>* tw, int* status) {
+#define PFC_X(LEN, VAR, PP, ...)                                                                          \
+  if (M == LEN && var == VAR) {                                                                           \
+    hipLaunchKernelGGL((k_row_fast<double, Sch<LEN, PP, __VA_ARGS__>>), grid, block, lds, st, S, T, Ny, q, tw, \
+                       status);                                                                           \
+    return true;                                                                                          \
+  }
+  PFC_SCHEDULES(PFC_X)
+#undef PFC_X
+  return false;
+}
+
+}  // namespace pfc

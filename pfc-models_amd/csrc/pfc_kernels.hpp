@@ -112,6 +112,16 @@ __device__ __forceinline__ V2<T> row_unpack(const V2<T>* Z, int k, int M, const 
   return V{T(0.5) * (s.x + wd.y), T(0.5) * (s.y - wd.x)};
 }
 
+// The same unpack from the two spectrum values and w = exp(-2 pi i k / Nx).
+template <typename T>
+__device__ __forceinline__ V2<T> row_unpack_pair(V2<T> zk, V2<T> zm, V2<T> w) {
+  using V = V2<T>;
+  V s = V{zk.x + zm.x, zk.y - zm.y};
+  V d = V{zk.x - zm.x, zk.y + zm.y};
+  V wd = cmul(w, d);
+  return V{T(0.5) * (s.x + wd.y), T(0.5) * (s.y - wd.x)};
+}
+
 // ---------------------------------------------------------------------------
 // row kernel: S~ -> n -> N(n) -> T
 // ---------------------------------------------------------------------------
@@ -552,7 +562,7 @@ __global__ void __launch_bounds__(256) k_fe_col(const V2<Tin>* __restrict__ S, C
 
 // one block: fixed-order sum of the row and column partials.
 // out[0] = free energy per site, out[1] = mean, out[2] = min, out[3] = max
-__global__ void __launch_bounds__(256) k_fe_final(const Partial* __restrict__ part, int nrow, int ncol, int Nx,
+static __global__ void __launch_bounds__(256) k_fe_final(const Partial* __restrict__ part, int nrow, int ncol, int Nx,
                                                   int Ny, double* __restrict__ out, double* __restrict__ pin_out) {
   Partial mine;
   for (int i = 0; i < 3; ++i) mine.s[i] = DD{0.0, 0.0};

@@ -14,7 +14,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(os.path.dirname(HERE), "csrc")
 OUT = os.path.join(HERE, "libpfc_hip.so")
-SOURCES = ["pfc_capi.hip"]
+SOURCES = ["pfc_capi.hip", "pfc_inst_row64.hip", "pfc_inst_row32.hip", "pfc_inst_col64.hip", "pfc_inst_col32.hip"]
 ARCH = os.environ.get("PFC_HIP_ARCH", "gfx950")
 
 FLAGS = [
@@ -31,16 +31,32 @@ def hipcc() -> str:
     raise RuntimeError("hipcc not found")
 
 
-def build(verbose: bool = False) -> str:
-    srcs = [os.path.join(CSRC, s) for s in SOURCES]
-    tmp = OUT + ".tmp"
-    cmd = [hipcc(), *FLAGS, "-o", tmp, *srcs]
+def _run(cmd, verbose):
     if verbose:
         print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         sys.stderr.write(r.stdout + r.stderr)
-        raise RuntimeError(f"hipcc failed ({r.returncode})")
+        raise RuntimeError(f"hipcc failed ({r.returncode}): {' '.join(cmd)}")
+
+
+def build(verbose: bool = False, jobs: int | None = None) -> str:
+    """Compile each translation unit to an object in parallel, then link."""
+    from concurrent.futures import ThreadPoolExecutor
+    objdir = os.path.join(os.path.dirname(HERE), "_build")
+    os.makedirs(objdir, exist_ok=True)
+    cc = hipcc()
+    compile_flags = [f for f in FLAGS if f != "-shared"]
+    objs, cmds = [], []
+    for src in SOURCES:
+        obj = os.path.join(objdir, os.path.splitext(src)[0] + ".o")
+        objs.append(obj)
+        cmds.append([cc, *compile_flags, "-c", os.path.join(CSRC, src), "-o", obj])
+    jobs = jobs or min(len(cmds), max(1, min(8, os.cpu_count() or 1)))
+    with ThreadPoolExecutor(max_workers=jobs) as ex:
+        list(ex.map(lambda c: _run(c, verbose), cmds))
+    tmp = OUT + ".tmp"
+    _run([cc, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp, *objs], verbose)
     os.replace(tmp, OUT)
     return OUT
 

@@ -62,6 +62,7 @@ _SIGNATURES = [
     ("pfc_local_field", ctypes.c_int, [_P, _P, ctypes.c_int, _P]),
     ("pfc_step_profile", ctypes.c_int, [_P, ctypes.c_longlong, _P, ctypes.POINTER(ctypes.c_double),
                                         ctypes.POINTER(ctypes.c_longlong)]),
+    ("pfc_plan_schedule", ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_int)]),
     ("pfc_plan_bytes", ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_longlong)]),
     ("pfc_rfft2", ctypes.c_int, [_P, _P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _P]),
     ("pfc_irfft2", ctypes.c_int, [_P, _P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _P]),

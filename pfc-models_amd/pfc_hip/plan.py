@@ -153,6 +153,12 @@ class Plan:
         _lib.check(self._lib.pfc_get_status(self._h, ctypes.byref(v), current_stream_handle()))
         return v.value
 
+    def schedule(self):
+        """Kernel schedule in use: {'row': (fast, variant, P, lines), 'col': (...)}."""
+        info = (ctypes.c_int * 8)()
+        _lib.check(self._lib.pfc_plan_schedule(self._h, info))
+        return {"row": tuple(info[0:4]), "col": tuple(info[4:8])}
+
     def device_bytes(self):
         v = ctypes.c_longlong()
         _lib.check(self._lib.pfc_plan_bytes(self._h, ctypes.byref(v)))

@@ -180,3 +180,51 @@ def test_fp32_step_tolerance(hip_device):
     assert err <= 1e-5, err
     assert abs(fe - sim.free_energy()) <= 1e-6 * abs(sim.free_energy())
     assert abs(mean - m0) <= 1e-7
+
+
+# schedule variants per length (mirrors the library's table): {L: [(variant, points per lane)]}
+SCHEDULES = {32: [(0, 8)], 48: [(0, 12)], 64: [(0, 8), (1, 4)], 96: [(0, 12)], 128: [(0, 8), (1, 4)],
+             192: [(0, 12)], 256: [(0, 8), (1, 4), (2, 16)], 384: [(0, 12)], 512: [(0, 8), (1, 4), (2, 16)],
+             768: [(0, 12)], 1024: [(0, 16), (1, 8), (2, 4)], 2048: [(0, 16), (1, 8)]}
+
+
+def _shapes(L, lds_elems, cs):
+    """(variant, lines) pairs the library accepts for a length-L line."""
+    out = []
+    for var, P in SCHEDULES[L]:
+        tpl = L // P
+        w = 1
+        while w <= 256:
+            if 64 <= tpl * w <= 256 and lds_elems(w) * cs <= 163840:
+                out.append((var, w))
+            w *= 2
+    return out
+
+
+@pytest.mark.parametrize("precision", [64, 32])
+@pytest.mark.parametrize("shape", [(1024, 1024), (256, 512), (192, 384)])
+def test_every_schedule_matches_generic(hip_device, shape, precision, monkeypatch):
+    """Each (schedule variant, lines per block) the autotuner may pick computes the
+    same ETDRK2 step as the generic kernels (FP64 <= 1e-13, FP32 <= 2e-5 after 4 steps)."""
+    from pfc_hip.plan import Plan
+    ny, nx = shape
+    cs = 16 if precision == 64 else 8
+    psi0 = F.uniform_noise(shape, -0.3, 0.05, seed=8)
+    kw = dict(Nx=nx, Ny=ny, Lx=nx * np.pi / 4, Ly=ny * np.pi / 4, epsilon=-0.3, g=0.2, dt=0.5, scheme="etdrk2",
+              precision=precision)
+    monkeypatch.setenv("PFC_HIP_GENERIC", "1")
+    ref = Plan(**kw).set_field(psi0).step(4).get_field().cpu().numpy()
+    monkeypatch.delenv("PFC_HIP_GENERIC")
+    lpad = lambda i: i + (i >> 4)  # LDS padding of the library's fast kernels
+    rows = _shapes(nx // 2, lambda w: nx + w * (lpad(nx // 2) + 2), cs)
+    cols = _shapes(ny, lambda w: ny + w * (lpad(ny - 1) + 2), cs)
+    tol = 1e-13 if precision == 64 else 2e-5
+    combos = [(r, cols[0]) for r in rows] + [(rows[0], c) for c in cols[1:]]
+    for (rv, rl), (cv, cl) in combos:
+        monkeypatch.setenv("PFC_HIP_SCHEDULE", f"{rv}:{rl}:{cv}:{cl}")
+        pl = Plan(**kw).set_field(psi0).step(4)
+        sched = pl.schedule()
+        assert sched["row"] == (1, rv, dict(SCHEDULES[nx // 2])[rv], rl)
+        assert sched["col"] == (1, cv, dict(SCHEDULES[ny])[cv], cl)
+        d = np.abs(pl.get_field().cpu().numpy() - ref).max()
+        assert d <= tol, (rv, rl, cv, cl, d)
