@@ -113,6 +113,36 @@ int pfc_plan_schedule(const pfc_plan* plan, int info[8]);
 /* Device bytes held by a plan (state, tables, scratch). */
 int pfc_plan_bytes(const pfc_plan* plan, long long* bytes);
 
+/* ---- batches: many same-shape instances stepped by the same launches ----
+ * A batch holds `capacity` slots sharing the grid, model, scheme, precision
+ * and every parameter of pfc_params except epsilon and M, which are per slot
+ * (pfc_batch_set_slot; initially the shared values).  Each slot has its own
+ * field, ETD tables, k = 0 pin and status word.  pfc_batch_step advances the
+ * active slots (initially all) together: one row launch and one column launch
+ * per nonlinear evaluation, whatever the number of slots.  A slot's results do
+ * not depend on which other slots are active.  (No reference counterpart: the
+ * reference relaxes one Simulation at a time, phase_diagram/q0star.py:409-477
+ * and :566-635; this is the batched submission the phase-diagram driver and a
+ * serving loop use.) */
+typedef struct pfc_batch pfc_batch;
+int pfc_batch_create(const pfc_params* shared, int capacity, pfc_batch** out);
+int pfc_batch_destroy(pfc_batch* batch);
+/* Per-slot epsilon and stabilising shift M; rebuilds that slot's tables (synchronises `stream`). */
+int pfc_batch_set_slot(pfc_batch* batch, int slot, double epsilon, double M, void* stream);
+int pfc_batch_set_field(pfc_batch* batch, int slot, const void* field, int dtype, void* stream);
+int pfc_batch_get_field(pfc_batch* batch, int slot, void* field, int dtype, void* stream);
+/* Slots stepped and reduced by the next calls, in this order (synchronises `stream`). */
+int pfc_batch_set_active(pfc_batch* batch, const int* slots, int n, void* stream);
+int pfc_batch_step(pfc_batch* batch, long long nsteps, void* stream);
+/* FP64 reductions of the active slots: out[4 i + {0,1,2,3}] = free energy per
+ * site, mean, min, max of the i-th active slot.  Synchronises `stream`. */
+int pfc_batch_reduce(pfc_batch* batch, double* out, void* stream);
+/* Read and clear every slot's PFC_STATUS_* flags: flags[capacity]. */
+int pfc_batch_get_status(pfc_batch* batch, int* flags, void* stream);
+int pfc_batch_bytes(const pfc_batch* batch, long long* bytes);
+/* Device bytes one slot of these parameters needs (state + tables); for admission control. */
+int pfc_slot_bytes(const pfc_params* params, long long* bytes);
+
 /* Batched 2D real transforms over (batch, Ny, Nx) device arrays.
  * pfc_rfft2:  out[b] = rfft2(in[b])             (unnormalised)
  * pfc_irfft2: out[b] = irfft2(in[b], s=(Ny,Nx)) (scaled 1/(Nx Ny); `in` is not modified) */

@@ -1,4 +1,6 @@
-// Host side of libpfc_hip.so: plans, FP64 tables, launch shapes, C ABI.
+// Host side of libpfc_hip.so: instance sets ("engines"), launch shapes, the
+// autotuner and the C ABI.  A pfc_plan is an engine of one slot; a pfc_batch
+// is an engine of many same-shape slots stepped by the same launches.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -275,34 +277,38 @@ FastShape pick_default(const std::vector<FastShape>& cands, int target_blocks) {
 
 template <typename T> struct FastLaunch;
 template <> struct FastLaunch<double> {
-  static void row(int M, const FastShape& f, hipStream_t st, const V2<double>* S, V2<double>* T, int Ny, const Nonlin& q,
-                  const V2<double>* tw, int* status) {
-    if (!row_fast_launch_f64(M, f.var, dim3(f.blocks), dim3(f.lines * f.tpl), f.lds, st, S, T, Ny, q, tw, status))
+  static void row(int M, const FastShape& f, int ny_inst, hipStream_t st, const V2<double>* S, V2<double>* T, int Ny,
+                  const Nonlin& q, const V2<double>* tw, const Slots& sl) {
+    if (!row_fast_launch_f64(M, f.var, dim3(f.blocks, ny_inst), dim3(f.lines * f.tpl), f.lds, st, S, T, Ny, q, tw,
+                             sl))
       fail(PFC_EUNSUPPORTED, "no fast row schedule");
   }
-  static void col(int Ny, int mode, const FastShape& f, hipStream_t st, V2<double>* S, const V2<double>* T, V2<double>* D,
-                  int Nc, const ColTables& tb, const V2<double>* tw) {
-    if (!col_fast_launch_f64(Ny, f.var, mode, dim3(f.blocks), dim3(f.lines * f.tpl), f.lds, st, S, T, D, Nc, tb,
-                             tw))
+  static void col(int Ny, int mode, const FastShape& f, int ny_inst, hipStream_t st, V2<double>* S,
+                  const V2<double>* T, V2<double>* D, int Nc, const ColTables& tb, const V2<double>* tw,
+                  const Slots& sl) {
+    if (!col_fast_launch_f64(Ny, f.var, mode, dim3(f.blocks, ny_inst), dim3(f.lines * f.tpl), f.lds, st, S, T, D,
+                             Nc, tb, tw, sl))
       fail(PFC_EUNSUPPORTED, "no fast column schedule");
   }
 };
 template <> struct FastLaunch<float> {
-  static void row(int M, const FastShape& f, hipStream_t st, const V2<float>* S, V2<float>* T, int Ny, const Nonlin& q,
-                  const V2<float>* tw, int* status) {
-    if (!row_fast_launch_f32(M, f.var, dim3(f.blocks), dim3(f.lines * f.tpl), f.lds, st, S, T, Ny, q, tw, status))
+  static void row(int M, const FastShape& f, int ny_inst, hipStream_t st, const V2<float>* S, V2<float>* T, int Ny,
+                  const Nonlin& q, const V2<float>* tw, const Slots& sl) {
+    if (!row_fast_launch_f32(M, f.var, dim3(f.blocks, ny_inst), dim3(f.lines * f.tpl), f.lds, st, S, T, Ny, q, tw,
+                             sl))
       fail(PFC_EUNSUPPORTED, "no fast row schedule");
   }
-  static void col(int Ny, int mode, const FastShape& f, hipStream_t st, V2<float>* S, const V2<float>* T, V2<float>* D,
-                  int Nc, const ColTables& tb, const V2<float>* tw) {
-    if (!col_fast_launch_f32(Ny, f.var, mode, dim3(f.blocks), dim3(f.lines * f.tpl), f.lds, st, S, T, D, Nc, tb,
-                             tw))
+  static void col(int Ny, int mode, const FastShape& f, int ny_inst, hipStream_t st, V2<float>* S,
+                  const V2<float>* T, V2<float>* D, int Nc, const ColTables& tb, const V2<float>* tw,
+                  const Slots& sl) {
+    if (!col_fast_launch_f32(Ny, f.var, mode, dim3(f.blocks, ny_inst), dim3(f.lines * f.tpl), f.lds, st, S, T, D,
+                             Nc, tb, tw, sl))
       fail(PFC_EUNSUPPORTED, "no fast column schedule");
   }
 };
 
 // ---------------------------------------------------------------------------
-// FP64 k-grids and ETD tables (the reference's host arithmetic, restated)
+// FP64 k-grids (the reference's host arithmetic, restated)
 // ---------------------------------------------------------------------------
 // kx = 2 pi * rfftfreq(Nx, dx), ky = 2 pi * fftfreq(Ny, dy)  (geometry_2d.py:154-155)
 std::vector<double> rfft_k(int n, double d) {
@@ -322,118 +328,15 @@ std::vector<double> fft_k(int n, double d) {
   return k;
 }
 
-// ETDRK2 coefficient dt*phi2(z), phi2 = (e^z - 1 - z)/z^2 (Cox & Matthews 2002):
-// Taylor series for |z| < 0.1, expm1 otherwise.
-double etd2_coeff(double z, double dt) {
-  if (std::fabs(z) < 0.1) {
-    double s = 0.0, t = 0.5;
-    for (int j = 0; j < 16; ++j) {
-      s += t;
-      t = t * z / (j + 3);
-    }
-    return dt * s;
-  }
-  return dt * ((std::expm1(z) - z) / (z * z));
-}
-
-struct Tables {
-  std::vector<double> E1, Q1, Q2;  // column-major [Nc][Ny]
-};
-
-// E = exp(L dt), P = (E-1)/L or dt(1 + z/2 + z^2/6) (kernels.py:55-71) with
-// L = Gamma lap (beta (q0^2 + lap)^2 + eps + M), lap = -(kx^2 + ky^2)
-// (stdpfc_cpu/simulation.py:95-108; M as logpfc_extv/simulation.py:100-116).
-// Stored pre-scaled for the row-spectrum state: E1 = E/Ny, Q1 = P Gamma lap/(Nx Ny).
-Tables build_tables(const pfc_params& p, const std::vector<double>& kx, const std::vector<double>& ky, double M) {
-  const int Nc = (int)kx.size(), Ny = (int)ky.size();
-  const double inv_ny = 1.0 / Ny, inv_n = 1.0 / ((double)p.Nx * (double)Ny);
-  Tables t;
-  t.E1.resize((size_t)Nc * Ny);
-  t.Q1.resize((size_t)Nc * Ny);
-  if (p.scheme == PFC_SCHEME_ETDRK2) t.Q2.resize((size_t)Nc * Ny);
-  for (int c = 0; c < Nc; ++c) {
-    for (int j = 0; j < Ny; ++j) {
-      const double lap = -(kx[c] * kx[c] + ky[j] * ky[j]);
-      const double s = p.q0 * p.q0 + lap;
-      double lmu = p.beta * (s * s) + p.epsilon;
-      if (M != 0.0) lmu = lmu + M;
-      const double L = p.Gamma * lap * lmu;
-      const double z = L * p.dt;
-      const double E = std::exp(z);
-      const double P = std::fabs(z) > p.smallZTol ? (E - 1) / L : p.dt * (1 + z / 2 + z * z / 6);
-      const size_t i = (size_t)c * Ny + j;
-      t.E1[i] = E * inv_ny;
-      t.Q1[i] = P * (p.Gamma * lap) * inv_n;
-      if (!t.Q2.empty()) t.Q2[i] = etd2_coeff(z, p.dt) * (p.Gamma * lap) * inv_n;
-    }
-  }
-  return t;
-}
-
-template <typename X>
-std::vector<X> cast_vec(const std::vector<double>& v) {
-  std::vector<X> o(v.size());
-  for (size_t i = 0; i < v.size(); ++i) o[i] = (X)v[i];
-  return o;
-}
-
-}  // namespace
-
-// ---------------------------------------------------------------------------
-// plan
-// ---------------------------------------------------------------------------
-struct pfc_plan {
-  pfc_params p{};
-  int Nc = 0;
-  size_t csize = 0;  // bytes per complex of the bulk precision
-  RowGeom rg{}, rg64{};
-  ColGeom cg{}, cg64{};
-  FastShape frow{}, fcol{};      // compile-time-schedule kernels, when the sizes have one
-  bool tuned = false;            // autotuner has run (or was skipped) for this plan
-  DevBuf S, Tb, D;               // state, row spectrum of N, RK2 k-space carry
-  DevBuf E1, Q1, Q2;             // ETD tables (bulk precision)
-  DevBuf twN, twY;               // twiddles (bulk precision)
-  DevBuf twN64, twY64;           // FP64 twiddles for the reductions
-  DevBuf kx2, ky2;               // FP64 k^2 grids for the free energy
-  DevBuf part;                   // reduction partials
-  DevBuf diag;                   // two row spectra for the chemical-potential diagnostics
-  DevBuf scal;                   // [0..3] reduce out, [4] pin, int status at [5]
-  std::vector<double> kx, ky;
-  double M = 0.0;
-  bool field_set = false;
-  double* host_out = nullptr;    // pinned host mirror of scal[0..3]
-  ~pfc_plan() {
-    if (host_out) (void)hipHostFree(host_out);
-  }
-  bool f64() const { return p.precision == 64; }
-  double* pin() const { return reinterpret_cast<double*>(scal.p) + 4; }
-  int* status() const { return reinterpret_cast<int*>(reinterpret_cast<double*>(scal.p) + 5); }
-};
-
-namespace {
-
-void upload_tables(pfc_plan* pl) {
-  Tables t = build_tables(pl->p, pl->kx, pl->ky, pl->M);
-  if (pl->f64()) {
-    pl->E1.upload(t.E1);
-    pl->Q1.upload(t.Q1);
-    if (!t.Q2.empty()) pl->Q2.upload(t.Q2);
-  } else {
-    pl->E1.upload(cast_vec<float>(t.E1));
-    pl->Q1.upload(cast_vec<float>(t.Q1));
-    if (!t.Q2.empty()) pl->Q2.upload(cast_vec<float>(t.Q2));
-  }
-}
-
-Nonlin nonlin_of(const pfc_plan* pl) {
-  Nonlin q{};
-  q.model = pl->p.model;
-  q.clamp = pl->p.clamp_log;
-  q.g = pl->p.g;
-  q.minPhi = pl->p.minPhi;
-  q.logMin = std::log(pl->p.minPhi > 0 ? pl->p.minPhi : 1e-300);
-  q.M = pl->M;
-  return q;
+void validate(const pfc_params* p) {
+  if (!p) fail(PFC_EINVAL, "params is NULL");
+  if (p->model != PFC_MODEL_STD && p->model != PFC_MODEL_LOG) fail(PFC_EINVAL, "model must be PFC_MODEL_STD or PFC_MODEL_LOG");
+  if (p->Nx < 4 || p->Ny < 2 || (p->Nx & 1)) fail(PFC_EINVAL, "Nx must be even and >= 4, Ny >= 2");
+  if (!(p->Lx > 0) || !(p->Ly > 0)) fail(PFC_EINVAL, "Lx and Ly must be positive");
+  if (p->precision != 64 && p->precision != 32) fail(PFC_EINVAL, "precision must be 64 or 32");
+  if (p->scheme != PFC_SCHEME_ETD1 && p->scheme != PFC_SCHEME_ETDRK2) fail(PFC_EINVAL, "scheme must be ETD1 or ETDRK2");
+  if (!std::isfinite(p->dt) || !(p->dt > 0)) fail(PFC_EINVAL, "dt must be positive");
+  if (p->model == PFC_MODEL_LOG && p->clamp_log && !(p->minPhi > 0)) fail(PFC_EINVAL, "minPhi must be positive");
 }
 
 // Optional per-launch event timing (pfc_step_profile): kind 0 row, 1 col, 2 col stage 2.
@@ -453,60 +356,218 @@ struct Prof {
   }
 };
 
-template <typename T>
-void launch_step(pfc_plan* pl, long long nsteps, hipStream_t st, Prof* prof = nullptr) {
-  using V = V2<T>;
-  V* S = reinterpret_cast<V*>(pl->S.p);
-  V* Tb = reinterpret_cast<V*>(pl->Tb.p);
-  V* D = reinterpret_cast<V*>(pl->D.p);
-  const V* twN = reinterpret_cast<const V*>(pl->twN.p);
-  const V* twY = reinterpret_cast<const V*>(pl->twY.p);
-  ColTables tb{pl->E1.p, pl->Q1.p, pl->Q2.p, pl->p.pin_mean ? pl->pin() : nullptr};
-  const Nonlin q = nonlin_of(pl);
-  const size_t rl = row_lds(pl->rg, sizeof(V)), cl = col_lds(pl->cg, sizeof(V));
-  const int rb = row_blocks(pl->rg), cb = col_blocks(pl->cg);
-  auto B = [&]() { if (prof) prof->begin(); };
-  auto E = [&](int k) { if (prof) prof->end(k); };
-  const int M = pl->p.Nx / 2, Ny = pl->p.Ny, Nc = pl->Nc;
-  auto row = [&]() {
-    B();
-    if (pl->frow.on)
-      FastLaunch<T>::row(M, pl->frow, st, S, Tb, Ny, q, twN, pl->status());
-    else
-      hipLaunchKernelGGL((k_row_nonlin<T>), dim3(rb), dim3(256), rl, st, S, Tb, pl->rg, q, twN, pl->status());
-    E(0);
-  };
-  auto col = [&](auto mode_tag, int kind) {
-    constexpr int MODE = decltype(mode_tag)::value;
-    B();
-    if (pl->fcol.on)
-      FastLaunch<T>::col(Ny, MODE, pl->fcol, st, S, Tb, D, Nc, tb, twY);
-    else
-      hipLaunchKernelGGL((k_col_etd<T, MODE>), dim3(cb), dim3(256), cl, st, S, Tb, D, pl->cg, tb, twY);
-    E(kind);
-  };
-  for (long long s = 0; s < nsteps; ++s) {
-    row();
-    if (pl->p.scheme == PFC_SCHEME_ETD1) {
-      col(std::integral_constant<int, COL_ETD1>{}, 1);
-    } else {
-      col(std::integral_constant<int, COL_RK2_STAGE1>{}, 1);
-      row();
-      col(std::integral_constant<int, COL_RK2_STAGE2>{}, 2);
+// ---------------------------------------------------------------------------
+// Engine: `cap` same-shape instances (slots).  Per slot: state S~ (row
+// spectrum), T (row spectrum of N), D (ETDRK2 carry), ETD tables, epsilon and
+// M, the k = 0 pin and a status word.  Steps act on the active slot list.
+// ---------------------------------------------------------------------------
+struct Engine {
+  pfc_params p{};
+  int cap = 0, Nc = 0;
+  size_t csize = 0, nc = 0;  // bytes per complex; complex elements per slot array
+  RowGeom rg{}, rg64{};
+  ColGeom cg{}, cg64{};
+  FastShape frow{}, fcol{};
+  bool tuned = false, batched = false;
+  DevBuf S, Tb, D, E1, Q1, Q2;       // [cap][...]
+  DevBuf twN, twY, twN64, twY64, kx2, ky2;
+  DevBuf part, out, pin, status, eps, Ms, active, diag;
+  std::vector<double> h_eps, h_M;
+  std::vector<int> h_active;
+  std::vector<char> field_set;
+  double* host_out = nullptr;   // pinned [4 cap]
+  int* host_status = nullptr;   // pinned [cap]
+  ~Engine() {
+    if (host_out) (void)hipHostFree(host_out);
+    if (host_status) (void)hipHostFree(host_status);
+  }
+  bool f64() const { return p.precision == 64; }
+  int nact() const { return (int)h_active.size(); }
+  int pstride() const { return row_blocks(rg64) + col_blocks(cg64); }
+  Slots slots_active() const {
+    return Slots{(const int*)active.p, (long long)nc, (long long)nc, p.pin_mean ? (const double*)pin.p : nullptr,
+                 (int*)status.p, (const double*)Ms.p};
+  }
+  // a single slot addressed directly (list = null, blockIdx.y = 0 is that slot)
+  Slots slots_one(int slot) const {
+    return Slots{nullptr, 0, 0, p.pin_mean ? (const double*)pin.p + slot : nullptr, (int*)status.p + slot,
+                 (const double*)Ms.p + slot};
+  }
+  void check_slot(int slot) const {
+    if (slot < 0 || slot >= cap) fail(PFC_EINVAL, "slot out of range");
+  }
+};
+
+void engine_init(Engine& e, const pfc_params* params, int cap, bool batched) {
+  validate(params);
+  if (cap < 1) fail(PFC_EINVAL, "capacity must be >= 1");
+  e.p = *params;
+  e.cap = cap;
+  e.batched = batched;
+  const int Nx = params->Nx, Ny = params->Ny;
+  e.Nc = Nx / 2 + 1;
+  e.nc = (size_t)e.Nc * Ny;
+  e.csize = e.f64() ? sizeof(V2<double>) : sizeof(V2<float>);
+  e.rg = row_geom(Nx, Ny, e.csize);
+  e.cg = col_geom(Nx, Ny, e.csize);
+  e.rg64 = row_geom(Nx, Ny, sizeof(V2<double>));
+  e.cg64 = col_geom(Nx, Ny, sizeof(V2<double>));
+  if (!std::getenv("PFC_HIP_GENERIC")) {
+    auto rc = row_candidates(Nx, Ny, e.csize);
+    auto cc = col_candidates(Nx, Ny, e.csize);
+    e.frow = pick_default(rc, 512);
+    e.fcol = pick_default(cc, 256);
+    // PFC_HIP_SCHEDULE="rowvar:rowlines:colvar:collines" pins a schedule
+    // (tests and tuning studies); the autotuner is then skipped.
+    if (const char* fx = std::getenv("PFC_HIP_SCHEDULE")) {
+      int v[4] = {-1, -1, -1, -1};
+      if (std::sscanf(fx, "%d:%d:%d:%d", &v[0], &v[1], &v[2], &v[3]) == 4) {
+        bool okr = false, okc = false;
+        for (const FastShape& f : rc)
+          if (f.var == v[0] && f.lines == v[1]) e.frow = f, okr = true;
+        for (const FastShape& f : cc)
+          if (f.var == v[2] && f.lines == v[3]) e.fcol = f, okc = true;
+        if (!okr && e.frow.on) fail(PFC_EINVAL, "PFC_HIP_SCHEDULE: no such row schedule");
+        if (!okc && e.fcol.on) fail(PFC_EINVAL, "PFC_HIP_SCHEDULE: no such column schedule");
+        e.tuned = true;
+      }
     }
   }
-  hip_check(hipGetLastError(), "step launch");
+  const size_t bytes = (size_t)cap * e.nc * e.csize;
+  const size_t tbytes = (size_t)cap * e.nc * (e.f64() ? sizeof(double) : sizeof(float));
+  e.S.alloc(bytes);
+  e.Tb.alloc(bytes);
+  if (params->scheme == PFC_SCHEME_ETDRK2) e.D.alloc(bytes);
+  e.E1.alloc(tbytes);
+  e.Q1.alloc(tbytes);
+  if (params->scheme == PFC_SCHEME_ETDRK2) e.Q2.alloc(tbytes);
+  auto kx = rfft_k(Nx, params->Lx / Nx);
+  auto ky = fft_k(Ny, params->Ly / Ny);
+  std::vector<double> kx2(kx.size()), ky2(ky.size());
+  for (size_t i = 0; i < kx2.size(); ++i) kx2[i] = kx[i] * kx[i];
+  for (size_t i = 0; i < ky2.size(); ++i) ky2[i] = ky[i] * ky[i];
+  e.kx2.upload(kx2);
+  e.ky2.upload(ky2);
+  auto tn = twiddles(Nx), ty = twiddles(Ny);
+  e.twN64.upload(tn);
+  e.twY64.upload(ty);
+  if (e.f64()) {
+    e.twN.upload(tn);
+    e.twY.upload(ty);
+  } else {
+    e.twN.upload(to_f32(tn));
+    e.twY.upload(to_f32(ty));
+  }
+  e.part.alloc((size_t)cap * e.pstride() * sizeof(Partial));
+  e.out.alloc((size_t)cap * 4 * sizeof(double));
+  e.pin.alloc((size_t)cap * sizeof(double));
+  e.status.alloc((size_t)cap * sizeof(int));
+  e.active.alloc((size_t)cap * sizeof(int));
+  e.h_eps.assign(cap, params->epsilon);
+  e.h_M.assign(cap, params->M);
+  e.eps.upload(e.h_eps);
+  e.Ms.upload(e.h_M);
+  hip_check(hipMemset(e.pin.p, 0, e.pin.n), "memset");
+  hip_check(hipMemset(e.status.p, 0, e.status.n), "memset");
+  hip_check(hipMemset(e.S.p, 0, bytes), "memset");
+  e.field_set.assign(cap, 0);
+  hip_check(hipHostMalloc((void**)&e.host_out, (size_t)cap * 4 * sizeof(double), hipHostMallocDefault), "hipHostMalloc");
+  hip_check(hipHostMalloc((void**)&e.host_status, (size_t)cap * sizeof(int), hipHostMallocDefault), "hipHostMalloc");
+  // every slot starts with tables for the shared epsilon and M, and is active
+  e.h_active.resize(cap);
+  for (int i = 0; i < cap; ++i) e.h_active[i] = i;
+  e.active.upload(e.h_active);
 }
 
+// ETD tables of slots [slot0, slot0 + n), built in FP64 on the device
+void engine_tables(Engine& e, int slot0, int n, hipStream_t st) {
+  TabParams tp{e.p.Gamma, e.p.beta, e.p.q0, 0.0, 0.0, e.p.dt, e.p.smallZTol,
+               1.0 / e.p.Ny, 1.0 / ((double)e.p.Nx * (double)e.p.Ny), e.Nc, e.p.Ny};
+  const int blocks = (int)std::min<size_t>((e.nc + 255) / 256, 1024);
+  if (e.f64())
+    hipLaunchKernelGGL(k_tables<double>, dim3(blocks, n), dim3(256), 0, st, (double*)e.E1.p, (double*)e.Q1.p,
+                       (double*)e.Q2.p, (long long)e.nc, (const double*)e.kx2.p, (const double*)e.ky2.p, tp,
+                       (const double*)e.eps.p, (const double*)e.Ms.p, slot0);
+  else
+    hipLaunchKernelGGL(k_tables<float>, dim3(blocks, n), dim3(256), 0, st, (float*)e.E1.p, (float*)e.Q1.p,
+                       (float*)e.Q2.p, (long long)e.nc, (const double*)e.kx2.p, (const double*)e.ky2.p, tp,
+                       (const double*)e.eps.p, (const double*)e.Ms.p, slot0);
+  hip_check(hipGetLastError(), "table launch");
+}
+
+Nonlin nonlin_of(const Engine& e) {
+  Nonlin q{};
+  q.model = e.p.model;
+  q.clamp = e.p.clamp_log;
+  q.g = e.p.g;
+  q.minPhi = e.p.minPhi;
+  q.logMin = std::log(e.p.minPhi > 0 ? e.p.minPhi : 1e-300);
+  q.M = 0.0;  // per slot, from Slots.M
+  return q;
+}
+
+// one nonlinear row pass and one column pass (MODE) over `ny` instances
+template <typename T>
+struct Steps {
+  using V = V2<T>;
+  static void row(Engine& e, hipStream_t st, const Slots& sl, int ny) {
+    const V* S = (const V*)e.S.p;
+    V* Tb = (V*)e.Tb.p;
+    const Nonlin q = nonlin_of(e);
+    if (e.frow.on)
+      FastLaunch<T>::row(e.p.Nx / 2, e.frow, ny, st, S, Tb, e.p.Ny, q, (const V*)e.twN.p, sl);
+    else
+      hipLaunchKernelGGL((k_row_nonlin<T>), dim3(row_blocks(e.rg), ny), dim3(256), row_lds(e.rg, sizeof(V)), st, S,
+                         Tb, e.rg, q, (const V*)e.twN.p, sl);
+  }
+  static void col(Engine& e, int mode, hipStream_t st, const Slots& sl, int ny) {
+    ColTables tb{e.E1.p, e.Q1.p, e.Q2.p};
+    V* S = (V*)e.S.p;
+    V* D = (V*)e.D.p;
+    const V* Tb = (const V*)e.Tb.p;
+    if (e.fcol.on) {
+      FastLaunch<T>::col(e.p.Ny, mode, e.fcol, ny, st, S, Tb, D, e.Nc, tb, (const V*)e.twY.p, sl);
+      return;
+    }
+    const dim3 g(col_blocks(e.cg), ny);
+    const size_t lds = col_lds(e.cg, sizeof(V));
+    const V* tw = (const V*)e.twY.p;
+    if (mode == COL_ETD1)
+      hipLaunchKernelGGL((k_col_etd<T, COL_ETD1>), g, dim3(256), lds, st, S, Tb, D, e.cg, tb, tw, sl);
+    else if (mode == COL_RK2_STAGE1)
+      hipLaunchKernelGGL((k_col_etd<T, COL_RK2_STAGE1>), g, dim3(256), lds, st, S, Tb, D, e.cg, tb, tw, sl);
+    else
+      hipLaunchKernelGGL((k_col_etd<T, COL_RK2_STAGE2>), g, dim3(256), lds, st, S, Tb, D, e.cg, tb, tw, sl);
+  }
+  static void step(Engine& e, long long nsteps, hipStream_t st, Prof* prof) {
+    const int ny = e.nact();
+    if (ny == 0) return;
+    const Slots sl = e.slots_active();
+    auto B = [&]() { if (prof) prof->begin(); };
+    auto E = [&](int k) { if (prof) prof->end(k); };
+    for (long long s = 0; s < nsteps; ++s) {
+      B(); row(e, st, sl, ny); E(0);
+      if (e.p.scheme == PFC_SCHEME_ETD1) {
+        B(); col(e, COL_ETD1, st, sl, ny); E(1);
+      } else {
+        B(); col(e, COL_RK2_STAGE1, st, sl, ny); E(1);
+        B(); row(e, st, sl, ny); E(0);
+        B(); col(e, COL_RK2_STAGE2, st, sl, ny); E(2);
+      }
+    }
+    hip_check(hipGetLastError(), "step launch");
+  }
+};
+
 // ---------------------------------------------------------------------------
-// autotuner: at a plan's first step, time every (schedule variant, lines per
-// block) candidate of the fast row and column kernels on scratch copies of the
-// live state and keep the fastest.  Choices are cached per (device, kernel
-// kind, lengths, precision, mode) for the life of the process.  PFC_HIP_TUNE=0
-// keeps the defaults.
+// autotuner: at an engine's first step, time every (schedule variant, lines
+// per block) candidate of the fast row and column kernels on scratch copies
+// of the live state and keep the fastest.  Choices are cached per (device,
+// kernel kind, lengths, precision, mode, instances per launch) for the life of
+// the process.  PFC_HIP_TUNE=0 keeps the defaults.
 // ---------------------------------------------------------------------------
 std::mutex g_tune_mu;
-std::map<std::array<int, 6>, std::pair<int, int>> g_tune;  // -> (variant, lines)
+std::map<std::array<int, 7>, std::pair<int, int>> g_tune;  // -> (variant, lines)
 
 template <typename F>
 float time_launches(hipStream_t st, int reps, F&& launch) {
@@ -526,104 +587,295 @@ float time_launches(hipStream_t st, int reps, F&& launch) {
   return ms / reps;
 }
 
+int bucket(int n) {
+  int b = 1;
+  while (b < n) b *= 2;
+  return b;
+}
+
 template <typename T>
-void autotune(pfc_plan* pl, hipStream_t st) {
-  pl->tuned = true;
+void autotune(Engine& e, hipStream_t st) {
+  e.tuned = true;
   const char* env = std::getenv("PFC_HIP_TUNE");
   if (env && env[0] == '0') return;
   using V = V2<T>;
+  const int ny = e.nact();
+  if (ny == 0) return;
   int dev = 0;
   hip_check(hipGetDevice(&dev), "hipGetDevice");
-  const int Nx = pl->p.Nx, Ny = pl->p.Ny, M = Nx / 2, Nc = pl->Nc;
-  const size_t nc = (size_t)Nc * Ny;
-  const int reps = 5;
+  const int Nx = e.p.Nx, Ny = e.p.Ny, M = Nx / 2, Nc = e.Nc;
+  const int reps = ny > 8 ? 2 : 5;
+  const Slots sl = e.slots_active();
   std::lock_guard<std::mutex> lk(g_tune_mu);
   auto apply = [](FastShape& cur, const std::vector<FastShape>& cands, std::pair<int, int> vl) {
     for (const FastShape& f : cands)
       if (f.var == vl.first && f.lines == vl.second) cur = f;
   };
-  if (pl->frow.on) {
-    const std::array<int, 6> key{dev, 0, M, Ny, pl->p.precision, pl->p.model * 2 + pl->p.clamp_log};
-    auto cands = row_candidates(Nx, Ny, pl->csize);
+  // scratch arrays for every slot index the launches may touch
+  const size_t bytes = (size_t)e.cap * e.nc * sizeof(V);
+  if (e.frow.on) {
+    const std::array<int, 7> key{dev, 0, M, Ny, e.p.precision, e.p.model * 2 + e.p.clamp_log, bucket(ny)};
+    auto cands = row_candidates(Nx, Ny, e.csize);
     auto it = g_tune.find(key);
     if (it != g_tune.end()) {
-      apply(pl->frow, cands, it->second);
+      apply(e.frow, cands, it->second);
     } else if (cands.size() > 1) {
       DevBuf T2;
-      T2.alloc(nc * sizeof(V));
-      const Nonlin q = nonlin_of(pl);
+      T2.alloc(bytes);
+      const Nonlin q = nonlin_of(e);
       float best = 1e30f;
-      FastShape pick = pl->frow;
+      FastShape pick = e.frow;
       for (const FastShape& f : cands) {
         float ms = time_launches(st, reps, [&] {
-          FastLaunch<T>::row(M, f, st, (const V*)pl->S.p, (V*)T2.p, Ny, q, (const V*)pl->twN.p, pl->status());
+          FastLaunch<T>::row(M, f, ny, st, (const V*)e.S.p, (V*)T2.p, Ny, q, (const V*)e.twN.p, sl);
         });
         if (ms < best) best = ms, pick = f;
       }
-      pl->frow = pick;
+      e.frow = pick;
       g_tune[key] = {pick.var, pick.lines};
     }
   }
-  if (pl->fcol.on) {
-    const int mode = pl->p.scheme == PFC_SCHEME_ETD1 ? COL_ETD1 : COL_RK2_STAGE1;
-    const std::array<int, 6> key{dev, 1, Ny, Nc, pl->p.precision, mode};
-    auto cands = col_candidates(Nx, Ny, pl->csize);
+  if (e.fcol.on) {
+    const int mode = e.p.scheme == PFC_SCHEME_ETD1 ? COL_ETD1 : COL_RK2_STAGE1;
+    const std::array<int, 7> key{dev, 1, Ny, Nc, e.p.precision, mode, bucket(ny)};
+    auto cands = col_candidates(Nx, Ny, e.csize);
     auto it = g_tune.find(key);
     if (it != g_tune.end()) {
-      apply(pl->fcol, cands, it->second);
+      apply(e.fcol, cands, it->second);
     } else if (cands.size() > 1) {
       DevBuf S2, T2, D2;
-      S2.alloc(nc * sizeof(V));
-      T2.alloc(nc * sizeof(V));
-      D2.alloc(nc * sizeof(V));
-      hip_check(hipMemcpyAsync(S2.p, pl->S.p, nc * sizeof(V), hipMemcpyDeviceToDevice, st), "copy");
-      hip_check(hipMemcpyAsync(T2.p, pl->S.p, nc * sizeof(V), hipMemcpyDeviceToDevice, st), "copy");
-      ColTables tb{pl->E1.p, pl->Q1.p, pl->Q2.p, pl->p.pin_mean ? pl->pin() : nullptr};
+      S2.alloc(bytes);
+      T2.alloc(bytes);
+      D2.alloc(bytes);
+      hip_check(hipMemcpyAsync(S2.p, e.S.p, bytes, hipMemcpyDeviceToDevice, st), "copy");
+      hip_check(hipMemcpyAsync(T2.p, e.S.p, bytes, hipMemcpyDeviceToDevice, st), "copy");
+      ColTables tb{e.E1.p, e.Q1.p, e.Q2.p};
       float best = 1e30f;
-      FastShape pick = pl->fcol;
+      FastShape pick = e.fcol;
       for (const FastShape& f : cands) {
         float ms = time_launches(st, reps, [&] {
-          FastLaunch<T>::col(Ny, mode, f, st, (V*)S2.p, (const V*)T2.p, (V*)D2.p, Nc, tb, (const V*)pl->twY.p);
+          FastLaunch<T>::col(Ny, mode, f, ny, st, (V*)S2.p, (const V*)T2.p, (V*)D2.p, Nc, tb, (const V*)e.twY.p, sl);
         });
         if (ms < best) best = ms, pick = f;
       }
-      pl->fcol = pick;
+      e.fcol = pick;
       g_tune[key] = {pick.var, pick.lines};
       hip_check(hipStreamSynchronize(st), "sync after tuning");
     }
   }
 }
 
-// reductions of the current state; pin_out (device) receives the mean if non-null
+void engine_step(Engine& e, long long nsteps, hipStream_t st, Prof* prof) {
+  if (nsteps < 0) fail(PFC_EINVAL, "nsteps < 0");
+  for (int s : e.h_active)
+    if (!e.field_set[s]) fail(PFC_EINVAL, "a stepped slot has no field (pfc_set_field / pfc_batch_set_field)");
+  if (!e.tuned) {
+    if (e.f64())
+      autotune<double>(e, st);
+    else
+      autotune<float>(e, st);
+  }
+  if (e.f64())
+    Steps<double>::step(e, nsteps, st, prof);
+  else
+    Steps<float>::step(e, nsteps, st, prof);
+}
+
+// FP64 reductions of `ny` instances (the active list, or one slot when
+// Sbase is offset to it and sl.list is null); results land in e.out[4 i ..];
+// pin_out (device, per slot) receives the means.
 template <typename T>
-void launch_reduce(pfc_plan* pl, hipStream_t st, double* pin_out) {
+void launch_reduce(Engine& e, const void* Sbase, const double* eps, hipStream_t st, const Slots& sl, int ny,
+                   double* pin_out) {
   using V = V2<T>;
-  const V* S = reinterpret_cast<const V*>(pl->S.p);
-  FeParams fp{pl->p.model, pl->p.beta, pl->p.q0, pl->p.epsilon, pl->p.g,
-              reinterpret_cast<const double*>(pl->kx2.p), reinterpret_cast<const double*>(pl->ky2.p)};
-  Partial* part = reinterpret_cast<Partial*>(pl->part.p);
-  const int rb = row_blocks(pl->rg64), cb = col_blocks(pl->cg64);
+  const V* S = (const V*)Sbase;
+  FeParams fp{e.p.model, e.p.beta, e.p.q0, e.p.epsilon, e.p.g, (const double*)e.kx2.p, (const double*)e.ky2.p, eps};
+  Partial* part = (Partial*)e.part.p;
+  const int rb = row_blocks(e.rg64), cb = col_blocks(e.cg64), ps = e.pstride();
   const size_t red = 256 * sizeof(Partial);
-  const size_t rl = std::max(row_lds(pl->rg64, sizeof(V2<double>)), red);
-  const size_t cl = std::max(col_lds(pl->cg64, sizeof(V2<double>)), red);
-  hipLaunchKernelGGL((k_fe_row<T>), dim3(rb), dim3(256), rl, st, S, pl->rg64, fp,
-                     reinterpret_cast<const V2<double>*>(pl->twN64.p), part);
-  hipLaunchKernelGGL((k_fe_col<T>), dim3(cb), dim3(256), cl, st, S, pl->cg64, fp, pl->p.Nx,
-                     reinterpret_cast<const V2<double>*>(pl->twY64.p), part + rb);
-  hipLaunchKernelGGL(k_fe_final, dim3(1), dim3(256), 0, st, part, rb, cb, pl->p.Nx, pl->p.Ny,
-                     reinterpret_cast<double*>(pl->scal.p), pin_out);
+  const size_t rl = std::max(row_lds(e.rg64, sizeof(V2<double>)), red);
+  const size_t cl = std::max(col_lds(e.cg64, sizeof(V2<double>)), red);
+  hipLaunchKernelGGL((k_fe_row<T>), dim3(rb, ny), dim3(256), rl, st, S, e.rg64, fp, (const V2<double>*)e.twN64.p,
+                     part, sl, ps);
+  hipLaunchKernelGGL((k_fe_col<T>), dim3(cb, ny), dim3(256), cl, st, S, e.cg64, fp, e.p.Nx,
+                     (const V2<double>*)e.twY64.p, part + rb, sl, ps);
+  hipLaunchKernelGGL(k_fe_final, dim3(ny), dim3(256), 0, st, part, rb, cb, e.p.Nx, e.p.Ny, (double*)e.out.p, pin_out,
+                     sl, ps);
   hip_check(hipGetLastError(), "reduce launch");
 }
 
-void validate(const pfc_params* p) {
-  if (!p) fail(PFC_EINVAL, "params is NULL");
-  if (p->model != PFC_MODEL_STD && p->model != PFC_MODEL_LOG) fail(PFC_EINVAL, "model must be PFC_MODEL_STD or PFC_MODEL_LOG");
-  if (p->Nx < 4 || p->Ny < 2 || (p->Nx & 1)) fail(PFC_EINVAL, "Nx must be even and >= 4, Ny >= 2");
-  if (!(p->Lx > 0) || !(p->Ly > 0)) fail(PFC_EINVAL, "Lx and Ly must be positive");
-  if (p->precision != 64 && p->precision != 32) fail(PFC_EINVAL, "precision must be 64 or 32");
-  if (p->scheme != PFC_SCHEME_ETD1 && p->scheme != PFC_SCHEME_ETDRK2) fail(PFC_EINVAL, "scheme must be ETD1 or ETDRK2");
-  if (!std::isfinite(p->dt) || !(p->dt > 0)) fail(PFC_EINVAL, "dt must be positive");
-  if (p->model == PFC_MODEL_LOG && p->clamp_log && !(p->minPhi > 0)) fail(PFC_EINVAL, "minPhi must be positive");
+// reduce one slot; optionally capture its mean as the slot's k = 0 pin
+void reduce_slot(Engine& e, int slot, hipStream_t st, bool set_pin) {
+  const Slots one{nullptr, 0, 0, nullptr, nullptr, nullptr};
+  const void* S = (const char*)e.S.p + (size_t)slot * e.nc * e.csize;
+  double* pin_out = set_pin ? (double*)e.pin.p + slot : nullptr;
+  const double* eps = (const double*)e.eps.p + slot;
+  if (e.f64())
+    launch_reduce<double>(e, S, eps, st, one, 1, pin_out);
+  else
+    launch_reduce<float>(e, S, eps, st, one, 1, pin_out);
+}
+
+void engine_set_field(Engine& e, int slot, const void* field, int dtype, hipStream_t st) {
+  e.check_slot(slot);
+  if (!field) fail(PFC_EINVAL, "NULL field");
+  if (dtype != 64 && dtype != 32) fail(PFC_EINVAL, "dtype must be 64 or 32");
+  const int rb = row_blocks(e.rg);
+  const size_t rl = row_lds(e.rg, e.csize);
+  const size_t off = (size_t)slot * e.nc;
+  if (e.f64()) {
+    const double inv = 1.0 / e.p.Nx;
+    V2<double>* S = (V2<double>*)e.S.p + off;
+    if (dtype == 64)
+      hipLaunchKernelGGL((k_row_r2c<double, double>), dim3(rb), dim3(256), rl, st, (const double*)field, S, e.rg,
+                         (const V2<double>*)e.twN.p, inv, (size_t)0, (size_t)0);
+    else
+      hipLaunchKernelGGL((k_row_r2c<double, float>), dim3(rb), dim3(256), rl, st, (const float*)field, S, e.rg,
+                         (const V2<double>*)e.twN.p, inv, (size_t)0, (size_t)0);
+  } else {
+    const float inv = 1.0f / e.p.Nx;
+    V2<float>* S = (V2<float>*)e.S.p + off;
+    if (dtype == 64)
+      hipLaunchKernelGGL((k_row_r2c<float, double>), dim3(rb), dim3(256), rl, st, (const double*)field, S, e.rg,
+                         (const V2<float>*)e.twN.p, inv, (size_t)0, (size_t)0);
+    else
+      hipLaunchKernelGGL((k_row_r2c<float, float>), dim3(rb), dim3(256), rl, st, (const float*)field, S, e.rg,
+                         (const V2<float>*)e.twN.p, inv, (size_t)0, (size_t)0);
+  }
+  hip_check(hipGetLastError(), "set_field launch");
+  reduce_slot(e, slot, st, true);
+  e.field_set[slot] = 1;
+}
+
+void engine_get_field(Engine& e, int slot, void* field, int dtype, hipStream_t st) {
+  e.check_slot(slot);
+  if (!field) fail(PFC_EINVAL, "NULL field");
+  if (dtype != 64 && dtype != 32) fail(PFC_EINVAL, "dtype must be 64 or 32");
+  const int rb = row_blocks(e.rg);
+  const size_t rl = row_lds(e.rg, e.csize);
+  const size_t off = (size_t)slot * e.nc;
+  if (e.f64()) {
+    const V2<double>* S = (const V2<double>*)e.S.p + off;
+    if (dtype == 64)
+      hipLaunchKernelGGL((k_row_c2r<double, double>), dim3(rb), dim3(256), rl, st, S, (double*)field, e.rg,
+                         (const V2<double>*)e.twN.p, 1.0, (size_t)0, (size_t)0);
+    else
+      hipLaunchKernelGGL((k_row_c2r<double, float>), dim3(rb), dim3(256), rl, st, S, (float*)field, e.rg,
+                         (const V2<double>*)e.twN.p, 1.0, (size_t)0, (size_t)0);
+  } else {
+    const V2<float>* S = (const V2<float>*)e.S.p + off;
+    if (dtype == 64)
+      hipLaunchKernelGGL((k_row_c2r<float, double>), dim3(rb), dim3(256), rl, st, S, (double*)field, e.rg,
+                         (const V2<float>*)e.twN.p, 1.0f, (size_t)0, (size_t)0);
+    else
+      hipLaunchKernelGGL((k_row_c2r<float, float>), dim3(rb), dim3(256), rl, st, S, (float*)field, e.rg,
+                         (const V2<float>*)e.twN.p, 1.0f, (size_t)0, (size_t)0);
+  }
+  hip_check(hipGetLastError(), "get_field launch");
+}
+
+// (epsilon, M) of one slot; rebuilds its tables on `st`
+void engine_set_slot(Engine& e, int slot, double epsilon, double M, hipStream_t st) {
+  e.check_slot(slot);
+  if (!std::isfinite(epsilon) || !std::isfinite(M)) fail(PFC_EINVAL, "epsilon and M must be finite");
+  e.h_eps[slot] = epsilon;
+  e.h_M[slot] = M;
+  hip_check(hipMemcpyAsync((double*)e.eps.p + slot, &e.h_eps[slot], sizeof(double), hipMemcpyHostToDevice, st), "copy");
+  hip_check(hipMemcpyAsync((double*)e.Ms.p + slot, &e.h_M[slot], sizeof(double), hipMemcpyHostToDevice, st), "copy");
+  engine_tables(e, slot, 1, st);
+  // the host values must outlive the asynchronous copies
+  hip_check(hipStreamSynchronize(st), "sync");
+}
+
+void engine_set_active(Engine& e, const int* slots, int n, hipStream_t st) {
+  if (n < 0 || n > e.cap) fail(PFC_EINVAL, "active count out of range");
+  if (n > 0 && !slots) fail(PFC_EINVAL, "NULL slot list");
+  std::vector<int> seen(e.cap, 0), list(slots, slots + n);
+  for (int s : list) {
+    e.check_slot(s);
+    if (seen[s]++) fail(PFC_EINVAL, "duplicate slot in the active list");
+  }
+  e.h_active = list;
+  if (n) {
+    hip_check(hipMemcpyAsync(e.active.p, e.h_active.data(), n * sizeof(int), hipMemcpyHostToDevice, st), "copy");
+    hip_check(hipStreamSynchronize(st), "sync");
+  }
+}
+
+// reductions of the active list into out[4 i ..] (host); synchronises
+void engine_reduce_active(Engine& e, double* out, hipStream_t st) {
+  const int ny = e.nact();
+  if (ny == 0) return;
+  const Slots sl = e.slots_active();
+  if (e.f64())
+    launch_reduce<double>(e, e.S.p, (const double*)e.eps.p, st, sl, ny, nullptr);
+  else
+    launch_reduce<float>(e, e.S.p, (const double*)e.eps.p, st, sl, ny, nullptr);
+  hip_check(hipMemcpyAsync(e.host_out, e.out.p, (size_t)ny * 4 * sizeof(double), hipMemcpyDeviceToHost, st), "copy");
+  hip_check(hipStreamSynchronize(st), "sync");
+  std::memcpy(out, e.host_out, (size_t)ny * 4 * sizeof(double));
+}
+
+void engine_reduce_one(Engine& e, int slot, double out[4], hipStream_t st) {
+  e.check_slot(slot);
+  reduce_slot(e, slot, st, false);
+  hip_check(hipMemcpyAsync(e.host_out, e.out.p, 4 * sizeof(double), hipMemcpyDeviceToHost, st), "copy");
+  hip_check(hipStreamSynchronize(st), "sync");
+  std::memcpy(out, e.host_out, 4 * sizeof(double));
+}
+
+void engine_status(Engine& e, int* flags, hipStream_t st) {
+  hip_check(hipMemcpyAsync(e.host_status, e.status.p, e.cap * sizeof(int), hipMemcpyDeviceToHost, st), "copy");
+  hip_check(hipStreamSynchronize(st), "sync");
+  std::memcpy(flags, e.host_status, e.cap * sizeof(int));
+  hip_check(hipMemsetAsync(e.status.p, 0, e.cap * sizeof(int), st), "clear status");
+}
+
+template <typename T>
+void launch_filter(Engine& e, int slot, int mode, void* out0, void* out1, int dtype, hipStream_t st) {
+  using V = V2<T>;
+  if (e.diag.n < 2 * e.nc * sizeof(V)) {
+    hip_check(hipStreamSynchronize(st), "sync before scratch growth");
+    e.diag.alloc(2 * e.nc * sizeof(V));
+  }
+  V* d0 = (V*)e.diag.p;
+  V* d1 = d0 + e.nc;
+  const V* S = (const V*)e.S.p + (size_t)slot * e.nc;
+  V* Tb = (V*)e.Tb.p + (size_t)slot * e.nc;
+  const V* twN = (const V*)e.twN.p;
+  const V* twY = (const V*)e.twY.p;
+  FilterParams fp{e.p.beta, e.p.q0, e.h_eps[slot], e.h_M[slot], 6.283185307179586, e.p.Nx,
+                  (const double*)e.kx2.p, (const double*)e.ky2.p};
+  const int rb = row_blocks(e.rg), cb = col_blocks(e.cg);
+  const size_t rl = row_lds(e.rg, sizeof(V)), cl = col_lds(e.cg, sizeof(V));
+  if (mode == FILTER_MU) {
+    Slots one = e.slots_one(slot);
+    hipLaunchKernelGGL((k_row_nonlin<T>), dim3(rb), dim3(256), rl, st, S, Tb, e.rg, nonlin_of(e), twN, one);
+    hipLaunchKernelGGL((k_col_filter<T, FILTER_MU>), dim3(cb), dim3(256), cl, st, S, Tb, d0, d1, e.cg, fp, twY);
+  } else {
+    hipLaunchKernelGGL((k_col_filter<T, FILTER_LOCAL>), dim3(cb), dim3(256), cl, st, S, Tb, d0, d1, e.cg, fp, twY);
+  }
+  const T scale = T(1.0 / ((double)e.p.Nx * (double)e.p.Ny));
+  const int nout = mode == FILTER_MU ? 2 : 1;
+  for (int o = 0; o < nout; ++o) {
+    const V* src = o == 0 ? d0 : d1;
+    void* dst = o == 0 ? out0 : out1;
+    if (dtype == 64)
+      hipLaunchKernelGGL((k_row_c2r<T, double>), dim3(rb), dim3(256), rl, st, src, (double*)dst, e.rg, twN, scale,
+                         (size_t)0, (size_t)0);
+    else
+      hipLaunchKernelGGL((k_row_c2r<T, float>), dim3(rb), dim3(256), rl, st, src, (float*)dst, e.rg, twN, scale,
+                         (size_t)0, (size_t)0);
+  }
+  hip_check(hipGetLastError(), "diagnostic launch");
+}
+
+size_t engine_bytes(const Engine& e) {
+  size_t b = 0;
+  for (const DevBuf* d : {&e.S, &e.Tb, &e.D, &e.E1, &e.Q1, &e.Q2, &e.twN, &e.twY, &e.twN64, &e.twY64, &e.kx2, &e.ky2,
+                          &e.part, &e.out, &e.pin, &e.status, &e.eps, &e.Ms, &e.active, &e.diag})
+    b += d->n;
+  return b;
 }
 
 // cache of transform-only twiddles/geometry for pfc_rfft2 / pfc_irfft2
@@ -660,47 +912,13 @@ FftEntry* fft_entry(int Ny, int Nx, int dtype) {
 
 }  // namespace
 
-namespace {
+struct pfc_plan {
+  Engine e;
+};
 
-template <typename T>
-void launch_filter(pfc_plan* pl, int mode, void* out0, void* out1, int dtype, hipStream_t st) {
-  using V = V2<T>;
-  const size_t nc = (size_t)pl->Nc * pl->p.Ny;
-  if (pl->diag.n < 2 * nc * sizeof(V)) pl->diag.alloc(2 * nc * sizeof(V));
-  V* d0 = reinterpret_cast<V*>(pl->diag.p);
-  V* d1 = d0 + nc;
-  const V* S = reinterpret_cast<const V*>(pl->S.p);
-  V* Tb = reinterpret_cast<V*>(pl->Tb.p);
-  const V* twN = reinterpret_cast<const V*>(pl->twN.p);
-  const V* twY = reinterpret_cast<const V*>(pl->twY.p);
-  FilterParams fp{pl->p.beta, pl->p.q0, pl->p.epsilon, pl->M, 6.283185307179586, pl->p.Nx,
-                  reinterpret_cast<const double*>(pl->kx2.p), reinterpret_cast<const double*>(pl->ky2.p)};
-  const int rb = row_blocks(pl->rg), cb = col_blocks(pl->cg);
-  const size_t rl = row_lds(pl->rg, sizeof(V)), cl = col_lds(pl->cg, sizeof(V));
-  if (mode == FILTER_MU) {
-    hipLaunchKernelGGL((k_row_nonlin<T>), dim3(rb), dim3(256), rl, st, S, Tb, pl->rg, nonlin_of(pl), twN,
-                       pl->status());
-    hipLaunchKernelGGL((k_col_filter<T, FILTER_MU>), dim3(cb), dim3(256), cl, st, S, Tb, d0, d1, pl->cg, fp, twY);
-  } else {
-    hipLaunchKernelGGL((k_col_filter<T, FILTER_LOCAL>), dim3(cb), dim3(256), cl, st, S, Tb, d0, d1, pl->cg, fp,
-                       twY);
-  }
-  const T scale = T(1.0 / ((double)pl->p.Nx * (double)pl->p.Ny));
-  const int nout = mode == FILTER_MU ? 2 : 1;
-  for (int o = 0; o < nout; ++o) {
-    const V* src = o == 0 ? d0 : d1;
-    void* dst = o == 0 ? out0 : out1;
-    if (dtype == 64)
-      hipLaunchKernelGGL((k_row_c2r<T, double>), dim3(rb), dim3(256), rl, st, src, (double*)dst, pl->rg, twN, scale,
-                         (size_t)0, (size_t)0);
-    else
-      hipLaunchKernelGGL((k_row_c2r<T, float>), dim3(rb), dim3(256), rl, st, src, (float*)dst, pl->rg, twN, scale,
-                         (size_t)0, (size_t)0);
-  }
-  hip_check(hipGetLastError(), "diagnostic launch");
-}
-
-}  // namespace
+struct pfc_batch {
+  Engine e;
+};
 
 // ---------------------------------------------------------------------------
 // C ABI
@@ -709,71 +927,16 @@ extern "C" {
 
 const char* pfc_last_error(void) { return g_err.c_str(); }
 
-int pfc_version(void) { return 1; }
+int pfc_version(void) { return 2; }
 
 int pfc_create(const pfc_params* params, pfc_plan** out) {
   return guarded([&] {
     if (!out) fail(PFC_EINVAL, "out is NULL");
     *out = nullptr;
-    validate(params);
     auto pl = std::make_unique<pfc_plan>();
-    pl->p = *params;
-    pl->M = params->M;
-    const int Nx = params->Nx, Ny = params->Ny;
-    pl->Nc = Nx / 2 + 1;
-    pl->csize = pl->f64() ? sizeof(V2<double>) : sizeof(V2<float>);
-    pl->rg = row_geom(Nx, Ny, pl->csize);
-    pl->cg = col_geom(Nx, Ny, pl->csize);
-    pl->rg64 = row_geom(Nx, Ny, sizeof(V2<double>));
-    pl->cg64 = col_geom(Nx, Ny, sizeof(V2<double>));
-    if (!std::getenv("PFC_HIP_GENERIC")) {
-      auto rc = row_candidates(Nx, Ny, pl->csize);
-      auto cc = col_candidates(Nx, Ny, pl->csize);
-      pl->frow = pick_default(rc, 512);
-      pl->fcol = pick_default(cc, 256);
-      // PFC_HIP_SCHEDULE="rowvar:rowlines:colvar:collines" pins a schedule
-      // (tests and tuning studies); the autotuner is then skipped.
-      if (const char* fx = std::getenv("PFC_HIP_SCHEDULE")) {
-        int v[4] = {-1, -1, -1, -1};
-        if (std::sscanf(fx, "%d:%d:%d:%d", &v[0], &v[1], &v[2], &v[3]) == 4) {
-          bool okr = false, okc = false;
-          for (const FastShape& f : rc)
-            if (f.var == v[0] && f.lines == v[1]) pl->frow = f, okr = true;
-          for (const FastShape& f : cc)
-            if (f.var == v[2] && f.lines == v[3]) pl->fcol = f, okc = true;
-          if (!okr && pl->frow.on) fail(PFC_EINVAL, "PFC_HIP_SCHEDULE: no such row schedule");
-          if (!okc && pl->fcol.on) fail(PFC_EINVAL, "PFC_HIP_SCHEDULE: no such column schedule");
-          pl->tuned = true;
-        }
-      }
-    }
-    const size_t nc = (size_t)pl->Nc * Ny;
-    pl->S.alloc(nc * pl->csize);
-    pl->Tb.alloc(nc * pl->csize);
-    if (params->scheme == PFC_SCHEME_ETDRK2) pl->D.alloc(nc * pl->csize);
-    pl->kx = rfft_k(Nx, params->Lx / Nx);
-    pl->ky = fft_k(Ny, params->Ly / Ny);
-    upload_tables(pl.get());
-    auto tn = twiddles(Nx), ty = twiddles(Ny);
-    pl->twN64.upload(tn);
-    pl->twY64.upload(ty);
-    if (pl->f64()) {
-      pl->twN.upload(tn);
-      pl->twY.upload(ty);
-    } else {
-      pl->twN.upload(to_f32(tn));
-      pl->twY.upload(to_f32(ty));
-    }
-    std::vector<double> kx2(pl->kx.size()), ky2(pl->ky.size());
-    for (size_t i = 0; i < kx2.size(); ++i) kx2[i] = pl->kx[i] * pl->kx[i];
-    for (size_t i = 0; i < ky2.size(); ++i) ky2[i] = pl->ky[i] * pl->ky[i];
-    pl->kx2.upload(kx2);
-    pl->ky2.upload(ky2);
-    pl->part.alloc((size_t)(row_blocks(pl->rg64) + col_blocks(pl->cg64)) * sizeof(Partial));
-    pl->scal.alloc(8 * sizeof(double));
-    hip_check(hipMemset(pl->scal.p, 0, 8 * sizeof(double)), "memset");
-    hip_check(hipHostMalloc((void**)&pl->host_out, 8 * sizeof(double), hipHostMallocDefault), "hipHostMalloc");
-    hip_check(hipMemset(pl->S.p, 0, nc * pl->csize), "memset");
+    engine_init(pl->e, params, 1, false);
+    engine_tables(pl->e, 0, 1, nullptr);
+    hip_check(hipStreamSynchronize(nullptr), "sync");
     *out = pl.release();
   });
 }
@@ -784,114 +947,44 @@ int pfc_destroy(pfc_plan* plan) {
 
 int pfc_set_field(pfc_plan* pl, const void* field, int dtype, void* stream) {
   return guarded([&] {
-    if (!pl || !field) fail(PFC_EINVAL, "NULL plan or field");
-    if (dtype != 64 && dtype != 32) fail(PFC_EINVAL, "dtype must be 64 or 32");
-    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
-    const int rb = row_blocks(pl->rg);
-    const size_t rl = row_lds(pl->rg, pl->csize);
-    if (pl->f64()) {
-      const double inv = 1.0 / pl->p.Nx;
-      if (dtype == 64)
-        hipLaunchKernelGGL((k_row_r2c<double, double>), dim3(rb), dim3(256), rl, st, (const double*)field,
-                           (V2<double>*)pl->S.p, pl->rg, (const V2<double>*)pl->twN.p, inv, (size_t)0, (size_t)0);
-      else
-        hipLaunchKernelGGL((k_row_r2c<double, float>), dim3(rb), dim3(256), rl, st, (const float*)field,
-                           (V2<double>*)pl->S.p, pl->rg, (const V2<double>*)pl->twN.p, inv, (size_t)0, (size_t)0);
-      hip_check(hipGetLastError(), "set_field launch");
-      launch_reduce<double>(pl, st, pl->pin());
-    } else {
-      const float inv = 1.0f / pl->p.Nx;
-      if (dtype == 64)
-        hipLaunchKernelGGL((k_row_r2c<float, double>), dim3(rb), dim3(256), rl, st, (const double*)field,
-                           (V2<float>*)pl->S.p, pl->rg, (const V2<float>*)pl->twN.p, inv, (size_t)0, (size_t)0);
-      else
-        hipLaunchKernelGGL((k_row_r2c<float, float>), dim3(rb), dim3(256), rl, st, (const float*)field,
-                           (V2<float>*)pl->S.p, pl->rg, (const V2<float>*)pl->twN.p, inv, (size_t)0, (size_t)0);
-      hip_check(hipGetLastError(), "set_field launch");
-      launch_reduce<float>(pl, st, pl->pin());
-    }
-    pl->field_set = true;
+    if (!pl) fail(PFC_EINVAL, "NULL plan");
+    engine_set_field(pl->e, 0, field, dtype, reinterpret_cast<hipStream_t>(stream));
   });
 }
 
 int pfc_get_field(pfc_plan* pl, void* field, int dtype, void* stream) {
   return guarded([&] {
-    if (!pl || !field) fail(PFC_EINVAL, "NULL plan or field");
-    if (dtype != 64 && dtype != 32) fail(PFC_EINVAL, "dtype must be 64 or 32");
-    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
-    const int rb = row_blocks(pl->rg);
-    const size_t rl = row_lds(pl->rg, pl->csize);
-    if (pl->f64()) {
-      if (dtype == 64)
-        hipLaunchKernelGGL((k_row_c2r<double, double>), dim3(rb), dim3(256), rl, st, (const V2<double>*)pl->S.p,
-                           (double*)field, pl->rg, (const V2<double>*)pl->twN.p, 1.0, (size_t)0, (size_t)0);
-      else
-        hipLaunchKernelGGL((k_row_c2r<double, float>), dim3(rb), dim3(256), rl, st, (const V2<double>*)pl->S.p,
-                           (float*)field, pl->rg, (const V2<double>*)pl->twN.p, 1.0, (size_t)0, (size_t)0);
-    } else {
-      if (dtype == 64)
-        hipLaunchKernelGGL((k_row_c2r<float, double>), dim3(rb), dim3(256), rl, st, (const V2<float>*)pl->S.p,
-                           (double*)field, pl->rg, (const V2<float>*)pl->twN.p, 1.0f, (size_t)0, (size_t)0);
-      else
-        hipLaunchKernelGGL((k_row_c2r<float, float>), dim3(rb), dim3(256), rl, st, (const V2<float>*)pl->S.p,
-                           (float*)field, pl->rg, (const V2<float>*)pl->twN.p, 1.0f, (size_t)0, (size_t)0);
-    }
-    hip_check(hipGetLastError(), "get_field launch");
+    if (!pl) fail(PFC_EINVAL, "NULL plan");
+    engine_get_field(pl->e, 0, field, dtype, reinterpret_cast<hipStream_t>(stream));
   });
 }
 
 int pfc_step(pfc_plan* pl, long long nsteps, void* stream) {
   return guarded([&] {
     if (!pl) fail(PFC_EINVAL, "NULL plan");
-    if (nsteps < 0) fail(PFC_EINVAL, "nsteps < 0");
-    if (!pl->field_set) fail(PFC_EINVAL, "pfc_set_field has not been called");
-    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
-    if (!pl->tuned) {
-      if (pl->f64())
-        autotune<double>(pl, st);
-      else
-        autotune<float>(pl, st);
-    }
-    if (pl->f64())
-      launch_step<double>(pl, nsteps, st);
-    else
-      launch_step<float>(pl, nsteps, st);
+    engine_step(pl->e, nsteps, reinterpret_cast<hipStream_t>(stream), nullptr);
   });
 }
 
 int pfc_reduce(pfc_plan* pl, double out[4], void* stream) {
   return guarded([&] {
     if (!pl || !out) fail(PFC_EINVAL, "NULL plan or out");
-    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
-    if (pl->f64())
-      launch_reduce<double>(pl, st, nullptr);
-    else
-      launch_reduce<float>(pl, st, nullptr);
-    hip_check(hipMemcpyAsync(pl->host_out, pl->scal.p, 4 * sizeof(double), hipMemcpyDeviceToHost, st), "copy");
-    hip_check(hipStreamSynchronize(st), "sync");
-    std::memcpy(out, pl->host_out, 4 * sizeof(double));
+    engine_reduce_one(pl->e, 0, out, reinterpret_cast<hipStream_t>(stream));
   });
 }
 
 int pfc_set_shift(pfc_plan* pl, double M) {
   return guarded([&] {
     if (!pl) fail(PFC_EINVAL, "NULL plan");
-    if (!std::isfinite(M)) fail(PFC_EINVAL, "M must be finite");
     hip_check(hipDeviceSynchronize(), "sync before table rebuild");
-    pl->M = M;
-    upload_tables(pl);
+    engine_set_slot(pl->e, 0, pl->e.h_eps[0], M, nullptr);
   });
 }
 
 int pfc_get_status(pfc_plan* pl, int* flags, void* stream) {
   return guarded([&] {
     if (!pl || !flags) fail(PFC_EINVAL, "NULL plan or flags");
-    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
-    int v = 0;
-    hip_check(hipMemcpyAsync(&v, pl->status(), sizeof(int), hipMemcpyDeviceToHost, st), "copy");
-    hip_check(hipStreamSynchronize(st), "sync");
-    hip_check(hipMemsetAsync(pl->status(), 0, sizeof(int), st), "clear status");
-    *flags = v;
+    engine_status(pl->e, flags, reinterpret_cast<hipStream_t>(stream));
   });
 }
 
@@ -899,12 +992,12 @@ int pfc_chemical_potential(pfc_plan* pl, void* mu, void* local_mu, int dtype, vo
   return guarded([&] {
     if (!pl || !mu || !local_mu) fail(PFC_EINVAL, "NULL argument");
     if (dtype != 64 && dtype != 32) fail(PFC_EINVAL, "dtype must be 64 or 32");
-    if (!pl->field_set) fail(PFC_EINVAL, "pfc_set_field has not been called");
+    if (!pl->e.field_set[0]) fail(PFC_EINVAL, "pfc_set_field has not been called");
     hipStream_t st = reinterpret_cast<hipStream_t>(stream);
-    if (pl->f64())
-      launch_filter<double>(pl, FILTER_MU, mu, local_mu, dtype, st);
+    if (pl->e.f64())
+      launch_filter<double>(pl->e, 0, FILTER_MU, mu, local_mu, dtype, st);
     else
-      launch_filter<float>(pl, FILTER_MU, mu, local_mu, dtype, st);
+      launch_filter<float>(pl->e, 0, FILTER_MU, mu, local_mu, dtype, st);
   });
 }
 
@@ -912,12 +1005,12 @@ int pfc_local_field(pfc_plan* pl, void* out, int dtype, void* stream) {
   return guarded([&] {
     if (!pl || !out) fail(PFC_EINVAL, "NULL argument");
     if (dtype != 64 && dtype != 32) fail(PFC_EINVAL, "dtype must be 64 or 32");
-    if (!pl->field_set) fail(PFC_EINVAL, "pfc_set_field has not been called");
+    if (!pl->e.field_set[0]) fail(PFC_EINVAL, "pfc_set_field has not been called");
     hipStream_t st = reinterpret_cast<hipStream_t>(stream);
-    if (pl->f64())
-      launch_filter<double>(pl, FILTER_LOCAL, out, nullptr, dtype, st);
+    if (pl->e.f64())
+      launch_filter<double>(pl->e, 0, FILTER_LOCAL, out, nullptr, dtype, st);
     else
-      launch_filter<float>(pl, FILTER_LOCAL, out, nullptr, dtype, st);
+      launch_filter<float>(pl->e, 0, FILTER_LOCAL, out, nullptr, dtype, st);
   });
 }
 
@@ -925,35 +1018,24 @@ int pfc_step_profile(pfc_plan* pl, long long nsteps, void* stream, double ms[3],
   Prof prof;
   int rc = guarded([&] {
     if (!pl || !ms || !calls) fail(PFC_EINVAL, "NULL argument");
-    if (nsteps < 0) fail(PFC_EINVAL, "nsteps < 0");
-    if (!pl->field_set) fail(PFC_EINVAL, "pfc_set_field has not been called");
     prof.st = reinterpret_cast<hipStream_t>(stream);
-    if (!pl->tuned) {
-      if (pl->f64())
-        autotune<double>(pl, prof.st);
-      else
-        autotune<float>(pl, prof.st);
-    }
     hip_check(hipEventCreate(&prof.ev[0]), "event create");
     hip_check(hipEventCreate(&prof.ev[1]), "event create");
-    if (pl->f64())
-      launch_step<double>(pl, nsteps, prof.st, &prof);
-    else
-      launch_step<float>(pl, nsteps, prof.st, &prof);
+    engine_step(pl->e, nsteps, prof.st, &prof);
     for (int k = 0; k < 3; ++k) {
       ms[k] = prof.ms[k];
       calls[k] = prof.calls[k];
     }
   });
-  for (hipEvent_t e : prof.ev)
-    if (e) (void)hipEventDestroy(e);
+  for (hipEvent_t ev : prof.ev)
+    if (ev) (void)hipEventDestroy(ev);
   return rc;
 }
 
 int pfc_plan_schedule(const pfc_plan* pl, int info[8]) {
   return guarded([&] {
     if (!pl || !info) fail(PFC_EINVAL, "NULL argument");
-    const FastShape* fs[2] = {&pl->frow, &pl->fcol};
+    const FastShape* fs[2] = {&pl->e.frow, &pl->e.fcol};
     for (int k = 0; k < 2; ++k) {
       info[4 * k + 0] = fs[k]->on ? 1 : 0;
       info[4 * k + 1] = fs[k]->var;
@@ -966,13 +1048,96 @@ int pfc_plan_schedule(const pfc_plan* pl, int info[8]) {
 int pfc_plan_bytes(const pfc_plan* pl, long long* bytes) {
   return guarded([&] {
     if (!pl || !bytes) fail(PFC_EINVAL, "NULL plan or bytes");
-    size_t b = 0;
-    for (const DevBuf* d : {&pl->S, &pl->Tb, &pl->D, &pl->E1, &pl->Q1, &pl->Q2, &pl->twN, &pl->twY, &pl->twN64,
-                            &pl->twY64, &pl->kx2, &pl->ky2, &pl->part, &pl->scal, &pl->diag})
-      b += d->n;
-    *bytes = (long long)b;
+    *bytes = (long long)engine_bytes(pl->e);
   });
 }
+
+// ---- batches ---------------------------------------------------------------
+
+int pfc_batch_create(const pfc_params* shared, int capacity, pfc_batch** out) {
+  return guarded([&] {
+    if (!out) fail(PFC_EINVAL, "out is NULL");
+    *out = nullptr;
+    auto b = std::make_unique<pfc_batch>();
+    engine_init(b->e, shared, capacity, true);
+    engine_tables(b->e, 0, capacity, nullptr);
+    hip_check(hipStreamSynchronize(nullptr), "sync");
+    *out = b.release();
+  });
+}
+
+int pfc_batch_destroy(pfc_batch* b) {
+  return guarded([&] { delete b; });
+}
+
+int pfc_batch_set_slot(pfc_batch* b, int slot, double epsilon, double M, void* stream) {
+  return guarded([&] {
+    if (!b) fail(PFC_EINVAL, "NULL batch");
+    engine_set_slot(b->e, slot, epsilon, M, reinterpret_cast<hipStream_t>(stream));
+  });
+}
+
+int pfc_batch_set_field(pfc_batch* b, int slot, const void* field, int dtype, void* stream) {
+  return guarded([&] {
+    if (!b) fail(PFC_EINVAL, "NULL batch");
+    engine_set_field(b->e, slot, field, dtype, reinterpret_cast<hipStream_t>(stream));
+  });
+}
+
+int pfc_batch_get_field(pfc_batch* b, int slot, void* field, int dtype, void* stream) {
+  return guarded([&] {
+    if (!b) fail(PFC_EINVAL, "NULL batch");
+    engine_get_field(b->e, slot, field, dtype, reinterpret_cast<hipStream_t>(stream));
+  });
+}
+
+int pfc_batch_set_active(pfc_batch* b, const int* slots, int n, void* stream) {
+  return guarded([&] {
+    if (!b) fail(PFC_EINVAL, "NULL batch");
+    engine_set_active(b->e, slots, n, reinterpret_cast<hipStream_t>(stream));
+  });
+}
+
+int pfc_batch_step(pfc_batch* b, long long nsteps, void* stream) {
+  return guarded([&] {
+    if (!b) fail(PFC_EINVAL, "NULL batch");
+    engine_step(b->e, nsteps, reinterpret_cast<hipStream_t>(stream), nullptr);
+  });
+}
+
+int pfc_batch_reduce(pfc_batch* b, double* out, void* stream) {
+  return guarded([&] {
+    if (!b || !out) fail(PFC_EINVAL, "NULL batch or out");
+    engine_reduce_active(b->e, out, reinterpret_cast<hipStream_t>(stream));
+  });
+}
+
+int pfc_batch_get_status(pfc_batch* b, int* flags, void* stream) {
+  return guarded([&] {
+    if (!b || !flags) fail(PFC_EINVAL, "NULL batch or flags");
+    engine_status(b->e, flags, reinterpret_cast<hipStream_t>(stream));
+  });
+}
+
+int pfc_batch_bytes(const pfc_batch* b, long long* bytes) {
+  return guarded([&] {
+    if (!b || !bytes) fail(PFC_EINVAL, "NULL batch or bytes");
+    *bytes = (long long)engine_bytes(b->e);
+  });
+}
+
+int pfc_slot_bytes(const pfc_params* params, long long* bytes) {
+  return guarded([&] {
+    validate(params);
+    if (!bytes) fail(PFC_EINVAL, "NULL bytes");
+    const size_t nc = (size_t)(params->Nx / 2 + 1) * params->Ny;
+    const size_t cs = params->precision == 64 ? 16 : 8, ts = cs / 2;
+    const bool rk2 = params->scheme == PFC_SCHEME_ETDRK2;
+    *bytes = (long long)(nc * cs * (rk2 ? 3 : 2) + nc * ts * (rk2 ? 3 : 2));
+  });
+}
+
+// ---- plain transforms --------------------------------------------------------
 
 int pfc_rfft2(const void* in, void* out, int batch, int Ny, int Nx, int dtype, void* stream) {
   return guarded([&] {
@@ -987,8 +1152,8 @@ int pfc_rfft2(const void* in, void* out, int batch, int Ny, int Nx, int dtype, v
     if (dtype == 64) {
       hipLaunchKernelGGL((k_row_r2c<double, double>), rgrid, dim3(256), row_lds(e->rg, 16), st, (const double*)in,
                          (V2<double>*)out, e->rg, (const V2<double>*)e->twN.p, 1.0, nreal, ncplx);
-      hipLaunchKernelGGL((k_col_fft<double, false>), cgrid, dim3(256), col_lds(e->cg, 16), st, (V2<double>*)out, e->cg,
-                         (const V2<double>*)e->twY.p, 1.0, ncplx);
+      hipLaunchKernelGGL((k_col_fft<double, false>), cgrid, dim3(256), col_lds(e->cg, 16), st, (V2<double>*)out,
+                         e->cg, (const V2<double>*)e->twY.p, 1.0, ncplx);
     } else {
       hipLaunchKernelGGL((k_row_r2c<float, float>), rgrid, dim3(256), row_lds(e->rg, 8), st, (const float*)in,
                          (V2<float>*)out, e->rg, (const V2<float>*)e->twN.p, 1.0f, nreal, ncplx);
@@ -1018,17 +1183,17 @@ int pfc_irfft2(const void* in, void* out, int batch, int Ny, int Nx, int dtype, 
     dim3 rgrid(row_blocks(e->rg), batch), cgrid(col_blocks(e->cg), batch);
     const double scale = 1.0 / ((double)Nx * (double)Ny);
     if (dtype == 64) {
-      hipLaunchKernelGGL((k_col_fft<double, true>), cgrid, dim3(256), col_lds(e->cg, 16), st, (V2<double>*)e->scratch.p,
-                         e->cg, (const V2<double>*)e->twY.p, 1.0, ncplx);
+      hipLaunchKernelGGL((k_col_fft<double, true>), cgrid, dim3(256), col_lds(e->cg, 16), st,
+                         (V2<double>*)e->scratch.p, e->cg, (const V2<double>*)e->twY.p, 1.0, ncplx);
       hipLaunchKernelGGL((k_row_c2r<double, double>), rgrid, dim3(256), row_lds(e->rg, 16), st,
-                         (const V2<double>*)e->scratch.p, (double*)out, e->rg, (const V2<double>*)e->twN.p, scale, ncplx,
-                         nreal);
+                         (const V2<double>*)e->scratch.p, (double*)out, e->rg, (const V2<double>*)e->twN.p, scale,
+                         ncplx, nreal);
     } else {
       hipLaunchKernelGGL((k_col_fft<float, true>), cgrid, dim3(256), col_lds(e->cg, 8), st, (V2<float>*)e->scratch.p,
                          e->cg, (const V2<float>*)e->twY.p, 1.0f, ncplx);
       hipLaunchKernelGGL((k_row_c2r<float, float>), rgrid, dim3(256), row_lds(e->rg, 8), st,
-                         (const V2<float>*)e->scratch.p, (float*)out, e->rg, (const V2<float>*)e->twN.p, (float)scale,
-                         ncplx, nreal);
+                         (const V2<float>*)e->scratch.p, (float*)out, e->rg, (const V2<float>*)e->twN.p,
+                         (float)scale, ncplx, nreal);
     }
     hip_check(hipGetLastError(), "irfft2 launch");
   });

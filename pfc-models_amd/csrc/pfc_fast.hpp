@@ -176,11 +176,14 @@ __device__ __forceinline__ void reg_fft(V2<T>* v, V2<T>* line, int lane, const V
 // ---------------------------------------------------------------------------
 template <typename T, class S>
 __global__ void __launch_bounds__(256) k_row_fast(const V2<T>* __restrict__ Sg, V2<T>* __restrict__ Tout, int Ny,
-                                                  Nonlin q, const V2<T>* __restrict__ twN,
-                                                  int* __restrict__ status) {
+                                                  Nonlin q, const V2<T>* __restrict__ twN, Slots sl) {
   using V = V2<T>;
   constexpr int M = S::L, P = S::P, TPL = S::TPL, Nc = M + 1, LD = row_ld(M);
   extern __shared__ __align__(16) unsigned char smem_raw[];
+  const int slot = slot_of(sl, blockIdx.y);
+  Sg += (size_t)slot * sl.sstride;
+  Tout += (size_t)slot * sl.sstride;
+  if (sl.M) q.M = sl.M[slot];
   V* tw = reinterpret_cast<V*>(smem_raw);
   const int wl = threadIdx.x / TPL;
   const int lane = threadIdx.x - wl * TPL;
@@ -224,7 +227,7 @@ __global__ void __launch_bounds__(256) k_row_fast(const V2<T>* __restrict__ Sg, 
     bad |= !(isfinite(v[qq].x) && isfinite(v[qq].y));
     v[qq] = V{nonlin_mu<T>(v[qq].x, q), nonlin_mu<T>(v[qq].y, q)};
   }
-  if (bad && live) atomicOr(status, 1);
+  if (bad && live && sl.status) atomicOr(sl.status + slot, 1);
   reg_fft<T, S, false>(v, line, lane, tw, 2);
   line_sync<S>();
 #pragma unroll
@@ -252,7 +255,7 @@ __global__ void __launch_bounds__(256) k_row_fast(const V2<T>* __restrict__ Sg, 
 template <typename T, class S, int MODE>
 __global__ void __launch_bounds__(256) k_col_fast(V2<T>* __restrict__ Sg, const V2<T>* __restrict__ Tin,
                                                   V2<T>* __restrict__ D, int Nc, ColTables tb,
-                                                  const V2<T>* __restrict__ twY) {
+                                                  const V2<T>* __restrict__ twY, Slots sl) {
   using V = V2<T>;
   constexpr int NY = S::L, P = S::P, TPL = S::TPL, LD = col_ld(NY);
   extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -262,15 +265,23 @@ __global__ void __launch_bounds__(256) k_col_fast(V2<T>* __restrict__ Sg, const 
   const int lgC = __ffs(C) - 1;
   const int wl = threadIdx.x / TPL;
   const int lane = threadIdx.x - wl * TPL;
-  const int ntile = (Nc + C - 1) / C;
-  const int c0 = xcd_remap(blockIdx.x, ntile) * C;
+  // XCD-aware order over (instance, tile): consecutive tiles of an instance
+  // land on the same XCD
+  const int ntile = gridDim.x;
+  const int lin = xcd_remap(blockIdx.x + blockIdx.y * ntile, ntile * gridDim.y);
+  const int yi = lin / ntile;
+  const int slot = slot_of(sl, yi);
+  const int c0 = (lin - yi * ntile) * C;
+  Sg += (size_t)slot * sl.sstride;
+  Tin += (size_t)slot * sl.sstride;
+  if (D) D += (size_t)slot * sl.sstride;
   const int ncols = min(C, Nc - c0);
   const int col = c0 + wl;
   const bool live = wl < ncols;
   V* line = lds + wl * LD;
-  const T* E1 = reinterpret_cast<const T*>(tb.E1);
-  const T* Q1 = reinterpret_cast<const T*>(tb.Q1);
-  const T* Q2 = reinterpret_cast<const T*>(tb.Q2);
+  const T* E1 = reinterpret_cast<const T*>(tb.E1) + (size_t)slot * sl.tstride;
+  const T* Q1 = reinterpret_cast<const T*>(tb.Q1) + (size_t)slot * sl.tstride;
+  const T* Q2 = reinterpret_cast<const T*>(tb.Q2) + (size_t)slot * sl.tstride;
 
   // Tile I/O: thread t moves column tc = t % C of rows ty + TPL*q, so
   // consecutive threads touch consecutive columns of a row (coalesced).
@@ -326,7 +337,8 @@ __global__ void __launch_bounds__(256) k_col_fast(V2<T>* __restrict__ Sg, const 
       if constexpr (MODE == COL_RK2_STAGE1) q2r[qq] = Q2[ti];
     }
   }
-  const double pin = tb.pin ? *tb.pin : 0.0;
+  const double* pinp = sl.pin ? sl.pin + slot : nullptr;
+  const double pin = pinp ? *pinp : 0.0;
   V b[P];
   reg_gather<T, S>(b, line, lane);
   reg_fft<T, S, false>(b, line, lane, tw, 1);
@@ -343,7 +355,7 @@ __global__ void __launch_bounds__(256) k_col_fast(V2<T>* __restrict__ Sg, const 
         if constexpr (MODE == COL_RK2_STAGE1)
           D[tbase + ky] = V{out.x - q2r[qq] * b[qq].x, out.y - q2r[qq] * b[qq].y};
       }
-      if (tb.pin && col == 0 && ky == 0) out = V{T(pin), T(0)};
+      if (pinp && col == 0 && ky == 0) out = V{T(pin), T(0)};
       b[qq] = out;
     }
   }
@@ -403,12 +415,14 @@ struct SchedInfo {
 // Launchers, instantiated in the pfc_inst_*.hip translation units.  Each
 // returns false if (L, variant) has no schedule.
 bool row_fast_launch_f64(int M, int var, dim3 grid, dim3 block, size_t lds, hipStream_t st, const V2<double>* S,
-                         V2<double>* T, int Ny, const Nonlin& q, const V2<double>* tw, int* status);
+                         V2<double>* T, int Ny, const Nonlin& q, const V2<double>* tw, const Slots& sl);
 bool row_fast_launch_f32(int M, int var, dim3 grid, dim3 block, size_t lds, hipStream_t st, const V2<float>* S,
-                         V2<float>* T, int Ny, const Nonlin& q, const V2<float>* tw, int* status);
+                         V2<float>* T, int Ny, const Nonlin& q, const V2<float>* tw, const Slots& sl);
 bool col_fast_launch_f64(int Ny, int var, int mode, dim3 grid, dim3 block, size_t lds, hipStream_t st, V2<double>* S,
-                         const V2<double>* T, V2<double>* D, int Nc, const ColTables& tb, const V2<double>* tw);
+                         const V2<double>* T, V2<double>* D, int Nc, const ColTables& tb, const V2<double>* tw,
+                         const Slots& sl);
 bool col_fast_launch_f32(int Ny, int var, int mode, dim3 grid, dim3 block, size_t lds, hipStream_t st, V2<float>* S,
-                         const V2<float>* T, V2<float>* D, int Nc, const ColTables& tb, const V2<float>* tw);
+                         const V2<float>* T, V2<float>* D, int Nc, const ColTables& tb, const V2<float>* tw,
+                         const Slots& sl);
 
 }  // namespace pfc

@@ -46,6 +46,20 @@ __device__ __forceinline__ T nonlin_mu(T f, const Nonlin& q) {
   return lg + poly - T(q.M) * f;
 }
 
+// Instances stepped together by one launch.  blockIdx.y indexes the active
+// list (or is the slot itself when the list is null); a slot's arrays sit
+// `sstride` (state) / `tstride` (tables) elements from slot 0's.
+struct Slots {
+  const int* list;    // active slot ids in device memory, or null
+  long long sstride;  // complex elements between slots' S~, T and D
+  long long tstride;  // elements between slots' ETD tables
+  const double* pin;  // per-slot pinned k = 0 value, or null (no pin)
+  int* status;        // per-slot status flags, or null
+  const double* M;    // per-slot stabilising shift, or null (use Nonlin.M)
+};
+
+__device__ __forceinline__ int slot_of(const Slots& s, int y) { return s.list ? s.list[y] : y; }
+
 struct RowGeom {
   int Nx, M, Nc, Ny;
   int nl;   // rows per block
@@ -127,12 +141,15 @@ __device__ __forceinline__ V2<T> row_unpack_pair(V2<T> zk, V2<T> zm, V2<T> w) {
 // ---------------------------------------------------------------------------
 template <typename T>
 __global__ void __launch_bounds__(256) k_row_nonlin(const V2<T>* __restrict__ S, V2<T>* __restrict__ Tout,
-                                                    RowGeom g, Nonlin q, const V2<T>* __restrict__ twN,
-                                                    int* __restrict__ status) {
+                                                    RowGeom g, Nonlin q, const V2<T>* __restrict__ twN, Slots sl) {
   using V = V2<T>;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   V* b0 = reinterpret_cast<V*>(smem_raw);
   V* b1 = b0 + g.nl * g.ld;
+  const int slot = slot_of(sl, blockIdx.y);
+  S += (size_t)slot * sl.sstride;
+  Tout += (size_t)slot * sl.sstride;
+  if (sl.M) q.M = sl.M[slot];
   const int row0 = blockIdx.x * g.nl;
   const int nrows = min(g.nl, g.Ny - row0);
   row_load_pack<T, T>(S, b1, b0, g, row0, nrows, twN);
@@ -145,7 +162,7 @@ __global__ void __launch_bounds__(256) k_row_nonlin(const V2<T>* __restrict__ S,
     bad |= !(isfinite(v.x) && isfinite(v.y));
     z[line * g.ld + m] = V{nonlin_mu<T>(v.x, q), nonlin_mu<T>(v.y, q)};
   }
-  if (bad) atomicOr(status, 1);
+  if (bad && sl.status) atomicOr(sl.status + slot, 1);
   __syncthreads();
   V* other = (z == b0) ? b1 : b0;
   V* Z = fft_lines<T, false>(z, other, g.plM, nrows, g.ld, twN, 2);
@@ -162,10 +179,9 @@ __global__ void __launch_bounds__(256) k_row_nonlin(const V2<T>* __restrict__ S,
 enum { COL_ETD1 = 0, COL_RK2_STAGE1 = 1, COL_RK2_STAGE2 = 2 };
 
 struct ColTables {
-  const void* E1;  // column-major [Nc][Ny], storage type
+  const void* E1;  // column-major [Nc][Ny] per slot, storage type
   const void* Q1;
   const void* Q2;
-  const double* pin;  // device scalar: pinned k=0 value (the mean), or null
 };
 
 constexpr int kColRegs = 16;  // A values held per thread between the two forward FFTs
@@ -191,18 +207,24 @@ __device__ __forceinline__ void col_load(const V2<T>* __restrict__ src, V2<T>* b
 template <typename T, int MODE>
 __global__ void __launch_bounds__(256) k_col_etd(V2<T>* __restrict__ S, const V2<T>* __restrict__ Tin,
                                                  V2<T>* __restrict__ D, ColGeom g, ColTables tb,
-                                                 const V2<T>* __restrict__ twY) {
+                                                 const V2<T>* __restrict__ twY, Slots sl) {
   using V = V2<T>;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   V* b0 = reinterpret_cast<V*>(smem_raw);
   V* b1 = b0 + g.nc * g.ld;
-  const int ntile = (g.Nc + g.nc - 1) / g.nc;
-  const int c0 = xcd_remap(blockIdx.x, ntile) * g.nc;
+  const int ntile = gridDim.x;
+  const int lin = xcd_remap(blockIdx.x + blockIdx.y * ntile, ntile * gridDim.y);
+  const int yi = lin / ntile;
+  const int slot = slot_of(sl, yi);
+  const int c0 = (lin - yi * ntile) * g.nc;
+  S += (size_t)slot * sl.sstride;
+  Tin += (size_t)slot * sl.sstride;
+  if (D) D += (size_t)slot * sl.sstride;
   const int ncols = min(g.nc, g.Nc - c0);
   const int nel = ncols * g.Ny;
-  const T* E1 = reinterpret_cast<const T*>(tb.E1);
-  const T* Q1 = reinterpret_cast<const T*>(tb.Q1);
-  const T* Q2 = reinterpret_cast<const T*>(tb.Q2);
+  const T* E1 = reinterpret_cast<const T*>(tb.E1) + (size_t)slot * sl.tstride;
+  const T* Q1 = reinterpret_cast<const T*>(tb.Q1) + (size_t)slot * sl.tstride;
+  const T* Q2 = reinterpret_cast<const T*>(tb.Q2) + (size_t)slot * sl.tstride;
 
   V a[kColRegs];
   if (MODE != COL_RK2_STAGE2) {
@@ -223,7 +245,8 @@ __global__ void __launch_bounds__(256) k_col_etd(V2<T>* __restrict__ S, const V2
   col_load<T>(Tin, b0, g, c0, ncols);
   __syncthreads();
   V* B = fft_lines<T, false>(b0, b1, g.plY, ncols, g.ld, twY, 1);
-  const double pin = tb.pin ? *tb.pin : 0.0;
+  const double* pinp = sl.pin ? sl.pin + slot : nullptr;
+  const double pin = pinp ? *pinp : 0.0;
 #pragma unroll
   for (int i = 0; i < kColRegs; ++i) {
     const int e = threadIdx.x + i * blockDim.x;
@@ -245,7 +268,7 @@ __global__ void __launch_bounds__(256) k_col_etd(V2<T>* __restrict__ S, const V2
           D[ti] = V{out.x - q2 * b.x, out.y - q2 * b.y};
         }
       }
-      if (tb.pin && c0 + line == 0 && ky == 0) out = V{T(pin), T(0)};
+      if (pinp && c0 + line == 0 && ky == 0) out = V{T(pin), T(0)};
       B[line * g.ld + ky] = out;
     }
   }
@@ -450,6 +473,63 @@ __global__ void __launch_bounds__(256) k_col_filter(const V2<T>* __restrict__ S,
 }
 
 // ---------------------------------------------------------------------------
+// ETD tables, built on the device in FP64 and stored in the bulk precision
+// ---------------------------------------------------------------------------
+// E = exp(L dt), P = (E - 1)/L where |L dt| > smallZTol else dt (1 + z/2 + z^2/6)
+// (the ETD1 coefficients of kernels.py:55-71), L = Gamma lap (beta (q0^2 + lap)^2
+// + eps + M), lap = -(kx^2 + ky^2); P2 = dt (e^z - 1 - z)/z^2 (ETDRK2).
+// Stored column-major [Nc][Ny], pre-scaled for the row-spectrum state:
+// E1 = E/Ny, Q1 = P Gamma lap/(Nx Ny), Q2 = P2 Gamma lap/(Nx Ny).
+struct TabParams {
+  double Gamma, beta, q0, epsilon, M, dt, smallZTol;
+  double inv_ny, inv_n;
+  int Nc, Ny;
+};
+
+__device__ __forceinline__ double etd2_phi_dt(double z, double dt) {
+  if (fabs(z) < 0.1) {
+    double s = 0.0, t = 0.5;
+    for (int j = 0; j < 16; ++j) {
+      s += t;
+      t = t * z / (j + 3);
+    }
+    return dt * s;
+  }
+  return dt * ((expm1(z) - z) / (z * z));
+}
+
+// blockIdx.y = slot - slot0; per-slot epsilon and M come from eps[] and Ms[].
+template <typename T>
+__global__ void __launch_bounds__(256) k_tables(T* __restrict__ E1, T* __restrict__ Q1, T* __restrict__ Q2,
+                                                long long tstride, const double* __restrict__ kx2,
+                                                const double* __restrict__ ky2, TabParams p,
+                                                const double* __restrict__ eps, const double* __restrict__ Ms,
+                                                int slot0) {
+  const int slot = slot0 + blockIdx.y;
+  p.epsilon = eps[slot];
+  p.M = Ms[slot];
+  E1 += (size_t)slot * tstride;
+  Q1 += (size_t)slot * tstride;
+  if (Q2) Q2 += (size_t)slot * tstride;
+  const size_t n = (size_t)p.Nc * p.Ny;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+    const int c = (int)(i / p.Ny);
+    const int j = (int)(i - (size_t)c * p.Ny);
+    const double lap = -(kx2[c] + ky2[j]);
+    const double sq = p.q0 * p.q0 + lap;
+    double lmu = p.beta * (sq * sq) + p.epsilon;
+    if (p.M != 0.0) lmu = lmu + p.M;
+    const double L = p.Gamma * lap * lmu;
+    const double z = L * p.dt;
+    const double E = exp(z);
+    const double P = fabs(z) > p.smallZTol ? (E - 1) / L : p.dt * (1 + z / 2 + z * z / 6);
+    E1[i] = T(E * p.inv_ny);
+    Q1[i] = T(P * (p.Gamma * lap) * p.inv_n);
+    if (Q2) Q2[i] = T(etd2_phi_dt(z, p.dt) * (p.Gamma * lap) * p.inv_n);
+  }
+}
+
+// ---------------------------------------------------------------------------
 // FP64 reductions: free energy per site, mean, min, max
 // ---------------------------------------------------------------------------
 struct FeParams {
@@ -457,6 +537,7 @@ struct FeParams {
   double beta, q0, epsilon, g;
   const double* kx2;  // [Nc]  kx^2
   const double* ky2;  // [Ny]  ky^2
+  const double* eps;  // per-slot epsilon (indexed like Slots), or null: use `epsilon`
 };
 
 // per-block partials: [0] sum n, [1] sum f_local, [2] sum w |A|^2 Lf  (each DD),
@@ -494,11 +575,16 @@ __device__ __forceinline__ void block_reduce_partial(Partial& mine, Partial* red
 // rows of S~ (storage Tin) -> n in FP64 -> sums of n and of the local FE density
 template <typename Tin>
 __global__ void __launch_bounds__(256) k_fe_row(const V2<Tin>* __restrict__ S, RowGeom g, FeParams p,
-                                                const V2<double>* __restrict__ twN, Partial* __restrict__ part) {
+                                                const V2<double>* __restrict__ twN, Partial* __restrict__ part,
+                                                Slots sl, int pstride) {
   using V = V2<double>;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   V* b0 = reinterpret_cast<V*>(smem_raw);
   V* b1 = b0 + g.nl * g.ld;
+  const int slot = slot_of(sl, blockIdx.y);
+  S += (size_t)slot * sl.sstride;
+  if (p.eps) p.epsilon = p.eps[slot];
+  part += (size_t)blockIdx.y * pstride;
   const int row0 = blockIdx.x * g.nl;
   const int nrows = min(g.nl, g.Ny - row0);
   row_load_pack<double, Tin>(S, b1, b0, g, row0, nrows, twN);
@@ -526,11 +612,14 @@ __global__ void __launch_bounds__(256) k_fe_row(const V2<Tin>* __restrict__ S, R
 // w(kx) |A|^2 (beta/2)(q0^2 - k^2)^2, w = 1 on kx = 0 and kx = Nx/2, else 2.
 template <typename Tin>
 __global__ void __launch_bounds__(256) k_fe_col(const V2<Tin>* __restrict__ S, ColGeom g, FeParams p, int Nx,
-                                                const V2<double>* __restrict__ twY, Partial* __restrict__ part) {
+                                                const V2<double>* __restrict__ twY, Partial* __restrict__ part,
+                                                Slots sl, int pstride) {
   using V = V2<double>;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   V* b0 = reinterpret_cast<V*>(smem_raw);
   V* b1 = b0 + g.nc * g.ld;
+  S += (size_t)slot_of(sl, blockIdx.y) * sl.sstride;
+  part += (size_t)blockIdx.y * pstride;
   const int c0 = blockIdx.x * g.nc;
   const int ncols = min(g.nc, g.Nc - c0);
   for (int e = threadIdx.x; e < ncols * g.Ny; e += blockDim.x) {
@@ -560,10 +649,16 @@ __global__ void __launch_bounds__(256) k_fe_col(const V2<Tin>* __restrict__ S, C
   block_reduce_partial(mine, reinterpret_cast<Partial*>(smem_raw), part + blockIdx.x);
 }
 
-// one block: fixed-order sum of the row and column partials.
-// out[0] = free energy per site, out[1] = mean, out[2] = min, out[3] = max
+// one block per active instance (blockIdx.x indexes the active list):
+// fixed-order sum of its row and column partials into out[4*i ..], i the
+// position in the active list:
+// free energy per site, mean, min, max.  pin_out[slot] receives the mean.
 static __global__ void __launch_bounds__(256) k_fe_final(const Partial* __restrict__ part, int nrow, int ncol, int Nx,
-                                                  int Ny, double* __restrict__ out, double* __restrict__ pin_out) {
+                                                  int Ny, double* __restrict__ out, double* __restrict__ pin_out,
+                                                  Slots sl, int pstride) {
+  part += (size_t)blockIdx.x * pstride;
+  out += 4 * (size_t)blockIdx.x;
+  if (pin_out) pin_out += slot_of(sl, blockIdx.x);
   Partial mine;
   for (int i = 0; i < 3; ++i) mine.s[i] = DD{0.0, 0.0};
   mine.mn = INFINITY;

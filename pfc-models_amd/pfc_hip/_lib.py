@@ -64,6 +64,17 @@ _SIGNATURES = [
                                         ctypes.POINTER(ctypes.c_longlong)]),
     ("pfc_plan_schedule", ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_int)]),
     ("pfc_plan_bytes", ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_longlong)]),
+    ("pfc_batch_create", ctypes.c_int, [ctypes.POINTER(PfcParams), ctypes.c_int, ctypes.POINTER(_P)]),
+    ("pfc_batch_destroy", ctypes.c_int, [_P]),
+    ("pfc_batch_set_slot", ctypes.c_int, [_P, ctypes.c_int, ctypes.c_double, ctypes.c_double, _P]),
+    ("pfc_batch_set_field", ctypes.c_int, [_P, ctypes.c_int, _P, ctypes.c_int, _P]),
+    ("pfc_batch_get_field", ctypes.c_int, [_P, ctypes.c_int, _P, ctypes.c_int, _P]),
+    ("pfc_batch_set_active", ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_int), ctypes.c_int, _P]),
+    ("pfc_batch_step", ctypes.c_int, [_P, ctypes.c_longlong, _P]),
+    ("pfc_batch_reduce", ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_double), _P]),
+    ("pfc_batch_get_status", ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_int), _P]),
+    ("pfc_batch_bytes", ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_longlong)]),
+    ("pfc_slot_bytes", ctypes.c_int, [ctypes.POINTER(PfcParams), ctypes.POINTER(ctypes.c_longlong)]),
     ("pfc_rfft2", ctypes.c_int, [_P, _P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _P]),
     ("pfc_irfft2", ctypes.c_int, [_P, _P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _P]),
 ]

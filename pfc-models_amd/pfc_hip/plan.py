@@ -27,6 +27,37 @@ def current_stream_handle():
     return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
 
 
+def make_params(*, model="std", Nx, Ny, Lx, Ly, Gamma=1.0, beta=1.0, q0=1.0, epsilon=1.0, g=1.0, dt=0.01,
+                minPhi=1e-6, smallZTol=2e-5, M=0.0, precision=64, scheme="etd1", clamp_log=False, pin_mean=True):
+    """`pfc_params` from keyword arguments (names of Build2DGeometry and the model builders)."""
+    if scheme not in SCHEMES:
+        raise ValueError(f"scheme must be one of {sorted(SCHEMES)}")
+    if model not in MODELS:
+        raise ValueError(f"model must be one of {sorted(MODELS)}")
+    return _lib.PfcParams(
+        model=MODELS[model], Nx=int(Nx), Ny=int(Ny), Lx=float(Lx), Ly=float(Ly),
+        Gamma=float(Gamma), beta=float(beta), q0=float(q0), epsilon=float(epsilon), g=float(g),
+        dt=float(dt), minPhi=float(minPhi), smallZTol=float(smallZTol), M=float(M),
+        precision=int(precision), scheme=SCHEMES[scheme], clamp_log=int(bool(clamp_log)),
+        pin_mean=int(bool(pin_mean)))
+
+
+def slot_bytes(**kw):
+    """Device bytes one instance of these parameters needs (state and tables)."""
+    v = ctypes.c_longlong()
+    p = make_params(**kw)
+    _lib.check(_lib.load().pfc_slot_bytes(ctypes.byref(p), ctypes.byref(v)))
+    return v.value
+
+
+def _field_tensor(torch, x, Ny, Nx):
+    if isinstance(x, torch.Tensor):
+        t = x.to(device="cuda", dtype=torch.float64 if x.dtype == torch.float64 else torch.float32)
+    else:
+        t = torch.as_tensor(np.ascontiguousarray(np.asarray(x, dtype=np.float64)), device="cuda")
+    return t.reshape(Ny, Nx).contiguous()
+
+
 class Plan:
     """One Ny x Nx instance resident in HBM: row-spectrum state, FP64-built
     ETD tables and reduction scratch, owned by the library."""
@@ -36,20 +67,13 @@ class Plan:
                  clamp_log=False, pin_mean=True):
         self._lib = _lib.load()
         self.torch = _torch()
-        if scheme not in SCHEMES:
-            raise ValueError(f"scheme must be one of {sorted(SCHEMES)}")
-        if model not in MODELS:
-            raise ValueError(f"model must be one of {sorted(MODELS)}")
+        p = make_params(model=model, Nx=Nx, Ny=Ny, Lx=Lx, Ly=Ly, Gamma=Gamma, beta=beta, q0=q0, epsilon=epsilon,
+                        g=g, dt=dt, minPhi=minPhi, smallZTol=smallZTol, M=M, precision=precision, scheme=scheme,
+                        clamp_log=clamp_log, pin_mean=pin_mean)
         self.Nx, self.Ny = int(Nx), int(Ny)
         self.precision = int(precision)
         self.scheme = scheme
         self.model = model
-        p = _lib.PfcParams(
-            model=MODELS[model], Nx=self.Nx, Ny=self.Ny, Lx=float(Lx), Ly=float(Ly),
-            Gamma=float(Gamma), beta=float(beta), q0=float(q0), epsilon=float(epsilon), g=float(g),
-            dt=float(dt), minPhi=float(minPhi), smallZTol=float(smallZTol), M=float(M),
-            precision=self.precision, scheme=SCHEMES[scheme], clamp_log=int(bool(clamp_log)),
-            pin_mean=int(bool(pin_mean)))
         self.params = p
         h = ctypes.c_void_p()
         _lib.check(self._lib.pfc_create(ctypes.byref(p), ctypes.byref(h)))
@@ -75,13 +99,7 @@ class Plan:
 
     # -- field I/O ----------------------------------------------------------
     def _device_field(self, x):
-        torch = self.torch
-        if isinstance(x, torch.Tensor):
-            t = x.to(device="cuda", dtype=torch.float64 if x.dtype == torch.float64 else torch.float32)
-        else:
-            t = torch.as_tensor(np.ascontiguousarray(np.asarray(x, dtype=np.float64)), device="cuda")
-        t = t.reshape(self.Ny, self.Nx).contiguous()
-        return t
+        return _field_tensor(self.torch, x, self.Ny, self.Nx)
 
     def set_field(self, x):
         t = self._device_field(x)
@@ -162,6 +180,94 @@ class Plan:
     def device_bytes(self):
         v = ctypes.c_longlong()
         _lib.check(self._lib.pfc_plan_bytes(self._h, ctypes.byref(v)))
+        return v.value
+
+
+class Batch:
+    """`capacity` same-shape instances stepped by the same launches.
+
+    Every pfc_params field is shared except epsilon and M, which are per slot
+    (`set_slot`).  `step(n)` advances the active slots (initially all)
+    together; `reduce()` returns an (n_active, 4) array of FP64
+    (free energy per site, mean, min, max)."""
+
+    def __init__(self, capacity, **kw):
+        self._lib = _lib.load()
+        self.torch = _torch()
+        p = make_params(**kw)
+        self.params = p
+        self.capacity = int(capacity)
+        self.Nx, self.Ny = p.Nx, p.Ny
+        self.precision = p.precision
+        h = ctypes.c_void_p()
+        _lib.check(self._lib.pfc_batch_create(ctypes.byref(p), self.capacity, ctypes.byref(h)))
+        self._h = h
+        self._active = list(range(self.capacity))
+        self._keep = {}
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            self.torch.cuda.synchronize()
+            _lib.check(self._lib.pfc_batch_destroy(self._h))
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_slot(self, slot, epsilon, M=0.0):
+        _lib.check(self._lib.pfc_batch_set_slot(self._h, int(slot), float(epsilon), float(M),
+                                                current_stream_handle()))
+        return self
+
+    def set_field(self, slot, x):
+        t = _field_tensor(self.torch, x, self.Ny, self.Nx)
+        dt = 64 if t.dtype == self.torch.float64 else 32
+        _lib.check(self._lib.pfc_batch_set_field(self._h, int(slot), ctypes.c_void_p(t.data_ptr()), dt,
+                                                 current_stream_handle()))
+        self._keep[int(slot)] = t  # must outlive the asynchronous upload
+        return self
+
+    def get_field(self, slot, out=None, dtype=None):
+        torch = self.torch
+        if out is None:
+            out = torch.empty((self.Ny, self.Nx), device="cuda", dtype=dtype or torch.float64)
+        dt = 64 if out.dtype == torch.float64 else 32
+        _lib.check(self._lib.pfc_batch_get_field(self._h, int(slot), ctypes.c_void_p(out.data_ptr()), dt,
+                                                 current_stream_handle()))
+        return out
+
+    def set_active(self, slots):
+        slots = [int(s) for s in slots]
+        arr = (ctypes.c_int * max(1, len(slots)))(*slots)
+        _lib.check(self._lib.pfc_batch_set_active(self._h, arr, len(slots), current_stream_handle()))
+        self._active = slots
+        return self
+
+    @property
+    def active(self):
+        return list(self._active)
+
+    def step(self, nsteps=1):
+        _lib.check(self._lib.pfc_batch_step(self._h, int(nsteps), current_stream_handle()))
+        return self
+
+    def reduce(self):
+        n = len(self._active)
+        out = (ctypes.c_double * max(4, 4 * n))()
+        _lib.check(self._lib.pfc_batch_reduce(self._h, out, current_stream_handle()))
+        return np.array(out[: 4 * n], dtype=np.float64).reshape(n, 4)
+
+    def status(self):
+        v = (ctypes.c_int * self.capacity)()
+        _lib.check(self._lib.pfc_batch_get_status(self._h, v, current_stream_handle()))
+        return np.array(v[:], dtype=np.int32)
+
+    def device_bytes(self):
+        v = ctypes.c_longlong()
+        _lib.check(self._lib.pfc_batch_bytes(self._h, ctypes.byref(v)))
         return v.value
 
 

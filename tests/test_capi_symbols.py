@@ -23,7 +23,7 @@ def test_library_exports_every_declared_symbol():
     for name in _declared():
         assert hasattr(lib, name), name
     bound = _lib.load()
-    assert bound.pfc_version() == 1
+    assert bound.pfc_version() == 2
 
 
 def test_params_struct_layout_matches_c_compiler(tmp_path):
