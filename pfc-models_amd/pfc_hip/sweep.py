@@ -1,0 +1,168 @@
+"""Phase-diagram sweeps: many stdPFC relaxations at once.
+
+The workload is a 16 x 16 grid of (r, psibar) state points, r in
+[-0.5, -0.05] and psibar in [-0.6, 0], each relaxed from psibar +
+U(-0.05, 0.05) for a fixed number of ETD1 steps on its own grid.  Grid sides
+are drawn from {64, 128, 192, 256, 384, 512} (seeded), and each box holds an
+integer number of triangular periods (`fields.commensurate_box`).
+
+Two ways to run it:
+
+* `run_plain`   one instance at a time, one Plan each, FP64 (the plain
+                one-job-at-a-time GPU step);
+* `run_batched` instances grouped by grid size into Batches (FP32 bulk,
+                FP64 tables and reductions); every group advances with one
+                row and one column launch per step, groups on their own HIP
+                streams so their launches overlap.
+
+Both return per-instance FP64 free energy per site and mean density, which is
+what the common-tangent construction consumes (`(phi, fmin)` tables, cf.
+phase_diagram/common_tangent.py:336-403 in the reference).
+"""
+from __future__ import annotations
+
+import math
+import time
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import fields as F
+
+SIZES = (64, 128, 192, 256, 384, 512)
+
+
+@dataclass(frozen=True)
+class Instance:
+    index: int
+    r: float
+    psibar: float
+    N: int
+    Lx: float
+    Ly: float
+    seed: int
+
+
+def sweep_instances(n_r=16, n_psi=16, r_range=(-0.5, -0.05), psi_range=(-0.6, 0.0), size_seed=4,
+                    sizes=SIZES, dx=math.pi / 4):
+    """The (r, psibar, N) grid; instance i uses noise seed 1000 + i."""
+    rng = np.random.default_rng(size_seed)
+    rs = np.linspace(*r_range, n_r)
+    ps = np.linspace(*psi_range, n_psi)
+    out = []
+    k = 0
+    for r in rs:
+        for pb in ps:
+            n = int(rng.choice(sizes))
+            lx, ly = F.commensurate_box(n, n, dx=dx)
+            out.append(Instance(k, float(r), float(pb), n, lx, ly, 1000 + k))
+            k += 1
+    return out
+
+
+def initial_field(inst: Instance):
+    return F.uniform_noise((inst.N, inst.N), inst.psibar, 0.05, seed=inst.seed)
+
+
+def _model_kw(inst: Instance, dt, precision, scheme):
+    # F = integral psi/2 (r + (1 + lap)^2) psi + psi^4/4: beta = q0 = Gamma = 1, epsilon = r, g = 0
+    return dict(model="std", Nx=inst.N, Ny=inst.N, Lx=inst.Lx, Ly=inst.Ly, Gamma=1.0, beta=1.0, q0=1.0,
+                epsilon=inst.r, g=0.0, dt=dt, precision=precision, scheme=scheme, pin_mean=True)
+
+
+@dataclass
+class SweepResult:
+    fe: np.ndarray        # free energy per site (FP64), by instance index
+    mean: np.ndarray      # mean density (FP64)
+    wall_s: float
+    solves_per_s: float
+    nonfinite: np.ndarray  # status flags
+
+
+def run_plain(instances, steps, dt=0.5, precision=64, scheme="etd1"):
+    """One instance at a time, one Plan each."""
+    import torch
+    from .plan import Plan
+    n = len(instances)
+    fe, mean, bad = np.zeros(n), np.zeros(n), np.zeros(n, dtype=np.int32)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for j, inst in enumerate(instances):
+        pl = Plan(**_model_kw(inst, dt, precision, scheme))
+        pl.set_field(initial_field(inst))
+        pl.step(steps)
+        fe[j], mean[j], _, _ = pl.reduce()
+        bad[j] = pl.status()
+        pl.close()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    return SweepResult(fe, mean, wall, n / wall, bad)
+
+
+class BatchedSweep:
+    """Instances grouped by grid size, one Batch (and one HIP stream) per group."""
+
+    def __init__(self, instances, dt=0.5, precision=32, scheme="etd1"):
+        import torch
+        from .plan import Batch
+        self.instances = list(instances)
+        self.groups = {}
+        for j, inst in enumerate(self.instances):
+            self.groups.setdefault(inst.N, []).append(j)
+        self.batches, self.streams = {}, {}
+        for n, idx in self.groups.items():
+            first = self.instances[idx[0]]
+            b = Batch(len(idx), **_model_kw(first, dt, precision, scheme))
+            s = torch.cuda.Stream()
+            with torch.cuda.stream(s):
+                for slot, j in enumerate(idx):
+                    inst = self.instances[j]
+                    if (inst.Lx, inst.Ly) != (first.Lx, first.Ly):
+                        raise ValueError("instances of one grid size must share the box")
+                    b.set_slot(slot, inst.r)
+                    b.set_field(slot, initial_field(inst))
+            self.batches[n], self.streams[n] = b, s
+        torch.cuda.synchronize()
+
+    def step(self, steps):
+        import torch
+        for n, b in self.batches.items():
+            with torch.cuda.stream(self.streams[n]):
+                b.step(steps)
+
+    def results(self):
+        import torch
+        m = len(self.instances)
+        fe, mean, bad = np.zeros(m), np.zeros(m), np.zeros(m, dtype=np.int32)
+        for n, b in self.batches.items():
+            with torch.cuda.stream(self.streams[n]):
+                red = b.reduce()
+                st = b.status()
+            for slot, j in enumerate(self.groups[n]):
+                fe[j], mean[j] = red[slot, 0], red[slot, 1]
+                bad[j] = st[slot]
+        return fe, mean, bad
+
+    def close(self):
+        for b in self.batches.values():
+            b.close()
+
+
+def run_batched(instances, steps, dt=0.5, precision=32, scheme="etd1", chunk=None):
+    """All instances at once; the wall time covers setup (allocation, tables,
+    upload, kernel tuning), stepping and the reductions, as run_plain's does."""
+    import torch
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    sw = BatchedSweep(instances, dt=dt, precision=precision, scheme=scheme)
+    done = 0
+    chunk = chunk or steps
+    while done < steps:
+        k = min(chunk, steps - done)
+        sw.step(k)
+        done += k
+    fe, mean, bad = sw.results()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    sw.close()
+    return SweepResult(fe, mean, wall, len(instances) / wall, bad)
