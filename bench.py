@@ -72,7 +72,10 @@ def run_gpu(precision, steps, warmup):
     avg_ms = kinds[dom][0]
     ab = _algorithmic_bytes(dom, precision)
     achieved = ab / (avg_ms * 1e-3) / 1e9
-    names = {"row": "k_row_nonlin", "col": "k_col_etd<COL_RK2_STAGE1>", "col2": "k_col_etd<COL_RK2_STAGE2>"}
+    sched = pl.schedule()
+    rk = "k_row_fast" if sched["row"][0] else "k_row_nonlin"
+    ck = "k_col_fast" if sched["col"][0] else "k_col_etd"
+    names = {"row": rk, "col": f"{ck}<COL_RK2_STAGE1>", "col2": f"{ck}<COL_RK2_STAGE2>"}
     step_bytes = 2 * 4 * N * (N // 2 + 1) * (16 if precision == 64 else 8)
     return {
         "value": N * N * steps / wall / 1e6,
@@ -91,6 +94,7 @@ def run_gpu(precision, steps, warmup):
             "frac": step_bytes / (wall / steps) / 1e9 / HBM_PEAK_GBS,
         },
         "device_bytes": pl.device_bytes(),
+        "schedule": sched,
     }
 
 
@@ -117,6 +121,33 @@ def run_cpu(max_seconds=20.0):
                       f"(NumPy/SciPy restatement of the reference CPU step, scipy.fft workers={workers})"}
 
 
+def run_sweep(plain_fraction=8):
+    """The 256-instance (r, psibar) stdPFC sweep, 5000 ETD1 steps each: batched
+    FP32 (and FP64) against the plain FP64 one-job-at-a-time step, the latter
+    timed on every `plain_fraction`-th instance and reported per solve."""
+    import numpy as np
+    from pfc_hip import sweep as SW
+    inst = SW.sweep_instances()
+    b32 = SW.run_batched(inst, 5000, precision=32)
+    b64 = SW.run_batched(inst, 5000, precision=64)
+    sub = inst[::plain_fraction]
+    plain = SW.run_plain(sub, 5000)
+    idx = [i.index for i in sub]
+    rel32 = np.abs(b32.fe[idx] - plain.fe) / np.abs(plain.fe)
+    rel64 = np.abs(b64.fe[idx] - plain.fe) / np.abs(plain.fe)
+    return {
+        "metric": "sweep solves/s (256 stdPFC instances, 64^2..512^2, ETD1 dt=0.5, 5000 steps)",
+        "batched_fp32_solves_per_s": round(b32.solves_per_s, 2),
+        "batched_fp64_solves_per_s": round(b64.solves_per_s, 2),
+        "plain_fp64_solves_per_s": round(plain.solves_per_s, 2),
+        "speedup_fp32_vs_plain_fp64": round(b32.solves_per_s / plain.solves_per_s, 2),
+        "plain_sample": f"every {plain_fraction}th instance ({len(sub)} solves)",
+        "fp32_rel_fe_err": {"median": float(np.median(rel32)), "max": float(rel32.max())},
+        "fp64_batched_rel_fe_err_max": float(rel64.max()),
+        "nonfinite": int((b32.nonfinite != 0).sum()),
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -124,13 +155,15 @@ def main():
     ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--no-sweep", action="store_true", help="skip the batched-sweep leg")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py: this build measures one GPU; --gpus above 1 is not supported "
-              "(instances shard trivially, one per GPU; see DESIGN.md)", file=sys.stderr)
+              "(independent instances would shard one scheduler per GPU)", file=sys.stderr)
         sys.exit(2)
     f64 = run_gpu(64, a.steps, a.warmup)
     f32 = run_gpu(32, a.steps, a.warmup)
+    sweep = None if a.no_sweep else run_sweep()
     cpu = None if a.no_cpu else run_cpu(a.cpu_seconds)
     line = {
         "metric": "Mpts*steps/s at 1024^2 (stdPFC ETDRK2 step, FP64; FP32 in 'fp32')",
@@ -155,6 +188,7 @@ def main():
                  "kernels_us": f32["kernels_us"],
                  "rel_fe_vs_fp64": abs(f32["free_energy"] - f64["free_energy"]) / abs(f64["free_energy"])},
         "cpu_baseline": cpu,
+        "sweep": sweep,
     }
     print(json.dumps(line), flush=True)
 
