@@ -434,7 +434,7 @@ void engine_init(Engine& e, const pfc_params* params, int cap, bool batched) {
     }
   }
   const size_t bytes = (size_t)cap * e.nc * e.csize;
-  const size_t tbytes = (size_t)cap * e.nc * (e.f64() ? sizeof(double) : sizeof(float));
+  const size_t tbytes = (size_t)cap * e.nc * sizeof(double);  // ETD tables are FP64 in every precision
   e.S.alloc(bytes);
   e.Tb.alloc(bytes);
   if (params->scheme == PFC_SCHEME_ETDRK2) e.D.alloc(bytes);
@@ -484,14 +484,9 @@ void engine_tables(Engine& e, int slot0, int n, hipStream_t st) {
   TabParams tp{e.p.Gamma, e.p.beta, e.p.q0, 0.0, 0.0, e.p.dt, e.p.smallZTol,
                1.0 / e.p.Ny, 1.0 / ((double)e.p.Nx * (double)e.p.Ny), e.Nc, e.p.Ny};
   const int blocks = (int)std::min<size_t>((e.nc + 255) / 256, 1024);
-  if (e.f64())
-    hipLaunchKernelGGL(k_tables<double>, dim3(blocks, n), dim3(256), 0, st, (double*)e.E1.p, (double*)e.Q1.p,
-                       (double*)e.Q2.p, (long long)e.nc, (const double*)e.kx2.p, (const double*)e.ky2.p, tp,
-                       (const double*)e.eps.p, (const double*)e.Ms.p, slot0);
-  else
-    hipLaunchKernelGGL(k_tables<float>, dim3(blocks, n), dim3(256), 0, st, (float*)e.E1.p, (float*)e.Q1.p,
-                       (float*)e.Q2.p, (long long)e.nc, (const double*)e.kx2.p, (const double*)e.ky2.p, tp,
-                       (const double*)e.eps.p, (const double*)e.Ms.p, slot0);
+  hipLaunchKernelGGL(k_tables<double>, dim3(blocks, n), dim3(256), 0, st, (double*)e.E1.p, (double*)e.Q1.p,
+                     (double*)e.Q2.p, (long long)e.nc, (const double*)e.kx2.p, (const double*)e.ky2.p, tp,
+                     (const double*)e.eps.p, (const double*)e.Ms.p, slot0);
   hip_check(hipGetLastError(), "table launch");
 }
 
@@ -1131,7 +1126,7 @@ int pfc_slot_bytes(const pfc_params* params, long long* bytes) {
     validate(params);
     if (!bytes) fail(PFC_EINVAL, "NULL bytes");
     const size_t nc = (size_t)(params->Nx / 2 + 1) * params->Ny;
-    const size_t cs = params->precision == 64 ? 16 : 8, ts = cs / 2;
+    const size_t cs = params->precision == 64 ? 16 : 8, ts = sizeof(double);
     const bool rk2 = params->scheme == PFC_SCHEME_ETDRK2;
     *bytes = (long long)(nc * cs * (rk2 ? 3 : 2) + nc * ts * (rk2 ? 3 : 2));
   });

@@ -279,9 +279,11 @@ __global__ void __launch_bounds__(256) k_col_fast(V2<T>* __restrict__ Sg, const 
   const int col = c0 + wl;
   const bool live = wl < ncols;
   V* line = lds + wl * LD;
-  const T* E1 = reinterpret_cast<const T*>(tb.E1) + (size_t)slot * sl.tstride;
-  const T* Q1 = reinterpret_cast<const T*>(tb.Q1) + (size_t)slot * sl.tstride;
-  const T* Q2 = reinterpret_cast<const T*>(tb.Q2) + (size_t)slot * sl.tstride;
+  // ETD tables are FP64 in every precision; the combine runs in FP64 and is
+  // rounded once to the bulk type
+  const double* E1 = reinterpret_cast<const double*>(tb.E1) + (size_t)slot * sl.tstride;
+  const double* Q1 = reinterpret_cast<const double*>(tb.Q1) + (size_t)slot * sl.tstride;
+  const double* Q2 = reinterpret_cast<const double*>(tb.Q2) + (size_t)slot * sl.tstride;
 
   // Tile I/O: thread t moves column tc = t % C of rows ty + TPL*q, so
   // consecutive threads touch consecutive columns of a row (coalesced).
@@ -323,7 +325,7 @@ __global__ void __launch_bounds__(256) k_col_fast(V2<T>* __restrict__ Sg, const 
   // arrive while the second forward transform runs; dead columns of a
   // partial tile read column c0 (in bounds, unused)
   const size_t tbase = (size_t)(live ? col : c0) * NY;
-  T e1r[P], q1r[P], q2r[P];
+  double e1r[P], q1r[P], q2r[P];
   V dr[P];
 #pragma unroll
   for (int qq = 0; qq < P; ++qq) {
@@ -347,16 +349,18 @@ __global__ void __launch_bounds__(256) k_col_fast(V2<T>* __restrict__ Sg, const 
 #pragma unroll
     for (int qq = 0; qq < P; ++qq) {
       const int ky = lane + TPL * qq;
-      V out;
+      const double bx = b[qq].x, by = b[qq].y;
+      double ox, oy;
       if constexpr (MODE == COL_RK2_STAGE2) {
-        out = V{dr[qq].x + q2r[qq] * b[qq].x, dr[qq].y + q2r[qq] * b[qq].y};
+        ox = double(dr[qq].x) + q2r[qq] * bx;
+        oy = double(dr[qq].y) + q2r[qq] * by;
       } else {
-        out = V{e1r[qq] * a[qq].x + q1r[qq] * b[qq].x, e1r[qq] * a[qq].y + q1r[qq] * b[qq].y};
-        if constexpr (MODE == COL_RK2_STAGE1)
-          D[tbase + ky] = V{out.x - q2r[qq] * b[qq].x, out.y - q2r[qq] * b[qq].y};
+        ox = e1r[qq] * double(a[qq].x) + q1r[qq] * bx;
+        oy = e1r[qq] * double(a[qq].y) + q1r[qq] * by;
+        if constexpr (MODE == COL_RK2_STAGE1) D[tbase + ky] = V{T(ox - q2r[qq] * bx), T(oy - q2r[qq] * by)};
       }
-      if (pinp && col == 0 && ky == 0) out = V{T(pin), T(0)};
-      b[qq] = out;
+      if (pinp && col == 0 && ky == 0) ox = pin, oy = 0.0;
+      b[qq] = V{T(ox), T(oy)};
     }
   }
   reg_fft<T, S, true>(b, line, lane, tw, 1);

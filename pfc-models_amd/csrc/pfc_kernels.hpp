@@ -179,7 +179,7 @@ __global__ void __launch_bounds__(256) k_row_nonlin(const V2<T>* __restrict__ S,
 enum { COL_ETD1 = 0, COL_RK2_STAGE1 = 1, COL_RK2_STAGE2 = 2 };
 
 struct ColTables {
-  const void* E1;  // column-major [Nc][Ny] per slot, storage type
+  const void* E1;  // column-major [Nc][Ny] per slot, FP64 in every precision
   const void* Q1;
   const void* Q2;
 };
@@ -222,9 +222,9 @@ __global__ void __launch_bounds__(256) k_col_etd(V2<T>* __restrict__ S, const V2
   if (D) D += (size_t)slot * sl.sstride;
   const int ncols = min(g.nc, g.Nc - c0);
   const int nel = ncols * g.Ny;
-  const T* E1 = reinterpret_cast<const T*>(tb.E1) + (size_t)slot * sl.tstride;
-  const T* Q1 = reinterpret_cast<const T*>(tb.Q1) + (size_t)slot * sl.tstride;
-  const T* Q2 = reinterpret_cast<const T*>(tb.Q2) + (size_t)slot * sl.tstride;
+  const double* E1 = reinterpret_cast<const double*>(tb.E1) + (size_t)slot * sl.tstride;
+  const double* Q1 = reinterpret_cast<const double*>(tb.Q1) + (size_t)slot * sl.tstride;
+  const double* Q2 = reinterpret_cast<const double*>(tb.Q2) + (size_t)slot * sl.tstride;
 
   V a[kColRegs];
   if (MODE != COL_RK2_STAGE2) {
@@ -255,21 +255,24 @@ __global__ void __launch_bounds__(256) k_col_etd(V2<T>* __restrict__ S, const V2
       const int ky = e - line * g.Ny;
       const size_t ti = (size_t)(c0 + line) * g.Ny + ky;
       V b = B[line * g.ld + ky];
-      V out;
+      const double bx = b.x, by = b.y;
+      double ox, oy;
       if (MODE == COL_RK2_STAGE2) {
-        const T q2 = Q2[ti];
+        const double q2 = Q2[ti];
         V d = D[ti];
-        out = V{d.x + q2 * b.x, d.y + q2 * b.y};
+        ox = double(d.x) + q2 * bx;
+        oy = double(d.y) + q2 * by;
       } else {
-        const T e1 = E1[ti], q1 = Q1[ti];
-        out = V{e1 * a[i].x + q1 * b.x, e1 * a[i].y + q1 * b.y};
+        const double e1 = E1[ti], q1 = Q1[ti];
+        ox = e1 * double(a[i].x) + q1 * bx;
+        oy = e1 * double(a[i].y) + q1 * by;
         if (MODE == COL_RK2_STAGE1) {
-          const T q2 = Q2[ti];
-          D[ti] = V{out.x - q2 * b.x, out.y - q2 * b.y};
+          const double q2 = Q2[ti];
+          D[ti] = V{T(ox - q2 * bx), T(oy - q2 * by)};
         }
       }
-      if (pinp && c0 + line == 0 && ky == 0) out = V{T(pin), T(0)};
-      B[line * g.ld + ky] = out;
+      if (pinp && c0 + line == 0 && ky == 0) ox = pin, oy = 0.0;
+      B[line * g.ld + ky] = V{T(ox), T(oy)};
     }
   }
   __syncthreads();

@@ -127,6 +127,6 @@ def test_slot_bytes_and_mixed_sizes(hip_device):
     from pfc_hip.plan import Batch, slot_bytes
     per = slot_bytes(**_kw())
     nc = N * (N // 2 + 1)
-    assert per == nc * 16 * 3 + nc * 8 * 3
+    assert per == nc * 16 * 3 + nc * 8 * 3  # state (S, T, D) + FP64 tables (E1, Q1, Q2)
     b = Batch(3, **_kw())
     assert b.device_bytes() >= 3 * per

@@ -2,7 +2,7 @@
 
 Tolerances (stated per test):
   FFT round trips            rel L2 <= 1e-14 (FP64), 3e-6 (FP32)
-  FP64 step, 100 steps        max|d psi| <= 1e-10          (north-star check (a))
+  FP64 step, 100 steps        max|d psi| <= 1e-10
   recorded FEs (FP64)         rtol 1e-10                    (the reference's own tolerance)
   free energy / mean (FP64)   rel 1e-12 against the oracle on the same field
 """
@@ -49,7 +49,7 @@ def test_rfft2_fp32(hip_device):
     assert np.linalg.norm(back - x) / np.linalg.norm(x) < 3e-6
 
 
-def _config1(scheme="etd1"):
+def _std128(scheme="etd1"):
     geom = O.build_geometry(128, 128, 128 * np.pi / 4, 128 * np.pi / 4)
     p = O.Params(model="std", Gamma=1.0, beta=1.0, q0=1.0, epsilon=-0.25, g=0.0, dt=0.5, scheme=scheme)
     psi0 = F.uniform_noise((128, 128), -0.25, 0.05, seed=1)
@@ -59,7 +59,7 @@ def _config1(scheme="etd1"):
 @pytest.mark.parametrize("scheme", ["etd1", "etdrk2"])
 def test_fp64_step_matches_oracle_100_steps(hip_device, scheme):
     """North-star check (a): max|d psi| <= 1e-10 after 100 FP64 steps (128^2, r = psibar = -0.25, dt = 0.5)."""
-    geom, p, psi0 = _config1(scheme)
+    geom, p, psi0 = _std128(scheme)
     sim = O.Simulation(p, geom)
     sim.set_field(psi0)
     sim.step(100)
@@ -167,7 +167,7 @@ def test_fp32_step_tolerance(hip_device):
     """FP32 bulk against the FP64 oracle, 128^2 ETD1 x 100 (r = psibar = -0.25, dt = 0.5):
     max|d psi| <= 1e-5, relative FE <= 1e-6, mean held exactly by the k = 0 pin (1e-7)."""
     from pfc_hip.plan import Plan
-    geom, p, psi0 = _config1("etd1")
+    geom, p, psi0 = _std128("etd1")
     sim = O.Simulation(p, geom)
     sim.set_field(psi0)
     sim.step(100)
