@@ -156,6 +156,7 @@ RowGeom row_geom(int Nx, int Ny, size_t csize) {
   g.M = Nx / 2;
   g.Nc = Nx / 2 + 1;
   g.Ny = Ny;
+  g.lay = row_major(g.Nc);
   g.ld = g.Nc;
   if (!factor(g.M, g.plM)) fail(PFC_EUNSUPPORTED, "Nx/2 = " + std::to_string(g.M) + " is not 2,3,5,7,11,13-smooth");
   const size_t per_row = 2 * (size_t)g.ld * csize;
@@ -171,6 +172,7 @@ ColGeom col_geom(int Nx, int Ny, size_t csize) {
   ColGeom g{};
   g.Nc = Nx / 2 + 1;
   g.Ny = Ny;
+  g.lay = row_major(g.Nc);
   if (!factor(Ny, g.plY)) fail(PFC_EUNSUPPORTED, "Ny = " + std::to_string(Ny) + " is not 2,3,5,7,11,13-smooth");
   const int maxc = (kColRegs * 256) / Ny;
   if (maxc < 1) fail(PFC_EUNSUPPORTED, "Ny too large for the column kernel");
@@ -182,6 +184,15 @@ ColGeom col_geom(int Nx, int Ny, size_t csize) {
   g.nc = nc;
   g.ld = fits(nc, Ny + 1) ? Ny + 1 : Ny;
   return g;
+}
+
+// Storage order of an engine's state arrays: 128-byte tiles of 2 kx x 4 rows
+// (FP64) or 4 kx x 4 rows (FP32); row-major when Ny is not a multiple of 4.
+Lay state_layout(int Nx, int Ny, int precision) {
+  const int Nc = Nx / 2 + 1;
+  const int ltx = precision == 64 ? 1 : 2, lty = 2;
+  if (Ny % (1 << lty)) return row_major(Nc);
+  return Lay{ltx, lty, (Nc + (1 << ltx) - 1) >> ltx};
 }
 
 size_t row_lds(const RowGeom& g, size_t csize) { return 2 * (size_t)g.nl * g.ld * csize; }
@@ -211,7 +222,7 @@ struct FastShape {
   size_t lds = 0;
 };
 
-// Row kernel shape: LDS = twiddles (Nx) + lines * (M + 1).
+// Row kernel shape: LDS = lines * (M + 1), padded.
 FastShape row_shape(int Nx, int Ny, size_t csize, const SchedInfo& si, int lines) {
   FastShape f;
   const int M = Nx / 2;
@@ -220,12 +231,12 @@ FastShape row_shape(int Nx, int Ny, size_t csize, const SchedInfo& si, int lines
   f.tpl = M / si.P;
   f.lines = lines;
   f.blocks = (Ny + lines - 1) / lines;
-  f.lds = ((size_t)Nx + (size_t)lines * row_ld(M)) * csize;
+  f.lds = (size_t)lines * row_ld(M) * csize;
   f.on = f.tpl * lines <= 256 && f.tpl * lines >= 64 && f.lds <= (size_t)lds_limit();
   return f;
 }
 
-// Column kernel shape: LDS = twiddles (Ny) + lines * Ny.
+// Column kernel shape: LDS = lines * Ny, padded.
 FastShape col_shape(int Nx, int Ny, size_t csize, const SchedInfo& si, int lines) {
   FastShape f;
   const int Nc = Nx / 2 + 1;
@@ -234,7 +245,7 @@ FastShape col_shape(int Nx, int Ny, size_t csize, const SchedInfo& si, int lines
   f.tpl = Ny / si.P;
   f.lines = lines;
   f.blocks = (Nc + lines - 1) / lines;
-  f.lds = ((size_t)Ny + (size_t)lines * col_ld(Ny)) * csize;
+  f.lds = (size_t)lines * col_ld(Ny) * csize;
   f.on = f.tpl * lines <= 256 && f.tpl * lines >= 64 && f.lds <= (size_t)lds_limit();
   return f;
 }
@@ -278,31 +289,31 @@ FastShape pick_default(const std::vector<FastShape>& cands, int target_blocks) {
 template <typename T> struct FastLaunch;
 template <> struct FastLaunch<double> {
   static void row(int M, const FastShape& f, int ny_inst, hipStream_t st, const V2<double>* S, V2<double>* T, int Ny,
-                  const Nonlin& q, const V2<double>* tw, const Slots& sl) {
-    if (!row_fast_launch_f64(M, f.var, dim3(f.blocks, ny_inst), dim3(f.lines * f.tpl), f.lds, st, S, T, Ny, q, tw,
-                             sl))
+                  const Lay& lay, const Nonlin& q, const V2<double>* tw, const Slots& sl) {
+    if (!row_fast_launch_f64(M, f.var, dim3(f.blocks, ny_inst), dim3(f.lines * f.tpl), f.lds, st, S, T, Ny, lay, q,
+                             tw, sl))
       fail(PFC_EUNSUPPORTED, "no fast row schedule");
   }
   static void col(int Ny, int mode, const FastShape& f, int ny_inst, hipStream_t st, V2<double>* S,
-                  const V2<double>* T, V2<double>* D, int Nc, const ColTables& tb, const V2<double>* tw,
+                  const V2<double>* T, V2<double>* D, int Nc, const Lay& lay, const ColTables& tb, const V2<double>* tw,
                   const Slots& sl) {
     if (!col_fast_launch_f64(Ny, f.var, mode, dim3(f.blocks, ny_inst), dim3(f.lines * f.tpl), f.lds, st, S, T, D,
-                             Nc, tb, tw, sl))
+                             Nc, lay, tb, tw, sl))
       fail(PFC_EUNSUPPORTED, "no fast column schedule");
   }
 };
 template <> struct FastLaunch<float> {
   static void row(int M, const FastShape& f, int ny_inst, hipStream_t st, const V2<float>* S, V2<float>* T, int Ny,
-                  const Nonlin& q, const V2<float>* tw, const Slots& sl) {
-    if (!row_fast_launch_f32(M, f.var, dim3(f.blocks, ny_inst), dim3(f.lines * f.tpl), f.lds, st, S, T, Ny, q, tw,
-                             sl))
+                  const Lay& lay, const Nonlin& q, const V2<float>* tw, const Slots& sl) {
+    if (!row_fast_launch_f32(M, f.var, dim3(f.blocks, ny_inst), dim3(f.lines * f.tpl), f.lds, st, S, T, Ny, lay, q,
+                             tw, sl))
       fail(PFC_EUNSUPPORTED, "no fast row schedule");
   }
   static void col(int Ny, int mode, const FastShape& f, int ny_inst, hipStream_t st, V2<float>* S,
-                  const V2<float>* T, V2<float>* D, int Nc, const ColTables& tb, const V2<float>* tw,
+                  const V2<float>* T, V2<float>* D, int Nc, const Lay& lay, const ColTables& tb, const V2<float>* tw,
                   const Slots& sl) {
     if (!col_fast_launch_f32(Ny, f.var, mode, dim3(f.blocks, ny_inst), dim3(f.lines * f.tpl), f.lds, st, S, T, D,
-                             Nc, tb, tw, sl))
+                             Nc, lay, tb, tw, sl))
       fail(PFC_EUNSUPPORTED, "no fast column schedule");
   }
 };
@@ -365,6 +376,7 @@ struct Engine {
   pfc_params p{};
   int cap = 0, Nc = 0;
   size_t csize = 0, nc = 0;  // bytes per complex; complex elements per slot array
+  Lay lay{};  // storage order of S~, T (and the diagnostic scratch)
   RowGeom rg{}, rg64{};
   ColGeom cg{}, cg64{};
   FastShape frow{}, fcol{};
@@ -406,12 +418,14 @@ void engine_init(Engine& e, const pfc_params* params, int cap, bool batched) {
   e.batched = batched;
   const int Nx = params->Nx, Ny = params->Ny;
   e.Nc = Nx / 2 + 1;
-  e.nc = (size_t)e.Nc * Ny;
+  e.lay = state_layout(Nx, Ny, params->precision);
+  e.nc = e.lay.size(Ny);  // >= Nc * Ny, the column-major tables' extent
   e.csize = e.f64() ? sizeof(V2<double>) : sizeof(V2<float>);
   e.rg = row_geom(Nx, Ny, e.csize);
   e.cg = col_geom(Nx, Ny, e.csize);
   e.rg64 = row_geom(Nx, Ny, sizeof(V2<double>));
   e.cg64 = col_geom(Nx, Ny, sizeof(V2<double>));
+  e.rg.lay = e.cg.lay = e.rg64.lay = e.cg64.lay = e.lay;
   if (!std::getenv("PFC_HIP_GENERIC")) {
     auto rc = row_candidates(Nx, Ny, e.csize);
     auto cc = col_candidates(Nx, Ny, e.csize);
@@ -510,7 +524,7 @@ struct Steps {
     V* Tb = (V*)e.Tb.p;
     const Nonlin q = nonlin_of(e);
     if (e.frow.on)
-      FastLaunch<T>::row(e.p.Nx / 2, e.frow, ny, st, S, Tb, e.p.Ny, q, (const V*)e.twN.p, sl);
+      FastLaunch<T>::row(e.p.Nx / 2, e.frow, ny, st, S, Tb, e.p.Ny, e.lay, q, (const V*)e.twN.p, sl);
     else
       hipLaunchKernelGGL((k_row_nonlin<T>), dim3(row_blocks(e.rg), ny), dim3(256), row_lds(e.rg, sizeof(V)), st, S,
                          Tb, e.rg, q, (const V*)e.twN.p, sl);
@@ -521,7 +535,7 @@ struct Steps {
     V* D = (V*)e.D.p;
     const V* Tb = (const V*)e.Tb.p;
     if (e.fcol.on) {
-      FastLaunch<T>::col(e.p.Ny, mode, e.fcol, ny, st, S, Tb, D, e.Nc, tb, (const V*)e.twY.p, sl);
+      FastLaunch<T>::col(e.p.Ny, mode, e.fcol, ny, st, S, Tb, D, e.Nc, e.lay, tb, (const V*)e.twY.p, sl);
       return;
     }
     const dim3 g(col_blocks(e.cg), ny);
@@ -622,7 +636,7 @@ void autotune(Engine& e, hipStream_t st) {
       FastShape pick = e.frow;
       for (const FastShape& f : cands) {
         float ms = time_launches(st, reps, [&] {
-          FastLaunch<T>::row(M, f, ny, st, (const V*)e.S.p, (V*)T2.p, Ny, q, (const V*)e.twN.p, sl);
+          FastLaunch<T>::row(M, f, ny, st, (const V*)e.S.p, (V*)T2.p, Ny, e.lay, q, (const V*)e.twN.p, sl);
         });
         if (ms < best) best = ms, pick = f;
       }
@@ -649,7 +663,8 @@ void autotune(Engine& e, hipStream_t st) {
       FastShape pick = e.fcol;
       for (const FastShape& f : cands) {
         float ms = time_launches(st, reps, [&] {
-          FastLaunch<T>::col(Ny, mode, f, ny, st, (V*)S2.p, (const V*)T2.p, (V*)D2.p, Nc, tb, (const V*)e.twY.p, sl);
+          FastLaunch<T>::col(Ny, mode, f, ny, st, (V*)S2.p, (const V*)T2.p, (V*)D2.p, Nc, e.lay, tb,
+                             (const V*)e.twY.p, sl);
         });
         if (ms < best) best = ms, pick = f;
       }
@@ -1125,7 +1140,7 @@ int pfc_slot_bytes(const pfc_params* params, long long* bytes) {
   return guarded([&] {
     validate(params);
     if (!bytes) fail(PFC_EINVAL, "NULL bytes");
-    const size_t nc = (size_t)(params->Nx / 2 + 1) * params->Ny;
+    const size_t nc = state_layout(params->Nx, params->Ny, params->precision).size(params->Ny);
     const size_t cs = params->precision == 64 ? 16 : 8, ts = sizeof(double);
     const bool rk2 = params->scheme == PFC_SCHEME_ETDRK2;
     *bytes = (long long)(nc * cs * (rk2 ? 3 : 2) + nc * ts * (rk2 ? 3 : 2));

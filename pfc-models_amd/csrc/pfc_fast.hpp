@@ -89,9 +89,10 @@ template <int L_, int P_, int... Rs> struct Sch {
   }
 };
 
-// ordering of LDS traffic between the lanes of one line
-template <class S> __device__ __forceinline__ void line_sync() {
-  if constexpr (S::WAVE) {
+// Ordering of LDS traffic between the lanes of one line: a wave-level fence
+// when the line's lanes share one wavefront, else a block barrier.
+template <bool BLOCK> __device__ __forceinline__ void line_sync() {
+  if constexpr (!BLOCK) {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
@@ -108,8 +109,36 @@ __host__ __device__ constexpr int lpad(int i) { return i + (i >> 4); }
 __host__ __device__ constexpr int row_ld(int M) { return lpad(M) + 2; }
 __host__ __device__ constexpr int col_ld(int Ny) { return lpad(Ny - 1) + 2; }
 
+// Twiddles of the lane's butterflies in passes 1..NP-1, held in registers:
+// pass PI contributes (P / R) * (R - 1) roots w_L^(k r L / (NS R)), k = j % NS.
+// The same roots serve the forward and (conjugated) inverse transforms.
+template <class S> __host__ __device__ constexpr int tw_offset(int pi) {
+  int n = 0;
+  for (int p = 1; p < pi; ++p) n += (S::P / S::R[p]) * (S::R[p] - 1);
+  return n;
+}
+template <class S> __host__ __device__ constexpr int tw_count() { return tw_offset<S>(S::NP) > 0 ? tw_offset<S>(S::NP) : 1; }
+
+// Load the lane's roots from a table exp(-2 pi i m / (L * tws)).
+template <typename T, class S, int PI = 1>
+__device__ __forceinline__ void tw_load(V2<T>* w, int lane, const V2<T>* __restrict__ tw, int tws) {
+  if constexpr (PI < S::NP) {
+    constexpr int R = S::R[PI];
+    constexpr int NS = S::ns(PI);
+    constexpr int NB = S::P / R;
+    constexpr int OFF = tw_offset<S>(PI);
+#pragma unroll
+    for (int i = 0; i < NB; ++i) {
+      const int k = (lane + S::TPL * i) % NS;
+#pragma unroll
+      for (int r = 1; r < R; ++r) w[OFF + i * (R - 1) + r - 1] = tw[(k * r * (S::L / (NS * R))) * tws];
+    }
+    tw_load<T, S, PI + 1>(w, lane, tw, tws);
+  }
+}
+
 template <typename T, class S, int PI, bool INV>
-__device__ __forceinline__ void reg_pass(V2<T>* v, int lane, const V2<T>* tw, int tws) {
+__device__ __forceinline__ void reg_pass(V2<T>* v, const V2<T>* w) {
   using V = V2<T>;
   constexpr int R = S::R[PI];
   constexpr int NS = S::ns(PI);
@@ -120,16 +149,15 @@ __device__ __forceinline__ void reg_pass(V2<T>* v, int lane, const V2<T>* tw, in
 #pragma unroll
     for (int r = 0; r < R; ++r) x[r] = v[i + r * NB];
     if constexpr (NS > 1) {
-      const int j = lane + S::TPL * i;
-      const int k = j % NS;
+      constexpr int OFF = tw_offset<S>(PI);
 #pragma unroll
       for (int r = 1; r < R; ++r) {
-        V t = tw[(k * r * (S::L / (NS * R))) * tws];
+        V t = w[OFF + i * (R - 1) + r - 1];
         if (INV) t.y = -t.y;
         x[r] = cmul(x[r], t);
       }
     }
-    Dft<T, R, INV>::run(x, tw, (S::L / R) * tws);
+    Dft<T, R, INV>::run(x, nullptr, 0);
 #pragma unroll
     for (int r = 0; r < R; ++r) v[i + r * NB] = x[r];
   }
@@ -157,53 +185,64 @@ __device__ __forceinline__ void reg_gather(V2<T>* v, const V2<T>* line, int lane
 }
 
 // Full FFT of the lane's strided registers, exchanging through `line` (S::L
-// LDS elements owned by this line).  Every lane of the block must call it.
-template <typename T, class S, bool INV, int PI = 0>
-__device__ __forceinline__ void reg_fft(V2<T>* v, V2<T>* line, int lane, const V2<T>* tw, int tws) {
-  reg_pass<T, S, PI, INV>(v, lane, tw, tws);
+// LDS elements owned by this line); `w` holds the lane's roots (tw_load).
+// BLOCK: the line's lanes span several wavefronts.  Every lane of the block
+// must call it.
+template <typename T, class S, bool INV, bool BLOCK, int PI = 0>
+__device__ __forceinline__ void reg_fft(V2<T>* v, V2<T>* line, int lane, const V2<T>* w) {
+  reg_pass<T, S, PI, INV>(v, w);
   if constexpr (PI + 1 < S::NP) {
-    line_sync<S>();  // earlier readers of `line` are done
+    line_sync<BLOCK>();  // earlier readers of `line` are done
     reg_scatter<T, S, PI>(v, line, lane);
-    line_sync<S>();
+    line_sync<BLOCK>();
     reg_gather<T, S>(v, line, lane);
-    reg_fft<T, S, INV, PI + 1>(v, line, lane, tw, tws);
+    reg_fft<T, S, INV, BLOCK, PI + 1>(v, line, lane, w);
   }
 }
 
 // ---------------------------------------------------------------------------
 // fast row kernel: S~ -> C2R -> N(n) -> R2C -> T; W rows per block
-// LDS: twiddles exp(-2 pi i m / Nx), m < Nx, then W lines of M + 1.
+// LDS: W lines of M + 1 (padded).  Roots come from the table
+// exp(-2 pi i m / Nx), m < Nx, straight into registers.
 // ---------------------------------------------------------------------------
 template <typename T, class S>
 __global__ void __launch_bounds__(256) k_row_fast(const V2<T>* __restrict__ Sg, V2<T>* __restrict__ Tout, int Ny,
-                                                  Nonlin q, const V2<T>* __restrict__ twN, Slots sl) {
+                                                  Lay lay, Nonlin q, const V2<T>* __restrict__ twN, Slots sl) {
   using V = V2<T>;
-  constexpr int M = S::L, P = S::P, TPL = S::TPL, Nc = M + 1, LD = row_ld(M);
+  constexpr int M = S::L, P = S::P, TPL = S::TPL, LD = row_ld(M);
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int slot = slot_of(sl, blockIdx.y);
   Sg += (size_t)slot * sl.sstride;
   Tout += (size_t)slot * sl.sstride;
   if (sl.M) q.M = sl.M[slot];
-  V* tw = reinterpret_cast<V*>(smem_raw);
   const int wl = threadIdx.x / TPL;
   const int lane = threadIdx.x - wl * TPL;
   const int row = blockIdx.x * (blockDim.x / TPL) + wl;
   const bool live = row < Ny;
-  V* line = tw + 2 * M + wl * LD;
+  V* line = reinterpret_cast<V*>(smem_raw) + wl * LD;
 
+  // Loads are unconditional (rows past Ny read row Ny - 1; results are not
+  // stored): a load under a branch makes the join wait for it, which would
+  // serialise it with the root loads below.
+  const int rowc = live ? row : Ny - 1;
   V x[P];
-  V xM = V{T(0), T(0)};
-  if (live) {
-    const V* src = Sg + (size_t)row * Nc;
 #pragma unroll
-    for (int qq = 0; qq < P; ++qq) x[qq] = src[lane + TPL * qq];
-    if (lane == 0) xM = src[M];
-  }
-  for (int i = threadIdx.x; i < 2 * M; i += blockDim.x) tw[i] = twN[i];
+  for (int qq = 0; qq < P; ++qq) x[qq] = Sg[lay.at(rowc, lane + TPL * qq)];
+  const V xM = Sg[lay.at(rowc, M)];
+  // roots: w_Nx^k for the (un)packing of k = lane + TPL q, and the FFT roots
+  V wk[P];
+#pragma unroll
+  for (int qq = 0; qq < P; ++qq) wk[qq] = twN[lane + TPL * qq];
+  const V wM = twN[M];
+  V wt[tw_count<S>()];
+  tw_load<T, S>(wt, lane, twN, 2);
+  // keep every load above in flight together: no arithmetic is scheduled
+  // ahead of the last one
+  __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
   for (int qq = 0; qq < P; ++qq) line[lpad(lane + TPL * qq)] = x[qq];
-  if (lane == 0) line[lpad(M)] = xM;
-  __syncthreads();
+  line[lpad(M)] = xM;  // every lane stores the same value: no branch for the load to sink into
+  line_sync<!S::WAVE>();
   V v[P];
 #pragma unroll
   for (int qq = 0; qq < P; ++qq) {
@@ -216,11 +255,10 @@ __global__ void __launch_bounds__(256) k_row_fast(const V2<T>* __restrict__ Sg, 
     }
     V s = V{xk.x + xm.x, xk.y - xm.y};
     V d = V{xk.x - xm.x, xk.y + xm.y};
-    V w = tw[k];
-    V wd = cmul(cconj(w), d);
+    V wd = cmul(cconj(wk[qq]), d);
     v[qq] = V{s.x - wd.y, s.y + wd.x};
   }
-  reg_fft<T, S, true>(v, line, lane, tw, 2);
+  reg_fft<T, S, true, !S::WAVE>(v, line, lane, wt);
   bool bad = false;
 #pragma unroll
   for (int qq = 0; qq < P; ++qq) {
@@ -228,43 +266,46 @@ __global__ void __launch_bounds__(256) k_row_fast(const V2<T>* __restrict__ Sg, 
     v[qq] = V{nonlin_mu<T>(v[qq].x, q), nonlin_mu<T>(v[qq].y, q)};
   }
   if (bad && live && sl.status) atomicOr(sl.status + slot, 1);
-  reg_fft<T, S, false>(v, line, lane, tw, 2);
-  line_sync<S>();
+  reg_fft<T, S, false, !S::WAVE>(v, line, lane, wt);
+  line_sync<!S::WAVE>();
 #pragma unroll
   for (int qq = 0; qq < P; ++qq) line[lpad(lane + TPL * qq)] = v[qq];
-  line_sync<S>();
+  line_sync<!S::WAVE>();
   if (live) {
-    V* dst = Tout + (size_t)row * Nc;
 #pragma unroll
     for (int qq = 0; qq < P; ++qq) {
       // unpack Y[k] from Z[k] (in registers) and Z[M-k] (from LDS)
       const int k = lane + TPL * qq;
       const V zk = v[qq];
       const V zm = line[lpad(k == 0 ? 0 : M - k)];
-      dst[k] = row_unpack_pair<T>(zk, zm, tw[k]);
+      Tout[lay.at(row, k)] = row_unpack_pair<T>(zk, zm, wk[qq]);
     }
-    if (lane == 0) dst[M] = row_unpack_pair<T>(line[0], line[0], tw[M]);
+    if (lane == 0) Tout[lay.at(row, M)] = row_unpack_pair<T>(line[0], line[0], wM);
   }
 }
 
 // ---------------------------------------------------------------------------
 // fast column kernel: forward column FFTs of S~ and T, ETD combine in
 // registers, inverse column FFT, S~ written in place.  C columns per block.
-// LDS: twiddles exp(-2 pi i m / Ny), then C lines of Ny.
+// LDS: C lines of Ny (padded); roots from exp(-2 pi i m / Ny) in registers.
 // ---------------------------------------------------------------------------
 template <typename T, class S, int MODE>
 __global__ void __launch_bounds__(256) k_col_fast(V2<T>* __restrict__ Sg, const V2<T>* __restrict__ Tin,
-                                                  V2<T>* __restrict__ D, int Nc, ColTables tb,
+                                                  V2<T>* __restrict__ D, int Nc, Lay lay, ColTables tb,
                                                   const V2<T>* __restrict__ twY, Slots sl) {
   using V = V2<T>;
   constexpr int NY = S::L, P = S::P, TPL = S::TPL, LD = col_ld(NY);
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  V* tw = reinterpret_cast<V*>(smem_raw);
-  V* lds = tw + NY;
+  V* lds = reinterpret_cast<V*>(smem_raw);
+  // Thread t is transform lane t / C of column t % C of the tile, holding rows
+  // lane + TPL q.  The lanes of one load or store then cover C columns of
+  // consecutive rows (whole 128-byte lines when C is a multiple of the
+  // layout's tile width), and the loaded registers are already in the
+  // transforms' strided order: no LDS transpose on the way in or out.
   const int C = blockDim.x / TPL;  // columns per block, a power of two
   const int lgC = __ffs(C) - 1;
-  const int wl = threadIdx.x / TPL;
-  const int lane = threadIdx.x - wl * TPL;
+  const int wl = threadIdx.x & (C - 1);
+  const int lane = threadIdx.x >> lgC;
   // XCD-aware order over (instance, tile): consecutive tiles of an instance
   // land on the same XCD
   const int ntile = gridDim.x;
@@ -276,8 +317,10 @@ __global__ void __launch_bounds__(256) k_col_fast(V2<T>* __restrict__ Sg, const 
   Tin += (size_t)slot * sl.sstride;
   if (D) D += (size_t)slot * sl.sstride;
   const int ncols = min(C, Nc - c0);
-  const int col = c0 + wl;
   const bool live = wl < ncols;
+  // lanes past the last column of a partial tile shadow that column (in
+  // bounds, results discarded) so loads and barriers stay uniform
+  const int col = c0 + (live ? wl : ncols - 1);
   V* line = lds + wl * LD;
   // ETD tables are FP64 in every precision; the combine runs in FP64 and is
   // rounded once to the bulk type
@@ -285,46 +328,21 @@ __global__ void __launch_bounds__(256) k_col_fast(V2<T>* __restrict__ Sg, const 
   const double* Q1 = reinterpret_cast<const double*>(tb.Q1) + (size_t)slot * sl.tstride;
   const double* Q2 = reinterpret_cast<const double*>(tb.Q2) + (size_t)slot * sl.tstride;
 
-  // Tile I/O: thread t moves column tc = t % C of rows ty + TPL*q, so
-  // consecutive threads touch consecutive columns of a row (coalesced).
-  const int tc = threadIdx.x & (C - 1);
-  const int ty = threadIdx.x >> lgC;
-  const bool tlive = tc < ncols;
-  // threads past the last column of a partial tile read the last column again
-  // (in bounds, discarded) so the loads stay branch-free
-  const int tcc = tlive ? tc : ncols - 1;
-  V ra[P], rb[P];
-  {
-    const V* ps = Sg + c0 + tcc;
-    const V* pt = Tin + c0 + tcc;
+  V a[P], b[P];
 #pragma unroll
-    for (int qq = 0; qq < P; ++qq) {
-      const size_t off = (size_t)(ty + TPL * qq) * Nc;
-      if constexpr (MODE != COL_RK2_STAGE2) ra[qq] = ps[off];
-      rb[qq] = pt[off];  // in flight while the first transform runs
-    }
+  for (int qq = 0; qq < P; ++qq) {
+    const size_t off = lay.at(lane + TPL * qq, col);
+    if constexpr (MODE != COL_RK2_STAGE2) a[qq] = Sg[off];
+    b[qq] = Tin[off];  // in flight while the first transform runs
   }
-  for (int i = threadIdx.x; i < NY; i += blockDim.x) tw[i] = twY[i];
-
-  V a[P];
-  if constexpr (MODE != COL_RK2_STAGE2) {
-#pragma unroll
-    for (int qq = 0; qq < P; ++qq) lds[tcc * LD + lpad(ty + TPL * qq)] = ra[qq];
-    __syncthreads();
-    reg_gather<T, S>(a, line, lane);
-    reg_fft<T, S, false>(a, line, lane, tw, 1);
-    __syncthreads();
-  } else {
-    __syncthreads();  // twiddles visible
-  }
-  // (threads past a partial tile write the same value as the owner of tcc)
-#pragma unroll
-  for (int qq = 0; qq < P; ++qq) lds[tcc * LD + lpad(ty + TPL * qq)] = rb[qq];
-  __syncthreads();
+  V wt[tw_count<S>()];
+  tw_load<T, S>(wt, lane, twY, 1);
+  // every load above is issued before the first transform starts
+  __builtin_amdgcn_sched_barrier(0);
+  if constexpr (MODE != COL_RK2_STAGE2) reg_fft<T, S, false, true>(a, line, lane, wt);
   // ETD tables (and the stage-2 carry) for this lane's ky, loaded now so they
-  // arrive while the second forward transform runs; dead columns of a
-  // partial tile read column c0 (in bounds, unused)
-  const size_t tbase = (size_t)(live ? col : c0) * NY;
+  // arrive while the second forward transform runs
+  const size_t tbase = (size_t)col * NY;
   double e1r[P], q1r[P], q2r[P];
   V dr[P];
 #pragma unroll
@@ -341,9 +359,8 @@ __global__ void __launch_bounds__(256) k_col_fast(V2<T>* __restrict__ Sg, const 
   }
   const double* pinp = sl.pin ? sl.pin + slot : nullptr;
   const double pin = pinp ? *pinp : 0.0;
-  V b[P];
-  reg_gather<T, S>(b, line, lane);
-  reg_fft<T, S, false>(b, line, lane, tw, 1);
+  __builtin_amdgcn_sched_barrier(0);  // table loads issued before the second transform
+  reg_fft<T, S, false, true>(b, line, lane, wt);
 
   if (live) {
 #pragma unroll
@@ -363,23 +380,15 @@ __global__ void __launch_bounds__(256) k_col_fast(V2<T>* __restrict__ Sg, const 
       b[qq] = V{T(ox), T(oy)};
     }
   }
-  reg_fft<T, S, true>(b, line, lane, tw, 1);
-  line_sync<S>();
-  // real projection of the kx = 0 and kx = Nx/2 columns (see k_col_etd)
-  const bool realcol = (col == 0 || col == Nc - 1);
-#pragma unroll
-  for (int qq = 0; qq < P; ++qq) {
-    V xv = b[qq];
-    if (realcol) xv.y = T(0);
-    line[lpad(lane + TPL * qq)] = xv;
-  }
-  __syncthreads();
-  if (tlive) {
-    V* po = Sg + c0 + tc;
+  reg_fft<T, S, true, true>(b, line, lane, wt);
+  if (live) {
+    // real projection of the kx = 0 and kx = Nx/2 columns (see k_col_etd)
+    const bool realcol = (col == 0 || col == Nc - 1);
 #pragma unroll
     for (int qq = 0; qq < P; ++qq) {
-      const int y = ty + TPL * qq;
-      po[(size_t)y * Nc] = lds[tc * LD + lpad(y)];
+      V xv = b[qq];
+      if (realcol) xv.y = T(0);
+      Sg[lay.at(lane + TPL * qq, col)] = xv;
     }
   }
 }
@@ -419,14 +428,16 @@ struct SchedInfo {
 // Launchers, instantiated in the pfc_inst_*.hip translation units.  Each
 // returns false if (L, variant) has no schedule.
 bool row_fast_launch_f64(int M, int var, dim3 grid, dim3 block, size_t lds, hipStream_t st, const V2<double>* S,
-                         V2<double>* T, int Ny, const Nonlin& q, const V2<double>* tw, const Slots& sl);
+                         V2<double>* T, int Ny, const Lay& lay, const Nonlin& q, const V2<double>* tw,
+                         const Slots& sl);
 bool row_fast_launch_f32(int M, int var, dim3 grid, dim3 block, size_t lds, hipStream_t st, const V2<float>* S,
-                         V2<float>* T, int Ny, const Nonlin& q, const V2<float>* tw, const Slots& sl);
+                         V2<float>* T, int Ny, const Lay& lay, const Nonlin& q, const V2<float>* tw,
+                         const Slots& sl);
 bool col_fast_launch_f64(int Ny, int var, int mode, dim3 grid, dim3 block, size_t lds, hipStream_t st, V2<double>* S,
-                         const V2<double>* T, V2<double>* D, int Nc, const ColTables& tb, const V2<double>* tw,
-                         const Slots& sl);
+                         const V2<double>* T, V2<double>* D, int Nc, const Lay& lay, const ColTables& tb,
+                         const V2<double>* tw, const Slots& sl);
 bool col_fast_launch_f32(int Ny, int var, int mode, dim3 grid, dim3 block, size_t lds, hipStream_t st, V2<float>* S,
-                         const V2<float>* T, V2<float>* D, int Nc, const ColTables& tb, const V2<float>* tw,
-                         const Slots& sl);
+                         const V2<float>* T, V2<float>* D, int Nc, const Lay& lay, const ColTables& tb,
+                         const V2<float>* tw, const Slots& sl);
 
 }  // namespace pfc

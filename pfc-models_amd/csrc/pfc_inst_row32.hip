@@ -9,10 +9,11 @@ PFC_SCHEDULES(PFC_X)
 #undef PFC_X
 
 bool row_fast_launch_f32(int M, int var, dim3 grid, dim3 block, size_t lds, hipStream_t st, const V2<float>* S,
-                         V2<float>* T, int Ny, const Nonlin& q, const V2<float>* tw, const Slots& sl) {
+                         V2<float>* T, int Ny, const Lay& lay, const Nonlin& q, const V2<float>* tw,
+                         const Slots& sl) {
 #define PFC_X(LEN, VAR, PP, ...)                                                                          \
   if (M == LEN && var == VAR) {                                                                           \
-    hipLaunchKernelGGL((k_row_fast<float, Sch<LEN, PP, __VA_ARGS__>>), grid, block, lds, st, S, T, Ny, q, tw, \
+    hipLaunchKernelGGL((k_row_fast<float, Sch<LEN, PP, __VA_ARGS__>>), grid, block, lds, st, S, T, Ny, lay, q, tw, \
                        sl);                                                                           \
     return true;                                                                                          \
   }

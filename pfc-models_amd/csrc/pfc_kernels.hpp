@@ -2,8 +2,9 @@
 //
 // State layout in HBM.  An instance of Ny x Nx real points is held as the
 // row spectrum S~[y][kx] = (1/Nx) * sum_x n[y][x] exp(-2 pi i kx x / Nx),
-// kx = 0..Nx/2 (row-major, Nc = Nx/2+1 complex per row).  Its column FFT is
-// n_hat/Nx, with n_hat = numpy.fft.rfft2(n) (ky in fftfreq order).
+// kx = 0..Nx/2 (Nc = Nx/2+1 complex per row), stored in the line-tiled order
+// of `Lay` below.  Its column FFT is n_hat/Nx, with n_hat =
+// numpy.fft.rfft2(n) (ky in fftfreq order).
 //
 // One ETD1 step = two kernels:
 //   row kernel : S~ row -> C2R (length-Nx/2 complex IFFT + pack) -> n
@@ -60,8 +61,28 @@ struct Slots {
 
 __device__ __forceinline__ int slot_of(const Slots& s, int y) { return s.list ? s.list[y] : y; }
 
+// Storage order of a (Ny, Nc) half spectrum.  A tile of TX consecutive kx of
+// TY consecutive rows fills one 128-byte line (TX * TY elements); the tiles of
+// one TY-row group follow each other in kx order, and the groups in y.  A
+// block of TY rows (row pass) or of TX columns (column pass) then moves whole
+// cache lines, which a row-major array only gives the row pass.  TX = TY = 1
+// is row-major with row pitch lpg.
+struct Lay {
+  int ltx, lty;  // log2 TX, log2 TY
+  int lpg;       // tiles per row group: ceil(Nc / TX)
+  __host__ __device__ size_t at(int y, int kx) const {
+    const size_t tile = (size_t)(y >> lty) * lpg + (kx >> ltx);
+    return (tile << (ltx + lty)) + ((size_t)(y & ((1 << lty) - 1)) << ltx) + (kx & ((1 << ltx) - 1));
+  }
+  // elements of one array of Ny rows (Ny a multiple of TY)
+  __host__ __device__ size_t size(int Ny) const { return (size_t)Ny * lpg << ltx; }
+};
+
+__host__ __device__ inline Lay row_major(int Nc) { return Lay{0, 0, Nc}; }
+
 struct RowGeom {
   int Nx, M, Nc, Ny;
+  Lay lay;  // storage of the half spectrum
   int nl;   // rows per block
   int ld;   // LDS line stride (complex elements), >= Nc
   Radices plM;
@@ -69,6 +90,7 @@ struct RowGeom {
 
 struct ColGeom {
   int Nc, Ny;
+  Lay lay;  // storage of the half spectrum
   int nc;   // columns per block
   int ld;   // LDS line stride, >= Ny
   Radices plY;
@@ -90,7 +112,7 @@ __device__ __forceinline__ void row_load_pack(const V2<Tin>* __restrict__ S, V2<
   for (int e = threadIdx.x; e < nrows * Nc; e += blockDim.x) {
     const int line = e / Nc;
     const int k = e - line * Nc;
-    V2<Tin> v = S[(size_t)(row0 + line) * Nc + k];
+    V2<Tin> v = S[g.lay.at(row0 + line, k)];
     stage[line * ld + k] = V{T(v.x), T(v.y)};
   }
   __syncthreads();
@@ -169,7 +191,7 @@ __global__ void __launch_bounds__(256) k_row_nonlin(const V2<T>* __restrict__ S,
   for (int e = threadIdx.x; e < nrows * g.Nc; e += blockDim.x) {
     const int line = e / g.Nc;
     const int k = e - line * g.Nc;
-    Tout[(size_t)(row0 + line) * g.Nc + k] = row_unpack<T>(Z + line * g.ld, k, g.M, twN);
+    Tout[g.lay.at(row0 + line, k)] = row_unpack<T>(Z + line * g.ld, k, g.M, twN);
   }
 }
 
@@ -200,7 +222,7 @@ __device__ __forceinline__ void col_load(const V2<T>* __restrict__ src, V2<T>* b
   for (int e = threadIdx.x; e < ncols * g.Ny; e += blockDim.x) {
     const int y = e / ncols;
     const int line = e - y * ncols;
-    buf[line * g.ld + y] = src[(size_t)y * g.Nc + c0 + line];
+    buf[line * g.ld + y] = src[g.lay.at(y, c0 + line)];
   }
 }
 
@@ -288,7 +310,7 @@ __global__ void __launch_bounds__(256) k_col_etd(V2<T>* __restrict__ S, const V2
     // grow them as exp(L dt)^n.  Storing only the real part is the projection
     // onto real fields that irfft2 performs every step in the reference.
     if (c0 + line == 0 || c0 + line == g.Nc - 1) v.y = T(0);
-    S[(size_t)y * g.Nc + c0 + line] = v;
+    S[g.lay.at(y, c0 + line)] = v;
   }
 }
 
@@ -320,7 +342,7 @@ __global__ void __launch_bounds__(256) k_row_r2c(const Tin* __restrict__ x, V2<T
     const int line = e / g.Nc;
     const int k = e - line * g.Nc;
     V y = row_unpack<T>(Z + line * g.ld, k, g.M, twN);
-    S[(size_t)(row0 + line) * g.Nc + k] = V{y.x * scale, y.y * scale};
+    S[g.lay.at(row0 + line, k)] = V{y.x * scale, y.y * scale};
   }
 }
 
@@ -349,7 +371,7 @@ __global__ void __launch_bounds__(256) k_row_c2r(const V2<T>* __restrict__ S, To
   }
 }
 
-// column FFT of a row-major (Ny, Nc) complex array, in place, times `scale`.
+// column FFT of a (Ny, Nc) complex array stored as g.lay, in place, times `scale`.
 template <typename T, bool INV>
 __global__ void __launch_bounds__(256) k_col_fft(V2<T>* __restrict__ S, ColGeom g, const V2<T>* __restrict__ twY,
                                                  T scale, size_t bstride) {
@@ -367,7 +389,7 @@ __global__ void __launch_bounds__(256) k_col_fft(V2<T>* __restrict__ S, ColGeom 
     const int y = e / ncols;
     const int line = e - y * ncols;
     V v = R[line * g.ld + y];
-    S[(size_t)y * g.Nc + c0 + line] = V{v.x * scale, v.y * scale};
+    S[g.lay.at(y, c0 + line)] = V{v.x * scale, v.y * scale};
   }
 }
 
@@ -452,7 +474,7 @@ __global__ void __launch_bounds__(256) k_col_filter(const V2<T>* __restrict__ S,
   for (int e = threadIdx.x; e < nel; e += blockDim.x) {
     const int y = e / ncols;
     const int line = e - y * ncols;
-    out0[(size_t)y * g.Nc + c0 + line] = R[line * g.ld + y];
+    out0[g.lay.at(y, c0 + line)] = R[line * g.ld + y];
   }
   if (MODE == FILTER_MU) {
     __syncthreads();
@@ -470,7 +492,7 @@ __global__ void __launch_bounds__(256) k_col_filter(const V2<T>* __restrict__ S,
     for (int e = threadIdx.x; e < nel; e += blockDim.x) {
       const int y = e / ncols;
       const int line = e - y * ncols;
-      out1[(size_t)y * g.Nc + c0 + line] = R2[line * g.ld + y];
+      out1[g.lay.at(y, c0 + line)] = R2[line * g.ld + y];
     }
   }
 }
@@ -628,7 +650,7 @@ __global__ void __launch_bounds__(256) k_fe_col(const V2<Tin>* __restrict__ S, C
   for (int e = threadIdx.x; e < ncols * g.Ny; e += blockDim.x) {
     const int y = e / ncols;
     const int line = e - y * ncols;
-    V2<Tin> v = S[(size_t)y * g.Nc + c0 + line];
+    V2<Tin> v = S[g.lay.at(y, c0 + line)];
     b0[line * g.ld + y] = V{double(v.x), double(v.y)};
   }
   __syncthreads();

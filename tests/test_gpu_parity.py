@@ -216,8 +216,8 @@ def test_every_schedule_matches_generic(hip_device, shape, precision, monkeypatc
     ref = Plan(**kw).set_field(psi0).step(4).get_field().cpu().numpy()
     monkeypatch.delenv("PFC_HIP_GENERIC")
     lpad = lambda i: i + (i >> 4)  # LDS padding of the library's fast kernels
-    rows = _shapes(nx // 2, lambda w: nx + w * (lpad(nx // 2) + 2), cs)
-    cols = _shapes(ny, lambda w: ny + w * (lpad(ny - 1) + 2), cs)
+    rows = _shapes(nx // 2, lambda w: w * (lpad(nx // 2) + 2), cs)
+    cols = _shapes(ny, lambda w: w * (lpad(ny - 1) + 2), cs)
     tol = 1e-13 if precision == 64 else 2e-5
     combos = [(r, cols[0]) for r in rows] + [(rows[0], c) for c in cols[1:]]
     for (rv, rl), (cv, cl) in combos:
