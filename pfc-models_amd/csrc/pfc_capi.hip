@@ -304,13 +304,13 @@ template <> struct FastLaunch<double> {
 };
 template <> struct FastLaunch<float> {
   static void row(int M, const FastShape& f, int ny_inst, hipStream_t st, const V2<float>* S, V2<float>* T, int Ny,
-                  const Lay& lay, const Nonlin& q, const V2<float>* tw, const Slots& sl) {
+                  const Lay& lay, const Nonlin& q, const V2<double>* tw, const Slots& sl) {
     if (!row_fast_launch_f32(M, f.var, dim3(f.blocks, ny_inst), dim3(f.lines * f.tpl), f.lds, st, S, T, Ny, lay, q,
                              tw, sl))
       fail(PFC_EUNSUPPORTED, "no fast row schedule");
   }
   static void col(int Ny, int mode, const FastShape& f, int ny_inst, hipStream_t st, V2<float>* S,
-                  const V2<float>* T, V2<float>* D, int Nc, const Lay& lay, const ColTables& tb, const V2<float>* tw,
+                  const V2<float>* T, V2<float>* D, int Nc, const Lay& lay, const ColTables& tb, const V2<double>* tw,
                   const Slots& sl) {
     if (!col_fast_launch_f32(Ny, f.var, mode, dim3(f.blocks, ny_inst), dim3(f.lines * f.tpl), f.lds, st, S, T, D,
                              Nc, lay, tb, tw, sl))
@@ -376,6 +376,7 @@ struct Engine {
   pfc_params p{};
   int cap = 0, Nc = 0;
   size_t csize = 0, nc = 0;  // bytes per complex; complex elements per slot array
+  size_t ntab = 0;           // doubles per slot ETD table: Nc * (Ny/2 + 1)
   Lay lay{};  // storage order of S~, T (and the diagnostic scratch)
   RowGeom rg{}, rg64{};
   ColGeom cg{}, cg64{};
@@ -397,7 +398,7 @@ struct Engine {
   int nact() const { return (int)h_active.size(); }
   int pstride() const { return row_blocks(rg64) + col_blocks(cg64); }
   Slots slots_active() const {
-    return Slots{(const int*)active.p, (long long)nc, (long long)nc, p.pin_mean ? (const double*)pin.p : nullptr,
+    return Slots{(const int*)active.p, (long long)nc, (long long)ntab, p.pin_mean ? (const double*)pin.p : nullptr,
                  (int*)status.p, (const double*)Ms.p};
   }
   // a single slot addressed directly (list = null, blockIdx.y = 0 is that slot)
@@ -419,7 +420,8 @@ void engine_init(Engine& e, const pfc_params* params, int cap, bool batched) {
   const int Nx = params->Nx, Ny = params->Ny;
   e.Nc = Nx / 2 + 1;
   e.lay = state_layout(Nx, Ny, params->precision);
-  e.nc = e.lay.size(Ny);  // >= Nc * Ny, the column-major tables' extent
+  e.nc = e.lay.size(Ny);  // >= Nc * Ny, the ETDRK2 carry's extent
+  e.ntab = (size_t)e.Nc * table_rows(Ny);
   e.csize = e.f64() ? sizeof(V2<double>) : sizeof(V2<float>);
   e.rg = row_geom(Nx, Ny, e.csize);
   e.cg = col_geom(Nx, Ny, e.csize);
@@ -448,7 +450,7 @@ void engine_init(Engine& e, const pfc_params* params, int cap, bool batched) {
     }
   }
   const size_t bytes = (size_t)cap * e.nc * e.csize;
-  const size_t tbytes = (size_t)cap * e.nc * sizeof(double);  // ETD tables are FP64 in every precision
+  const size_t tbytes = (size_t)cap * e.ntab * sizeof(double);  // ETD tables are FP64 in every precision
   e.S.alloc(bytes);
   e.Tb.alloc(bytes);
   if (params->scheme == PFC_SCHEME_ETDRK2) e.D.alloc(bytes);
@@ -496,10 +498,10 @@ void engine_init(Engine& e, const pfc_params* params, int cap, bool batched) {
 // ETD tables of slots [slot0, slot0 + n), built in FP64 on the device
 void engine_tables(Engine& e, int slot0, int n, hipStream_t st) {
   TabParams tp{e.p.Gamma, e.p.beta, e.p.q0, 0.0, 0.0, e.p.dt, e.p.smallZTol,
-               1.0 / e.p.Ny, 1.0 / ((double)e.p.Nx * (double)e.p.Ny), e.Nc, e.p.Ny};
-  const int blocks = (int)std::min<size_t>((e.nc + 255) / 256, 1024);
+               1.0 / e.p.Ny, 1.0 / ((double)e.p.Nx * (double)e.p.Ny), e.Nc, e.p.Ny, table_rows(e.p.Ny)};
+  const int blocks = (int)std::min<size_t>((e.ntab + 255) / 256, 1024);
   hipLaunchKernelGGL(k_tables<double>, dim3(blocks, n), dim3(256), 0, st, (double*)e.E1.p, (double*)e.Q1.p,
-                     (double*)e.Q2.p, (long long)e.nc, (const double*)e.kx2.p, (const double*)e.ky2.p, tp,
+                     (double*)e.Q2.p, (long long)e.ntab, (const double*)e.kx2.p, (const double*)e.ky2.p, tp,
                      (const double*)e.eps.p, (const double*)e.Ms.p, slot0);
   hip_check(hipGetLastError(), "table launch");
 }
@@ -524,18 +526,18 @@ struct Steps {
     V* Tb = (V*)e.Tb.p;
     const Nonlin q = nonlin_of(e);
     if (e.frow.on)
-      FastLaunch<T>::row(e.p.Nx / 2, e.frow, ny, st, S, Tb, e.p.Ny, e.lay, q, (const V*)e.twN.p, sl);
+      FastLaunch<T>::row(e.p.Nx / 2, e.frow, ny, st, S, Tb, e.p.Ny, e.lay, q, (const V2<double>*)e.twN64.p, sl);
     else
       hipLaunchKernelGGL((k_row_nonlin<T>), dim3(row_blocks(e.rg), ny), dim3(256), row_lds(e.rg, sizeof(V)), st, S,
                          Tb, e.rg, q, (const V*)e.twN.p, sl);
   }
   static void col(Engine& e, int mode, hipStream_t st, const Slots& sl, int ny) {
-    ColTables tb{e.E1.p, e.Q1.p, e.Q2.p};
+    ColTables tb{e.E1.p, e.Q1.p, e.Q2.p, table_rows(e.p.Ny)};
     V* S = (V*)e.S.p;
     V* D = (V*)e.D.p;
     const V* Tb = (const V*)e.Tb.p;
     if (e.fcol.on) {
-      FastLaunch<T>::col(e.p.Ny, mode, e.fcol, ny, st, S, Tb, D, e.Nc, e.lay, tb, (const V*)e.twY.p, sl);
+      FastLaunch<T>::col(e.p.Ny, mode, e.fcol, ny, st, S, Tb, D, e.Nc, e.lay, tb, (const V2<double>*)e.twY64.p, sl);
       return;
     }
     const dim3 g(col_blocks(e.cg), ny);
@@ -636,7 +638,8 @@ void autotune(Engine& e, hipStream_t st) {
       FastShape pick = e.frow;
       for (const FastShape& f : cands) {
         float ms = time_launches(st, reps, [&] {
-          FastLaunch<T>::row(M, f, ny, st, (const V*)e.S.p, (V*)T2.p, Ny, e.lay, q, (const V*)e.twN.p, sl);
+          FastLaunch<T>::row(M, f, ny, st, (const V*)e.S.p, (V*)T2.p, Ny, e.lay, q, (const V2<double>*)e.twN64.p,
+                             sl);
         });
         if (ms < best) best = ms, pick = f;
       }
@@ -658,13 +661,13 @@ void autotune(Engine& e, hipStream_t st) {
       D2.alloc(bytes);
       hip_check(hipMemcpyAsync(S2.p, e.S.p, bytes, hipMemcpyDeviceToDevice, st), "copy");
       hip_check(hipMemcpyAsync(T2.p, e.S.p, bytes, hipMemcpyDeviceToDevice, st), "copy");
-      ColTables tb{e.E1.p, e.Q1.p, e.Q2.p};
+      ColTables tb{e.E1.p, e.Q1.p, e.Q2.p, table_rows(e.p.Ny)};
       float best = 1e30f;
       FastShape pick = e.fcol;
       for (const FastShape& f : cands) {
         float ms = time_launches(st, reps, [&] {
           FastLaunch<T>::col(Ny, mode, f, ny, st, (V*)S2.p, (const V*)T2.p, (V*)D2.p, Nc, e.lay, tb,
-                             (const V*)e.twY.p, sl);
+                             (const V2<double>*)e.twY64.p, sl);
         });
         if (ms < best) best = ms, pick = f;
       }
@@ -1141,9 +1144,10 @@ int pfc_slot_bytes(const pfc_params* params, long long* bytes) {
     validate(params);
     if (!bytes) fail(PFC_EINVAL, "NULL bytes");
     const size_t nc = state_layout(params->Nx, params->Ny, params->precision).size(params->Ny);
+    const size_t nt = (size_t)(params->Nx / 2 + 1) * table_rows(params->Ny);
     const size_t cs = params->precision == 64 ? 16 : 8, ts = sizeof(double);
     const bool rk2 = params->scheme == PFC_SCHEME_ETDRK2;
-    *bytes = (long long)(nc * cs * (rk2 ? 3 : 2) + nc * ts * (rk2 ? 3 : 2));
+    *bytes = (long long)(nc * cs * (rk2 ? 3 : 2) + nt * ts * (rk2 ? 3 : 2));
   });
 }
 

@@ -119,9 +119,20 @@ template <class S> __host__ __device__ constexpr int tw_offset(int pi) {
 }
 template <class S> __host__ __device__ constexpr int tw_count() { return tw_offset<S>(S::NP) > 0 ? tw_offset<S>(S::NP) : 1; }
 
-// Load the lane's roots from a table exp(-2 pi i m / (L * tws)).
+// w[r - 1] = u^r, r = 1..R-1, by binary splitting (u^r = u^(r/2) u^(r - r/2)):
+// at most ceil(log2 R) rounded products separate u^r from the loaded root.
+template <int R>
+__device__ __forceinline__ void root_powers(V2<double> u, V2<double>* w) {
+  w[0] = u;
+#pragma unroll
+  for (int r = 2; r < R; ++r) w[r - 1] = cmul(w[r / 2 - 1], w[r - r / 2 - 1]);
+}
+
+// The lane's roots for passes PI..NP-1 from an FP64 table exp(-2 pi i m /
+// (L * tws)): one load per butterfly (its base root w^(k L / (NS R))), the
+// other R - 2 powers computed in FP64 and rounded once to T.
 template <typename T, class S, int PI = 1>
-__device__ __forceinline__ void tw_load(V2<T>* w, int lane, const V2<T>* __restrict__ tw, int tws) {
+__device__ __forceinline__ void tw_load(V2<T>* w, int lane, const V2<double>* __restrict__ tw, int tws) {
   if constexpr (PI < S::NP) {
     constexpr int R = S::R[PI];
     constexpr int NS = S::ns(PI);
@@ -130,8 +141,10 @@ __device__ __forceinline__ void tw_load(V2<T>* w, int lane, const V2<T>* __restr
 #pragma unroll
     for (int i = 0; i < NB; ++i) {
       const int k = (lane + S::TPL * i) % NS;
+      V2<double> pw[R - 1];
+      root_powers<R>(tw[(k * (S::L / (NS * R))) * tws], pw);
 #pragma unroll
-      for (int r = 1; r < R; ++r) w[OFF + i * (R - 1) + r - 1] = tw[(k * r * (S::L / (NS * R))) * tws];
+      for (int r = 1; r < R; ++r) w[OFF + i * (R - 1) + r - 1] = V2<T>{T(pw[r - 1].x), T(pw[r - 1].y)};
     }
     tw_load<T, S, PI + 1>(w, lane, tw, tws);
   }
@@ -202,12 +215,13 @@ __device__ __forceinline__ void reg_fft(V2<T>* v, V2<T>* line, int lane, const V
 
 // ---------------------------------------------------------------------------
 // fast row kernel: S~ -> C2R -> N(n) -> R2C -> T; W rows per block
-// LDS: W lines of M + 1 (padded).  Roots come from the table
-// exp(-2 pi i m / Nx), m < Nx, straight into registers.
+// LDS: W lines of M + 1 (padded).  Roots come from the FP64 table
+// exp(-2 pi i m / Nx), m < Nx, into registers.
 // ---------------------------------------------------------------------------
 template <typename T, class S>
 __global__ void __launch_bounds__(256) k_row_fast(const V2<T>* __restrict__ Sg, V2<T>* __restrict__ Tout, int Ny,
-                                                  Lay lay, Nonlin q, const V2<T>* __restrict__ twN, Slots sl) {
+                                                  Lay lay, Nonlin q, const V2<double>* __restrict__ twN,
+                                                  Slots sl) {
   using V = V2<T>;
   constexpr int M = S::L, P = S::P, TPL = S::TPL, LD = row_ld(M);
   extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -229,11 +243,16 @@ __global__ void __launch_bounds__(256) k_row_fast(const V2<T>* __restrict__ Sg, 
 #pragma unroll
   for (int qq = 0; qq < P; ++qq) x[qq] = Sg[lay.at(rowc, lane + TPL * qq)];
   const V xM = Sg[lay.at(rowc, M)];
-  // roots: w_Nx^k for the (un)packing of k = lane + TPL q, and the FFT roots
+  // roots: w_Nx^k for the (un)packing of k = lane + TPL q, as w_Nx^lane times
+  // the wave-uniform w_Nx^(TPL q) (scalar loads), in FP64; and the FFT roots
+  const V2<double> wl0 = twN[lane];
   V wk[P];
 #pragma unroll
-  for (int qq = 0; qq < P; ++qq) wk[qq] = twN[lane + TPL * qq];
-  const V wM = twN[M];
+  for (int qq = 0; qq < P; ++qq) {
+    const V2<double> c = cmul(wl0, twN[TPL * qq]);
+    wk[qq] = V{T(c.x), T(c.y)};
+  }
+  const V wM = V{T(twN[M].x), T(twN[M].y)};
   V wt[tw_count<S>()];
   tw_load<T, S>(wt, lane, twN, 2);
   // keep every load above in flight together: no arithmetic is scheduled
@@ -287,12 +306,13 @@ __global__ void __launch_bounds__(256) k_row_fast(const V2<T>* __restrict__ Sg, 
 // ---------------------------------------------------------------------------
 // fast column kernel: forward column FFTs of S~ and T, ETD combine in
 // registers, inverse column FFT, S~ written in place.  C columns per block.
-// LDS: C lines of Ny (padded); roots from exp(-2 pi i m / Ny) in registers.
+// LDS: C lines of Ny (padded); roots from the FP64 table exp(-2 pi i m / Ny)
+// in registers.
 // ---------------------------------------------------------------------------
 template <typename T, class S, int MODE>
 __global__ void __launch_bounds__(256) k_col_fast(V2<T>* __restrict__ Sg, const V2<T>* __restrict__ Tin,
                                                   V2<T>* __restrict__ D, int Nc, Lay lay, ColTables tb,
-                                                  const V2<T>* __restrict__ twY, Slots sl) {
+                                                  const V2<double>* __restrict__ twY, Slots sl) {
   using V = V2<T>;
   constexpr int NY = S::L, P = S::P, TPL = S::TPL, LD = col_ld(NY);
   extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -342,15 +362,16 @@ __global__ void __launch_bounds__(256) k_col_fast(V2<T>* __restrict__ Sg, const 
   if constexpr (MODE != COL_RK2_STAGE2) reg_fft<T, S, false, true>(a, line, lane, wt);
   // ETD tables (and the stage-2 carry) for this lane's ky, loaded now so they
   // arrive while the second forward transform runs
-  const size_t tbase = (size_t)col * NY;
+  const size_t tbase = (size_t)col * NY;         // carry, [Nc][Ny]
+  const size_t hbase = (size_t)col * tb.hy;      // tables, [Nc][Ny/2 + 1]
   double e1r[P], q1r[P], q2r[P];
   V dr[P];
 #pragma unroll
   for (int qq = 0; qq < P; ++qq) {
-    const size_t ti = tbase + lane + TPL * qq;
+    const size_t ti = hbase + ky_half(lane + TPL * qq, NY);
     if constexpr (MODE == COL_RK2_STAGE2) {
       q2r[qq] = Q2[ti];
-      dr[qq] = D[ti];
+      dr[qq] = D[tbase + lane + TPL * qq];
     } else {
       e1r[qq] = E1[ti];
       q1r[qq] = Q1[ti];
@@ -431,13 +452,13 @@ bool row_fast_launch_f64(int M, int var, dim3 grid, dim3 block, size_t lds, hipS
                          V2<double>* T, int Ny, const Lay& lay, const Nonlin& q, const V2<double>* tw,
                          const Slots& sl);
 bool row_fast_launch_f32(int M, int var, dim3 grid, dim3 block, size_t lds, hipStream_t st, const V2<float>* S,
-                         V2<float>* T, int Ny, const Lay& lay, const Nonlin& q, const V2<float>* tw,
+                         V2<float>* T, int Ny, const Lay& lay, const Nonlin& q, const V2<double>* tw,
                          const Slots& sl);
 bool col_fast_launch_f64(int Ny, int var, int mode, dim3 grid, dim3 block, size_t lds, hipStream_t st, V2<double>* S,
                          const V2<double>* T, V2<double>* D, int Nc, const Lay& lay, const ColTables& tb,
                          const V2<double>* tw, const Slots& sl);
 bool col_fast_launch_f32(int Ny, int var, int mode, dim3 grid, dim3 block, size_t lds, hipStream_t st, V2<float>* S,
                          const V2<float>* T, V2<float>* D, int Nc, const Lay& lay, const ColTables& tb,
-                         const V2<float>* tw, const Slots& sl);
+                         const V2<double>* tw, const Slots& sl);
 
 }  // namespace pfc

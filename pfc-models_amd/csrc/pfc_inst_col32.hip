@@ -5,7 +5,7 @@ namespace pfc {
 
 bool col_fast_launch_f32(int Ny, int var, int mode, dim3 grid, dim3 block, size_t lds, hipStream_t st, V2<float>* S,
                          const V2<float>* T, V2<float>* D, int Nc, const Lay& lay, const ColTables& tb,
-                         const V2<float>* tw, const Slots& sl) {
+                         const V2<double>* tw, const Slots& sl) {
 #define PFC_X(LEN, VAR, PP, ...)                                                                               \
   if (Ny == LEN && var == VAR) {                                                                               \
     using Sc = Sch<LEN, PP, __VA_ARGS__>;                                                                      \

@@ -9,7 +9,7 @@ PFC_SCHEDULES(PFC_X)
 #undef PFC_X
 
 bool row_fast_launch_f32(int M, int var, dim3 grid, dim3 block, size_t lds, hipStream_t st, const V2<float>* S,
-                         V2<float>* T, int Ny, const Lay& lay, const Nonlin& q, const V2<float>* tw,
+                         V2<float>* T, int Ny, const Lay& lay, const Nonlin& q, const V2<double>* tw,
                          const Slots& sl) {
 #define PFC_X(LEN, VAR, PP, ...)                                                                          \
   if (M == LEN && var == VAR) {                                                                           \

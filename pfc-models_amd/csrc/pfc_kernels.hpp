@@ -200,10 +200,17 @@ __global__ void __launch_bounds__(256) k_row_nonlin(const V2<T>* __restrict__ S,
 // ---------------------------------------------------------------------------
 enum { COL_ETD1 = 0, COL_RK2_STAGE1 = 1, COL_RK2_STAGE2 = 2 };
 
+// The ETD coefficients depend on k^2 only, and ky^2 of rows ky and Ny - ky
+// are the same double (fftfreq's negative frequencies negate exactly), so a
+// column's table stores rows 0..Ny/2 and row ky reads row ky_half(ky).
+__host__ __device__ __forceinline__ int ky_half(int ky, int Ny) { return ky <= Ny - ky ? ky : Ny - ky; }
+__host__ __device__ __forceinline__ int table_rows(int Ny) { return Ny / 2 + 1; }
+
 struct ColTables {
-  const void* E1;  // column-major [Nc][Ny] per slot, FP64 in every precision
+  const void* E1;  // column-major [Nc][hy] per slot, FP64 in every precision
   const void* Q1;
   const void* Q2;
+  int hy;          // rows stored per column (table_rows(Ny))
 };
 
 constexpr int kColRegs = 16;  // A values held per thread between the two forward FFTs
@@ -275,13 +282,14 @@ __global__ void __launch_bounds__(256) k_col_etd(V2<T>* __restrict__ S, const V2
     if (e < nel) {
       const int line = e / g.Ny;
       const int ky = e - line * g.Ny;
-      const size_t ti = (size_t)(c0 + line) * g.Ny + ky;
+      const size_t di = (size_t)(c0 + line) * g.Ny + ky;                      // carry
+      const size_t ti = (size_t)(c0 + line) * tb.hy + ky_half(ky, g.Ny);  // tables
       V b = B[line * g.ld + ky];
       const double bx = b.x, by = b.y;
       double ox, oy;
       if (MODE == COL_RK2_STAGE2) {
         const double q2 = Q2[ti];
-        V d = D[ti];
+        V d = D[di];
         ox = double(d.x) + q2 * bx;
         oy = double(d.y) + q2 * by;
       } else {
@@ -290,7 +298,7 @@ __global__ void __launch_bounds__(256) k_col_etd(V2<T>* __restrict__ S, const V2
         oy = e1 * double(a[i].y) + q1 * by;
         if (MODE == COL_RK2_STAGE1) {
           const double q2 = Q2[ti];
-          D[ti] = V{T(ox - q2 * bx), T(oy - q2 * by)};
+          D[di] = V{T(ox - q2 * bx), T(oy - q2 * by)};
         }
       }
       if (pinp && c0 + line == 0 && ky == 0) ox = pin, oy = 0.0;
@@ -509,6 +517,7 @@ struct TabParams {
   double Gamma, beta, q0, epsilon, M, dt, smallZTol;
   double inv_ny, inv_n;
   int Nc, Ny;
+  int hy;  // stored rows per column: Ny/2 + 1 (see ky_half)
 };
 
 __device__ __forceinline__ double etd2_phi_dt(double z, double dt) {
@@ -536,10 +545,10 @@ __global__ void __launch_bounds__(256) k_tables(T* __restrict__ E1, T* __restric
   E1 += (size_t)slot * tstride;
   Q1 += (size_t)slot * tstride;
   if (Q2) Q2 += (size_t)slot * tstride;
-  const size_t n = (size_t)p.Nc * p.Ny;
+  const size_t n = (size_t)p.Nc * p.hy;
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
-    const int c = (int)(i / p.Ny);
-    const int j = (int)(i - (size_t)c * p.Ny);
+    const int c = (int)(i / p.hy);
+    const int j = (int)(i - (size_t)c * p.hy);
     const double lap = -(kx2[c] + ky2[j]);
     const double sq = p.q0 * p.q0 + lap;
     double lmu = p.beta * (sq * sq) + p.epsilon;

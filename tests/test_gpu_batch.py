@@ -126,8 +126,10 @@ def test_set_slot_rebuilds_tables(hip_device):
 def test_slot_bytes_and_mixed_sizes(hip_device):
     from pfc_hip.plan import Batch, slot_bytes
     per = slot_bytes(**_kw())
-    # FP64 state is stored in tiles of 2 kx x 4 rows: kx padded to even
+    # FP64 state is stored in tiles of 2 kx x 4 rows: kx padded to even;
+    # ETD tables keep rows ky <= N/2 (they depend on ky^2 only)
     nc = N * (2 * ((N // 2 + 2) // 2))
-    assert per == nc * 16 * 3 + nc * 8 * 3  # state (S, T, D) + FP64 tables (E1, Q1, Q2)
+    nt = (N // 2 + 1) * (N // 2 + 1)
+    assert per == nc * 16 * 3 + nt * 8 * 3  # state (S, T, D) + FP64 tables (E1, Q1, Q2)
     b = Batch(3, **_kw())
     assert b.device_bytes() >= 3 * per

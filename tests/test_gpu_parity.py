@@ -58,7 +58,7 @@ def _std128(scheme="etd1"):
 
 @pytest.mark.parametrize("scheme", ["etd1", "etdrk2"])
 def test_fp64_step_matches_oracle_100_steps(hip_device, scheme):
-    """North-star check (a): max|d psi| <= 1e-10 after 100 FP64 steps (128^2, r = psibar = -0.25, dt = 0.5)."""
+    """max|d psi| <= 1e-10 after 100 FP64 steps (128^2, r = psibar = -0.25, dt = 0.5)."""
     geom, p, psi0 = _std128(scheme)
     sim = O.Simulation(p, geom)
     sim.set_field(psi0)
