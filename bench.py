@@ -10,9 +10,10 @@ evaluations, four kernels).
 
 Prints ONE JSON line.  `value` is Mpts*steps/s of the plain FP64 step;
 `fp32` carries the FP32-bulk / FP64-reduction variant of the same workload.
-`roofline` is measured for the dominant kernel with HIP events on the stream
-the kernels run on; `cpu_baseline` times the FP64 NumPy/SciPy CPU restatement
-of the same step on this host's cores over a bounded sample.
+`roofline` is measured for the dominant kernel (largest total time) with a HIP
+event recorded after every launch on the stream the kernels run on, launches
+back to back; `cpu_baseline` times the FP64 NumPy/SciPy CPU restatement of the
+same step on this host's cores over a bounded sample.
 """
 from __future__ import annotations
 
@@ -65,9 +66,15 @@ def run_gpu(precision, steps, warmup):
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     fe, mean, _, _ = pl.reduce()
-    # per-kernel device time with HIP events (separate pass, same workload)
-    prof = pl.step_profile(min(steps, 200))
-    kinds = {k: (ms / calls, calls) for k, (ms, calls) in prof.items() if calls}
+    # per-kernel device time: each kernel kind of the schedule in use, launched
+    # back to back on scratch copies of the live state, one HIP event pair per
+    # kind on the plan's stream; weighted by launches per ETDRK2 step (two row
+    # passes, one of each column pass) to pick the dominant kernel
+    nprof = min(steps, 500)
+    prof = pl.step_profile(nprof)
+    per_step = {"row": 2, "col": 1, "col2": 1}
+    kinds = {k: (ms / calls, per_step[k]) for k, (ms, calls) in prof.items() if calls}
+    kernel_ms_per_step = sum(avg * n for avg, n in kinds.values())
     dom = max(kinds, key=lambda k: kinds[k][0] * kinds[k][1])
     avg_ms = kinds[dom][0]
     ab = _algorithmic_bytes(dom, precision)
@@ -77,6 +84,7 @@ def run_gpu(precision, steps, warmup):
     ck = "k_col_fast" if sched["col"][0] else "k_col_etd"
     names = {"row": rk, "col": f"{ck}<COL_RK2_STAGE1>", "col2": f"{ck}<COL_RK2_STAGE2>"}
     step_bytes = 2 * 4 * N * (N // 2 + 1) * (16 if precision == 64 else 8)
+    traffic = _traffic(names[dom], precision)
     return {
         "value": N * N * steps / wall / 1e6,
         "ms_per_step": wall / steps * 1e3,
@@ -85,8 +93,9 @@ def run_gpu(precision, steps, warmup):
         "kernels_us": {k: round(v[0] * 1e3, 3) for k, v in kinds.items()},
         "roofline": {
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": names[dom],
+            "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": names[dom],
             "algorithmic_bytes_per_launch": ab, "avg_launch_us": avg_ms * 1e3,
+            "kernels_ms_per_step": kernel_ms_per_step,
         },
         "step_roofline": {
             "bytes_per_step": step_bytes,
@@ -96,6 +105,34 @@ def run_gpu(precision, steps, warmup):
         "device_bytes": pl.device_bytes(),
         "schedule": sched,
     }
+
+
+def _traffic(kernel, precision):
+    """HBM bytes per launch of `kernel` from the committed counter record
+    (rocprofv3 FETCH_SIZE and WRITE_SIZE passes, each scaled by the same
+    counter on a plain copy of known size; every schedule variant measured
+    agreed within 1 %), when it covers this kernel and precision; else None."""
+    path = os.path.join(ROOT, "profiles", f"traffic_{N}sq_fp{precision}.json")
+    try:
+        with open(path) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    return t.get("hbm_bytes_per_launch", {}).get(kernel)
+
+
+def run_copy_calibration(reps=50):
+    """A plain device copy of one FP64 row-spectrum array (the row pass's read
+    footprint), for calibrating rocprofv3's FETCH_SIZE / WRITE_SIZE counters."""
+    import torch
+    nbytes = N * (N // 2 + 1) * 16
+    x = torch.ones(nbytes // 8, dtype=torch.float64, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(reps):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration": {"bytes_read": nbytes, "bytes_written": nbytes, "launches": reps}}),
+          flush=True)
 
 
 def run_cpu(max_seconds=20.0):
@@ -156,13 +193,19 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-sweep", action="store_true", help="skip the batched-sweep leg")
+    ap.add_argument("--no-fp32", action="store_true", help="skip the FP32 leg")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a plain device copy of known size (counter calibration) and exit")
     a = ap.parse_args()
+    if a.copy_calibration:
+        run_copy_calibration()
+        return
     if a.gpus != 1:
         print("bench.py: this build measures one GPU; --gpus above 1 is not supported "
               "(independent instances would shard one scheduler per GPU)", file=sys.stderr)
         sys.exit(2)
     f64 = run_gpu(64, a.steps, a.warmup)
-    f32 = run_gpu(32, a.steps, a.warmup)
+    f32 = None if a.no_fp32 else run_gpu(32, a.steps, a.warmup)
     sweep = None if a.no_sweep else run_sweep()
     cpu = None if a.no_cpu else run_cpu(a.cpu_seconds)
     line = {
@@ -183,10 +226,11 @@ def main():
         "roofline": f64["roofline"],
         "step_roofline": f64["step_roofline"],
         "kernels_us": f64["kernels_us"],
-        "fp32": {"value": round(f32["value"], 1), "ms_per_step": round(f32["ms_per_step"], 5),
-                 "dtype": "f32 bulk, f64 tables and reductions", "roofline": f32["roofline"],
-                 "kernels_us": f32["kernels_us"],
-                 "rel_fe_vs_fp64": abs(f32["free_energy"] - f64["free_energy"]) / abs(f64["free_energy"])},
+        "fp32": None if f32 is None else {
+            "value": round(f32["value"], 1), "ms_per_step": round(f32["ms_per_step"], 5),
+            "dtype": "f32 bulk, f64 tables, combine and reductions", "roofline": f32["roofline"],
+            "kernels_us": f32["kernels_us"],
+            "rel_fe_vs_fp64": abs(f32["free_energy"] - f64["free_energy"]) / abs(f64["free_energy"])},
         "cpu_baseline": cpu,
         "sweep": sweep,
     }

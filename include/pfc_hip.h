@@ -100,10 +100,13 @@ int pfc_chemical_potential(pfc_plan* plan, void* mu, void* local_mu, int dtype, 
 /* Gaussian-smoothed field irfft2(G n_hat) into a real (Ny, Nx) device output.
  * Replaces Get_local_n / Get_local_phi (models/stdpfc/simulation.py:237-247). */
 int pfc_local_field(pfc_plan* plan, void* out, int dtype, void* stream);
-/* pfc_step with a HIP event pair around every kernel launch, for roofline
- * figures: ms[k] accumulates the device time of kernel kind k and calls[k]
- * its launch count; k = 0 row pass, 1 first column pass, 2 ETDRK2 second
- * column pass.  Synchronises `stream`.  (No reference counterpart: profiling
+/* Kernel timing for roofline figures: each kernel kind of the plan's current
+ * schedule runs `nsteps` times back to back on scratch copies of the live
+ * state (the plan's state is not advanced), bracketed by one HIP event pair on
+ * `stream`; ms[k] is the total and calls[k] the launch count of kind k (0 row
+ * pass, 1 first column pass, 2 ETDRK2 second column pass; 0 calls when the
+ * scheme has no such pass).  Runs the autotuner first if no step has.
+ * Synchronises `stream`.  (No reference counterpart: profiling
  * instrumentation lives outside the model code, README.md:59-60.) */
 int pfc_step_profile(pfc_plan* plan, long long nsteps, void* stream, double ms[3], long long calls[3]);
 /* Kernel schedule in use (after the first step has run the autotuner):

@@ -350,23 +350,6 @@ void validate(const pfc_params* p) {
   if (p->model == PFC_MODEL_LOG && p->clamp_log && !(p->minPhi > 0)) fail(PFC_EINVAL, "minPhi must be positive");
 }
 
-// Optional per-launch event timing (pfc_step_profile): kind 0 row, 1 col, 2 col stage 2.
-struct Prof {
-  hipEvent_t ev[2] = {nullptr, nullptr};
-  double ms[3] = {0, 0, 0};
-  long long calls[3] = {0, 0, 0};
-  hipStream_t st = nullptr;
-  void begin() { hip_check(hipEventRecord(ev[0], st), "event record"); }
-  void end(int kind) {
-    hip_check(hipEventRecord(ev[1], st), "event record");
-    hip_check(hipEventSynchronize(ev[1]), "event sync");
-    float t = 0.f;
-    hip_check(hipEventElapsedTime(&t, ev[0], ev[1]), "event elapsed");
-    ms[kind] += t;
-    calls[kind] += 1;
-  }
-};
-
 // ---------------------------------------------------------------------------
 // Engine: `cap` same-shape instances (slots).  Per slot: state S~ (row
 // spectrum), T (row spectrum of N), D (ETDRK2 carry), ETD tables, epsilon and
@@ -517,13 +500,18 @@ Nonlin nonlin_of(const Engine& e) {
   return q;
 }
 
-// one nonlinear row pass and one column pass (MODE) over `ny` instances
+// one nonlinear row pass and one column pass (MODE) over `ny` instances, on
+// the engine's arrays or on caller-given ones of the same shape
 template <typename T>
 struct Steps {
   using V = V2<T>;
   static void row(Engine& e, hipStream_t st, const Slots& sl, int ny) {
-    const V* S = (const V*)e.S.p;
-    V* Tb = (V*)e.Tb.p;
+    row(e, st, sl, ny, (const V*)e.S.p, (V*)e.Tb.p);
+  }
+  static void col(Engine& e, int mode, hipStream_t st, const Slots& sl, int ny) {
+    col(e, mode, st, sl, ny, (V*)e.S.p, (const V*)e.Tb.p, (V*)e.D.p);
+  }
+  static void row(Engine& e, hipStream_t st, const Slots& sl, int ny, const V* S, V* Tb) {
     const Nonlin q = nonlin_of(e);
     if (e.frow.on)
       FastLaunch<T>::row(e.p.Nx / 2, e.frow, ny, st, S, Tb, e.p.Ny, e.lay, q, (const V2<double>*)e.twN64.p, sl);
@@ -531,11 +519,8 @@ struct Steps {
       hipLaunchKernelGGL((k_row_nonlin<T>), dim3(row_blocks(e.rg), ny), dim3(256), row_lds(e.rg, sizeof(V)), st, S,
                          Tb, e.rg, q, (const V*)e.twN.p, sl);
   }
-  static void col(Engine& e, int mode, hipStream_t st, const Slots& sl, int ny) {
+  static void col(Engine& e, int mode, hipStream_t st, const Slots& sl, int ny, V* S, const V* Tb, V* D) {
     ColTables tb{e.E1.p, e.Q1.p, e.Q2.p, table_rows(e.p.Ny)};
-    V* S = (V*)e.S.p;
-    V* D = (V*)e.D.p;
-    const V* Tb = (const V*)e.Tb.p;
     if (e.fcol.on) {
       FastLaunch<T>::col(e.p.Ny, mode, e.fcol, ny, st, S, Tb, D, e.Nc, e.lay, tb, (const V2<double>*)e.twY64.p, sl);
       return;
@@ -550,20 +535,18 @@ struct Steps {
     else
       hipLaunchKernelGGL((k_col_etd<T, COL_RK2_STAGE2>), g, dim3(256), lds, st, S, Tb, D, e.cg, tb, tw, sl);
   }
-  static void step(Engine& e, long long nsteps, hipStream_t st, Prof* prof) {
+  static void step(Engine& e, long long nsteps, hipStream_t st) {
     const int ny = e.nact();
     if (ny == 0) return;
     const Slots sl = e.slots_active();
-    auto B = [&]() { if (prof) prof->begin(); };
-    auto E = [&](int k) { if (prof) prof->end(k); };
     for (long long s = 0; s < nsteps; ++s) {
-      B(); row(e, st, sl, ny); E(0);
+      row(e, st, sl, ny);
       if (e.p.scheme == PFC_SCHEME_ETD1) {
-        B(); col(e, COL_ETD1, st, sl, ny); E(1);
+        col(e, COL_ETD1, st, sl, ny);
       } else {
-        B(); col(e, COL_RK2_STAGE1, st, sl, ny); E(1);
-        B(); row(e, st, sl, ny); E(0);
-        B(); col(e, COL_RK2_STAGE2, st, sl, ny); E(2);
+        col(e, COL_RK2_STAGE1, st, sl, ny);
+        row(e, st, sl, ny);
+        col(e, COL_RK2_STAGE2, st, sl, ny);
       }
     }
     hip_check(hipGetLastError(), "step launch");
@@ -678,7 +661,7 @@ void autotune(Engine& e, hipStream_t st) {
   }
 }
 
-void engine_step(Engine& e, long long nsteps, hipStream_t st, Prof* prof) {
+void engine_step(Engine& e, long long nsteps, hipStream_t st) {
   if (nsteps < 0) fail(PFC_EINVAL, "nsteps < 0");
   for (int s : e.h_active)
     if (!e.field_set[s]) fail(PFC_EINVAL, "a stepped slot has no field (pfc_set_field / pfc_batch_set_field)");
@@ -689,9 +672,43 @@ void engine_step(Engine& e, long long nsteps, hipStream_t st, Prof* prof) {
       autotune<float>(e, st);
   }
   if (e.f64())
-    Steps<double>::step(e, nsteps, st, prof);
+    Steps<double>::step(e, nsteps, st);
   else
-    Steps<float>::step(e, nsteps, st, prof);
+    Steps<float>::step(e, nsteps, st);
+}
+
+// Average launch time of each kernel kind of the engine's current schedule:
+// n launches back to back on scratch copies of the live state, one event pair
+// per kind (kind 0 row pass, 1 first column pass, 2 ETDRK2 second column
+// pass).  The engine's own arrays are not touched.
+template <typename T>
+void kernel_times(Engine& e, long long n, hipStream_t st, double ms[3], long long calls[3]) {
+  using V = V2<T>;
+  for (int k = 0; k < 3; ++k) ms[k] = 0.0, calls[k] = 0;
+  const int ny = e.nact();
+  if (ny == 0 || n <= 0) return;
+  const Slots sl = e.slots_active();
+  const size_t bytes = (size_t)e.cap * e.nc * sizeof(V);
+  DevBuf S2, T2, D2;
+  S2.alloc(bytes);
+  T2.alloc(bytes);
+  D2.alloc(bytes);
+  hip_check(hipMemcpyAsync(S2.p, e.S.p, bytes, hipMemcpyDeviceToDevice, st), "copy");
+  hip_check(hipMemcpyAsync(T2.p, e.Tb.p, bytes, hipMemcpyDeviceToDevice, st), "copy");
+  if (e.D.p) hip_check(hipMemcpyAsync(D2.p, e.D.p, bytes, hipMemcpyDeviceToDevice, st), "copy");
+  const bool rk2 = e.p.scheme == PFC_SCHEME_ETDRK2;
+  for (int k = 0; k < (rk2 ? 3 : 2); ++k) {
+    const float m = time_launches(st, (int)n, [&] {
+      if (k == 0)
+        Steps<T>::row(e, st, sl, ny, (const V*)S2.p, (V*)T2.p);
+      else
+        Steps<T>::col(e, k == 1 ? (rk2 ? COL_RK2_STAGE1 : COL_ETD1) : COL_RK2_STAGE2, st, sl, ny, (V*)S2.p,
+                      (const V*)T2.p, (V*)D2.p);
+    });
+    ms[k] = (double)m * n;
+    calls[k] = n;
+  }
+  hip_check(hipStreamSynchronize(st), "sync");
 }
 
 // FP64 reductions of `ny` instances (the active list, or one slot when
@@ -975,7 +992,7 @@ int pfc_get_field(pfc_plan* pl, void* field, int dtype, void* stream) {
 int pfc_step(pfc_plan* pl, long long nsteps, void* stream) {
   return guarded([&] {
     if (!pl) fail(PFC_EINVAL, "NULL plan");
-    engine_step(pl->e, nsteps, reinterpret_cast<hipStream_t>(stream), nullptr);
+    engine_step(pl->e, nsteps, reinterpret_cast<hipStream_t>(stream));
   });
 }
 
@@ -1028,21 +1045,22 @@ int pfc_local_field(pfc_plan* pl, void* out, int dtype, void* stream) {
 }
 
 int pfc_step_profile(pfc_plan* pl, long long nsteps, void* stream, double ms[3], long long calls[3]) {
-  Prof prof;
-  int rc = guarded([&] {
+  return guarded([&] {
     if (!pl || !ms || !calls) fail(PFC_EINVAL, "NULL argument");
-    prof.st = reinterpret_cast<hipStream_t>(stream);
-    hip_check(hipEventCreate(&prof.ev[0]), "event create");
-    hip_check(hipEventCreate(&prof.ev[1]), "event create");
-    engine_step(pl->e, nsteps, prof.st, &prof);
-    for (int k = 0; k < 3; ++k) {
-      ms[k] = prof.ms[k];
-      calls[k] = prof.calls[k];
+    if (nsteps < 0) fail(PFC_EINVAL, "nsteps < 0");
+    if (!pl->e.field_set[0]) fail(PFC_EINVAL, "pfc_set_field has not been called");
+    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    if (!pl->e.tuned) {
+      if (pl->e.f64())
+        autotune<double>(pl->e, st);
+      else
+        autotune<float>(pl->e, st);
     }
+    if (pl->e.f64())
+      kernel_times<double>(pl->e, nsteps, st, ms, calls);
+    else
+      kernel_times<float>(pl->e, nsteps, st, ms, calls);
   });
-  for (hipEvent_t ev : prof.ev)
-    if (ev) (void)hipEventDestroy(ev);
-  return rc;
 }
 
 int pfc_plan_schedule(const pfc_plan* pl, int info[8]) {
@@ -1114,7 +1132,7 @@ int pfc_batch_set_active(pfc_batch* b, const int* slots, int n, void* stream) {
 int pfc_batch_step(pfc_batch* b, long long nsteps, void* stream) {
   return guarded([&] {
     if (!b) fail(PFC_EINVAL, "NULL batch");
-    engine_step(b->e, nsteps, reinterpret_cast<hipStream_t>(stream), nullptr);
+    engine_step(b->e, nsteps, reinterpret_cast<hipStream_t>(stream));
   });
 }
 

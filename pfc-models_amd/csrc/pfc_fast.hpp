@@ -101,6 +101,12 @@ template <bool BLOCK> __device__ __forceinline__ void line_sync() {
   }
 }
 
+// Stores of arrays that the next kernel reads on other XCDs (S~ and T cross
+// the row/column transpose): non-temporal, so they stream out while the
+// kernel runs instead of waiting as dirty L2 lines for the end-of-kernel
+// write-back.
+template <typename V> __device__ __forceinline__ void store_stream(V* p, V v) { __builtin_nontemporal_store(v, p); }
+
 // LDS line index with one pad slot per 16 elements: Stockham scatters write
 // at strides of R (or R*Ns) elements, which without padding put every lane of
 // a wavefront on the same bank.
@@ -225,13 +231,18 @@ __global__ void __launch_bounds__(256) k_row_fast(const V2<T>* __restrict__ Sg, 
   using V = V2<T>;
   constexpr int M = S::L, P = S::P, TPL = S::TPL, LD = row_ld(M);
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  const int slot = slot_of(sl, blockIdx.y);
+  // XCD-aware order over (instance, row block): the blocks that share a row
+  // group's 128-byte lines run on one XCD and meet in its L2
+  const int ntile = gridDim.x;
+  const int lin = xcd_remap(blockIdx.x + blockIdx.y * ntile, ntile * gridDim.y);
+  const int yi = lin / ntile;
+  const int slot = slot_of(sl, yi);
   Sg += (size_t)slot * sl.sstride;
   Tout += (size_t)slot * sl.sstride;
   if (sl.M) q.M = sl.M[slot];
   const int wl = threadIdx.x / TPL;
   const int lane = threadIdx.x - wl * TPL;
-  const int row = blockIdx.x * (blockDim.x / TPL) + wl;
+  const int row = (lin - yi * ntile) * (blockDim.x / TPL) + wl;
   const bool live = row < Ny;
   V* line = reinterpret_cast<V*>(smem_raw) + wl * LD;
 
@@ -297,9 +308,9 @@ __global__ void __launch_bounds__(256) k_row_fast(const V2<T>* __restrict__ Sg, 
       const int k = lane + TPL * qq;
       const V zk = v[qq];
       const V zm = line[lpad(k == 0 ? 0 : M - k)];
-      Tout[lay.at(row, k)] = row_unpack_pair<T>(zk, zm, wk[qq]);
+      store_stream(Tout + lay.at(row, k), row_unpack_pair<T>(zk, zm, wk[qq]));
     }
-    if (lane == 0) Tout[lay.at(row, M)] = row_unpack_pair<T>(line[0], line[0], wM);
+    if (lane == 0) store_stream(Tout + lay.at(row, M), row_unpack_pair<T>(line[0], line[0], wM));
   }
 }
 
@@ -409,7 +420,7 @@ __global__ void __launch_bounds__(256) k_col_fast(V2<T>* __restrict__ Sg, const 
     for (int qq = 0; qq < P; ++qq) {
       V xv = b[qq];
       if (realcol) xv.y = T(0);
-      Sg[lay.at(lane + TPL * qq, col)] = xv;
+      store_stream(Sg + lay.at(lane + TPL * qq, col), xv);
     }
   }
 }

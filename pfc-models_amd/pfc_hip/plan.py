@@ -124,8 +124,10 @@ class Plan:
         return self
 
     def step_profile(self, nsteps):
-        """Step with HIP events around every launch; returns {kind: (total_ms, calls)}
-        for kinds 'row', 'col', 'col2'."""
+        """Each kernel kind of the current schedule, `nsteps` launches back to
+        back on scratch copies of the state (the state is not advanced), timed
+        with one HIP event pair per kind; returns {kind: (total_ms, calls)} for
+        kinds 'row', 'col', 'col2'."""
         ms = (ctypes.c_double * 3)()
         calls = (ctypes.c_longlong * 3)()
         _lib.check(self._lib.pfc_step_profile(self._h, int(nsteps), current_stream_handle(), ms, calls))
