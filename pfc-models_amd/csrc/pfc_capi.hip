@@ -598,7 +598,19 @@ void autotune(Engine& e, hipStream_t st) {
   int dev = 0;
   hip_check(hipGetDevice(&dev), "hipGetDevice");
   const int Nx = e.p.Nx, Ny = e.p.Ny, M = Nx / 2, Nc = e.Nc;
-  const int reps = ny > 8 ? 2 : 5;
+  // launches per timing; every candidate is timed in two rounds (forward and
+  // reverse order) and keeps its faster one, so a transient stall of the
+  // device does not decide the choice
+  const int reps = ny > 8 ? 4 : 20;
+  auto pick_fastest = [&](const std::vector<FastShape>& cands, auto&& launch) {
+    std::vector<float> t(cands.size(), 1e30f);
+    for (int round = 0; round < 2; ++round)
+      for (size_t j = 0; j < cands.size(); ++j) {
+        const size_t i = round ? cands.size() - 1 - j : j;
+        t[i] = std::min(t[i], time_launches(st, reps, [&] { launch(cands[i]); }));
+      }
+    return cands[std::min_element(t.begin(), t.end()) - t.begin()];
+  };
   const Slots sl = e.slots_active();
   std::lock_guard<std::mutex> lk(g_tune_mu);
   auto apply = [](FastShape& cur, const std::vector<FastShape>& cands, std::pair<int, int> vl) {
@@ -617,15 +629,9 @@ void autotune(Engine& e, hipStream_t st) {
       DevBuf T2;
       T2.alloc(bytes);
       const Nonlin q = nonlin_of(e);
-      float best = 1e30f;
-      FastShape pick = e.frow;
-      for (const FastShape& f : cands) {
-        float ms = time_launches(st, reps, [&] {
-          FastLaunch<T>::row(M, f, ny, st, (const V*)e.S.p, (V*)T2.p, Ny, e.lay, q, (const V2<double>*)e.twN64.p,
-                             sl);
-        });
-        if (ms < best) best = ms, pick = f;
-      }
+      const FastShape pick = pick_fastest(cands, [&](const FastShape& f) {
+        FastLaunch<T>::row(M, f, ny, st, (const V*)e.S.p, (V*)T2.p, Ny, e.lay, q, (const V2<double>*)e.twN64.p, sl);
+      });
       e.frow = pick;
       g_tune[key] = {pick.var, pick.lines};
     }
@@ -645,15 +651,10 @@ void autotune(Engine& e, hipStream_t st) {
       hip_check(hipMemcpyAsync(S2.p, e.S.p, bytes, hipMemcpyDeviceToDevice, st), "copy");
       hip_check(hipMemcpyAsync(T2.p, e.S.p, bytes, hipMemcpyDeviceToDevice, st), "copy");
       ColTables tb{e.E1.p, e.Q1.p, e.Q2.p, table_rows(e.p.Ny)};
-      float best = 1e30f;
-      FastShape pick = e.fcol;
-      for (const FastShape& f : cands) {
-        float ms = time_launches(st, reps, [&] {
-          FastLaunch<T>::col(Ny, mode, f, ny, st, (V*)S2.p, (const V*)T2.p, (V*)D2.p, Nc, e.lay, tb,
-                             (const V2<double>*)e.twY64.p, sl);
-        });
-        if (ms < best) best = ms, pick = f;
-      }
+      const FastShape pick = pick_fastest(cands, [&](const FastShape& f) {
+        FastLaunch<T>::col(Ny, mode, f, ny, st, (V*)S2.p, (const V*)T2.p, (V*)D2.p, Nc, e.lay, tb,
+                           (const V2<double>*)e.twY64.p, sl);
+      });
       e.fcol = pick;
       g_tune[key] = {pick.var, pick.lines};
       hip_check(hipStreamSynchronize(st), "sync after tuning");

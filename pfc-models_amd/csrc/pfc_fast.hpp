@@ -104,7 +104,8 @@ template <bool BLOCK> __device__ __forceinline__ void line_sync() {
 // Stores of arrays that the next kernel reads on other XCDs (S~ and T cross
 // the row/column transpose): non-temporal, so they stream out while the
 // kernel runs instead of waiting as dirty L2 lines for the end-of-kernel
-// write-back.
+// write-back.  (Measured faster per step also for tiles narrower than a
+// 128-byte line, whose partial lines are then written separately.)
 template <typename V> __device__ __forceinline__ void store_stream(V* p, V v) { __builtin_nontemporal_store(v, p); }
 
 // LDS line index with one pad slot per 16 elements: Stockham scatters write
