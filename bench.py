@@ -135,6 +135,67 @@ def run_copy_calibration(reps=50):
           flush=True)
 
 
+def run_serve(n=2000, load=0.8, sample=16):
+    """The served workload: `n` seeded requests (70 % stdPFC, 30 % logPFC,
+    sides 128..2048, 500..20000 ETD1 steps, FP32 bulk), HBM budget 200 GB.
+    First all requests are queued at once (capacity, solves/s); then the same
+    mix arrives as a Poisson stream at `load` x that capacity and the
+    time-to-result (arrival to result on the host) is reported.  Accuracy: a
+    sample of served results against the plain FP64 one-at-a-time solve."""
+    import numpy as np
+    import torch
+    from pfc_hip import serve as SV
+    from pfc_hip.plan import Plan
+
+    def note(tag):
+        return lambda d, tot, t: print(f"serve[{tag}]: {d}/{tot} done at {t:.0f} s", file=sys.stderr, flush=True)
+
+    srv = SV.Server(hbm_budget=200e9)
+    cap = srv.run(SV.poisson_stream(n=n, rate=1.0), realtime=False, progress=note("capacity"))
+    srv.close()
+    rate = load * cap.solves_per_s
+    srv = SV.Server(hbm_budget=200e9)
+    live = srv.run(SV.poisson_stream(n=n, rate=rate), realtime=True, progress=note("poisson"))
+    srv.close()
+    # accuracy and the plain FP64 rate on a sample of the live results
+    reqs = {r.id: r for r in SV.poisson_stream(n=n, rate=rate)}
+    rng = np.random.default_rng(11)
+    ids = rng.choice([r.id for r in live.results if reqs[r.id].N <= 1024], size=sample, replace=False)
+    rel, t_plain = [], 0.0
+    for i in ids:
+        q = reqs[int(i)]
+        kw = SV.request_params(q, precision=64)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        pl = Plan(**kw)
+        pl.set_field(SV.request_field(q))
+        pl.step(q.steps)
+        fe = pl.reduce()[0]
+        torch.cuda.synchronize()
+        t_plain += time.perf_counter() - t0
+        pl.close()
+        got = live.results[int(i)].fe
+        rel.append(abs(got - fe) / abs(fe))
+        print(f"serve[check]: request {int(i)} N={q.N} {q.model} steps={q.steps} rel FE {rel[-1]:.2e}",
+              file=sys.stderr, flush=True)
+    rel = np.array(rel)
+    return {
+        "metric": f"served time-to-result ({n} requests, 70/30 std/log, sides 128..2048, 500..20000 ETD1 steps, FP32)",
+        "capacity_solves_per_s": round(cap.solves_per_s, 3),
+        "capacity_wall_s": round(cap.wall_s, 2),
+        "poisson_rate_per_s": round(rate, 3),
+        "solves_per_s": round(live.solves_per_s, 3),
+        "p50_s": round(live.p50_s, 3),
+        "p99_s": round(live.p99_s, 3),
+        "wall_s": round(live.wall_s, 2),
+        "pools": len(live.pools),
+        "nonfinite": int(sum(r.nonfinite for r in live.results)),
+        "check_sample": f"{sample} served results with side <= 1024 against plain FP64 one-at-a-time",
+        "rel_fe_err": {"median": float(np.median(rel)), "max": float(rel.max())},
+        "plain_fp64_solves_per_s_on_sample": round(sample / t_plain, 3),
+    }
+
+
 def run_cpu(max_seconds=20.0):
     """FP64 CPU restatement of the same step (scipy.fft, all host cores)."""
     sys.path.insert(0, ROOT)
@@ -194,11 +255,17 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-sweep", action="store_true", help="skip the batched-sweep leg")
     ap.add_argument("--no-fp32", action="store_true", help="skip the FP32 leg")
+    ap.add_argument("--serve", action="store_true",
+                    help="only run the serving leg (minutes) and print its JSON line")
+    ap.add_argument("--serve-requests", type=int, default=2000)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain device copy of known size (counter calibration) and exit")
     a = ap.parse_args()
     if a.copy_calibration:
         run_copy_calibration()
+        return
+    if a.serve:
+        print(json.dumps({"serve": run_serve(n=a.serve_requests)}), flush=True)
         return
     if a.gpus != 1:
         print("bench.py: this build measures one GPU; --gpus above 1 is not supported "

@@ -203,14 +203,17 @@ class Server:
             self.used += pool.batch.device_bytes()
         return pool, key
 
-    def run(self, requests, realtime=True):
+    def run(self, requests, realtime=True, progress=None, progress_every=10.0):
         """Serve `requests` (sorted by arrival).  With realtime=True a request
-        becomes visible at its arrival time; otherwise all are queued at once."""
+        becomes visible at its arrival time; otherwise all are queued at once.
+        `progress(done, total, seconds)` is called about every
+        `progress_every` seconds."""
         import torch
         reqs = sorted(requests, key=lambda r: r.arrival)
         pending = deque(reqs)
         results = []
         t0 = time.perf_counter()
+        last = t0
         if not realtime:
             for r in reqs:
                 r.arrival = 0.0
@@ -234,6 +237,9 @@ class Server:
                 stepped += pool.tick(self.tick_steps)
             for pool in self.pools.values():
                 results.extend(pool.retire(t0, self.keep_fields))
+            if progress is not None and time.perf_counter() - last >= progress_every:
+                last = time.perf_counter()
+                progress(len(results), len(reqs), last - t0)
             if stepped == 0 and pending and not waiting:
                 time.sleep(max(0.0, min(pending[0].arrival - (time.perf_counter() - t0), 0.01)))
         torch.cuda.synchronize()
