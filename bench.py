@@ -135,7 +135,7 @@ def run_copy_calibration(reps=50):
           flush=True)
 
 
-def run_serve(n=2000, load=0.8, sample=16):
+def run_serve(n=2000, load=0.8, sample_per_side=2):
     """The served workload: `n` seeded requests (70 % stdPFC, 30 % logPFC,
     sides 128..2048, 500..20000 ETD1 steps, FP32 bulk), HBM budget 200 GB.
     First all requests are queued at once (capacity, solves/s); then the same
@@ -157,27 +157,40 @@ def run_serve(n=2000, load=0.8, sample=16):
     srv = SV.Server(hbm_budget=200e9)
     live = srv.run(SV.poisson_stream(n=n, rate=rate), realtime=True, progress=note("poisson"))
     srv.close()
-    # accuracy and the plain FP64 rate on a sample of the live results
+    # accuracy: two live results per grid side against the plain FP64
+    # one-at-a-time solve
     reqs = {r.id: r for r in SV.poisson_stream(n=n, rate=rate)}
     rng = np.random.default_rng(11)
-    ids = rng.choice([r.id for r in live.results if reqs[r.id].N <= 1024], size=sample, replace=False)
-    rel, t_plain = [], 0.0
+    ids = []
+    for side in sorted({q.N for q in reqs.values()}):
+        pool = [r.id for r in live.results if reqs[r.id].N == side]
+        ids += list(rng.choice(pool, size=min(sample_per_side, len(pool)), replace=False))
+    rel = []
     for i in ids:
         q = reqs[int(i)]
-        kw = SV.request_params(q, precision=64)
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        pl = Plan(**kw)
+        pl = Plan(**SV.request_params(q, precision=64))
         pl.set_field(SV.request_field(q))
         pl.step(q.steps)
         fe = pl.reduce()[0]
-        torch.cuda.synchronize()
-        t_plain += time.perf_counter() - t0
         pl.close()
-        got = live.results[int(i)].fe
-        rel.append(abs(got - fe) / abs(fe))
+        rel.append(abs(live.results[int(i)].fe - fe) / abs(fe))
         print(f"serve[check]: request {int(i)} N={q.N} {q.model} steps={q.steps} rel FE {rel[-1]:.2e}",
               file=sys.stderr, flush=True)
+    # the plain FP64 one-at-a-time rate on the whole mix: each (side, model)'s
+    # FP64 step time, measured on its own plan, times every request's steps
+    t_step = {}
+    for key in sorted({(q.N, q.model) for q in reqs.values()}):
+        q = next(r for r in reqs.values() if (r.N, r.model) == key)
+        pl = Plan(**SV.request_params(q, precision=64))
+        pl.set_field(SV.request_field(q))
+        pl.step(20)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        pl.step(200)
+        pl.reduce()
+        t_step[key] = (time.perf_counter() - t0) / 200
+        pl.close()
+    plain_total = sum(q.steps * t_step[(q.N, q.model)] for q in reqs.values())
     rel = np.array(rel)
     return {
         "metric": f"served time-to-result ({n} requests, 70/30 std/log, sides 128..2048, 500..20000 ETD1 steps, FP32)",
@@ -190,9 +203,11 @@ def run_serve(n=2000, load=0.8, sample=16):
         "wall_s": round(live.wall_s, 2),
         "pools": len(live.pools),
         "nonfinite": int(sum(r.nonfinite for r in live.results)),
-        "check_sample": f"{sample} served results with side <= 1024 against plain FP64 one-at-a-time",
+        "check_sample": f"{len(ids)} served results ({sample_per_side} per grid side) against plain FP64 one-at-a-time",
         "rel_fe_err": {"median": float(np.median(rel)), "max": float(rel.max())},
-        "plain_fp64_solves_per_s_on_sample": round(sample / t_plain, 3),
+        "plain_fp64_solves_per_s": round(n / plain_total, 3),
+        "plain_fp64_basis": "per-step FP64 time of each (side, model) on its own plan x every request's steps",
+        "capacity_vs_plain_fp64": round(cap.solves_per_s / (n / plain_total), 2),
     }
 
 
