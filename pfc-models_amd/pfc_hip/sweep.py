@@ -17,7 +17,8 @@ Two ways to run it:
 
 Both return per-instance FP64 free energy per site and mean density, which is
 what the common-tangent construction consumes (`(phi, fmin)` tables, cf.
-phase_diagram/common_tangent.py:336-403 in the reference).
+phase_diagram/common_tangent.py:336-403 in the reference); `coexistence_by_r`
+applies it (`coexistence.py`) at every r of the sweep.
 """
 from __future__ import annotations
 
@@ -77,26 +78,35 @@ class SweepResult:
     wall_s: float
     solves_per_s: float
     nonfinite: np.ndarray  # status flags
+    fe_prev: np.ndarray = None  # free energy `checkpoint` steps before the end (run_plain)
 
 
-def run_plain(instances, steps, dt=0.5, precision=64, scheme="etd1"):
-    """One instance at a time, one Plan each."""
+def run_plain(instances, steps, dt=0.5, precision=64, scheme="etd1", checkpoint=None):
+    """One instance at a time, one Plan each.  With `checkpoint`, the free
+    energy is also read `checkpoint` steps before the end (how far each
+    relaxation still moves)."""
     import torch
     from .plan import Plan
     n = len(instances)
     fe, mean, bad = np.zeros(n), np.zeros(n), np.zeros(n, dtype=np.int32)
+    prev = np.zeros(n) if checkpoint else None
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for j, inst in enumerate(instances):
         pl = Plan(**_model_kw(inst, dt, precision, scheme))
         pl.set_field(initial_field(inst))
-        pl.step(steps)
+        if checkpoint:
+            pl.step(steps - checkpoint)
+            prev[j] = pl.reduce()[0]
+            pl.step(checkpoint)
+        else:
+            pl.step(steps)
         fe[j], mean[j], _, _ = pl.reduce()
         bad[j] = pl.status()
         pl.close()
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
-    return SweepResult(fe, mean, wall, n / wall, bad)
+    return SweepResult(fe, mean, wall, n / wall, bad, prev)
 
 
 class BatchedSweep:
@@ -166,3 +176,14 @@ def run_batched(instances, steps, dt=0.5, precision=32, scheme="etd1", chunk=Non
     wall = time.perf_counter() - t0
     sw.close()
     return SweepResult(fe, mean, wall, len(instances) / wall, bad)
+
+
+def coexistence_by_r(instances, fe):
+    """Liquid-crystal coexistence (common tangent of the analytic liquid and
+    the tabulated relaxed crystals) at every r among `instances`, `fe[j]`
+    being the free energy of `instances[j]`: {r: Tangent}."""
+    from .coexistence import std_sweep_coexistence
+    groups = {}
+    for inst, f in zip(instances, fe):
+        groups.setdefault(inst.r, []).append((inst.psibar, f))
+    return {r: std_sweep_coexistence(r, [p for p, _ in g], [f for _, f in g]) for r, g in sorted(groups.items())}

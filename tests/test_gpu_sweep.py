@@ -27,3 +27,25 @@ def test_batched_fp32_sweep_matches_plain_fp64(hip_device):
     assert np.abs(b32.mean - plain.mean).max() <= 1e-6
     psibar = np.array([i.psibar for i in inst])
     assert np.abs(plain.mean - psibar).max() <= 0.01
+
+
+def test_coexistence_densities_fp32_vs_fp64(hip_device):
+    """Liquid-crystal coexistence densities (common tangent of the analytic
+    liquid and the relaxed crystals, 16 psibar values per r) from the FP32
+    batched sweep against the FP64 batched sweep at three r values, 5000 ETD1
+    steps: <= 1e-4 absolute wherever both find a tangent."""
+    from pfc_hip import sweep as SW
+    full = SW.sweep_instances()
+    rs = sorted({i.r for i in full})
+    pick = {rs[0], rs[4], rs[8]}
+    inst = [i for i in full if i.r in pick]
+    b32 = SW.run_batched(inst, 5000, precision=32)
+    b64 = SW.run_batched(inst, 5000, precision=64)
+    c32 = SW.coexistence_by_r(inst, b32.fe)
+    c64 = SW.coexistence_by_r(inst, b64.fe)
+    found = [r for r in pick if c64[r].status == "ok"]
+    assert found, {r: c64[r].status for r in pick}
+    for r in found:
+        assert c32[r].status == "ok"
+        assert abs(c32[r].phi_a - c64[r].phi_a) <= 1e-4, (r, c32[r], c64[r])
+        assert abs(c32[r].phi_b - c64[r].phi_b) <= 1e-4, (r, c32[r], c64[r])
