@@ -1,0 +1,159 @@
+"""Shared machinery of the phase-diagram adapters (stdpfc_hip, logpfc_hip).
+
+The adapter contract is the reference's (phase_diagram/q0star.py:5-20; adapters
+models/stdpfc_cpu/phase_diagram.py, models/logpfc_cpu/phase_diagram.py):
+structures as R2C (row, col) index sets of a rectangular unit cell of the
+triangular lattice, a box scaled by the lattice scale q00, a 3-mode ansatz and
+a 3-mode fitness.  Everything here runs on the host in FP64 on (Ny, Nx)
+arrays; the simulation itself is the hip Simulation.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass
+
+import numpy as np
+
+from ..discretization import Build2DGeometry
+from .. import phase_diagram as _pd
+
+
+@dataclass(frozen=True)
+class Structure:
+    """One target lattice: `kvectors` = (1-mode, 2-mode, 3-mode) sets of R2C
+    (row, col) = (ky, kx) indices for one rectangular cell; `amplitude` is the
+    ansatz scale (absolute for stdPFC, in units of phibar for logPFC);
+    `maxima_per_cell` the expected density maxima per cell (None: ridges)."""
+    name: str
+    kvectors: tuple
+    amplitudeSign: int
+    amplitude: float
+    maxima_per_cell: int | None
+
+
+# the three R2C 3-mode sets of the rectangular cell (models/stdpfc_cpu/phase_diagram.py:55-92)
+HEX_SETS = (((1, 1), (2, 0), (-2, 0), (-1, 1)), ((0, 2), (3, 1), (-3, 1)), ((2, 2), (4, 0), (-4, 0), (-2, 2)))
+STRIPE_SETS = (((2, 0), (-2, 0)), ((4, 0), (-4, 0)), ((6, 0), (-6, 0)))
+
+PARAM_KEYS = ('beta', 'g', 'v0', 'Gamma', 'q0')
+CELL_KEYS = ('uNx', 'uNy', 'Mx', 'My')
+ANSATZ_KEYS = ('mode', 'A1', 'A2', 'A3')
+
+
+def validate_params(slug, params):
+    """beta, g required; v0 must be 1 (the quartic coefficient of both models);
+    Gamma, q0 default 1."""
+    p = dict(params)
+    unknown = sorted(set(p) - set(PARAM_KEYS))
+    if unknown:
+        raise ValueError(f'{slug}: unknown model parameters {unknown}; allowed {list(PARAM_KEYS)}')
+    missing = [k for k in ('beta', 'g') if k not in p]
+    if missing:
+        raise ValueError(f'{slug}: missing model parameters {missing}')
+    v0 = float(p.get('v0', 1.0))
+    if v0 != 1.0:
+        raise ValueError(f'{slug}: v0={v0} not supported; the quartic coefficient is 1 in this model')
+    return {'beta': float(p['beta']), 'g': float(p['g']), 'v0': 1.0, 'Gamma': float(p.get('Gamma', 1.0)),
+            'q0': float(p.get('q0', 1.0))}
+
+
+def validate_cell(slug, cell):
+    """uNx, uNy: grid points per rectangular unit cell; Mx, My: cells per box side."""
+    c = dict(cell)
+    if sorted(c) != sorted(CELL_KEYS):
+        raise ValueError(f'{slug}: cell needs exactly {list(CELL_KEYS)}, got {sorted(c)}')
+    out = {}
+    for k in CELL_KEYS:
+        v = c[k]
+        if int(v) != v or int(v) < 1:
+            raise ValueError(f'{slug}: cell {k}={v!r} must be a positive integer')
+        out[k] = int(v)
+    return out
+
+
+def validate_ansatz(slug, ansatz):
+    """mode (a label such as 'L2H'); A1, A2, A3: 1-, 2-, 3-mode amplitudes in
+    units of the structure's ansatz scale."""
+    a = dict(ansatz)
+    if sorted(a) != sorted(ANSATZ_KEYS):
+        raise ValueError(f'{slug}: ansatz needs exactly {list(ANSATZ_KEYS)}, got {sorted(a)}')
+    if not isinstance(a['mode'], str):
+        raise ValueError(f'{slug}: ansatz mode must be a string label')
+    return {'mode': a['mode'], 'A1': float(a['A1']), 'A2': float(a['A2']), 'A3': float(a['A3'])}
+
+
+def mode_indices(structure, cell):
+    """The 1-, 2-, 3-mode (rows, cols) index arrays scaled to the Mx x My supercell."""
+    out = []
+    for pos in structure.kvectors:
+        rows, cols = np.array(pos).T
+        out.append((rows * cell['My'], cols * cell['Mx']))
+    return out
+
+
+def grid_shape(cell):
+    return cell['uNy'] * cell['My'], cell['uNx'] * cell['Mx']
+
+
+def box(cell, q00):
+    """Rectangular cell of the triangular lattice with its first shell at q00:
+    a = 4 pi / (sqrt 3 q00), h = 4 pi / q00; box Mx a x My h."""
+    Ny, Nx = grid_shape(cell)
+    a = 4 * np.pi / (3 ** 0.5 * q00)
+    h = 4 * np.pi / q00
+    return Build2DGeometry(Nx=Nx, Ny=Ny, Lx=cell['Mx'] * a, Ly=cell['My'] * h)
+
+
+def three_mode(structure, cell, mean, A, B, C, nonnegative=False):
+    """Field with the given mean and 1-, 2-, 3-mode amplitudes (irfft2 of a
+    spectrum set at the structure's index sets); `nonnegative` rescales the
+    modulation about the mean so that the minimum is zero when it would be
+    negative (the logPFC ansatz)."""
+    Ny, Nx = grid_shape(cell)
+    size = Nx * Ny
+    (r1, c1), (r2, c2), (r3, c3) = mode_indices(structure, cell)
+    spec = np.zeros((Ny, Nx // 2 + 1), dtype=complex)
+    spec[0, 0] = mean * size
+    spec[r1, c1] = A * size
+    spec[r2, c2] = B * size
+    spec[r3, c3] = C * size
+    field = np.fft.irfftn(spec, s=(Ny, Nx), axes=(0, 1)).real
+    if nonnegative:
+        lo = field.min()
+        if lo < 0:
+            field = mean + (field - mean) * mean / (mean - lo)
+    return field
+
+
+def fitness(field, structure, cell, relative=False, nonnegative=False):
+    """1 - mean((field - field_3mode)^2) and the amplitudes (A, B, C): mean
+    |rfft2(field)| over the 1-, 2-, 3-mode sets per site (divided by the mean
+    when `relative`), field_3mode rebuilt from them."""
+    Ny, Nx = grid_shape(cell)
+    size = Nx * Ny
+    spec = np.fft.rfft2(field)
+    mean = spec.flat[0].real / size
+    (r1, c1), (r2, c2), (r3, c3) = mode_indices(structure, cell)
+    s = structure.amplitudeSign
+    amps = []
+    for r, c in ((r1, c1), (r2, c2), (r3, c3)):
+        a = s * np.nanmean(np.abs(spec[r, c])) / size
+        amps.append(a / mean if relative else a)
+    A, B, C = amps
+    k = mean if relative else 1.0
+    rec = three_mode(structure, cell, mean, A * k, B * k, C * k, nonnegative=nonnegative)
+    return 1 - float(np.abs(np.power(field - rec, 2).mean())), np.abs(A), np.abs(B), np.abs(C)
+
+
+def maxima_per_cell(field, cell):
+    """Strict local maxima on the periodic grid (8 neighbours), per rectangular cell."""
+    is_max = np.ones(field.shape, dtype=bool)
+    for dy in (-1, 0, 1):
+        for dx in (-1, 0, 1):
+            if dy or dx:
+                is_max &= field > np.roll(field, (dy, dx), axis=(0, 1))
+    return int(is_max.sum()) / (cell['Mx'] * cell['My'])
+
+
+def diagnostics(field, structure, cell, relative=False, nonnegative=False):
+    f, A, B, C = fitness(field, structure, cell, relative=relative, nonnegative=nonnegative)
+    return _pd.Diagnostics(fitness=f, amplitudes=(A, B, C), details={'maxima_per_cell': maxima_per_cell(field, cell)})

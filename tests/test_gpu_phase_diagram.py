@@ -1,0 +1,148 @@
+"""The phase-diagram driver over the hip adapters (needs a GPU): logPFC
+evaluations and the q00 scan against the values the reference's driver tests
+hold (tests/test_phase_diagram_q0star.py:44-64, 486-556, 612-643), the
+uniform state, the stdPFC g = 0 mirror, and `run` against a step loop.
+
+Tolerances are the reference's own for these comparisons: f to 3e-14 and
+(fitness, A, B, C) to 1e-12 per evaluation (2e-14 and 1e-13 without ansatz
+noise), q0* to 1e-8 and fmin to 1e-13, the honeycomb's converged f to 1e-12.
+"""
+import dataclasses
+import json
+import os
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+REFS = json.load(open(os.path.join(HERE, "golden", "phase_diagram_refs.json")))
+
+
+def _settings():
+    from pfc_hip.phase_diagram import q0star as Q
+    return Q.RelaxSettings(**REFS["relax"]), Q.QScanSettings(**REFS["qscan"])
+
+
+def _log_minimizer(structure, **relax_changes):
+    from pfc_hip.models.logpfc_hip import phase_diagram as LOG
+    from pfc_hip.phase_diagram import q0star as Q
+    relax, qscan = _settings()
+    relax = dataclasses.replace(relax, **relax_changes)
+    return Q.MinimizeFreeEnergy(LOG, REFS["log_params"], structure, REFS["cell"], REFS["ansatz"], relax, qscan)
+
+
+def _ref(structure):
+    return next(r for r in REFS["get_f_abc"] if r["structure"] == structure)
+
+
+@pytest.mark.parametrize("structure", ["triangle", "stripe"])
+def test_log_get_f_abc_against_reference(hip_device, structure):
+    from pfc_hip.phase_diagram import q0star as Q
+    r = _ref(structure)
+    m = _log_minimizer(structure)
+    f, fit, A, B, C = (float(x) for x in m.Get_f_ABC(temp=r["eps"], meanDensity=r["phi"], q00=r["q00"]))
+    print(f"\n{structure}: df {f - r['f']:.3e} dfit {fit - r['fitness']:.3e} dA {A - r['A']:.3e} "
+          f"dB {B - r['B']:.3e} dC {C - r['C']:.3e} steps {m.last_steps}", flush=True)
+    assert m.last_stop == Q.STOP_CONVERGED
+    assert m.last_steps == r["steps"]
+    assert abs(f - r["f"]) <= r["tol_f"]
+    assert (fit, A, B, C) == pytest.approx((r["fitness"], r["A"], r["B"], r["C"]), abs=r["tol_diag"])
+
+
+@pytest.mark.parametrize("structure", ["triangle", "stripe"])
+def test_log_get_f_abc_without_noise(hip_device, structure):
+    r = _ref(structure)
+    m = _log_minimizer(structure, noise=0.0)
+    f, fit, A, B, C = (float(x) for x in m.Get_f_ABC(temp=r["eps"], meanDensity=r["phi"], q00=r["q00"]))
+    print(f"\n{structure} noise 0: df {f - r['f']:.3e} dfit {fit - r['fitness']:.3e} dA {A - r['A']:.3e} "
+          f"steps {m.last_steps}", flush=True)
+    assert m.last_steps == r["steps"]
+    assert abs(f - r["f"]) <= 2e-14
+    assert (fit, A, B, C) == pytest.approx((r["fitness"], r["A"], r["B"], r["C"]), abs=1e-13)
+
+
+def test_log_honeycomb_integrates_through_the_transient(hip_device):
+    # dt = 0.1 drives phi negative in the first block (NaN from the log): the
+    # rollback redoes it at reduced dt and the evaluation converges on the
+    # relaxed honeycomb, far below the transient the unrefined run stopped at
+    from pfc_hip.phase_diagram import q0star as Q
+    r = _ref("honeycomb")
+    m = _log_minimizer("honeycomb")
+    f = float(m.Get_f_ABC(temp=r["eps"], meanDensity=r["phi"], q00=r["q00"])[0])
+    print(f"\nhoneycomb: df {f - r['f']:.3e} steps {m.last_steps}", flush=True)
+    assert m.last_stop == Q.STOP_CONVERGED
+    assert abs(f - r["f"]) <= r["tol_f"]
+    assert f < r["f_original_transient"] - r["below_original_by"]
+    assert m.last_steps == r["steps"]
+
+
+def test_log_find_q0star_against_reference(hip_device):
+    ref = REFS["find_q0star"]
+    recs = []
+    q, f = _log_minimizer(ref["structure"]).Find_q0Star(temp=ref["eps"], meanDensity=ref["phi"],
+                                                        record=recs.append)
+    print(f"\nq0*: dq {float(q) - ref['q0star']:.3e} dfmin {float(f) - ref['fmin']:.3e} "
+          f"evaluations {len(recs)} steps {sum(r['steps'] for r in recs)}", flush=True)
+    assert abs(float(q) - ref["q0star"]) <= ref["tol_q"]
+    assert abs(float(f) - ref["fmin"]) <= ref["tol_f"]
+
+
+def test_log_points_without_a_local_minimum(hip_device):
+    from pfc_hip.phase_diagram import q0star as Q
+    ref = REFS["no_local_minimum"]
+    r = Q.scan_density(_log_minimizer(ref["structure"]), ref["eps"], ref["phi"])
+    for k in range(len(ref["phi"])):
+        assert (str(r["status"][k]), str(r["message"][k])) == (Q.STATUS_NO_LOCAL_MINIMUM, ref["message"])
+        assert np.isnan(r["q0star"][k]) and np.isnan(r["fmin"][k])
+
+
+@pytest.mark.parametrize("slug, density", [("logpfc_hip", 0.7), ("stdpfc_hip", -0.3)])
+def test_liquid_free_energy_is_the_uniform_state(hip_device, slug, density):
+    from pfc_hip.models.logpfc_hip import phase_diagram as LOG
+    from pfc_hip.models.stdpfc_hip import phase_diagram as STD
+    ad = {"logpfc_hip": LOG, "stdpfc_hip": STD}[slug]
+    params, eps = dict(beta=0.5, g=-2.0), -0.9
+    sim = ad.build(params, "triangle", REFS["cell"], REFS["ansatz"], eps, density, 1.0, 0.1,
+                   state=np.full((24, 28), density))
+    assert float(ad.free_energy(sim)) == pytest.approx(ad.liquid_free_energy(density, params, eps), rel=1e-14)
+
+
+def test_std_g0_triangle_mirrors_honeycomb(hip_device):
+    # at g = 0 f(-n) = f(n) and the dynamics are odd in n: the triangle at
+    # -nbar is the exact negative of the honeycomb at +nbar, step for step
+    from pfc_hip.models.stdpfc_hip import phase_diagram as STD
+    from pfc_hip.phase_diagram import q0star as Q
+    ref = REFS["std_mirror"]
+    relax, qscan = _settings()
+    tri = Q.MinimizeFreeEnergy(STD, ref["params"], "triangle", REFS["cell"], REFS["ansatz"], relax, qscan)
+    hc = Q.MinimizeFreeEnergy(STD, ref["params"], "honeycomb", REFS["cell"], REFS["ansatz"], relax, qscan)
+    a = [float(x) for x in tri.Get_f_ABC(temp=ref["eps"], meanDensity=-ref["nbar"], q00=1.0)]
+    b = [float(x) for x in hc.Get_f_ABC(temp=ref["eps"], meanDensity=+ref["nbar"], q00=1.0)]
+    print(f"\nmirror: df {a[0] - ref['f']:.3e} dfit {a[1] - ref['fitness']:.3e} dA {a[2] - ref['A']:.3e} "
+          f"steps {tri.last_steps}", flush=True)
+    assert a == b and tri.last_steps == hc.last_steps
+    assert a[1] > 0.999 and a[2] > 0.1
+    assert a[0] < STD.liquid_free_energy(ref["nbar"], ref["params"], ref["eps"])
+    # the reference's values at this point
+    assert tri.last_steps == ref["steps"]
+    assert abs(a[0] - ref["f"]) <= 1e-13
+    assert (a[1], a[2]) == pytest.approx((ref["fitness"], ref["A"]), abs=1e-11)
+
+
+def test_run_matches_a_step_loop(hip_device):
+    # the driver advances a block with the adapter's `run` (one device call);
+    # without it, with `step` per time step: the same evaluation
+    import types
+    from pfc_hip.models.logpfc_hip import phase_diagram as LOG
+    from pfc_hip.phase_diagram import q0star as Q
+    steps_only = types.SimpleNamespace(**{k: getattr(LOG, k) for k in dir(LOG) if not k.startswith("_")})
+    del steps_only.run
+    relax, qscan = _settings()
+    relax = dataclasses.replace(relax, totalT=300.0)
+    out = []
+    for ad in (LOG, steps_only):
+        m = Q.MinimizeFreeEnergy(ad, REFS["log_params"], "triangle", REFS["cell"], REFS["ansatz"], relax, qscan)
+        out.append(([float(x) for x in m.Get_f_ABC(temp=-0.9, meanDensity=0.63, q00=0.96)], m.last_steps))
+    assert out[0] == out[1]
