@@ -105,23 +105,28 @@ int pfc_local_field(pfc_plan* plan, void* out, int dtype, void* stream);
  * state (the plan's state is not advanced), bracketed by one HIP event pair on
  * `stream`; ms[k] is the total and calls[k] the launch count of kind k (0 row
  * pass, 1 first column pass, 2 ETDRK2 second column pass; 0 calls when the
- * scheme has no such pass).  Runs the autotuner first if no step has.
+ * scheme has no such pass; on the resident stepper kind 0 is one whole step
+ * per launch and kinds 1, 2 are empty).  Runs the autotuner first if no step has.
  * Synchronises `stream`.  (No reference counterpart: profiling
  * instrumentation lives outside the model code, README.md:59-60.) */
 int pfc_step_profile(pfc_plan* plan, long long nsteps, void* stream, double ms[3], long long calls[3]);
 /* Kernel schedule in use (after the first step has run the autotuner):
  * info[0..3] = row kernel  {compile-time schedule in use, variant, points per lane, rows per block},
  * info[4..7] = column kernel {same, columns per block}.  A zero in info[0] or
- * info[4] means the generic mixed-radix kernel runs.  (No reference counterpart.) */
+ * info[4] means the generic mixed-radix kernel runs; -1 in both means the
+ * resident stepper runs (small ETD1 instances: one workgroup per instance
+ * holds the whole row spectrum in LDS for all the steps of a call).
+ * (No reference counterpart.) */
 int pfc_plan_schedule(const pfc_plan* plan, int info[8]);
 /* Device bytes held by a plan (state, tables, scratch). */
 int pfc_plan_bytes(const pfc_plan* plan, long long* bytes);
 
 /* ---- batches: many same-shape instances stepped by the same launches ----
  * A batch holds `capacity` slots sharing the grid, model, scheme, precision
- * and every parameter of pfc_params except epsilon and M, which are per slot
- * (pfc_batch_set_slot; initially the shared values).  Each slot has its own
- * field, ETD tables, k = 0 pin and status word.  pfc_batch_step advances the
+ * and every parameter of pfc_params except epsilon and M (pfc_batch_set_slot)
+ * and the box Lx, Ly and time step dt (pfc_batch_set_slot_box), which are per
+ * slot, initially the shared values.  Each slot has its own field, k-grids,
+ * ETD tables, k = 0 pin and status word.  pfc_batch_step advances the
  * active slots (initially all) together: one row launch and one column launch
  * per nonlinear evaluation, whatever the number of slots.  A slot's results do
  * not depend on which other slots are active.  (No reference counterpart: the
@@ -133,6 +138,13 @@ int pfc_batch_create(const pfc_params* shared, int capacity, pfc_batch** out);
 int pfc_batch_destroy(pfc_batch* batch);
 /* Per-slot epsilon and stabilising shift M; rebuilds that slot's tables (synchronises `stream`). */
 int pfc_batch_set_slot(pfc_batch* batch, int slot, double epsilon, double M, void* stream);
+/* Per-slot box Lx x Ly and time step dt: the slot's k-grids (the k-grids of
+ * Build2DGeometry, discretization/spectral/geometry_2d.py:70-88) and ETD
+ * tables are rebuilt (synchronises `stream`); the field is kept as stored, so
+ * set it again when the box changes meaning.  The phase-diagram driver's
+ * evaluations at different lattice scales q00 (phase_diagram/q0star.py:409-477)
+ * each need their own box, and its rollback refinement its own dt (:261-307). */
+int pfc_batch_set_slot_box(pfc_batch* batch, int slot, double Lx, double Ly, double dt, void* stream);
 int pfc_batch_set_field(pfc_batch* batch, int slot, const void* field, int dtype, void* stream);
 int pfc_batch_get_field(pfc_batch* batch, int slot, void* field, int dtype, void* stream);
 /* Slots stepped and reduced by the next calls, in this order (synchronises `stream`). */

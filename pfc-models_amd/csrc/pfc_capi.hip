@@ -338,6 +338,11 @@ std::vector<double> fft_k(int n, double d) {
   }
   return k;
 }
+std::vector<double> k_squared(const std::vector<double>& k) {
+  std::vector<double> k2(k.size());
+  for (size_t i = 0; i < k.size(); ++i) k2[i] = k[i] * k[i];
+  return k2;
+}
 
 void validate(const pfc_params* p) {
   if (!p) fail(PFC_EINVAL, "params is NULL");
@@ -364,11 +369,13 @@ struct Engine {
   RowGeom rg{}, rg64{};
   ColGeom cg{}, cg64{};
   FastShape frow{}, fcol{};
+  ResGeom res{};
+  bool resident = false;  // small ETD1 instances: k_resident_etd1, all steps in LDS
   bool tuned = false, batched = false;
   DevBuf S, Tb, D, E1, Q1, Q2;       // [cap][...]
-  DevBuf twN, twY, twN64, twY64, kx2, ky2;
+  DevBuf twN, twY, twN64, twY64, kx2, ky2, dts;  // kx2 [cap][Nc], ky2 [cap][Ny], dts [cap]
   DevBuf part, out, pin, status, eps, Ms, active, diag;
-  std::vector<double> h_eps, h_M;
+  std::vector<double> h_eps, h_M, h_Lx, h_Ly, h_dt;
   std::vector<int> h_active;
   std::vector<char> field_set;
   double* host_out = nullptr;   // pinned [4 cap]
@@ -394,6 +401,19 @@ struct Engine {
   }
 };
 
+// The resident stepper holds four Ny x (Nx/2 + 1) complex arrays in LDS; it
+// is used for ETD1 when they fit in 64 KiB (e.g. up to 30 x 30 in FP64),
+// where the two-kernel step is launch- and latency-bound.  PFC_HIP_RESIDENT=0
+// turns it off.
+constexpr size_t kResidentLds = 64 * 1024;
+size_t resident_lds(int Nx, int Ny, size_t csize) { return 4 * (size_t)Ny * (Nx / 2 + 1) * csize; }
+bool resident_fits(const pfc_params& p) {
+  const char* env = std::getenv("PFC_HIP_RESIDENT");
+  if (env && std::string(env) == "0") return false;
+  const size_t lds = resident_lds(p.Nx, p.Ny, p.precision == 64 ? sizeof(V2<double>) : sizeof(V2<float>));
+  return p.scheme == PFC_SCHEME_ETD1 && lds <= std::min(kResidentLds, (size_t)lds_limit());
+}
+
 void engine_init(Engine& e, const pfc_params* params, int cap, bool batched) {
   validate(params);
   if (cap < 1) fail(PFC_EINVAL, "capacity must be >= 1");
@@ -411,7 +431,9 @@ void engine_init(Engine& e, const pfc_params* params, int cap, bool batched) {
   e.rg64 = row_geom(Nx, Ny, sizeof(V2<double>));
   e.cg64 = col_geom(Nx, Ny, sizeof(V2<double>));
   e.rg.lay = e.cg.lay = e.rg64.lay = e.cg64.lay = e.lay;
-  if (!std::getenv("PFC_HIP_GENERIC")) {
+  e.res = ResGeom{Nx / 2, e.Nc, Ny, table_rows(Ny), e.lay, e.rg.plM, e.cg.plY};
+  e.resident = resident_fits(*params);
+  if (!e.resident && !std::getenv("PFC_HIP_GENERIC")) {
     auto rc = row_candidates(Nx, Ny, e.csize);
     auto cc = col_candidates(Nx, Ny, e.csize);
     e.frow = pick_default(rc, 512);
@@ -440,13 +462,19 @@ void engine_init(Engine& e, const pfc_params* params, int cap, bool batched) {
   e.E1.alloc(tbytes);
   e.Q1.alloc(tbytes);
   if (params->scheme == PFC_SCHEME_ETDRK2) e.Q2.alloc(tbytes);
-  auto kx = rfft_k(Nx, params->Lx / Nx);
-  auto ky = fft_k(Ny, params->Ly / Ny);
-  std::vector<double> kx2(kx.size()), ky2(ky.size());
-  for (size_t i = 0; i < kx2.size(); ++i) kx2[i] = kx[i] * kx[i];
-  for (size_t i = 0; i < ky2.size(); ++i) ky2[i] = ky[i] * ky[i];
-  e.kx2.upload(kx2);
-  e.ky2.upload(ky2);
+  // every slot starts in the shared box with the shared dt
+  const std::vector<double> kx2 = k_squared(rfft_k(Nx, params->Lx / Nx)), ky2 = k_squared(fft_k(Ny, params->Ly / Ny));
+  std::vector<double> kx2s, ky2s;
+  for (int i = 0; i < cap; ++i) {
+    kx2s.insert(kx2s.end(), kx2.begin(), kx2.end());
+    ky2s.insert(ky2s.end(), ky2.begin(), ky2.end());
+  }
+  e.kx2.upload(kx2s);
+  e.ky2.upload(ky2s);
+  e.h_Lx.assign(cap, params->Lx);
+  e.h_Ly.assign(cap, params->Ly);
+  e.h_dt.assign(cap, params->dt);
+  e.dts.upload(e.h_dt);
   auto tn = twiddles(Nx), ty = twiddles(Ny);
   e.twN64.upload(tn);
   e.twY64.upload(ty);
@@ -483,9 +511,10 @@ void engine_tables(Engine& e, int slot0, int n, hipStream_t st) {
   TabParams tp{e.p.Gamma, e.p.beta, e.p.q0, 0.0, 0.0, e.p.dt, e.p.smallZTol,
                1.0 / e.p.Ny, 1.0 / ((double)e.p.Nx * (double)e.p.Ny), e.Nc, e.p.Ny, table_rows(e.p.Ny)};
   const int blocks = (int)std::min<size_t>((e.ntab + 255) / 256, 1024);
+  const SlotGrid sg{(const double*)e.kx2.p, (const double*)e.ky2.p, (const double*)e.dts.p};
   hipLaunchKernelGGL(k_tables<double>, dim3(blocks, n), dim3(256), 0, st, (double*)e.E1.p, (double*)e.Q1.p,
-                     (double*)e.Q2.p, (long long)e.ntab, (const double*)e.kx2.p, (const double*)e.ky2.p, tp,
-                     (const double*)e.eps.p, (const double*)e.Ms.p, slot0);
+                     (double*)e.Q2.p, (long long)e.ntab, sg, tp, (const double*)e.eps.p, (const double*)e.Ms.p,
+                     slot0);
   hip_check(hipGetLastError(), "table launch");
 }
 
@@ -535,10 +564,27 @@ struct Steps {
     else
       hipLaunchKernelGGL((k_col_etd<T, COL_RK2_STAGE2>), g, dim3(256), lds, st, S, Tb, D, e.cg, tb, tw, sl);
   }
+  // nsteps ETD1 steps of every active slot, one workgroup each, in launches
+  // of at most kResidentChunk steps
+  static void resident(Engine& e, long long nsteps, hipStream_t st, const Slots& sl, int ny, V* S) {
+    constexpr long long kResidentChunk = 4096;
+    ColTables tb{e.E1.p, e.Q1.p, e.Q2.p, table_rows(e.p.Ny)};
+    const size_t lds = resident_lds(e.p.Nx, e.p.Ny, sizeof(V));
+    for (long long done = 0; done < nsteps; done += kResidentChunk) {
+      const long long n = std::min(kResidentChunk, nsteps - done);
+      hipLaunchKernelGGL((k_resident_etd1<T>), dim3(ny), dim3(256), lds, st, S, e.res, nonlin_of(e), tb,
+                         (const V*)e.twN.p, (const V*)e.twY.p, sl, n);
+    }
+  }
   static void step(Engine& e, long long nsteps, hipStream_t st) {
     const int ny = e.nact();
     if (ny == 0) return;
     const Slots sl = e.slots_active();
+    if (e.resident) {
+      resident(e, nsteps, st, sl, ny, (V*)e.S.p);
+      hip_check(hipGetLastError(), "step launch");
+      return;
+    }
     for (long long s = 0; s < nsteps; ++s) {
       row(e, st, sl, ny);
       if (e.p.scheme == PFC_SCHEME_ETD1) {
@@ -591,7 +637,7 @@ template <typename T>
 void autotune(Engine& e, hipStream_t st) {
   e.tuned = true;
   const char* env = std::getenv("PFC_HIP_TUNE");
-  if (env && env[0] == '0') return;
+  if (e.resident || (env && env[0] == '0')) return;
   using V = V2<T>;
   const int ny = e.nact();
   if (ny == 0) return;
@@ -666,7 +712,7 @@ void engine_step(Engine& e, long long nsteps, hipStream_t st) {
   if (nsteps < 0) fail(PFC_EINVAL, "nsteps < 0");
   for (int s : e.h_active)
     if (!e.field_set[s]) fail(PFC_EINVAL, "a stepped slot has no field (pfc_set_field / pfc_batch_set_field)");
-  if (!e.tuned) {
+  if (!e.tuned && !e.resident) {
     if (e.f64())
       autotune<double>(e, st);
     else
@@ -697,6 +743,12 @@ void kernel_times(Engine& e, long long n, hipStream_t st, double ms[3], long lon
   hip_check(hipMemcpyAsync(S2.p, e.S.p, bytes, hipMemcpyDeviceToDevice, st), "copy");
   hip_check(hipMemcpyAsync(T2.p, e.Tb.p, bytes, hipMemcpyDeviceToDevice, st), "copy");
   if (e.D.p) hip_check(hipMemcpyAsync(D2.p, e.D.p, bytes, hipMemcpyDeviceToDevice, st), "copy");
+  if (e.resident) {  // kind 0: one resident launch per step
+    ms[0] = (double)time_launches(st, (int)n, [&] { Steps<T>::resident(e, 1, st, sl, ny, (V*)S2.p); }) * n;
+    calls[0] = n;
+    hip_check(hipStreamSynchronize(st), "sync");
+    return;
+  }
   const bool rk2 = e.p.scheme == PFC_SCHEME_ETDRK2;
   for (int k = 0; k < (rk2 ? 3 : 2); ++k) {
     const float m = time_launches(st, (int)n, [&] {
@@ -716,11 +768,11 @@ void kernel_times(Engine& e, long long n, hipStream_t st, double ms[3], long lon
 // Sbase is offset to it and sl.list is null); results land in e.out[4 i ..];
 // pin_out (device, per slot) receives the means.
 template <typename T>
-void launch_reduce(Engine& e, const void* Sbase, const double* eps, hipStream_t st, const Slots& sl, int ny,
-                   double* pin_out) {
+void launch_reduce(Engine& e, const void* Sbase, const double* eps, const double* kx2, const double* ky2,
+                   hipStream_t st, const Slots& sl, int ny, double* pin_out) {
   using V = V2<T>;
   const V* S = (const V*)Sbase;
-  FeParams fp{e.p.model, e.p.beta, e.p.q0, e.p.epsilon, e.p.g, (const double*)e.kx2.p, (const double*)e.ky2.p, eps};
+  FeParams fp{e.p.model, e.p.beta, e.p.q0, e.p.epsilon, e.p.g, kx2, ky2, eps};
   Partial* part = (Partial*)e.part.p;
   const int rb = row_blocks(e.rg64), cb = col_blocks(e.cg64), ps = e.pstride();
   const size_t red = 256 * sizeof(Partial);
@@ -741,10 +793,12 @@ void reduce_slot(Engine& e, int slot, hipStream_t st, bool set_pin) {
   const void* S = (const char*)e.S.p + (size_t)slot * e.nc * e.csize;
   double* pin_out = set_pin ? (double*)e.pin.p + slot : nullptr;
   const double* eps = (const double*)e.eps.p + slot;
+  const double* kx2 = (const double*)e.kx2.p + (size_t)slot * e.Nc;
+  const double* ky2 = (const double*)e.ky2.p + (size_t)slot * e.p.Ny;
   if (e.f64())
-    launch_reduce<double>(e, S, eps, st, one, 1, pin_out);
+    launch_reduce<double>(e, S, eps, kx2, ky2, st, one, 1, pin_out);
   else
-    launch_reduce<float>(e, S, eps, st, one, 1, pin_out);
+    launch_reduce<float>(e, S, eps, kx2, ky2, st, one, 1, pin_out);
 }
 
 void engine_set_field(Engine& e, int slot, const void* field, int dtype, hipStream_t st) {
@@ -818,6 +872,24 @@ void engine_set_slot(Engine& e, int slot, double epsilon, double M, hipStream_t 
   hip_check(hipStreamSynchronize(st), "sync");
 }
 
+void engine_set_slot_box(Engine& e, int slot, double Lx, double Ly, double dt, hipStream_t st) {
+  e.check_slot(slot);
+  if (!(std::isfinite(Lx) && Lx > 0) || !(std::isfinite(Ly) && Ly > 0)) fail(PFC_EINVAL, "Lx and Ly must be positive");
+  if (!(std::isfinite(dt) && dt > 0)) fail(PFC_EINVAL, "dt must be positive");
+  const std::vector<double> kx2 = k_squared(rfft_k(e.p.Nx, Lx / e.p.Nx)), ky2 = k_squared(fft_k(e.p.Ny, Ly / e.p.Ny));
+  e.h_Lx[slot] = Lx;
+  e.h_Ly[slot] = Ly;
+  e.h_dt[slot] = dt;
+  hip_check(hipMemcpyAsync((double*)e.kx2.p + (size_t)slot * e.Nc, kx2.data(), kx2.size() * sizeof(double),
+                           hipMemcpyHostToDevice, st), "copy");
+  hip_check(hipMemcpyAsync((double*)e.ky2.p + (size_t)slot * e.p.Ny, ky2.data(), ky2.size() * sizeof(double),
+                           hipMemcpyHostToDevice, st), "copy");
+  hip_check(hipMemcpyAsync((double*)e.dts.p + slot, &e.h_dt[slot], sizeof(double), hipMemcpyHostToDevice, st), "copy");
+  engine_tables(e, slot, 1, st);
+  // the host vectors must outlive the asynchronous copies
+  hip_check(hipStreamSynchronize(st), "sync");
+}
+
 void engine_set_active(Engine& e, const int* slots, int n, hipStream_t st) {
   if (n < 0 || n > e.cap) fail(PFC_EINVAL, "active count out of range");
   if (n > 0 && !slots) fail(PFC_EINVAL, "NULL slot list");
@@ -839,9 +911,11 @@ void engine_reduce_active(Engine& e, double* out, hipStream_t st) {
   if (ny == 0) return;
   const Slots sl = e.slots_active();
   if (e.f64())
-    launch_reduce<double>(e, e.S.p, (const double*)e.eps.p, st, sl, ny, nullptr);
+    launch_reduce<double>(e, e.S.p, (const double*)e.eps.p, (const double*)e.kx2.p, (const double*)e.ky2.p, st, sl,
+                          ny, nullptr);
   else
-    launch_reduce<float>(e, e.S.p, (const double*)e.eps.p, st, sl, ny, nullptr);
+    launch_reduce<float>(e, e.S.p, (const double*)e.eps.p, (const double*)e.kx2.p, (const double*)e.ky2.p, st, sl,
+                         ny, nullptr);
   hip_check(hipMemcpyAsync(e.host_out, e.out.p, (size_t)ny * 4 * sizeof(double), hipMemcpyDeviceToHost, st), "copy");
   hip_check(hipStreamSynchronize(st), "sync");
   std::memcpy(out, e.host_out, (size_t)ny * 4 * sizeof(double));
@@ -876,7 +950,7 @@ void launch_filter(Engine& e, int slot, int mode, void* out0, void* out1, int dt
   const V* twN = (const V*)e.twN.p;
   const V* twY = (const V*)e.twY.p;
   FilterParams fp{e.p.beta, e.p.q0, e.h_eps[slot], e.h_M[slot], 6.283185307179586, e.p.Nx,
-                  (const double*)e.kx2.p, (const double*)e.ky2.p};
+                  (const double*)e.kx2.p + (size_t)slot * e.Nc, (const double*)e.ky2.p + (size_t)slot * e.p.Ny};
   const int rb = row_blocks(e.rg), cb = col_blocks(e.cg);
   const size_t rl = row_lds(e.rg, sizeof(V)), cl = col_lds(e.cg, sizeof(V));
   if (mode == FILTER_MU) {
@@ -904,7 +978,7 @@ void launch_filter(Engine& e, int slot, int mode, void* out0, void* out1, int dt
 size_t engine_bytes(const Engine& e) {
   size_t b = 0;
   for (const DevBuf* d : {&e.S, &e.Tb, &e.D, &e.E1, &e.Q1, &e.Q2, &e.twN, &e.twY, &e.twN64, &e.twY64, &e.kx2, &e.ky2,
-                          &e.part, &e.out, &e.pin, &e.status, &e.eps, &e.Ms, &e.active, &e.diag})
+                          &e.dts, &e.part, &e.out, &e.pin, &e.status, &e.eps, &e.Ms, &e.active, &e.diag})
     b += d->n;
   return b;
 }
@@ -1069,7 +1143,7 @@ int pfc_plan_schedule(const pfc_plan* pl, int info[8]) {
     if (!pl || !info) fail(PFC_EINVAL, "NULL argument");
     const FastShape* fs[2] = {&pl->e.frow, &pl->e.fcol};
     for (int k = 0; k < 2; ++k) {
-      info[4 * k + 0] = fs[k]->on ? 1 : 0;
+      info[4 * k + 0] = pl->e.resident ? -1 : fs[k]->on ? 1 : 0;
       info[4 * k + 1] = fs[k]->var;
       info[4 * k + 2] = fs[k]->P;
       info[4 * k + 3] = fs[k]->lines;
@@ -1106,6 +1180,13 @@ int pfc_batch_set_slot(pfc_batch* b, int slot, double epsilon, double M, void* s
   return guarded([&] {
     if (!b) fail(PFC_EINVAL, "NULL batch");
     engine_set_slot(b->e, slot, epsilon, M, reinterpret_cast<hipStream_t>(stream));
+  });
+}
+
+int pfc_batch_set_slot_box(pfc_batch* b, int slot, double Lx, double Ly, double dt, void* stream) {
+  return guarded([&] {
+    if (!b) fail(PFC_EINVAL, "NULL batch");
+    engine_set_slot_box(b->e, slot, Lx, Ly, dt, reinterpret_cast<hipStream_t>(stream));
   });
 }
 
@@ -1166,7 +1247,8 @@ int pfc_slot_bytes(const pfc_params* params, long long* bytes) {
     const size_t nt = (size_t)(params->Nx / 2 + 1) * table_rows(params->Ny);
     const size_t cs = params->precision == 64 ? 16 : 8, ts = sizeof(double);
     const bool rk2 = params->scheme == PFC_SCHEME_ETDRK2;
-    *bytes = (long long)(nc * cs * (rk2 ? 3 : 2) + nt * ts * (rk2 ? 3 : 2));
+    const size_t grid = (size_t)(params->Nx / 2 + 1) + params->Ny + 1;  // the slot's k-grids and dt
+    *bytes = (long long)(nc * cs * (rk2 ? 3 : 2) + nt * ts * (rk2 ? 3 : 2) + grid * ts);
   });
 }
 

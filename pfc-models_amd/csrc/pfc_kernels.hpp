@@ -323,6 +323,127 @@ __global__ void __launch_bounds__(256) k_col_etd(V2<T>* __restrict__ S, const V2
 }
 
 // ---------------------------------------------------------------------------
+// resident stepper: small instances, one workgroup each, many ETD1 steps per
+// launch with the whole instance in LDS
+// ---------------------------------------------------------------------------
+// A grid of a few thousand points is latency-bound on the two-kernel step:
+// every step pays two launches and two HBM round trips of the state for a
+// few microseconds of arithmetic.  Here a workgroup loads its instance's row
+// spectrum once, runs `nsteps` ETD1 steps (the row pass and the column pass
+// of k_row_nonlin / k_col_etd, with the same arithmetic, so the same
+// results) between four LDS buffers of Ny x Nc complex values, and stores it
+// once.  The phase-diagram driver's cells (e.g. 28 x 24) are this case.
+struct ResGeom {
+  int M, Nc, Ny;  // Nx/2, Nx/2 + 1, Ny
+  int hy;         // table rows per column
+  Lay lay;        // storage of S~ in device memory
+  Radices plM, plY;
+};
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_resident_etd1(V2<T>* __restrict__ S, ResGeom g, Nonlin q, ColTables tb,
+                                                       const V2<T>* __restrict__ twN, const V2<T>* __restrict__ twY,
+                                                       Slots sl, long long nsteps) {
+  using V = V2<T>;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  const int M = g.M, Nc = g.Nc, Ny = g.Ny, nsp = Ny * Nc;
+  V* Sl = reinterpret_cast<V*>(smem_raw);  // state S~, [y][k]
+  V* R1 = Sl + nsp;                        // T~ [y][k], then column lines
+  V* R2 = R1 + nsp;
+  V* R3 = R2 + nsp;
+  const int slot = slot_of(sl, blockIdx.x);
+  S += (size_t)slot * sl.sstride;
+  if (sl.M) q.M = sl.M[slot];
+  const double* E1 = reinterpret_cast<const double*>(tb.E1) + (size_t)slot * sl.tstride;
+  const double* Q1 = reinterpret_cast<const double*>(tb.Q1) + (size_t)slot * sl.tstride;
+  const bool pinned = sl.pin != nullptr;
+  const double pin = pinned ? sl.pin[slot] : 0.0;
+  for (int e = threadIdx.x; e < nsp; e += blockDim.x) {
+    const int y = e / Nc;
+    Sl[e] = S[g.lay.at(y, e - y * Nc)];
+  }
+  __syncthreads();
+  bool bad = false;
+  for (long long step = 0; step < nsteps; ++step) {
+    // row pass: S~ -> n -> N(n) -> T~ (k_row_nonlin's pack, FFTs and unpack)
+    for (int e = threadIdx.x; e < Ny * M; e += blockDim.x) {
+      const int line = e / M;
+      const int k = e - line * M;
+      V xk = Sl[line * Nc + k];
+      V xm = Sl[line * Nc + (M - k)];
+      if (k == 0) {
+        xk.y = T(0);
+        xm.y = T(0);
+      }
+      V sm = V{xk.x + xm.x, xk.y - xm.y};
+      V df = V{xk.x - xm.x, xk.y + xm.y};
+      V wd = cmul(cconj(twN[k]), df);
+      R2[line * Nc + k] = V{sm.x - wd.y, sm.y + wd.x};
+    }
+    __syncthreads();
+    V* z = fft_lines<T, true>(R2, R3, g.plM, Ny, Nc, twN, 2);
+    for (int e = threadIdx.x; e < Ny * M; e += blockDim.x) {
+      const int line = e / M;
+      const int m = e - line * M;
+      V v = z[line * Nc + m];
+      bad |= !(isfinite(v.x) && isfinite(v.y));
+      z[line * Nc + m] = V{nonlin_mu<T>(v.x, q), nonlin_mu<T>(v.y, q)};
+    }
+    __syncthreads();
+    V* Z = fft_lines<T, false>(z, z == R2 ? R3 : R2, g.plM, Ny, Nc, twN, 2);
+    for (int e = threadIdx.x; e < nsp; e += blockDim.x) {
+      const int line = e / Nc;
+      R1[e] = row_unpack<T>(Z + line * Nc, e - line * Nc, M, twN);
+    }
+    __syncthreads();
+    // column pass (k_col_etd, ETD1): B = FFT_y T~, A = FFT_y S~,
+    // out = E1 A + Q1 B, k = 0 pinned, inverse FFT_y
+    for (int e = threadIdx.x; e < nsp; e += blockDim.x) {
+      const int y = e / Nc;
+      const int line = e - y * Nc;
+      R2[line * Ny + y] = R1[e];
+    }
+    __syncthreads();
+    V* B = fft_lines<T, false>(R2, R3, g.plY, Nc, Ny, twY, 1);
+    V* F = B == R2 ? R3 : R2;  // free
+    for (int e = threadIdx.x; e < nsp; e += blockDim.x) {
+      const int y = e / Nc;
+      const int line = e - y * Nc;
+      R1[line * Ny + y] = Sl[e];
+    }
+    __syncthreads();
+    V* A = fft_lines<T, false>(R1, F, g.plY, Nc, Ny, twY, 1);
+    V* G = A == R1 ? F : R1;  // free
+    for (int e = threadIdx.x; e < nsp; e += blockDim.x) {
+      const int line = e / Ny;
+      const int ky = e - line * Ny;
+      const size_t ti = (size_t)line * g.hy + ky_half(ky, Ny);
+      const V a = A[e], b = B[e];
+      const double e1 = E1[ti], q1 = Q1[ti];
+      double ox = e1 * double(a.x) + q1 * double(b.x);
+      double oy = e1 * double(a.y) + q1 * double(b.y);
+      if (pinned && line == 0 && ky == 0) ox = pin, oy = 0.0;
+      B[e] = V{T(ox), T(oy)};
+    }
+    __syncthreads();
+    V* R = fft_lines<T, true>(B, G, g.plY, Nc, Ny, twY, 1);
+    for (int e = threadIdx.x; e < nsp; e += blockDim.x) {
+      const int y = e / Nc;
+      const int line = e - y * Nc;
+      V v = R[line * Ny + y];
+      if (line == 0 || line == Nc - 1) v.y = T(0);  // the real kx = 0 and kx = Nx/2 columns (see k_col_etd)
+      Sl[e] = v;
+    }
+    __syncthreads();
+  }
+  for (int e = threadIdx.x; e < nsp; e += blockDim.x) {
+    const int y = e / Nc;
+    S[g.lay.at(y, e - y * Nc)] = Sl[e];
+  }
+  if (bad && sl.status) atomicOr(sl.status + slot, 1);
+}
+
+// ---------------------------------------------------------------------------
 // plain transforms (field upload/download and the backend rfft2/irfft2)
 // ---------------------------------------------------------------------------
 // real rows (Nx) -> half spectrum rows, times `scale`.  blockIdx.y = batch.
@@ -520,6 +641,14 @@ struct TabParams {
   int hy;  // stored rows per column: Ny/2 + 1 (see ky_half)
 };
 
+// Per-slot box and time step: slot s reads kx^2 at kx2 + s*Nc, ky^2 at
+// ky2 + s*Ny and its dt at dt[s].
+struct SlotGrid {
+  const double* kx2;
+  const double* ky2;
+  const double* dt;
+};
+
 __device__ __forceinline__ double etd2_phi_dt(double z, double dt) {
   if (fabs(z) < 0.1) {
     double s = 0.0, t = 0.5;
@@ -532,16 +661,19 @@ __device__ __forceinline__ double etd2_phi_dt(double z, double dt) {
   return dt * ((expm1(z) - z) / (z * z));
 }
 
-// blockIdx.y = slot - slot0; per-slot epsilon and M come from eps[] and Ms[].
+// blockIdx.y = slot - slot0; per-slot epsilon, M, box and dt come from
+// eps[], Ms[] and the SlotGrid.
 template <typename T>
 __global__ void __launch_bounds__(256) k_tables(T* __restrict__ E1, T* __restrict__ Q1, T* __restrict__ Q2,
-                                                long long tstride, const double* __restrict__ kx2,
-                                                const double* __restrict__ ky2, TabParams p,
+                                                long long tstride, SlotGrid sg, TabParams p,
                                                 const double* __restrict__ eps, const double* __restrict__ Ms,
                                                 int slot0) {
   const int slot = slot0 + blockIdx.y;
   p.epsilon = eps[slot];
   p.M = Ms[slot];
+  p.dt = sg.dt[slot];
+  const double* __restrict__ kx2 = sg.kx2 + (size_t)slot * p.Nc;
+  const double* __restrict__ ky2 = sg.ky2 + (size_t)slot * p.Ny;
   E1 += (size_t)slot * tstride;
   Q1 += (size_t)slot * tstride;
   if (Q2) Q2 += (size_t)slot * tstride;
@@ -569,8 +701,8 @@ __global__ void __launch_bounds__(256) k_tables(T* __restrict__ E1, T* __restric
 struct FeParams {
   int model;
   double beta, q0, epsilon, g;
-  const double* kx2;  // [Nc]  kx^2
-  const double* ky2;  // [Ny]  ky^2
+  const double* kx2;  // [slot][Nc]  kx^2 (slot as indexed by Slots)
+  const double* ky2;  // [slot][Ny]  ky^2
   const double* eps;  // per-slot epsilon (indexed like Slots), or null: use `epsilon`
 };
 
@@ -652,7 +784,10 @@ __global__ void __launch_bounds__(256) k_fe_col(const V2<Tin>* __restrict__ S, C
   extern __shared__ __align__(16) unsigned char smem_raw[];
   V* b0 = reinterpret_cast<V*>(smem_raw);
   V* b1 = b0 + g.nc * g.ld;
-  S += (size_t)slot_of(sl, blockIdx.y) * sl.sstride;
+  const int slot = slot_of(sl, blockIdx.y);
+  S += (size_t)slot * sl.sstride;
+  const double* __restrict__ kx2 = p.kx2 + (size_t)slot * g.Nc;
+  const double* __restrict__ ky2 = p.ky2 + (size_t)slot * g.Ny;
   part += (size_t)blockIdx.y * pstride;
   const int c0 = blockIdx.x * g.nc;
   const int ncols = min(g.nc, g.Nc - c0);
@@ -673,7 +808,7 @@ __global__ void __launch_bounds__(256) k_fe_col(const V2<Tin>* __restrict__ S, C
     const int ky = e - line * g.Ny;
     const int kx = c0 + line;
     V a = A[line * g.ld + ky];
-    const double lap = -(p.kx2[kx] + p.ky2[ky]);
+    const double lap = -(kx2[kx] + ky2[ky]);
     const double t = p.q0 * p.q0 + lap;
     const double lf = (p.beta / 2.0) * (t * t);
     const double w = (kx == 0 || 2 * kx == Nx) ? 1.0 : 2.0;

@@ -67,6 +67,8 @@ _SIGNATURES = [
     ("pfc_batch_create", ctypes.c_int, [ctypes.POINTER(PfcParams), ctypes.c_int, ctypes.POINTER(_P)]),
     ("pfc_batch_destroy", ctypes.c_int, [_P]),
     ("pfc_batch_set_slot", ctypes.c_int, [_P, ctypes.c_int, ctypes.c_double, ctypes.c_double, _P]),
+    ("pfc_batch_set_slot_box", ctypes.c_int, [_P, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                              _P]),
     ("pfc_batch_set_field", ctypes.c_int, [_P, ctypes.c_int, _P, ctypes.c_int, _P]),
     ("pfc_batch_get_field", ctypes.c_int, [_P, ctypes.c_int, _P, ctypes.c_int, _P]),
     ("pfc_batch_set_active", ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_int), ctypes.c_int, _P]),

@@ -157,3 +157,53 @@ def maxima_per_cell(field, cell):
 def diagnostics(field, structure, cell, relative=False, nonnegative=False):
     f, A, B, C = fitness(field, structure, cell, relative=relative, nonnegative=nonnegative)
     return _pd.Diagnostics(fitness=f, amplitudes=(A, B, C), details={'maxima_per_cell': maxima_per_cell(field, cell)})
+
+
+class SlotBatch:
+    """Driver evaluations of one model in the slots of a device batch
+    (pfc_hip.plan.Batch): per slot the temperature (epsilon), the box of its
+    q00, its dt and its field; the rest of the model is shared.  A slot is
+    the batched counterpart of the Simulation `build` makes: the same library
+    parameters (`plan_kwargs`), the same k-grids and tables, the same field
+    transfers."""
+
+    def __init__(self, kind, model_params, cell, capacity, diagnose_state, precision=64):
+        from ..plan import Batch
+        from ._base import plan_kwargs
+        self.cell = cell
+        self._diagnose = diagnose_state
+        self.batch = Batch(capacity, **plan_kwargs(kind, model_params, box(cell, 1.0), precision=precision))
+        self._active = None
+
+    def place(self, slot, temp, meanDensity, q00, dt, state):
+        """Slot `slot` at temperature `temp` in the box of q00 with time step dt
+        and field `state` (which carries the mean density)."""
+        g = box(self.cell, q00)
+        self.batch.set_slot(slot, temp)
+        self.batch.set_slot_box(slot, g.Lx, g.Ly, dt)
+        self.set_state(slot, state)
+
+    def set_state(self, slot, state):
+        self.batch.set_field(slot, np.asarray(state, dtype=np.float64))
+
+    def state(self, slot):
+        return self.batch.get_field(slot).cpu().numpy().astype(np.float64)
+
+    def set_active(self, slots):
+        slots = list(slots)
+        if slots != self._active:
+            self.batch.set_active(slots)
+            self._active = slots
+
+    def step(self, n):
+        self.batch.step(n)
+
+    def free_energy(self):
+        """FP64 free energy per site of each active slot, in active order."""
+        return self.batch.reduce()[:, 0].copy()
+
+    def diagnose(self, slot, state, structure, cell):
+        return self._diagnose(state, structure, cell)
+
+    def close(self):
+        self.batch.close()

@@ -190,6 +190,16 @@ class _SimulationState:
         return rfft2(self._sim._plan.get_field())
 
 
+def plan_kwargs(kind, modelParameters, geometry, *, precision=64, scheme="etd1", clamp_log=False, pin_mean=True):
+    """The library parameters of one instance of model `kind` ('std' or 'log'):
+    what a Simulation's plan and a batch slot are built from."""
+    m = modelParameters
+    return dict(model=kind, Nx=geometry.Nx, Ny=geometry.Ny, Lx=geometry.Lx, Ly=geometry.Ly, Gamma=float(m.Gamma),
+                beta=float(m.beta), q0=float(m.q0), epsilon=float(m.epsilon), g=float(m.g), dt=float(m.dt),
+                minPhi=float(getattr(m, "minPhi", 1e-6) or 1e-6), smallZTol=float(m.smallZTol), M=0.0,
+                precision=int(precision), scheme=scheme, clamp_log=clamp_log, pin_mean=pin_mean)
+
+
 class Simulation:
     """One instance on the device.  Built by BuildHipGPU_Simulation."""
 
@@ -203,12 +213,8 @@ class Simulation:
         self.precision = int(precision)
         self.scheme = scheme
         self.stabilise = bool(stabilise)
-        m = modelParameters
-        self._plan = Plan(
-            model=self._model_kind, Nx=geometry.Nx, Ny=geometry.Ny, Lx=geometry.Lx, Ly=geometry.Ly,
-            Gamma=float(m.Gamma), beta=float(m.beta), q0=float(m.q0), epsilon=float(m.epsilon), g=float(m.g),
-            dt=float(m.dt), minPhi=float(getattr(m, "minPhi", 1e-6) or 1e-6), smallZTol=float(m.smallZTol),
-            M=0.0, precision=self.precision, scheme=scheme, clamp_log=clamp_log, pin_mean=pin_mean)
+        self._plan = Plan(**plan_kwargs(self._model_kind, modelParameters, geometry, precision=self.precision,
+                                        scheme=scheme, clamp_log=clamp_log, pin_mean=pin_mean))
         self.State = _SimulationState(self, self._field_name)
         self.mu = None
         # the field starts at zero, as the reference's preallocated buffers do

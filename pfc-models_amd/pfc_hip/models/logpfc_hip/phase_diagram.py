@@ -98,8 +98,27 @@ def diagnose(sim, structure, cell):
     """fitness = 1 - mean((phi - phi_3mode)^2); amplitudes (A, B, C) = mean
     |phi_hat| over the 1-, 2-, 3-mode sets per site in units of phibar;
     details: maxima_per_cell."""
-    return _A.diagnostics(get_state(sim), _structure(structure), validate_cell(cell), relative=True,
-                          nonnegative=True)
+    return diagnose_state(get_state(sim), structure, cell)
+
+
+# -- batched evaluations (pfc_hip.phase_diagram.batched) ---------------------------
+def open_batch(params, cell, capacity, precision=64):
+    """A device batch of `capacity` evaluation slots on the cell's grid."""
+    p = validate_params(params)
+    model = _simulation.BuildLogPFCModelParameters(Gamma=p['Gamma'], beta=p['beta'], q0=p['q0'], epsilon=0.0,
+                                                   g=p['g'], dt=1.0)
+    return _A.SlotBatch('log', model, validate_cell(cell), capacity, diagnose_state, precision=precision)
+
+
+def ansatz_state(structure, cell, ansatz, meanDensity):
+    """The fresh ansatz `build` starts from, as a host field."""
+    return initial_field(structure, validate_cell(cell), validate_ansatz(ansatz), meanDensity)
+
+
+def diagnose_state(state, structure, cell):
+    """`diagnose` of a host field."""
+    return _A.diagnostics(np.asarray(state, dtype=np.float64), _structure(structure), validate_cell(cell),
+                          relative=True, nonnegative=True)
 
 
 # -- uniform state and bounds ---------------------------------------------------------

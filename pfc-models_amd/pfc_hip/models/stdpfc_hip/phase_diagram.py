@@ -94,7 +94,26 @@ def free_energy(sim):
 def diagnose(sim, structure, cell):
     """fitness = 1 - mean((n - n_3mode)^2); amplitudes (A, B, C) = mean |n_hat|
     over the 1-, 2-, 3-mode sets per site; details: maxima_per_cell."""
-    return _A.diagnostics(get_state(sim), _structure(structure), validate_cell(cell))
+    return diagnose_state(get_state(sim), structure, cell)
+
+
+# -- batched evaluations (pfc_hip.phase_diagram.batched) ---------------------------
+def open_batch(params, cell, capacity, precision=64):
+    """A device batch of `capacity` evaluation slots on the cell's grid."""
+    p = validate_params(params)
+    model = _simulation.BuildStdPFCModelParameters(Gamma=p['Gamma'], beta=p['beta'], q0=p['q0'], epsilon=0.0,
+                                                   g=p['g'], dt=1.0)
+    return _A.SlotBatch('std', model, validate_cell(cell), capacity, diagnose_state, precision=precision)
+
+
+def ansatz_state(structure, cell, ansatz, meanDensity):
+    """The fresh ansatz `build` starts from, as a host field."""
+    return initial_field(structure, validate_cell(cell), validate_ansatz(ansatz), meanDensity)
+
+
+def diagnose_state(state, structure, cell):
+    """`diagnose` of a host field."""
+    return _A.diagnostics(np.asarray(state, dtype=np.float64), _structure(structure), validate_cell(cell))
 
 
 # -- uniform state and bounds (closed forms) --------------------------------------

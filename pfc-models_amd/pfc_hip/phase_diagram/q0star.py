@@ -122,19 +122,71 @@ def _advance(ad, sim, n):
             ad.step(sim)
 
 
-def add_ansatz_noise(ad, sim, seed, noise):
-    """Multiply each field of a fresh ansatz by 1 + noise N(0, 1) from
-    default_rng(seed) (one stream over the fields in order), restoring each
-    field's mean (q0star.py:245-256)."""
+def noisy_state(state, seed, noise):
+    """A fresh ansatz with each field multiplied by 1 + noise N(0, 1) from
+    default_rng(seed) (one stream over the fields in order) and its mean
+    restored (q0star.py:245-256)."""
     rng = np.random.default_rng(seed)
-    state = ad.get_state(sim)
     fields = state if isinstance(state, tuple) else (state,)
     noisy = []
     for phi in fields:
         mean = phi.mean()
         phi = phi * (1.0 + noise * rng.standard_normal(phi.shape))
         noisy.append(phi + (mean - phi.mean()))
-    ad.set_state(sim, tuple(noisy) if isinstance(state, tuple) else noisy[0])
+    return tuple(noisy) if isinstance(state, tuple) else noisy[0]
+
+
+def add_ansatz_noise(ad, sim, seed, noise):
+    ad.set_state(sim, noisy_state(ad.get_state(sim), seed, noise))
+
+
+@dataclass
+class Evaluation:
+    """One Get_f_ABC: the values at the stop, why it stopped, its time steps,
+    the largest dt it used and (optionally) the final state."""
+    f: float
+    fitness: float
+    amplitudes: tuple
+    stop: str
+    steps: int
+    dt_max: float
+    state: object = None
+
+
+def pass_decision(qs, rows, relax, minPoints):
+    """One q00 pass of Find_q0Star (q0star.py:447-476): the accepted points
+    (fitness >= requiredFitness unless the evaluation failed, A >= minimumA)
+    must number at least minPoints overall and on the contiguous run around
+    their argmin; the run is splined and the spline minimum nearest the argmin
+    returned with the spline.  Raises InsufficientPoints / NoLocalMinimum."""
+    f = np.array([r.f for r in rows], dtype=float)
+    fit = np.array([np.nan if r.stop in FAILED_STOPS else r.fitness for r in rows], dtype=float)
+    A = np.array([r.amplitudes[0] for r in rows], dtype=float)
+    rf = -np.inf if relax.requiredFitness is None else relax.requiredFitness
+    ok = (fit >= rf) & (A >= relax.minimumA)
+    if qs[ok].size < minPoints:
+        raise InsufficientPoints(MSG_INSUFFICIENT_POINTS)
+    i_min = np.flatnonzero(ok)[np.argmin(f[ok])]
+    q_min = qs[i_min]
+    lo = hi = i_min
+    while lo > 0 and ok[lo - 1]:
+        lo -= 1
+    while hi < ok.size - 1 and ok[hi + 1]:
+        hi += 1
+    if hi - lo + 1 < minPoints:
+        raise InsufficientPoints(MSG_INSUFFICIENT_POINTS)
+    spl = CubicSpline(qs[lo:hi + 1], f[lo:hi + 1])
+    roots = spl.derivative().roots(extrapolate=False)
+    roots = roots[spl.derivative(2)(roots) > 0]
+    if roots.size == 0:
+        raise NoLocalMinimum(MSG_NO_LOCAL_MINIMUM)
+    return np.abs(min(roots, key=lambda x: abs(x - q_min))), spl
+
+
+def pass_grid(q_best, Deltaq, qscan):
+    """The q00 points of one pass: q00_Steps points spaced Deltaq about q_best."""
+    half = qscan.q00_Steps // 2 * Deltaq
+    return np.linspace(q_best - half, q_best + half, qscan.q00_Steps, endpoint=True)
 
 
 class MinimizeFreeEnergy:
@@ -180,11 +232,22 @@ class MinimizeFreeEnergy:
         df = f - f_before
         if not (np.isnan(f) or df > self.relax.fTol * abs(f_before)):
             return df
-        # roll back and redo the block as `divs` sub-blocks at dt / divs
+        return self._rollback(sim, temp, meanDensity, q00, stepT, before, t_before, f_before, bool(np.isnan(f)),
+                              divs=divs, depth=depth, maxDepth=maxDepth, verbose=verbose)
+
+    def _rollback(self, sim, temp, meanDensity, q00, stepT, before, t_before, f_before, nan, divs=10, depth=0,
+                  maxDepth=None, verbose=False):
+        """A block that failed: restore `before` at t_before and redo it as
+        `divs` sub-blocks at dt / divs, or raise MaxDepthReached at maxDepth.
+        Returns the block's change of f."""
+        if maxDepth is None:
+            maxDepth = self.relax.maxDepth
+        ad = self.adapter
+        dt = ad.get_dt(sim)
         ad.set_state(sim, before)
         ad.set_time(sim, t_before)
         if depth >= maxDepth:
-            raise MaxDepthReached(f'Max recurrsion depth reached at t={ad.get_time(sim):0.6f}', nan=bool(np.isnan(f)))
+            raise MaxDepthReached(f'Max recurrsion depth reached at t={ad.get_time(sim):0.6f}', nan=nan)
         sub = self._generateSim(temp, meanDensity, q00, dt / divs, ad.get_state(sim), verbose=verbose)
         ad.set_time(sub, ad.get_time(sim))
         for _ in range(divs):
@@ -268,12 +331,19 @@ class MinimizeFreeEnergy:
         self.last_state = ad.get_state(sim)
         return (f, diag.fitness, *diag.amplitudes)
 
+    def evaluate(self, temp, meanDensity, q00, verbose=False, state=None):
+        """Get_f_ABC packed as an Evaluation (final state included)."""
+        out = self.Get_f_ABC(temp=temp, meanDensity=meanDensity, q00=q00, verbose=verbose, state=state)
+        return Evaluation(float(out[0]), float(out[1]), tuple(float(a) for a in out[2:]), self.last_stop,
+                          self.last_steps, self.last_dt_max, self.last_state)
+
     # -- the q00 scan ------------------------------------------------------------
     def Find_q0Star(self, temp, meanDensity, minPoints=None, verbose=False, record=None, evaluate=None):
         """q00 minimising f at (temp, meanDensity); returns (q0*, fmin).
         `evaluate(q00s)`, if given, evaluates a whole pass at once and returns
-        [(f, fitness, A, stop, steps)] (the batched form); by default each
-        point is a Get_f_ABC call."""
+        one Evaluation per point (the batched form); by default each point is a
+        Get_f_ABC call.  `record`, if given, receives one dict per evaluation
+        (pass, q00, f, fitness, amplitudes, stop, steps, dt_max)."""
         q = self.qscan
         if minPoints is None:
             minPoints = q.minPoints
@@ -281,46 +351,28 @@ class MinimizeFreeEnergy:
         q_best = q.q00_start
         spl = None
         for pass_idx in range(q.passes):
-            qs = np.linspace(q_best - q.q00_Steps // 2 * Deltaq, q_best + q.q00_Steps // 2 * Deltaq, q.q00_Steps,
-                             endpoint=True)
+            qs = pass_grid(q_best, Deltaq, q)
             if evaluate is not None:
-                rows = evaluate(qs)
+                rows = list(evaluate(qs))
+                for q00, r in zip(qs, rows):
+                    if record is not None:
+                        record(record_of(pass_idx, q00, r))
             else:
                 rows = []
-                for q00 in qs:
-                    out = self.Get_f_ABC(temp=temp, meanDensity=meanDensity, q00=q00, verbose=verbose)
-                    rows.append((out[0], out[1], out[2], self.last_stop, self.last_steps, tuple(out[2:])))
-            f = np.array([row[0] for row in rows], dtype=float)
-            fit = np.array([row[1] for row in rows], dtype=float)
-            A = np.array([row[2] for row in rows], dtype=float)
-            for i, row in enumerate(rows):
-                if row[3] in FAILED_STOPS:
-                    fit[i] = np.nan
-                if record is not None:
-                    record({'pass': pass_idx, 'q00': float(qs[i]), 'f': float(row[0]), 'fitness': float(row[1]),
-                            'amplitudes': tuple(float(a) for a in row[5]), 'stop': row[3], 'steps': row[4],
-                            'dt_max': self.last_dt_max})
-            rf = -np.inf if self.relax.requiredFitness is None else self.relax.requiredFitness
-            ok = (fit >= rf) & (A >= self.relax.minimumA)
-            if qs[ok].size < minPoints:
-                raise InsufficientPoints(MSG_INSUFFICIENT_POINTS)
-            i_min = np.flatnonzero(ok)[np.argmin(f[ok])]
-            q_min = qs[i_min]
-            lo = hi = i_min
-            while lo > 0 and ok[lo - 1]:
-                lo -= 1
-            while hi < ok.size - 1 and ok[hi + 1]:
-                hi += 1
-            if hi - lo + 1 < minPoints:
-                raise InsufficientPoints(MSG_INSUFFICIENT_POINTS)
-            spl = CubicSpline(qs[lo:hi + 1], f[lo:hi + 1])
-            roots = spl.derivative().roots(extrapolate=False)
-            roots = roots[spl.derivative(2)(roots) > 0]
-            if roots.size == 0:
-                raise NoLocalMinimum(MSG_NO_LOCAL_MINIMUM)
-            q_best = np.abs(min(roots, key=lambda x: abs(x - q_min)))
+                for q00 in qs:  # recorded as they finish: an exception keeps the earlier records
+                    rows.append(self.evaluate(temp, meanDensity, q00, verbose=verbose))
+                    if record is not None:
+                        record(record_of(pass_idx, q00, rows[-1]))
+            q_best, spl = pass_decision(qs, rows, self.relax, minPoints)
             Deltaq *= q.Deltaq_Refinement
         return q_best, spl(q_best)
+
+
+def record_of(pass_idx, q00, r):
+    """The q-scan record of one evaluation."""
+    return {'pass': pass_idx, 'q00': float(q00), 'f': float(r.f), 'fitness': float(r.fitness),
+            'amplitudes': tuple(float(a) for a in r.amplitudes), 'stop': r.stop, 'steps': r.steps,
+            'dt_max': r.dt_max}
 
 
 # ---------------------------------------------------------------------------

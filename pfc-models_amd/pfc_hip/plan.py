@@ -188,8 +188,9 @@ class Plan:
 class Batch:
     """`capacity` same-shape instances stepped by the same launches.
 
-    Every pfc_params field is shared except epsilon and M, which are per slot
-    (`set_slot`).  `step(n)` advances the active slots (initially all)
+    Every pfc_params field is shared except epsilon and M (`set_slot`) and
+    the box Lx, Ly and time step dt (`set_slot_box`), which are per slot.
+    `step(n)` advances the active slots (initially all)
     together; `reduce()` returns an (n_active, 4) array of FP64
     (free energy per site, mean, min, max)."""
 
@@ -222,6 +223,12 @@ class Batch:
     def set_slot(self, slot, epsilon, M=0.0):
         _lib.check(self._lib.pfc_batch_set_slot(self._h, int(slot), float(epsilon), float(M),
                                                 current_stream_handle()))
+        return self
+
+    def set_slot_box(self, slot, Lx, Ly, dt):
+        """The slot's box and time step (its k-grids and ETD tables are rebuilt)."""
+        _lib.check(self._lib.pfc_batch_set_slot_box(self._h, int(slot), float(Lx), float(Ly), float(dt),
+                                                    current_stream_handle()))
         return self
 
     def set_field(self, slot, x):

@@ -130,6 +130,7 @@ def test_slot_bytes_and_mixed_sizes(hip_device):
     # ETD tables keep rows ky <= N/2 (they depend on ky^2 only)
     nc = N * (2 * ((N // 2 + 2) // 2))
     nt = (N // 2 + 1) * (N // 2 + 1)
-    assert per == nc * 16 * 3 + nt * 8 * 3  # state (S, T, D) + FP64 tables (E1, Q1, Q2)
+    grid = (N // 2 + 1) + N + 1  # the slot's kx^2, ky^2 and dt
+    assert per == nc * 16 * 3 + nt * 8 * 3 + grid * 8  # state (S, T, D) + FP64 tables (E1, Q1, Q2) + grid
     b = Batch(3, **_kw())
     assert b.device_bytes() >= 3 * per

@@ -146,3 +146,76 @@ def test_run_matches_a_step_loop(hip_device):
         m = Q.MinimizeFreeEnergy(ad, REFS["log_params"], "triangle", REFS["cell"], REFS["ansatz"], relax, qscan)
         out.append(([float(x) for x in m.Get_f_ABC(temp=-0.9, meanDensity=0.63, q00=0.96)], m.last_steps))
     assert out[0] == out[1]
+
+
+# -- the batched driver ------------------------------------------------------------
+def test_batch_slot_box_matches_a_plan(hip_device):
+    # a batch slot given its own box and dt steps like a plan built with them
+    from pfc_hip import fields as F
+    from pfc_hip.plan import Batch, Plan
+    kw = dict(model="log", Nx=28, Ny=24, Gamma=1.0, beta=0.5, q0=1.0, epsilon=-0.9, g=-2.0, precision=64)
+    a, h = 4 * np.pi / (3 ** 0.5 * 0.97), 4 * np.pi / 0.97
+    phi0 = 0.63 + F.uniform_noise((24, 28), 0.0, 0.02, seed=3)
+    pl = Plan(Lx=2 * a, Ly=h, dt=0.05, **kw)
+    b = Batch(3, Lx=10.0, Ly=20.0, dt=0.1, **kw)
+    b.set_slot_box(1, 2 * a, h, 0.05)
+    for s in range(3):
+        b.set_field(s, phi0)
+    pl.set_field(phi0)
+    pl.step(200)
+    b.step(200)
+    x = pl.get_field().cpu().numpy()
+    y = b.get_field(1).cpu().numpy()
+    print(f"\nslot vs plan: max |d phi| {np.abs(x - y).max():.3e}", flush=True)
+    assert np.abs(x - y).max() <= 1e-14
+    assert b.reduce()[1, 0] == pytest.approx(pl.free_energy(), abs=1e-15)
+    # the other slots kept the shared box: they differ
+    assert np.abs(b.get_field(0).cpu().numpy() - x).max() > 1e-6
+
+
+def test_batched_pass_equals_serial_evaluations(hip_device):
+    from pfc_hip.phase_diagram import batched as B
+    from pfc_hip.phase_diagram import q0star as Q
+    ref = REFS["find_q0star"]
+    m = _log_minimizer(ref["structure"])
+    _, qscan = _settings()
+    qs = Q.pass_grid(qscan.q00_start, qscan.Deltaq, qscan)
+    import time
+    t0 = time.perf_counter()
+    serial = [m.evaluate(ref["eps"], ref["phi"], q) for q in qs]
+    t1 = time.perf_counter()
+    batched = B.evaluate_batched(m, [(ref["eps"], ref["phi"], q) for q in qs])
+    t2 = time.perf_counter()
+    df = max(abs(s.f - b.f) for s, b in zip(serial, batched))
+    dfit = max(abs(s.fitness - b.fitness) for s, b in zip(serial, batched))
+    print(f"\npass of {len(qs)}: serial {t1 - t0:.2f} s, batched {t2 - t1:.2f} s; max df {df:.3e} dfit {dfit:.3e}; "
+          f"steps {sum(s.steps for s in serial)}", flush=True)
+    assert [s.stop for s in serial] == [b.stop for b in batched]
+    assert [s.steps for s in serial] == [b.steps for b in batched]
+    assert df <= 1e-14 and dfit <= 1e-12
+
+
+def test_batched_honeycomb_rollback(hip_device):
+    from pfc_hip.phase_diagram import batched as B
+    from pfc_hip.phase_diagram import q0star as Q
+    r = _ref("honeycomb")
+    ev, = B.evaluate_batched(_log_minimizer("honeycomb"), [(r["eps"], r["phi"], r["q00"])])
+    print(f"\nbatched honeycomb: df {ev.f - r['f']:.3e} steps {ev.steps}", flush=True)
+    assert ev.stop == Q.STOP_CONVERGED and ev.steps == r["steps"]
+    assert abs(ev.f - r["f"]) <= r["tol_f"]
+
+
+def test_scan_density_batched_against_reference(hip_device):
+    import time
+    from pfc_hip.phase_diagram import batched as B
+    from pfc_hip.phase_diagram import q0star as Q
+    ref, nlm = REFS["find_q0star"], REFS["no_local_minimum"]
+    rhos = nlm["phi"] + [ref["phi"]]
+    t0 = time.perf_counter()
+    r = B.scan_density_batched(_log_minimizer(ref["structure"]), ref["eps"], rhos)
+    print(f"\nbatched scan of {len(rhos)} densities: {time.perf_counter() - t0:.2f} s, "
+          f"{int(r['qscan']['steps'].sum())} steps", flush=True)
+    assert list(r["status"]) == [Q.STATUS_NO_LOCAL_MINIMUM] * len(nlm["phi"]) + [Q.STATUS_OK]
+    assert all(str(msg) == nlm["message"] for msg in r["message"][:len(nlm["phi"])])
+    assert abs(r["q0star"][-1] - ref["q0star"]) <= ref["tol_q"]
+    assert abs(r["fmin"][-1] - ref["fmin"]) <= ref["tol_f"]
