@@ -401,12 +401,15 @@ struct Engine {
   }
 };
 
-// The resident stepper holds four Ny x (Nx/2 + 1) complex arrays in LDS; it
-// is used for ETD1 when they fit in 64 KiB (e.g. up to 30 x 30 in FP64),
-// where the two-kernel step is launch- and latency-bound.  PFC_HIP_RESIDENT=0
-// turns it off.
+// The resident stepper holds four Ny x (Nx/2 + 1) complex arrays, the
+// twiddles and the instance's ETD tables in LDS; it is used for ETD1 when
+// they fit in 64 KiB (e.g. up to 28 x 28 in FP64), where the two-kernel step
+// is launch- and latency-bound.  PFC_HIP_RESIDENT=0 turns it off.
 constexpr size_t kResidentLds = 64 * 1024;
-size_t resident_lds(int Nx, int Ny, size_t csize) { return 4 * (size_t)Ny * (Nx / 2 + 1) * csize; }
+size_t resident_lds(int Nx, int Ny, size_t csize) {
+  const int Nc = Nx / 2 + 1;
+  return resident_table_bytes(Nc, table_rows(Ny)) + ((size_t)Nx + Ny + 4 * (size_t)Ny * Nc) * csize;
+}
 bool resident_fits(const pfc_params& p) {
   const char* env = std::getenv("PFC_HIP_RESIDENT");
   if (env && std::string(env) == "0") return false;
@@ -431,7 +434,7 @@ void engine_init(Engine& e, const pfc_params* params, int cap, bool batched) {
   e.rg64 = row_geom(Nx, Ny, sizeof(V2<double>));
   e.cg64 = col_geom(Nx, Ny, sizeof(V2<double>));
   e.rg.lay = e.cg.lay = e.rg64.lay = e.cg64.lay = e.lay;
-  e.res = ResGeom{Nx / 2, e.Nc, Ny, table_rows(Ny), e.lay, e.rg.plM, e.cg.plY};
+  e.res = ResGeom{Nx, Nx / 2, e.Nc, Ny, table_rows(Ny), e.lay, e.rg.plM, e.cg.plY};
   e.resident = resident_fits(*params);
   if (!e.resident && !std::getenv("PFC_HIP_GENERIC")) {
     auto rc = row_candidates(Nx, Ny, e.csize);

@@ -329,16 +329,91 @@ __global__ void __launch_bounds__(256) k_col_etd(V2<T>* __restrict__ S, const V2
 // A grid of a few thousand points is latency-bound on the two-kernel step:
 // every step pays two launches and two HBM round trips of the state for a
 // few microseconds of arithmetic.  Here a workgroup loads its instance's row
-// spectrum once, runs `nsteps` ETD1 steps (the row pass and the column pass
-// of k_row_nonlin / k_col_etd, with the same arithmetic, so the same
-// results) between four LDS buffers of Ny x Nc complex values, and stores it
-// once.  The phase-diagram driver's cells (e.g. 28 x 24) are this case.
+// spectrum, twiddles and ETD tables into LDS once, runs `nsteps` ETD1 steps
+// (the arithmetic of k_row_nonlin and k_col_etd: the same Stockham passes,
+// pack/unpack, nonlinearity and FP64 combine) and stores the state once.
+// The phase-diagram driver's cells (e.g. 28 x 24) are this case.
+//
+// LDS: two regions of 2 Ny Nc complex values.  The state S~ ([y][k], row
+// pitch Nc) and T~ share one region; the other is the FFT ping-pong
+// partner.  Column transforms run in place of the [y][k] arrays (element
+// stride Nc), and the S~ and T~ columns go through one set of passes.
 struct ResGeom {
-  int M, Nc, Ny;  // Nx/2, Nx/2 + 1, Ny
-  int hy;         // table rows per column
-  Lay lay;        // storage of S~ in device memory
+  int Nx, M, Nc, Ny;  // Nx, Nx/2, Nx/2 + 1, Ny
+  int hy;             // table rows per column
+  Lay lay;            // storage of S~ in device memory
   Radices plM, plY;
 };
+
+// One Stockham pass (sk_pass's arithmetic) over nl lines of length L in
+// LDS: line l starts at (l % ng) * ld + (l / ng) * gs, elements es apart.
+template <typename T, int R, bool INV>
+__device__ __forceinline__ void rs_pass(const V2<T>* __restrict__ src, V2<T>* __restrict__ dst, int L, int Ns,
+                                        int nl, int ng, int ld, int gs, int es, const V2<T>* __restrict__ tw,
+                                        int tws) {
+  using V = V2<T>;
+  const int nb = L / R;
+  const int total = nb * nl;
+  const int twstep = (L / (Ns * R)) * tws;
+  const int rootstep = nb * tws;
+  for (int w = threadIdx.x; w < total; w += blockDim.x) {
+    const int line = w / nb;
+    const int j = w - line * nb;
+    const int grp = line / ng;
+    const int off = (line - grp * ng) * ld + grp * gs;
+    const V* s = src + off;
+    V* d = dst + off;
+    const int k = j % Ns;
+    V v[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) v[r] = s[(j + r * nb) * es];
+    if (Ns > 1) {
+#pragma unroll
+      for (int r = 1; r < R; ++r) {
+        V t = tw[k * r * twstep];
+        if (INV) t.y = -t.y;
+        v[r] = cmul(v[r], t);
+      }
+    }
+    Dft<T, R, INV>::run(v, tw, rootstep);
+    const int base = (j - k) * R + k;
+#pragma unroll
+    for (int r = 0; r < R; ++r) d[(base + r * Ns) * es] = v[r];
+  }
+}
+
+template <typename T, bool INV>
+__device__ V2<T>* rs_fft(V2<T>* buf0, V2<T>* buf1, const Radices& pl, int nl, int ng, int ld, int gs, int es,
+                         const V2<T>* __restrict__ tw, int tws) {
+  V2<T>* src = buf0;
+  V2<T>* dst = buf1;
+  int Ns = 1;
+  for (int p = 0; p < pl.np; ++p) {
+    const int R = pl.r[p];
+    switch (R) {
+      case 2: rs_pass<T, 2, INV>(src, dst, pl.L, Ns, nl, ng, ld, gs, es, tw, tws); break;
+      case 3: rs_pass<T, 3, INV>(src, dst, pl.L, Ns, nl, ng, ld, gs, es, tw, tws); break;
+      case 4: rs_pass<T, 4, INV>(src, dst, pl.L, Ns, nl, ng, ld, gs, es, tw, tws); break;
+      case 5: rs_pass<T, 5, INV>(src, dst, pl.L, Ns, nl, ng, ld, gs, es, tw, tws); break;
+      case 7: rs_pass<T, 7, INV>(src, dst, pl.L, Ns, nl, ng, ld, gs, es, tw, tws); break;
+      case 8: rs_pass<T, 8, INV>(src, dst, pl.L, Ns, nl, ng, ld, gs, es, tw, tws); break;
+      case 11: rs_pass<T, 11, INV>(src, dst, pl.L, Ns, nl, ng, ld, gs, es, tw, tws); break;
+      default: rs_pass<T, 13, INV>(src, dst, pl.L, Ns, nl, ng, ld, gs, es, tw, tws); break;
+    }
+    __syncthreads();
+    Ns *= R;
+    V2<T>* t = src;
+    src = dst;
+    dst = t;
+  }
+  return src;
+}
+
+// LDS bytes of the resident stepper: tables (FP64, padded to 16 B), twiddles
+// and the two state regions.
+__host__ __device__ inline size_t resident_table_bytes(int Nc, int hy) {
+  return ((size_t)2 * Nc * hy * sizeof(double) + 15) & ~(size_t)15;
+}
 
 template <typename T>
 __global__ void __launch_bounds__(256) k_resident_etd1(V2<T>* __restrict__ S, ResGeom g, Nonlin q, ColTables tb,
@@ -346,11 +421,13 @@ __global__ void __launch_bounds__(256) k_resident_etd1(V2<T>* __restrict__ S, Re
                                                        Slots sl, long long nsteps) {
   using V = V2<T>;
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  const int M = g.M, Nc = g.Nc, Ny = g.Ny, nsp = Ny * Nc;
-  V* Sl = reinterpret_cast<V*>(smem_raw);  // state S~, [y][k]
-  V* R1 = Sl + nsp;                        // T~ [y][k], then column lines
-  V* R2 = R1 + nsp;
-  V* R3 = R2 + nsp;
+  const int M = g.M, Nc = g.Nc, Ny = g.Ny, nsp = Ny * Nc, nt = Nc * g.hy;
+  double* tE = reinterpret_cast<double*>(smem_raw);
+  double* tQ = tE + nt;
+  V* twn = reinterpret_cast<V*>(smem_raw + resident_table_bytes(Nc, g.hy));
+  V* twy = twn + g.Nx;
+  V* P0 = twy + Ny;  // region 0: [S~ | T~] while the state lives here
+  V* P1 = P0 + 2 * nsp;
   const int slot = slot_of(sl, blockIdx.x);
   S += (size_t)slot * sl.sstride;
   if (sl.M) q.M = sl.M[slot];
@@ -358,30 +435,42 @@ __global__ void __launch_bounds__(256) k_resident_etd1(V2<T>* __restrict__ S, Re
   const double* Q1 = reinterpret_cast<const double*>(tb.Q1) + (size_t)slot * sl.tstride;
   const bool pinned = sl.pin != nullptr;
   const double pin = pinned ? sl.pin[slot] : 0.0;
+  for (int e = threadIdx.x; e < nt; e += blockDim.x) {
+    tE[e] = E1[e];
+    tQ[e] = Q1[e];
+  }
+  for (int e = threadIdx.x; e < g.Nx; e += blockDim.x) twn[e] = twN[e];
+  for (int e = threadIdx.x; e < Ny; e += blockDim.x) twy[e] = twY[e];
   for (int e = threadIdx.x; e < nsp; e += blockDim.x) {
     const int y = e / Nc;
-    Sl[e] = S[g.lay.at(y, e - y * Nc)];
+    P0[e] = S[g.lay.at(y, e - y * Nc)];
   }
   __syncthreads();
+  V* X = P0;  // the region holding S~ (and T~ after the row pass)
+  V* Y = P1;
   bool bad = false;
   for (long long step = 0; step < nsteps; ++step) {
-    // row pass: S~ -> n -> N(n) -> T~ (k_row_nonlin's pack, FFTs and unpack)
+    // row pass: S~ -> n -> N(n) -> T~.  The pack also stores the real
+    // projection of the kx = 0 and kx = Nx/2 columns (k_col_etd stores it
+    // after its inverse transform; the column pass below reads it here).
     for (int e = threadIdx.x; e < Ny * M; e += blockDim.x) {
       const int line = e / M;
       const int k = e - line * M;
-      V xk = Sl[line * Nc + k];
-      V xm = Sl[line * Nc + (M - k)];
+      V xk = X[line * Nc + k];
+      V xm = X[line * Nc + (M - k)];
       if (k == 0) {
         xk.y = T(0);
         xm.y = T(0);
+        X[line * Nc] = xk;
+        X[line * Nc + M] = xm;
       }
       V sm = V{xk.x + xm.x, xk.y - xm.y};
       V df = V{xk.x - xm.x, xk.y + xm.y};
-      V wd = cmul(cconj(twN[k]), df);
-      R2[line * Nc + k] = V{sm.x - wd.y, sm.y + wd.x};
+      V wd = cmul(cconj(twn[k]), df);
+      Y[line * Nc + k] = V{sm.x - wd.y, sm.y + wd.x};
     }
     __syncthreads();
-    V* z = fft_lines<T, true>(R2, R3, g.plM, Ny, Nc, twN, 2);
+    V* z = rs_fft<T, true>(Y, Y + nsp, g.plM, Ny, Ny, Nc, 0, 1, twn, 2);
     for (int e = threadIdx.x; e < Ny * M; e += blockDim.x) {
       const int line = e / M;
       const int m = e - line * M;
@@ -390,55 +479,37 @@ __global__ void __launch_bounds__(256) k_resident_etd1(V2<T>* __restrict__ S, Re
       z[line * Nc + m] = V{nonlin_mu<T>(v.x, q), nonlin_mu<T>(v.y, q)};
     }
     __syncthreads();
-    V* Z = fft_lines<T, false>(z, z == R2 ? R3 : R2, g.plM, Ny, Nc, twN, 2);
+    V* Z = rs_fft<T, false>(z, z == Y ? Y + nsp : Y, g.plM, Ny, Ny, Nc, 0, 1, twn, 2);
     for (int e = threadIdx.x; e < nsp; e += blockDim.x) {
       const int line = e / Nc;
-      R1[e] = row_unpack<T>(Z + line * Nc, e - line * Nc, M, twN);
+      X[nsp + e] = row_unpack<T>(Z + line * Nc, e - line * Nc, M, twn);
     }
     __syncthreads();
-    // column pass (k_col_etd, ETD1): B = FFT_y T~, A = FFT_y S~,
-    // out = E1 A + Q1 B, k = 0 pinned, inverse FFT_y
+    // column pass: the Nc columns of S~ and of T~ (2 Nc lines, stride Nc)
+    // in one forward transform, out = E1 A + Q1 B (k = 0 pinned) into A's
+    // place, inverse transform of those Nc columns
+    V* W = rs_fft<T, false>(X, Y, g.plY, 2 * Nc, Nc, 1, nsp, Nc, twy, 1);
     for (int e = threadIdx.x; e < nsp; e += blockDim.x) {
-      const int y = e / Nc;
-      const int line = e - y * Nc;
-      R2[line * Ny + y] = R1[e];
-    }
-    __syncthreads();
-    V* B = fft_lines<T, false>(R2, R3, g.plY, Nc, Ny, twY, 1);
-    V* F = B == R2 ? R3 : R2;  // free
-    for (int e = threadIdx.x; e < nsp; e += blockDim.x) {
-      const int y = e / Nc;
-      const int line = e - y * Nc;
-      R1[line * Ny + y] = Sl[e];
-    }
-    __syncthreads();
-    V* A = fft_lines<T, false>(R1, F, g.plY, Nc, Ny, twY, 1);
-    V* G = A == R1 ? F : R1;  // free
-    for (int e = threadIdx.x; e < nsp; e += blockDim.x) {
-      const int line = e / Ny;
-      const int ky = e - line * Ny;
-      const size_t ti = (size_t)line * g.hy + ky_half(ky, Ny);
-      const V a = A[e], b = B[e];
-      const double e1 = E1[ti], q1 = Q1[ti];
+      const int ky = e / Nc;
+      const int line = e - ky * Nc;
+      const int ti = line * g.hy + ky_half(ky, Ny);
+      const V a = W[e], b = W[nsp + e];
+      const double e1 = tE[ti], q1 = tQ[ti];
       double ox = e1 * double(a.x) + q1 * double(b.x);
       double oy = e1 * double(a.y) + q1 * double(b.y);
       if (pinned && line == 0 && ky == 0) ox = pin, oy = 0.0;
-      B[e] = V{T(ox), T(oy)};
+      W[e] = V{T(ox), T(oy)};
     }
     __syncthreads();
-    V* R = fft_lines<T, true>(B, G, g.plY, Nc, Ny, twY, 1);
-    for (int e = threadIdx.x; e < nsp; e += blockDim.x) {
-      const int y = e / Nc;
-      const int line = e - y * Nc;
-      V v = R[line * Ny + y];
-      if (line == 0 || line == Nc - 1) v.y = T(0);  // the real kx = 0 and kx = Nx/2 columns (see k_col_etd)
-      Sl[e] = v;
-    }
-    __syncthreads();
+    X = rs_fft<T, true>(W, W == X ? Y : X, g.plY, Nc, Nc, 1, 0, Nc, twy, 1);
+    Y = X == P0 ? P1 : P0;
   }
   for (int e = threadIdx.x; e < nsp; e += blockDim.x) {
     const int y = e / Nc;
-    S[g.lay.at(y, e - y * Nc)] = Sl[e];
+    const int k = e - y * Nc;
+    V v = X[e];
+    if (k == 0 || k == Nc - 1) v.y = T(0);
+    S[g.lay.at(y, k)] = v;
   }
   if (bad && sl.status) atomicOr(sl.status + slot, 1);
 }
