@@ -207,3 +207,66 @@ class SlotBatch:
 
     def close(self):
         self.batch.close()
+
+
+# -- frozen field (pfc_hip.phase_diagram.frozen_q) ---------------------------------
+def k2_ref(cell):
+    """|k|^2 on the full FFT grid of the q00 = 1 box, shape (Ny, Nx)."""
+    Ny, Nx = grid_shape(cell)
+    g = box(cell, 1.0)
+    kx = 2 * np.pi * np.fft.fftfreq(Nx, d=g.Lx / Nx)
+    ky = 2 * np.pi * np.fft.fftfreq(Ny, d=g.Ly / Ny)
+    return kx[None, :] ** 2 + ky[:, None] ** 2
+
+
+class FrozenField:
+    """f(q) of one field frozen on the grid while the box scale q00 = q
+    changes (every wavevector is q k_ref):
+        f(q) = <local> + beta/2 (q0^4 A0 - 2 q0^2 q^2 A1 + q^4 A2),
+        A_n = sum_k |phi_hat_k / N|^2 |k_ref|^(2n)  (full spectrum),
+    so q_opt = q0 sqrt(A1 / A2), the one minimum when A2 > 0
+    (models/logpfc_cpu/phase_diagram.py:475-500)."""
+
+    def __init__(self, state, beta, q0, local_mean, cell):
+        phi = np.asarray(state, dtype=np.float64)
+        power = np.abs(np.fft.fft2(phi)) ** 2 / phi.size ** 2
+        k2 = k2_ref(cell)
+        self.A = np.array([power.sum(), (power * k2).sum(), (power * k2 * k2).sum()])
+        self.local = float(local_mean)
+        self.beta, self.q0 = float(beta), float(q0)
+
+    def f(self, q):
+        s = np.asarray(q, dtype=float) ** 2
+        A0, A1, A2 = self.A
+        q2 = self.q0 ** 2
+        return self.local + self.beta / 2 * (q2 * q2 * A0 - 2 * q2 * s * A1 + s * s * A2)
+
+    def q_opt(self):
+        return self.q0 * np.sqrt(self.A[1] / self.A[2])
+
+
+PEAK_SPREAD_MAX = 0.20
+SUBQ_RATIO_MAX = 1e-3
+
+
+def defect_diagnostics(state, cell):
+    """Record-only defect screen (models/logpfc_cpu/phase_diagram.py:508-528):
+    peak_spread = (max - min) / mean of the heights above the mean of the
+    periodic 8-neighbour local maxima (>= every neighbour); subq_ratio = the
+    largest |phi_hat|^2 at 0 < |k_ref| < 0.95 over the largest |phi_hat|^2
+    (mean removed); defect_flag = either above its limit."""
+    phi = np.asarray(state, dtype=np.float64)
+    m = phi.mean()
+    top = phi > m
+    for dy in (-1, 0, 1):
+        for dx in (-1, 0, 1):
+            if dy or dx:
+                top &= phi >= np.roll(phi, (dy, dx), axis=(0, 1))
+    h = phi[top] - m
+    spread = float((h.max() - h.min()) / h.mean()) if h.size else float('nan')
+    power = np.abs(np.fft.fft2(phi - m)) ** 2
+    kr = np.sqrt(k2_ref(cell))
+    sub = (kr > 1e-9) & (kr < 0.95)
+    ratio = float(power[sub].max() / power.max()) if power.max() > 0 else float('nan')
+    return dict(peak_spread=spread, subq_ratio=ratio, n_peaks=int(top.sum()),
+                defect_flag=bool(spread > PEAK_SPREAD_MAX or ratio > SUBQ_RATIO_MAX))

@@ -121,6 +121,22 @@ def diagnose_state(state, structure, cell):
                           relative=True, nonnegative=True)
 
 
+# -- frozen field (pfc_hip.phase_diagram.frozen_q) ---------------------------------
+def frozen_field(state, params, temp, cell):
+    """f(q) and q_opt of `state` frozen on the grid while the box scale
+    changes: the local part (the box mean, epsilon = temp, no gradient) plus
+    beta/2 (q0^2 - q^2 k_ref^2)^2 |field_hat|^2 summed."""
+    p = validate_params(params)
+    phi = np.asarray(state, dtype=np.float64)
+    local = np.mean(phi * np.log(phi) + temp / 2 * phi ** 2 + p['g'] / 3 * phi ** 3 + p['v0'] / 4 * phi ** 4)
+    return _A.FrozenField(state, p['beta'], p['q0'], local, validate_cell(cell))
+
+
+def defect_diagnostics(state, cell):
+    """peak_spread, subq_ratio, n_peaks, defect_flag of a relaxed field (record only)."""
+    return _A.defect_diagnostics(state, validate_cell(cell))
+
+
 # -- uniform state and bounds ---------------------------------------------------------
 def liquid_free_energy(phi, params, temp):
     """phi ln phi + (beta q0^4 + eps)/2 phi^2 + g/3 phi^3 + v0/4 phi^4."""

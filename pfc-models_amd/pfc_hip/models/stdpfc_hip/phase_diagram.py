@@ -116,6 +116,17 @@ def diagnose_state(state, structure, cell):
     return _A.diagnostics(np.asarray(state, dtype=np.float64), _structure(structure), validate_cell(cell))
 
 
+# -- frozen field (pfc_hip.phase_diagram.frozen_q) ---------------------------------
+def frozen_field(state, params, temp, cell):
+    """f(q) and q_opt of `state` frozen on the grid while the box scale
+    changes: the local part (the box mean, epsilon = temp, no gradient) plus
+    beta/2 (q0^2 - q^2 k_ref^2)^2 |field_hat|^2 summed."""
+    p = validate_params(params)
+    n = np.asarray(state, dtype=np.float64)
+    local = np.mean(temp / 2 * n ** 2 + p['g'] / 3 * n ** 3 + p['v0'] / 4 * n ** 4)
+    return _A.FrozenField(state, p['beta'], p['q0'], local, validate_cell(cell))
+
+
 # -- uniform state and bounds (closed forms) --------------------------------------
 def liquid_free_energy(n, params, temp):
     """(beta q0^4 + eps)/2 n^2 + g/3 n^3 + v0/4 n^4."""

@@ -219,3 +219,71 @@ def test_scan_density_batched_against_reference(hip_device):
     assert all(str(msg) == nlm["message"] for msg in r["message"][:len(nlm["phi"])])
     assert abs(r["q0star"][-1] - ref["q0star"]) <= ref["tol_q"]
     assert abs(r["fmin"][-1] - ref["fmin"]) <= ref["tol_f"]
+
+
+# -- frozen_q on the hip logPFC adapter (the reference's
+#    tests/test_phase_diagram_frozen_q.py:10-83 on this backend) ----------------
+SMALL = {"uNx": 7, "uNy": 12, "Mx": 2, "My": 1}
+H2L = {"mode": "H2L", "A1": 0.9, "A2": 0.0, "A3": 0.0}
+
+
+def _frozen_relax():
+    from pfc_hip.phase_diagram import q0star as Q
+    return Q.RelaxSettings(dt=0.01, dtMax=0.1, dtRate=1e-5, dtGrow=2.0, dtChecks=10, totalT=1e6, stepT=100.0,
+                           divs=10, maxDepth=4, requiredFitness=0.99, minimumA=1e-6)
+
+
+def _f_adapter(phi, params, cell, temp, q):
+    from pfc_hip.models.logpfc_hip import phase_diagram as LOG
+    sim = LOG.build(params, "triangle", cell, H2L, temp, float(phi.mean()), q, 0.01, state=np.array(phi))
+    return float(LOG.free_energy(sim))
+
+
+@pytest.mark.parametrize("beta,g,temp", [(0.5, -2.0, -0.9), (0.25, 1.0, -3.0), (1.0, 0.0, -2.0)])
+def test_frozen_f_equals_the_hip_free_energy(hip_device, beta, g, temp):
+    from pfc_hip.models.logpfc_hip import phase_diagram as LOG
+    params = {"beta": beta, "g": g, "v0": 1.0}
+    rng = np.random.default_rng(1)
+    sim = LOG.build(params, "triangle", REFS["cell"], H2L, temp, 0.7, 1.0, 0.01)
+    phi = LOG.get_state(sim) * (1 + 0.05 * rng.standard_normal(LOG.get_state(sim).shape))
+    ff = LOG.frozen_field(phi, params, temp, REFS["cell"])
+    for q in (0.7, 0.93, 1.0, 1.08, 1.3):
+        fa = _f_adapter(phi, params, REFS["cell"], temp, q)
+        assert abs(float(ff.f(q)) - fa) <= 1e-12 * max(1.0, abs(fa))
+
+
+def test_get_f_abc_restarts_from_a_given_state(hip_device):
+    from pfc_hip.models.logpfc_hip import phase_diagram as LOG
+    from pfc_hip.phase_diagram import q0star as Q
+    params = {"beta": 0.5, "g": -2.0, "v0": 1.0}
+    _, qscan = _settings()
+    m = Q.MinimizeFreeEnergy(LOG, params, "triangle", SMALL, H2L, _frozen_relax(), qscan)
+    f, fit, *_ = m.Get_f_ABC(-0.9, 0.62, 1.0)
+    phi = m.last_state
+    assert phi is not None and abs(phi.mean() - 0.62) < 1e-12
+    assert abs(_f_adapter(phi, params, SMALL, -0.9, 1.0) - f) < 1e-14
+    f2, *_ = m.Get_f_ABC(-0.9, 0.62, 1.0, state=phi)
+    assert abs(f2 - f) <= 1e-11 and m.last_steps < 5000
+
+
+@pytest.mark.parametrize("phi", [0.62, 0.66])
+def test_frozen_q_agrees_with_find_q0star(hip_device, phi):
+    import time
+    from pfc_hip.models.logpfc_hip import phase_diagram as LOG
+    from pfc_hip.phase_diagram import frozen_q as FQ
+    from pfc_hip.phase_diagram import q0star as Q
+    params = {"beta": 0.5, "g": -2.0, "v0": 1.0}
+    _, qscan = _settings()
+    t0 = time.perf_counter()
+    q_f, f_f = Q.MinimizeFreeEnergy(LOG, params, "triangle", SMALL, H2L, _frozen_relax(), qscan).Find_q0Star(-0.9, phi)
+    t1 = time.perf_counter()
+    m = FQ.MinimizeFrozenQ(LOG, params, "triangle", SMALL, H2L, _frozen_relax())
+    q, f = m.Find_q0Star(-0.9, phi)
+    t2 = time.perf_counter()
+    print(f"\nphi {phi}: scan q* {q_f:.10f} ({t1 - t0:.1f} s), frozen q* {q:.10f} ({t2 - t1:.1f} s, "
+          f"{m.last_result['iters']} iterations); dq {q - q_f:.2e} df {f - f_f:.2e}", flush=True)
+    assert m.last_result["status"] == "ok"
+    assert abs(q - q_f) < 1e-5
+    assert abs(f - f_f) < 1e-9
+    assert abs(m.last_result["f_frozen"] - f) < 1e-10
+    assert "defect_flag" in m.last_result
