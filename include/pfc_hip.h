@@ -100,6 +100,17 @@ int pfc_chemical_potential(pfc_plan* plan, void* mu, void* local_mu, int dtype, 
 /* Gaussian-smoothed field irfft2(G n_hat) into a real (Ny, Nx) device output.
  * Replaces Get_local_n / Get_local_phi (models/stdpfc/simulation.py:237-247). */
 int pfc_local_field(pfc_plan* plan, void* out, int dtype, void* stream);
+/* Spectral diagnostics of the current field, from the row spectrum on the
+ * device (n_hat = rfft2(n), N = Nx Ny, sums over the full spectrum):
+ * out[0] = mean = n_hat(0,0)/N, out[1] = sum |n_hat/N|^2 (= mean of n^2),
+ * out[2] = sum |n_hat/N|^2 |k|^2, out[3] = sum |n_hat/N|^2 |k|^4, then
+ * out[4 + 2e], out[5 + 2e] = n_hat at (ky[e], kx[e]) for e < nent (<= 256;
+ * 0 <= ky < Ny, 0 <= kx <= Nx/2).  The phase-diagram adapters' 3-mode
+ * fitness and amplitudes (models/stdpfc_cpu/phase_diagram.py:193-205) and the
+ * frozen-field moments (models/logpfc_cpu/phase_diagram.py:475-500) follow
+ * from these without the field leaving the device.  Synchronises `stream`.
+ * PFC_EUNSUPPORTED if two FP64 copies of the half spectrum do not fit in LDS. */
+int pfc_diagnose(pfc_plan* plan, const int* ky, const int* kx, int nent, double* out, void* stream);
 /* Kernel timing for roofline figures: each kernel kind of the plan's current
  * schedule runs `nsteps` times back to back on scratch copies of the live
  * state (the plan's state is not advanced), bracketed by one HIP event pair on
@@ -147,6 +158,13 @@ int pfc_batch_set_slot(pfc_batch* batch, int slot, double epsilon, double M, voi
 int pfc_batch_set_slot_box(pfc_batch* batch, int slot, double Lx, double Ly, double dt, void* stream);
 int pfc_batch_set_field(pfc_batch* batch, int slot, const void* field, int dtype, void* stream);
 int pfc_batch_get_field(pfc_batch* batch, int slot, void* field, int dtype, void* stream);
+/* Fields of the active slots, in active order, into a device array
+ * (n_active, Ny, Nx) of `dtype` (one launch for all of them).  The batched
+ * phase-diagram driver reads every running evaluation's field per block
+ * (phase_diagram/q0star.py:261-307 reads one with get_state). */
+int pfc_batch_get_fields(pfc_batch* batch, void* fields, int dtype, void* stream);
+/* pfc_diagnose for every active slot: out[n_active][4 + 2 nent], active order. */
+int pfc_batch_diagnose(pfc_batch* batch, const int* ky, const int* kx, int nent, double* out, void* stream);
 /* Slots stepped and reduced by the next calls, in this order (synchronises `stream`). */
 int pfc_batch_set_active(pfc_batch* batch, const int* slots, int n, void* stream);
 int pfc_batch_step(pfc_batch* batch, long long nsteps, void* stream);

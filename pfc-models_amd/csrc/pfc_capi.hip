@@ -374,7 +374,7 @@ struct Engine {
   bool tuned = false, batched = false;
   DevBuf S, Tb, D, E1, Q1, Q2;       // [cap][...]
   DevBuf twN, twY, twN64, twY64, kx2, ky2, dts;  // kx2 [cap][Nc], ky2 [cap][Ny], dts [cap]
-  DevBuf part, out, pin, status, eps, Ms, active, diag;
+  DevBuf part, out, pin, status, eps, Ms, active, diag, dgi, dgo;  // dgi/dgo: diagnostics entries / results
   std::vector<double> h_eps, h_M, h_Lx, h_Ly, h_dt;
   std::vector<int> h_active;
   std::vector<char> field_set;
@@ -846,20 +846,85 @@ void engine_get_field(Engine& e, int slot, void* field, int dtype, hipStream_t s
     const V2<double>* S = (const V2<double>*)e.S.p + off;
     if (dtype == 64)
       hipLaunchKernelGGL((k_row_c2r<double, double>), dim3(rb), dim3(256), rl, st, S, (double*)field, e.rg,
-                         (const V2<double>*)e.twN.p, 1.0, (size_t)0, (size_t)0);
+                         (const V2<double>*)e.twN.p, 1.0, (size_t)0, (size_t)0, nullptr);
     else
       hipLaunchKernelGGL((k_row_c2r<double, float>), dim3(rb), dim3(256), rl, st, S, (float*)field, e.rg,
-                         (const V2<double>*)e.twN.p, 1.0, (size_t)0, (size_t)0);
+                         (const V2<double>*)e.twN.p, 1.0, (size_t)0, (size_t)0, nullptr);
   } else {
     const V2<float>* S = (const V2<float>*)e.S.p + off;
     if (dtype == 64)
       hipLaunchKernelGGL((k_row_c2r<float, double>), dim3(rb), dim3(256), rl, st, S, (double*)field, e.rg,
-                         (const V2<float>*)e.twN.p, 1.0f, (size_t)0, (size_t)0);
+                         (const V2<float>*)e.twN.p, 1.0f, (size_t)0, (size_t)0, nullptr);
     else
       hipLaunchKernelGGL((k_row_c2r<float, float>), dim3(rb), dim3(256), rl, st, S, (float*)field, e.rg,
-                         (const V2<float>*)e.twN.p, 1.0f, (size_t)0, (size_t)0);
+                         (const V2<float>*)e.twN.p, 1.0f, (size_t)0, (size_t)0, nullptr);
   }
   hip_check(hipGetLastError(), "get_field launch");
+}
+
+// fields of the active slots, in active order, into out[nact][Ny][Nx]
+template <typename T, typename Tout>
+void launch_get_fields(Engine& e, Tout* out, hipStream_t st) {
+  const dim3 grid(row_blocks(e.rg), e.nact());
+  hipLaunchKernelGGL((k_row_c2r<T, Tout>), grid, dim3(256), row_lds(e.rg, e.csize), st, (const V2<T>*)e.S.p, out, e.rg,
+                     (const V2<T>*)e.twN.p, T(1), e.nc, (size_t)e.p.Ny * e.p.Nx, (const int*)e.active.p);
+}
+
+void engine_get_fields(Engine& e, void* out, int dtype, hipStream_t st) {
+  if (!out) fail(PFC_EINVAL, "NULL out");
+  if (dtype != 64 && dtype != 32) fail(PFC_EINVAL, "dtype must be 64 or 32");
+  for (int s : e.h_active)
+    if (!e.field_set[s]) fail(PFC_EINVAL, "an active slot has no field");
+  if (e.nact() == 0) return;
+  if (e.f64()) {
+    if (dtype == 64) launch_get_fields<double, double>(e, (double*)out, st);
+    else launch_get_fields<double, float>(e, (float*)out, st);
+  } else {
+    if (dtype == 64) launch_get_fields<float, double>(e, (double*)out, st);
+    else launch_get_fields<float, float>(e, (float*)out, st);
+  }
+  hip_check(hipGetLastError(), "get_fields launch");
+}
+
+// spectral diagnostics of the active slots (k_diagnose): out[nact][4 + 2 nent]
+constexpr int kMaxDiagEntries = 256;
+size_t diag_lds(int Nx, int Ny) { return 2 * (size_t)Ny * (Nx / 2 + 1) * sizeof(V2<double>) + 3 * 256 * sizeof(DD); }
+
+void engine_diagnose(Engine& e, const int* ky, const int* kx, int nent, double* out, hipStream_t st) {
+  if (!out || (nent > 0 && (!ky || !kx))) fail(PFC_EINVAL, "NULL argument");
+  if (nent < 0 || nent > kMaxDiagEntries) fail(PFC_EINVAL, "entry count out of range (0..256)");
+  for (int i = 0; i < nent; ++i)
+    if (ky[i] < 0 || ky[i] >= e.p.Ny || kx[i] < 0 || kx[i] >= e.Nc) fail(PFC_EINVAL, "entry outside the half spectrum");
+  for (int s : e.h_active)
+    if (!e.field_set[s]) fail(PFC_EINVAL, "an active slot has no field");
+  const size_t lds = diag_lds(e.p.Nx, e.p.Ny);
+  if (lds > (size_t)lds_limit()) fail(PFC_EUNSUPPORTED, "grid too large for the device diagnostics");
+  const int n = e.nact();
+  if (n == 0) return;
+  const int per = 4 + 2 * nent;
+  const size_t ibytes = 2 * (size_t)std::max(nent, 1) * sizeof(int), obytes = (size_t)n * per * sizeof(double);
+  if (e.dgi.n < ibytes || e.dgo.n < obytes) {
+    hip_check(hipStreamSynchronize(st), "sync before scratch growth");
+    if (e.dgi.n < ibytes) e.dgi.alloc(ibytes);
+    if (e.dgo.n < obytes) e.dgo.alloc(obytes);
+  }
+  std::vector<int> idx(2 * (size_t)std::max(nent, 1), 0);
+  std::copy(ky, ky + nent, idx.begin());
+  std::copy(kx, kx + nent, idx.begin() + nent);
+  hip_check(hipMemcpyAsync(e.dgi.p, idx.data(), ibytes, hipMemcpyHostToDevice, st), "copy");
+  const int* dky = (const int*)e.dgi.p;
+  DiagParams dp{e.p.Nx, e.Nc, e.p.Ny, e.lay, e.cg64.plY, nent, dky, dky + nent, (const double*)e.kx2.p,
+                (const double*)e.ky2.p};
+  const Slots sl = e.slots_active();
+  if (e.f64())
+    hipLaunchKernelGGL(k_diagnose<double>, dim3(n), dim3(256), lds, st, (const V2<double>*)e.S.p, dp,
+                       (const V2<double>*)e.twY64.p, sl, (double*)e.dgo.p);
+  else
+    hipLaunchKernelGGL(k_diagnose<float>, dim3(n), dim3(256), lds, st, (const V2<float>*)e.S.p, dp,
+                       (const V2<double>*)e.twY64.p, sl, (double*)e.dgo.p);
+  hip_check(hipGetLastError(), "diagnose launch");
+  hip_check(hipMemcpyAsync(out, e.dgo.p, obytes, hipMemcpyDeviceToHost, st), "copy");
+  hip_check(hipStreamSynchronize(st), "sync");  // idx and out must outlive the copies
 }
 
 // (epsilon, M) of one slot; rebuilds its tables on `st`
@@ -970,10 +1035,10 @@ void launch_filter(Engine& e, int slot, int mode, void* out0, void* out1, int dt
     void* dst = o == 0 ? out0 : out1;
     if (dtype == 64)
       hipLaunchKernelGGL((k_row_c2r<T, double>), dim3(rb), dim3(256), rl, st, src, (double*)dst, e.rg, twN, scale,
-                         (size_t)0, (size_t)0);
+                         (size_t)0, (size_t)0, nullptr);
     else
       hipLaunchKernelGGL((k_row_c2r<T, float>), dim3(rb), dim3(256), rl, st, src, (float*)dst, e.rg, twN, scale,
-                         (size_t)0, (size_t)0);
+                         (size_t)0, (size_t)0, nullptr);
   }
   hip_check(hipGetLastError(), "diagnostic launch");
 }
@@ -981,7 +1046,7 @@ void launch_filter(Engine& e, int slot, int mode, void* out0, void* out1, int dt
 size_t engine_bytes(const Engine& e) {
   size_t b = 0;
   for (const DevBuf* d : {&e.S, &e.Tb, &e.D, &e.E1, &e.Q1, &e.Q2, &e.twN, &e.twY, &e.twN64, &e.twY64, &e.kx2, &e.ky2,
-                          &e.dts, &e.part, &e.out, &e.pin, &e.status, &e.eps, &e.Ms, &e.active, &e.diag})
+                          &e.dts, &e.part, &e.out, &e.pin, &e.status, &e.eps, &e.Ms, &e.active, &e.diag, &e.dgi, &e.dgo})
     b += d->n;
   return b;
 }
@@ -1122,6 +1187,13 @@ int pfc_local_field(pfc_plan* pl, void* out, int dtype, void* stream) {
   });
 }
 
+int pfc_diagnose(pfc_plan* pl, const int* ky, const int* kx, int nent, double* out, void* stream) {
+  return guarded([&] {
+    if (!pl) fail(PFC_EINVAL, "NULL plan");
+    engine_diagnose(pl->e, ky, kx, nent, out, reinterpret_cast<hipStream_t>(stream));
+  });
+}
+
 int pfc_step_profile(pfc_plan* pl, long long nsteps, void* stream, double ms[3], long long calls[3]) {
   return guarded([&] {
     if (!pl || !ms || !calls) fail(PFC_EINVAL, "NULL argument");
@@ -1204,6 +1276,20 @@ int pfc_batch_get_field(pfc_batch* b, int slot, void* field, int dtype, void* st
   return guarded([&] {
     if (!b) fail(PFC_EINVAL, "NULL batch");
     engine_get_field(b->e, slot, field, dtype, reinterpret_cast<hipStream_t>(stream));
+  });
+}
+
+int pfc_batch_get_fields(pfc_batch* b, void* fields, int dtype, void* stream) {
+  return guarded([&] {
+    if (!b) fail(PFC_EINVAL, "NULL batch");
+    engine_get_fields(b->e, fields, dtype, reinterpret_cast<hipStream_t>(stream));
+  });
+}
+
+int pfc_batch_diagnose(pfc_batch* b, const int* ky, const int* kx, int nent, double* out, void* stream) {
+  return guarded([&] {
+    if (!b) fail(PFC_EINVAL, "NULL batch");
+    engine_diagnose(b->e, ky, kx, nent, out, reinterpret_cast<hipStream_t>(stream));
   });
 }
 
@@ -1305,13 +1391,13 @@ int pfc_irfft2(const void* in, void* out, int batch, int Ny, int Nx, int dtype, 
                          (V2<double>*)e->scratch.p, e->cg, (const V2<double>*)e->twY.p, 1.0, ncplx);
       hipLaunchKernelGGL((k_row_c2r<double, double>), rgrid, dim3(256), row_lds(e->rg, 16), st,
                          (const V2<double>*)e->scratch.p, (double*)out, e->rg, (const V2<double>*)e->twN.p, scale,
-                         ncplx, nreal);
+                         ncplx, nreal, nullptr);
     } else {
       hipLaunchKernelGGL((k_col_fft<float, true>), cgrid, dim3(256), col_lds(e->cg, 8), st, (V2<float>*)e->scratch.p,
                          e->cg, (const V2<float>*)e->twY.p, 1.0f, ncplx);
       hipLaunchKernelGGL((k_row_c2r<float, float>), rgrid, dim3(256), row_lds(e->rg, 8), st,
                          (const V2<float>*)e->scratch.p, (float*)out, e->rg, (const V2<float>*)e->twN.p,
-                         (float)scale, ncplx, nreal);
+                         (float)scale, ncplx, nreal, nullptr);
     }
     hip_check(hipGetLastError(), "irfft2 launch");
   });

@@ -515,6 +515,87 @@ __global__ void __launch_bounds__(256) k_resident_etd1(V2<T>* __restrict__ S, Re
 }
 
 // ---------------------------------------------------------------------------
+// spectral diagnostics of the phase-diagram driver (one workgroup per slot)
+// ---------------------------------------------------------------------------
+// From the row spectrum S~ alone: A = FFT_y of each column in FP64 in LDS,
+// n_hat(ky, kx) = Nx A[kx][ky] = rfft2(n).  Per slot, out[0..3] =
+//   mean  = n_hat(0, 0) / N,
+//   m0    = sum_k |n_hat / N|^2            (= mean of n^2, Parseval),
+//   m1,m2 = sum_k |n_hat / N|^2 |k|^2, |k|^4 (the slot's own k-grid),
+// sums over the full spectrum (Hermitian weight 2 off the kx = 0 and Nx/2
+// columns), double-double in a fixed order; then out[4 + 2 e .. 5 + 2 e] =
+// n_hat at the e-th requested (ky, kx).  These give the driver's 3-mode
+// fitness and amplitudes and the frozen-field moments without moving the
+// field off the device.
+struct DiagParams {
+  int Nx, Nc, Ny;
+  Lay lay;
+  Radices plY;
+  int nent;           // requested entries
+  const int* ky;      // [nent] in 0..Ny-1
+  const int* kx;      // [nent] in 0..Nc-1
+  const double* kx2;  // [slot][Nc]
+  const double* ky2;  // [slot][Ny]
+};
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_diagnose(const V2<T>* __restrict__ S, DiagParams p,
+                                                  const V2<double>* __restrict__ twY, Slots sl, double* __restrict__ out) {
+  using V = V2<double>;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  const int Nc = p.Nc, Ny = p.Ny, nsp = Nc * Ny;
+  V* b0 = reinterpret_cast<V*>(smem_raw);
+  V* b1 = b0 + nsp;
+  DD* red = reinterpret_cast<DD*>(b1 + nsp);  // [3][blockDim.x]
+  const int slot = slot_of(sl, blockIdx.x);
+  S += (size_t)slot * sl.sstride;
+  const double* kx2 = p.kx2 + (size_t)slot * Nc;
+  const double* ky2 = p.ky2 + (size_t)slot * Ny;
+  out += (size_t)blockIdx.x * (4 + 2 * p.nent);
+  for (int e = threadIdx.x; e < nsp; e += blockDim.x) {
+    const int y = e / Nc;
+    const int k = e - y * Nc;
+    const V2<T> v = S[p.lay.at(y, k)];
+    b0[k * Ny + y] = V{double(v.x), double(v.y)};
+  }
+  __syncthreads();
+  const V* A = fft_lines<double, false>(b0, b1, p.plY, Nc, Ny, twY, 1);
+  DD m[3] = {DD{0.0, 0.0}, DD{0.0, 0.0}, DD{0.0, 0.0}};
+  for (int e = threadIdx.x; e < nsp; e += blockDim.x) {
+    const int k = e / Ny;
+    const int y = e - k * Ny;
+    const V a = A[e];
+    const double w = (k == 0 || 2 * k == p.Nx) ? 1.0 : 2.0;
+    const double pw = w * (a.x * a.x + a.y * a.y);
+    const double k2 = kx2[k] + ky2[y];
+    m[0] = dd_add(m[0], pw);
+    m[1] = dd_add(m[1], pw * k2);
+    m[2] = dd_add(m[2], pw * (k2 * k2));
+  }
+  for (int i = 0; i < 3; ++i) red[i * blockDim.x + threadIdx.x] = m[i];
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s)
+      for (int i = 0; i < 3; ++i) {
+        DD* r = red + i * blockDim.x + threadIdx.x;
+        r[0] = dd_add(r[0], r[s]);
+      }
+    __syncthreads();
+  }
+  // |n_hat / N|^2 = |A|^2 / Ny^2
+  const double inv = 1.0 / ((double)Ny * (double)Ny);
+  if (threadIdx.x == 0) {
+    out[0] = A[0].x / Ny;
+    for (int i = 0; i < 3; ++i) out[1 + i] = (red[i * blockDim.x].hi + red[i * blockDim.x].lo) * inv;
+  }
+  for (int e = threadIdx.x; e < p.nent; e += blockDim.x) {
+    const V a = A[p.kx[e] * Ny + p.ky[e]];
+    out[4 + 2 * e] = a.x * p.Nx;
+    out[5 + 2 * e] = a.y * p.Nx;
+  }
+}
+
+// ---------------------------------------------------------------------------
 // plain transforms (field upload/download and the backend rfft2/irfft2)
 // ---------------------------------------------------------------------------
 // real rows (Nx) -> half spectrum rows, times `scale`.  blockIdx.y = batch.
@@ -550,12 +631,12 @@ __global__ void __launch_bounds__(256) k_row_r2c(const Tin* __restrict__ x, V2<T
 template <typename T, typename Tout>
 __global__ void __launch_bounds__(256) k_row_c2r(const V2<T>* __restrict__ S, Tout* __restrict__ x, RowGeom g,
                                                  const V2<T>* __restrict__ twN, T scale, size_t in_bstride,
-                                                 size_t out_bstride) {
+                                                 size_t out_bstride, const int* __restrict__ list) {
   using V = V2<T>;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   V* b0 = reinterpret_cast<V*>(smem_raw);
   V* b1 = b0 + g.nl * g.ld;
-  S += blockIdx.y * in_bstride;
+  S += (size_t)(list ? list[blockIdx.y] : (int)blockIdx.y) * in_bstride;  // input instance: list entry or index
   x += blockIdx.y * out_bstride;
   const int row0 = blockIdx.x * g.nl;
   const int nrows = min(g.nl, g.Ny - row0);

@@ -144,6 +144,70 @@ def fitness(field, structure, cell, relative=False, nonnegative=False):
     return 1 - float(np.abs(np.power(field - rec, 2).mean())), np.abs(A), np.abs(B), np.abs(C)
 
 
+class SpectralFit:
+    """`fitness` computed from a field's spectral diagnostics (pfc_diagnose:
+    mean, m0 = mean of n^2, and n_hat = rfft2(n) at the structure's mode
+    entries) instead of the field: amplitudes = sign * mean |n_hat| / N over
+    each set (over the mean when relative), the 3-mode field rebuilt from
+    them (rescaled to a zero minimum when nonnegative and negative), and by
+    Parseval over the full spectrum
+        mean((n - n3)^2) = m0 - mean^2 + sum_e w_e (|n_hat_e - t_e|^2 - |n_hat_e|^2) / N^2,
+    t_e = the rebuilt field's real coefficient N a_set(e), w_e = 1 on the
+    kx = 0 and Nx/2 columns, else 2.  Vectorised over many fields."""
+
+    def __init__(self, structure, cell, relative=False, nonnegative=False):
+        Ny, Nx = grid_shape(cell)
+        self.N, self.relative, self.nonnegative = Nx * Ny, relative, nonnegative
+        self.sign = structure.amplitudeSign
+        ky, kx, which = [], [], []
+        for i, (rows, cols) in enumerate(mode_indices(structure, cell)):
+            ky += [int(r) % Ny for r in rows]
+            kx += [int(c) for c in cols]
+            which += [i] * len(rows)
+        self.ky, self.kx = np.array(ky, dtype=np.int32), np.array(kx, dtype=np.int32)
+        self.which = np.array(which)
+        self.weight = np.where((self.kx == 0) | (2 * self.kx == Nx), 1.0, 2.0)
+        self.nsets = len(structure.kvectors)
+        if nonnegative:  # unit-amplitude field of each set, as three_mode builds it
+            self.basis = np.empty((self.nsets, Ny, Nx))
+            for i in range(self.nsets):
+                spec = np.zeros((Ny, Nx // 2 + 1), dtype=complex)
+                sel = self.which == i
+                spec[self.ky[sel], self.kx[sel]] = self.N
+                self.basis[i] = np.fft.irfftn(spec, s=(Ny, Nx), axes=(0, 1))
+
+    def assemble(self, rows):
+        """[(mean, m0, m1, m2, n_hat entries)] -> (fitness, amplitudes) arrays."""
+        mean = np.array([r[0] for r in rows])
+        m0 = np.array([r[1] for r in rows])
+        ent = np.array([r[4] for r in rows]).reshape(len(rows), -1)
+        mag = np.abs(ent)
+        amps = np.stack([self.sign * mag[:, self.which == i].mean(axis=1) / self.N for i in range(self.nsets)], 1)
+        if self.relative:
+            amps = amps / mean[:, None]
+        a = amps * (mean[:, None] if self.relative else 1.0)  # what three_mode receives
+        if self.nonnegative:
+            n3 = mean[:, None, None] + np.einsum('fs,syx->fyx', a, self.basis)
+            lo = n3.min(axis=(1, 2))
+            scale = np.where(lo < 0, mean / np.where(lo < 0, mean - lo, 1.0), 1.0)
+            a = a * scale[:, None]
+        t = a[:, self.which] * self.N
+        extra = (self.weight * (np.abs(ent - t) ** 2 - mag ** 2)).sum(axis=1) / self.N ** 2
+        resid = m0 - mean ** 2 + extra
+        return 1.0 - np.abs(resid), np.abs(amps)
+
+
+_FITS = {}
+
+
+def spectral_fit(structure, cell, relative=False, nonnegative=False):
+    """A cached SpectralFit per (structure, cell, relative, nonnegative)."""
+    key = (structure, tuple(sorted(cell.items())), relative, nonnegative)
+    if key not in _FITS:
+        _FITS[key] = SpectralFit(structure, cell, relative, nonnegative)
+    return _FITS[key]
+
+
 def maxima_per_cell(field, cell):
     """Strict local maxima on the periodic grid (8 neighbours), per rectangular cell."""
     is_max = np.ones(field.shape, dtype=bool)
@@ -167,11 +231,11 @@ class SlotBatch:
     parameters (`plan_kwargs`), the same k-grids and tables, the same field
     transfers."""
 
-    def __init__(self, kind, model_params, cell, capacity, diagnose_state, precision=64):
+    def __init__(self, kind, model_params, cell, capacity, fit, precision=64):
         from ..plan import Batch
         from ._base import plan_kwargs
         self.cell = cell
-        self._diagnose = diagnose_state
+        self._fit = fit  # (structure, cell) -> SpectralFit
         self.batch = Batch(capacity, **plan_kwargs(kind, model_params, box(cell, 1.0), precision=precision))
         self._active = None
 
@@ -202,8 +266,16 @@ class SlotBatch:
         """FP64 free energy per site of each active slot, in active order."""
         return self.batch.reduce()[:, 0].copy()
 
-    def diagnose(self, slot, state, structure, cell):
-        return self._diagnose(state, structure, cell)
+    def states(self):
+        """Host fields of the active slots, in active order (one device call)."""
+        return list(self.batch.get_fields().cpu().numpy())
+
+    def diagnose_active(self, structure, cell):
+        """The adapter's diagnostics of every active slot, in active order,
+        computed on the device (details left empty)."""
+        fit = self._fit(structure, cell)
+        fitness, amps = fit.assemble(self.batch.diagnose(fit.ky, fit.kx))
+        return [_pd.Diagnostics(fitness=float(f), amplitudes=tuple(float(x) for x in a)) for f, a in zip(fitness, amps)]
 
     def close(self):
         self.batch.close()

@@ -13,6 +13,7 @@ from __future__ import annotations
 
 import numpy as np
 
+from ... import phase_diagram as _pd
 from .. import _adapter as _A
 from . import simulation as _simulation
 
@@ -92,9 +93,20 @@ def free_energy(sim):
 
 
 def diagnose(sim, structure, cell):
-    """fitness = 1 - mean((n - n_3mode)^2); amplitudes (A, B, C) = mean |n_hat|
-    over the 1-, 2-, 3-mode sets per site; details: maxima_per_cell."""
-    return diagnose_state(get_state(sim), structure, cell)
+    """fitness = 1 - mean((n - n_3mode)^2); amplitudes (A, B, C) = mean
+    |n_hat| over the 1-, 2-, 3-mode sets per site; details:
+    maxima_per_cell.  Fitness and amplitudes come from the device's spectral
+    diagnostics (`SpectralFit`); `diagnose_state` computes them from a host
+    field with host FFTs (the same values to rounding)."""
+    fit = _fit(structure, cell)
+    (fitness,), (amps,) = fit.assemble([sim._plan.diagnose(fit.ky, fit.kx)])
+    return _pd.Diagnostics(fitness=float(fitness), amplitudes=tuple(float(a) for a in amps),
+                           details={'maxima_per_cell': _A.maxima_per_cell(get_state(sim), validate_cell(cell))})
+
+
+def _fit(structure, cell):
+    """The SpectralFit of a structure on a cell (the device form of `diagnose_state`)."""
+    return _A.spectral_fit(_structure(structure), validate_cell(cell), relative=False, nonnegative=False)
 
 
 # -- batched evaluations (pfc_hip.phase_diagram.batched) ---------------------------
@@ -103,7 +115,7 @@ def open_batch(params, cell, capacity, precision=64):
     p = validate_params(params)
     model = _simulation.BuildStdPFCModelParameters(Gamma=p['Gamma'], beta=p['beta'], q0=p['q0'], epsilon=0.0,
                                                    g=p['g'], dt=1.0)
-    return _A.SlotBatch('std', model, validate_cell(cell), capacity, diagnose_state, precision=precision)
+    return _A.SlotBatch('std', model, validate_cell(cell), capacity, _fit, precision=precision)
 
 
 def ansatz_state(structure, cell, ansatz, meanDensity):
@@ -112,7 +124,7 @@ def ansatz_state(structure, cell, ansatz, meanDensity):
 
 
 def diagnose_state(state, structure, cell):
-    """`diagnose` of a host field."""
+    """Fitness and amplitudes of a host field, with host FFTs."""
     return _A.diagnostics(np.asarray(state, dtype=np.float64), _structure(structure), validate_cell(cell))
 
 

@@ -26,11 +26,12 @@ scan_density's results, statuses, records and checkpoint files.
 
 The adapter provides, besides the serial names: `open_batch(params, cell,
 capacity)`, an object with `place(slot, temp, meanDensity, q00, dt, state)`,
-`state(slot)`, `set_state(slot, state)`, `set_active(slots)`, `step(n)` (the
-active slots), `free_energy()` (per active slot), `diagnose(slot, state,
-structure, cell)` (the adapter's `diagnose` of that slot holding `state`)
-and `close()`; and `ansatz_state(structure, cell, ansatz, meanDensity)`, the
-fresh ansatz `build` starts from.
+`set_state(slot, state)`, `set_active(slots)`, `step(n)`, and, for the
+active slots in active order, `states()`, `free_energy()` and
+`diagnose_active(structure, cell)` (the adapter's `diagnose` of each), and
+`close()`; and `ansatz_state(structure, cell, ansatz, meanDensity)`, the
+fresh ansatz `build` starts from.  The hip adapters read all fields with one
+device call and compute the diagnostics on the device.
 """
 from __future__ import annotations
 
@@ -43,9 +44,9 @@ from . import q0star as Q
 
 
 class _Point:
-    """One evaluation's running values: time, f and A after the last block,
-    time steps, stop reason (None while running) and field."""
-    __slots__ = ("temp", "rho", "q00", "t", "f", "A", "steps", "stop", "state")
+    """One evaluation's running values: time, f, A and diagnostics after the
+    last block, time steps, stop reason (None while running) and field."""
+    __slots__ = ("temp", "rho", "q00", "t", "f", "A", "diag", "steps", "stop", "state")
 
     def __init__(self, temp, rho, q00):
         self.temp, self.rho, self.q00 = float(temp), float(rho), float(q00)
@@ -68,18 +69,19 @@ def evaluate_batched(minimizer, points, states=None, precision=64):
     sb = ad.open_batch(m.params, m.cell, len(points), precision=precision)
     try:
         pts = [_Point(*p) for p in points]
+        everyone = list(range(len(pts)))
         for j, (p, s) in enumerate(zip(pts, states)):
             if s is None:
                 sb.place(j, p.temp, p.rho, p.q00, dt0, ad.ansatz_state(m.structure, m.cell, m.ansatz, p.rho))
-                if r.noise:
-                    sb.set_state(j, Q.noisy_state(sb.state(j), r.seed, r.noise))
             else:
                 sb.place(j, p.temp, p.rho, p.q00, dt0, s)
-            p.state = sb.state(j)
-            p.A = sb.diagnose(j, p.state, m.structure, m.cell).amplitudes[0]
-        sb.set_active(range(len(pts)))
-        for j, f in enumerate(sb.free_energy()):
-            pts[j].f = float(f)
+        sb.set_active(everyone)
+        if r.noise:  # the noise multiplies the ansatz as read back from the device, as _generateSim's does
+            for j, x in enumerate(sb.states()):
+                if states[j] is None:
+                    sb.set_state(j, Q.noisy_state(x, r.seed, r.noise))
+        for p, x, f, d in zip(pts, sb.states(), sb.free_energy(), sb.diagnose_active(m.structure, m.cell)):
+            p.state, p.f, p.diag, p.A = x, float(f), d, d.amplitudes[0]
         for _ in range(blocks):
             act = [j for j, p in enumerate(pts) if p.stop is None]
             if not act:
@@ -87,54 +89,58 @@ def evaluate_batched(minimizer, points, states=None, precision=64):
             sb.set_active(act)
             sb.step(n)
             fe = sb.free_energy()
+            fields = sb.states()
+            df = {}
             for k, j in enumerate(act):
-                _after_block(m, ad, r, sb, pts[j], j, float(fe[k]), n, dt0)
-        for p in pts:
-            if p.stop is None:
-                p.stop = Q.STOP_TOTALT
-        out = []
-        for j, p in enumerate(pts):
-            diag = sb.diagnose(j, p.state, m.structure, m.cell)
-            out.append(Q.Evaluation(p.f, float(diag.fitness), tuple(float(a) for a in diag.amplitudes), p.stop,
-                                    p.steps, dt0, p.state))
-        return out
+                df[j] = _block(m, ad, r, sb, pts[j], j, float(fe[k]), fields[k], n, dt0)
+            for j, d in zip(act, sb.diagnose_active(m.structure, m.cell)):
+                p = pts[j]
+                p.diag = d
+                if p.stop is None:
+                    _stop_tests(r, p, df[j], d)
+        return [Q.Evaluation(p.f, float(p.diag.fitness), tuple(float(a) for a in p.diag.amplitudes),
+                             p.stop or Q.STOP_TOTALT, p.steps, dt0, p.state) for p in pts]
     finally:
         sb.close()
 
 
-def _after_block(m, ad, r, sb, p, j, f, n, dt):
-    """The serial driver's bookkeeping and stop tests for slot j after a block
-    of n steps that ended at free energy f."""
+def _block(m, ad, r, sb, p, j, f, field, n, dt):
+    """Slot j after a block of n steps ending at free energy f and `field`:
+    the serial driver's time and step bookkeeping, and its rollback of a
+    rising or NaN block (on a Simulation of its own, from the block's
+    starting state; the outcome goes back into the slot).  Returns the
+    block's change of f."""
     t_before, f_before, before = p.t, p.f, p.state
     for _ in range(n):  # t accumulates step by step, as the serial run does
         p.t += dt
     p.steps += n
     df = f - f_before
-    if np.isnan(f) or df > r.fTol * abs(f_before):
-        sim = m._generateSim(p.temp, p.rho, p.q00, dt, before)
-        ad.set_time(sim, t_before)
-        steps0 = m.last_steps
-        try:
-            df = m._rollback(sim, p.temp, p.rho, p.q00, r.stepT, before, t_before, f_before, bool(np.isnan(f)),
-                             divs=r.divs)
-        except Q.MaxDepthReached as e:
-            p.stop = Q.STOP_NAN if e.nan else Q.STOP_MAX_DEPTH
-            p.state, p.f, p.t = ad.get_state(sim), float(ad.free_energy(sim)), ad.get_time(sim)
-            return
-        finally:
-            p.steps += m.last_steps - steps0
-        p.state, f, p.t = ad.get_state(sim), float(ad.free_energy(sim)), ad.get_time(sim)
-        sb.set_state(j, p.state)
-    else:
-        p.state = sb.state(j)
-    p.f = f
-    diag = sb.diagnose(j, p.state, m.structure, m.cell)
+    if not (np.isnan(f) or df > r.fTol * abs(f_before)):
+        p.state, p.f = field, f
+        return df
+    sim = m._generateSim(p.temp, p.rho, p.q00, dt, before)
+    ad.set_time(sim, t_before)
+    steps0 = m.last_steps
+    try:
+        df = m._rollback(sim, p.temp, p.rho, p.q00, r.stepT, before, t_before, f_before, bool(np.isnan(f)),
+                         divs=r.divs)
+    except Q.MaxDepthReached as e:
+        p.stop = Q.STOP_NAN if e.nan else Q.STOP_MAX_DEPTH
+    finally:
+        p.steps += m.last_steps - steps0
+    p.state, p.f, p.t = ad.get_state(sim), float(ad.free_energy(sim)), ad.get_time(sim)
+    sb.set_state(j, p.state)
+    return df
+
+
+def _stop_tests(r, p, df, diag):
+    """Get_f_ABC's tests after a block, in its order."""
     fit, A = diag.fitness, diag.amplitudes[0]
     if r.requiredFitness is not None and fit < r.requiredFitness:
         p.stop = Q.STOP_POOR_FIT
     elif A < r.minimumA:
         p.stop = Q.STOP_MIN_AMPLITUDE
-    elif abs(df) <= r.fTol * abs(f) and abs(A - p.A) <= r.ATol * A:
+    elif abs(df) <= r.fTol * abs(p.f) and abs(A - p.A) <= r.ATol * A:
         p.stop = Q.STOP_CONVERGED
     p.A = A
 

@@ -123,6 +123,11 @@ class Plan:
         _lib.check(self._lib.pfc_step(self._h, int(nsteps), current_stream_handle()))
         return self
 
+    def diagnose(self, ky, kx):
+        """Spectral diagnostics of the field (pfc_diagnose): (mean, m0, m1, m2,
+        n_hat at the requested (ky, kx) as a complex array)."""
+        return _unpack_diag(_diagnose(self._lib.pfc_diagnose, self._h, ky, kx, 1))[0]
+
     def step_profile(self, nsteps):
         """Each kernel kind of the current schedule, `nsteps` launches back to
         back on scratch copies of the state (the state is not advanced), timed
@@ -248,6 +253,20 @@ class Batch:
                                                  current_stream_handle()))
         return out
 
+    def get_fields(self, dtype=None):
+        """Fields of the active slots in active order, one device array (n_active, Ny, Nx)."""
+        torch = self.torch
+        out = torch.empty((len(self._active), self.Ny, self.Nx), device="cuda", dtype=dtype or torch.float64)
+        if self._active:
+            dt = 64 if out.dtype == torch.float64 else 32
+            _lib.check(self._lib.pfc_batch_get_fields(self._h, ctypes.c_void_p(out.data_ptr()), dt,
+                                                      current_stream_handle()))
+        return out
+
+    def diagnose(self, ky, kx):
+        """pfc_batch_diagnose: per active slot (mean, m0, m1, m2, n_hat entries)."""
+        return _unpack_diag(_diagnose(self._lib.pfc_batch_diagnose, self._h, ky, kx, len(self._active)))
+
     def set_active(self, slots):
         slots = [int(s) for s in slots]
         arr = (ctypes.c_int * max(1, len(slots)))(*slots)
@@ -278,6 +297,24 @@ class Batch:
         v = ctypes.c_longlong()
         _lib.check(self._lib.pfc_batch_bytes(self._h, ctypes.byref(v)))
         return v.value
+
+
+def _diagnose(fn, h, ky, kx, n):
+    ky = np.asarray(ky, dtype=np.int32).ravel()
+    kx = np.asarray(kx, dtype=np.int32).ravel()
+    if ky.shape != kx.shape:
+        raise ValueError("ky and kx must have the same length")
+    ne = ky.size
+    out = np.zeros((max(n, 1), 4 + 2 * ne), dtype=np.float64)
+    iptr = ctypes.POINTER(ctypes.c_int)
+    _lib.check(fn(h, ky.ctypes.data_as(iptr), kx.ctypes.data_as(iptr), ne,
+                  out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), current_stream_handle()))
+    return out[:n]
+
+
+def _unpack_diag(raw):
+    """rows of pfc_diagnose output -> [(mean, m0, m1, m2, entries)]"""
+    return [(r[0], r[1], r[2], r[3], r[4::2] + 1j * r[5::2]) for r in raw]
 
 
 def rfft2(x):

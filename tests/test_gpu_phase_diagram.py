@@ -287,3 +287,58 @@ def test_frozen_q_agrees_with_find_q0star(hip_device, phi):
     assert abs(f - f_f) < 1e-9
     assert abs(m.last_result["f_frozen"] - f) < 1e-10
     assert "defect_flag" in m.last_result
+
+
+# -- device diagnostics -------------------------------------------------------------
+def test_device_diagnostics_equal_numpy_on_the_field(hip_device):
+    from pfc_hip import fields as F
+    from pfc_hip.models import _adapter as A
+    from pfc_hip.models.logpfc_hip import phase_diagram as LOG
+    from pfc_hip.plan import Batch, Plan
+    cell = REFS["cell"]
+    fit = A.spectral_fit(LOG.STRUCTURES["triangle"], cell, True, True)
+    qs = (0.97, 1.0, 1.04)
+    kw = dict(model="log", Nx=28, Ny=24, Gamma=1.0, beta=0.5, q0=1.0, epsilon=-0.9, g=-2.0, dt=0.1, precision=64)
+    base = LOG.initial_field("triangle", cell, H2L, 0.63)
+    fields = [base + F.uniform_noise(base.shape, 0.0, 0.01, seed=40 + i) for i in range(len(qs))]
+    b = Batch(len(qs), **kw, Lx=1.0, Ly=1.0)
+    plans = []
+    for i, q in enumerate(qs):
+        g = A.box(cell, q)
+        b.set_slot_box(i, g.Lx, g.Ly, 0.1)
+        b.set_field(i, fields[i])
+        plans.append(Plan(**kw, Lx=g.Lx, Ly=g.Ly).set_field(fields[i]).step(50))
+    b.step(50)
+    got = b.get_fields().cpu().numpy()
+    rows = b.diagnose(fit.ky, fit.kx)
+    N = 28 * 24
+    for i, q in enumerate(qs):
+        x = plans[i].get_field().cpu().numpy()
+        assert np.array_equal(got[i], x) and np.array_equal(b.get_field(i).cpu().numpy(), x)
+        mean, m0, m1, m2, ent = plans[i].diagnose(fit.ky, fit.kx)
+        bm = rows[i]
+        assert (bm[0], bm[1], bm[2], bm[3]) == (mean, m0, m1, m2) and np.array_equal(bm[4], ent)
+        spec = np.fft.rfft2(x)
+        full = np.abs(np.fft.fft2(x) / N) ** 2
+        k2 = A.k2_ref(cell) * q * q
+        assert mean == pytest.approx(spec[0, 0].real / N, abs=1e-15)
+        assert m0 == pytest.approx(np.mean(x ** 2), rel=1e-14)
+        assert (m1, m2) == pytest.approx(((full * k2).sum(), (full * k2 * k2).sum()), rel=1e-13)
+        assert np.abs(ent - spec[fit.ky, fit.kx]).max() <= 1e-12 * N
+
+
+@pytest.mark.parametrize("slug", ["logpfc_hip", "stdpfc_hip"])
+@pytest.mark.parametrize("structure", ["triangle", "honeycomb", "stripe"])
+def test_adapter_device_diagnose_equals_host(hip_device, slug, structure):
+    from pfc_hip.models.logpfc_hip import phase_diagram as LOG
+    from pfc_hip.models.stdpfc_hip import phase_diagram as STD
+    ad = {"logpfc_hip": LOG, "stdpfc_hip": STD}[slug]
+    rho = 0.63 if slug == "logpfc_hip" else -0.25
+    params = dict(beta=0.5, g=-2.0) if slug == "logpfc_hip" else dict(beta=1.0, g=0.0)
+    sim = ad.build(params, structure, REFS["cell"], H2L, -0.9 if slug == "logpfc_hip" else -0.25, rho, 1.0, 0.1)
+    ad.run(sim, 300)
+    d = ad.diagnose(sim, structure, REFS["cell"])
+    h = ad.diagnose_state(ad.get_state(sim), structure, REFS["cell"])
+    assert d.fitness == pytest.approx(h.fitness, abs=1e-13)
+    assert d.amplitudes == pytest.approx(h.amplitudes, abs=1e-14)
+    assert d.details == h.details

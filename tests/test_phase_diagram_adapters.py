@@ -175,3 +175,54 @@ def test_log_ansatz_is_positive_with_the_requested_mean():
         assert f.min() == pytest.approx(0.0, abs=1e-15) and f.mean() == pytest.approx(0.6, abs=1e-14)
         small = LOG.initial_field(name, c, ANSATZ, 0.6)
         assert small.min() > 0.59
+
+
+def _device_rows(field, fit, k2=None):
+    """What pfc_diagnose returns, computed with NumPy from the field."""
+    spec = np.fft.rfft2(field)
+    N = field.size
+    full = np.abs(np.fft.fft2(field) / N) ** 2
+    m1 = m2 = 0.0
+    if k2 is not None:
+        m1, m2 = (full * k2).sum(), (full * k2 * k2).sum()
+    return (spec[0, 0].real / N, float(np.mean(field ** 2)), m1, m2, spec[fit.ky, fit.kx])
+
+
+@pytest.mark.parametrize("name", ["triangle", "honeycomb", "stripe"])
+@pytest.mark.parametrize("model", ["std", "log"])
+def test_spectral_fit_equals_the_real_space_fitness(model, name):
+    # the Parseval form of the fitness (what the device diagnostics feed)
+    # against the real-space definition, with noise, and for logPFC with
+    # amplitudes large enough that the 3-mode rebuild is rescaled
+    ad = STD if model == "std" else LOG
+    s = ad.STRUCTURES[name]
+    rel = model == "log"
+    fit = A.spectral_fit(s, CELL, relative=rel, nonnegative=rel)
+    rng = np.random.default_rng(5)
+    fields = []
+    for amp in (0.05, 0.4, 1.5):
+        mean = 0.6 if rel else -0.2
+        base = A.three_mode(s, CELL, mean, amp * 0.2 * s.amplitudeSign, amp * 0.03, amp * 0.01)
+        fields.append(base + 0.02 * rng.standard_normal(base.shape))
+    fitness, amps = fit.assemble([_device_rows(f, fit) for f in fields])
+    rescaled = 0
+    for f, fi, am in zip(fields, fitness, amps):
+        ref = A.fitness(f, s, CELL, relative=rel, nonnegative=rel)
+        assert fi == pytest.approx(ref[0], abs=1e-13)
+        assert tuple(am) == pytest.approx(ref[1:], abs=1e-14)
+        if rel:
+            a = np.array(ref[1:]) * f.mean() * s.amplitudeSign
+            rescaled += A.three_mode(s, CELL, f.mean(), *a).min() < 0
+    if rel and name == "triangle":
+        assert rescaled >= 1
+
+
+def test_frozen_moments_from_device_style_sums():
+    # pfc_diagnose's m1, m2 (full-spectrum sums with |k|^2, |k|^4) give the
+    # FrozenField moments A1, A2 when taken on the q00 = 1 box
+    phi = LOG.initial_field("triangle", CELL, {"mode": "x", "A1": 0.9, "A2": 0.1, "A3": 0.0}, 0.7)
+    phi = phi + 0.01 * np.random.default_rng(3).standard_normal(phi.shape)
+    fit = A.spectral_fit(LOG.STRUCTURES["triangle"], CELL, True, True)
+    mean, m0, m1, m2, _ = _device_rows(phi, fit, A.k2_ref(CELL))
+    ff = LOG.frozen_field(phi, dict(beta=0.5, g=-2.0), -0.9, CELL)
+    assert (m0, m1, m2) == pytest.approx(tuple(ff.A), rel=1e-13)

@@ -63,8 +63,8 @@ class ToyBatch:
     def place(self, slot, temp, meanDensity, q00, dt, state):
         self.slots[slot] = self.ad.build({}, 'dots', {}, {}, temp, meanDensity, q00, dt, state=state)
 
-    def state(self, slot):
-        return self.ad.get_state(self.slots[slot])
+    def states(self):
+        return [self.ad.get_state(self.slots[j]) for j in self.active]
 
     def set_state(self, slot, state):
         self.ad.set_state(self.slots[slot], state)
@@ -80,8 +80,8 @@ class ToyBatch:
     def free_energy(self):
         return np.array([self.slots[j].free_energy() for j in self.active])
 
-    def diagnose(self, slot, state, structure, cell):
-        return self.slots[slot].diagnose(state[2])
+    def diagnose_active(self, structure, cell):
+        return [self.slots[j].diagnose() for j in self.active]
 
     def close(self):
         self.slots = None
