@@ -276,6 +276,53 @@ def run_sweep(plain_fraction=8):
     }
 
 
+def run_phase_diagram(n_dens=12, serial_every=1):
+    """q0* over a density list with the phase-diagram driver: logPFC triangle,
+    beta 0.5, g -2, eps -0.9 on the 28 x 24 cell with the driver tests'
+    relaxation and q00-scan settings (3 passes x 21 q00 per density).  All
+    densities scanned batched (pass p of every density in one batch) against
+    the serial driver, one density and one q00 at a time, on every
+    `serial_every`-th density (all by default); results must be identical.
+    Each pass lasts as long as its slowest relaxation (edge q00 points whose
+    seeded instability grows for up to ~10^6 steps before a poor-fit stop),
+    which bounds the batched rate."""
+    import numpy as np
+    from pfc_hip.models.logpfc_hip import phase_diagram as LOG
+    from pfc_hip.phase_diagram import batched as B
+    from pfc_hip.phase_diagram import q0star as Q
+    relax = Q.RelaxSettings(dt=0.1, totalT=1e6, stepT=100.0, divs=10, maxDepth=4, requiredFitness=0.99,
+                            minimumA=1e-6)
+    qscan = Q.QScanSettings(passes=3, Deltaq=0.02, q00_Steps=21, Deltaq_Refinement=0.05)
+    m = Q.MinimizeFreeEnergy(LOG, dict(beta=0.5, g=-2.0), "triangle", dict(uNx=14, uNy=24, Mx=2, My=1),
+                             dict(mode="L2H", A1=1e-5, A2=1e-6, A3=1e-7), relax, qscan)
+    rhos = np.round(np.linspace(0.54, 0.76, n_dens), 6)
+    t0 = time.perf_counter()
+    rb = B.scan_density_batched(m, -0.9, rhos)
+    tb = time.perf_counter() - t0
+    sub = rhos[::serial_every]
+    t0 = time.perf_counter()
+    rs = Q.scan_density(m, -0.9, sub)
+    ts = time.perf_counter() - t0
+    same = True
+    for k in range(len(sub)):
+        j = k * serial_every
+        same &= str(rb["status"][j]) == str(rs["status"][k])
+        same &= bool(np.isnan(rs["q0star"][k]) or (rb["q0star"][j] == rs["q0star"][k] and rb["fmin"][j] == rs["fmin"][k]))
+    n_eval = int(rb["qscan"]["q00"].size)
+    return {
+        "metric": "q0* density scans per second (logPFC triangle, 28 x 24 cell, eps -0.9, 3 x 21 q00 relaxations)",
+        "batched_scans_per_s": round(len(rhos) / tb, 3),
+        "serial_scans_per_s": round(len(sub) / ts, 3),
+        "speedup_batched_vs_serial": round((len(rhos) / tb) / (len(sub) / ts), 2),
+        "densities": [float(x) for x in rhos],
+        "batched_wall_s": round(tb, 2), "serial_wall_s": round(ts, 2), "serial_sample": [float(x) for x in sub],
+        "evaluations": n_eval, "time_steps": int(rb["qscan"]["steps"].sum()),
+        "statuses": [str(x) for x in rb["status"]],
+        "q0star": [None if np.isnan(x) else round(float(x), 10) for x in rb["q0star"]],
+        "batched_equals_serial": bool(same),
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -288,6 +335,9 @@ def main():
     ap.add_argument("--serve", action="store_true",
                     help="only run the serving leg (minutes) and print its JSON line")
     ap.add_argument("--serve-requests", type=int, default=2000)
+    ap.add_argument("--no-phase-diagram", action="store_true", help="skip the phase-diagram leg")
+    ap.add_argument("--phase-diagram", action="store_true",
+                    help="only run the phase-diagram leg and print its JSON line")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain device copy of known size (counter calibration) and exit")
     a = ap.parse_args()
@@ -297,6 +347,9 @@ def main():
     if a.serve:
         print(json.dumps({"serve": run_serve(n=a.serve_requests)}), flush=True)
         return
+    if a.phase_diagram:
+        print(json.dumps({"phase_diagram": run_phase_diagram()}), flush=True)
+        return
     if a.gpus != 1:
         print("bench.py: this build measures one GPU; --gpus above 1 is not supported "
               "(independent instances would shard one scheduler per GPU)", file=sys.stderr)
@@ -304,6 +357,7 @@ def main():
     f64 = run_gpu(64, a.steps, a.warmup)
     f32 = None if a.no_fp32 else run_gpu(32, a.steps, a.warmup)
     sweep = None if a.no_sweep else run_sweep()
+    pd = None if a.no_phase_diagram else run_phase_diagram()
     cpu = None if a.no_cpu else run_cpu(a.cpu_seconds)
     line = {
         "metric": "Mpts*steps/s at 1024^2 (stdPFC ETDRK2 step, FP64; FP32 in 'fp32')",
@@ -330,6 +384,7 @@ def main():
             "rel_fe_vs_fp64": abs(f32["free_energy"] - f64["free_energy"]) / abs(f64["free_energy"])},
         "cpu_baseline": cpu,
         "sweep": sweep,
+        "phase_diagram": pd,
     }
     print(json.dumps(line), flush=True)
 

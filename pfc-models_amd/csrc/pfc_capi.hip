@@ -573,10 +573,18 @@ struct Steps {
     constexpr long long kResidentChunk = 4096;
     ColTables tb{e.E1.p, e.Q1.p, e.Q2.p, table_rows(e.p.Ny)};
     const size_t lds = resident_lds(e.p.Nx, e.p.Ny, sizeof(V));
+    static const int threads = [] {
+      const char* v = std::getenv("PFC_HIP_RESIDENT_THREADS");
+      return v && std::string(v) == "256" ? 256 : 512;
+    }();
     for (long long done = 0; done < nsteps; done += kResidentChunk) {
       const long long n = std::min(kResidentChunk, nsteps - done);
-      hipLaunchKernelGGL((k_resident_etd1<T>), dim3(ny), dim3(256), lds, st, S, e.res, nonlin_of(e), tb,
-                         (const V*)e.twN.p, (const V*)e.twY.p, sl, n);
+      if (threads == 512)
+        hipLaunchKernelGGL((k_resident_etd1<T, 512>), dim3(ny), dim3(512), lds, st, S, e.res, nonlin_of(e), tb,
+                           (const V*)e.twN.p, (const V*)e.twY.p, sl, n);
+      else
+        hipLaunchKernelGGL((k_resident_etd1<T, 256>), dim3(ny), dim3(256), lds, st, S, e.res, nonlin_of(e), tb,
+                           (const V*)e.twN.p, (const V*)e.twY.p, sl, n);
     }
   }
   static void step(Engine& e, long long nsteps, hipStream_t st) {

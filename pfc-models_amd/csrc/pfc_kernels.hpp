@@ -415,8 +415,8 @@ __host__ __device__ inline size_t resident_table_bytes(int Nc, int hy) {
   return ((size_t)2 * Nc * hy * sizeof(double) + 15) & ~(size_t)15;
 }
 
-template <typename T>
-__global__ void __launch_bounds__(256) k_resident_etd1(V2<T>* __restrict__ S, ResGeom g, Nonlin q, ColTables tb,
+template <typename T, int BS>
+__global__ void __launch_bounds__(BS) k_resident_etd1(V2<T>* __restrict__ S, ResGeom g, Nonlin q, ColTables tb,
                                                        const V2<T>* __restrict__ twN, const V2<T>* __restrict__ twY,
                                                        Slots sl, long long nsteps) {
   using V = V2<T>;
