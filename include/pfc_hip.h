@@ -100,6 +100,18 @@ int pfc_chemical_potential(pfc_plan* plan, void* mu, void* local_mu, int dtype, 
 /* Gaussian-smoothed field irfft2(G n_hat) into a real (Ny, Nx) device output.
  * Replaces Get_local_n / Get_local_phi (models/stdpfc/simulation.py:237-247). */
 int pfc_local_field(pfc_plan* plan, void* out, int dtype, void* stream);
+/* External potential V, a real (Ny, Nx) device array of `dtype` (NULL removes
+ * it): the update becomes n_hat' = E n_hat + P (Gamma lap N_hat + lap V_hat),
+ * the forcing of models/logpfc_extv/simulation.py:557-577 (no Gamma on the V
+ * term; with ETDRK2 the constant term enters the first stage only).
+ * V_hat = rfft2(V) is kept in FP64 and the source P lap V_hat is rebuilt with
+ * the ETD tables when dt, epsilon, M or the box change.  The free energy does
+ * not include the potential, as the reference's does not
+ * (models/logpfc_extv/model.md, "The reported free energy excludes the
+ * external potential").  A plan with a potential steps with the generic
+ * column kernel, or the resident stepper where that applies.  Synchronises
+ * `stream`. */
+int pfc_set_external_potential(pfc_plan* plan, const void* V, int dtype, void* stream);
 /* Spectral diagnostics of the current field, from the row spectrum on the
  * device (n_hat = rfft2(n), N = Nx Ny, sums over the full spectrum):
  * out[0] = mean = n_hat(0,0)/N, out[1] = sum |n_hat/N|^2 (= mean of n^2),
@@ -163,6 +175,8 @@ int pfc_batch_get_field(pfc_batch* batch, int slot, void* field, int dtype, void
  * phase-diagram driver reads every running evaluation's field per block
  * (phase_diagram/q0star.py:261-307 reads one with get_state). */
 int pfc_batch_get_fields(pfc_batch* batch, void* fields, int dtype, void* stream);
+/* pfc_set_external_potential for one slot of a batch. */
+int pfc_batch_set_external_potential(pfc_batch* batch, int slot, const void* V, int dtype, void* stream);
 /* pfc_diagnose for every active slot: out[n_active][4 + 2 nent], active order. */
 int pfc_batch_diagnose(pfc_batch* batch, const int* ky, const int* kx, int nent, double* out, void* stream);
 /* Slots stepped and reduced by the next calls, in this order (synchronises `stream`). */

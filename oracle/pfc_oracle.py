@@ -33,7 +33,10 @@ How each function is pinned (see tests/test_oracle_*.py):
   kernel_etdrk2 + ETDRK2 step -- linear-regime exactness and a measured
       second-order convergence ratio (~4) under dt halving;
   clamp / M-shift / pin -- invariants (mass, positivity) and the identity
-      that an M shift changes an ETD1 trajectory only at O(dt).
+      that an M shift changes an ETD1 trajectory only at O(dt);
+  external potential (logpfc_extv's lap V_hat forcing) -- the closed-form
+      solution of the linear problem with constant forcing, which ETD1 and
+      ETDRK2 reproduce exactly, and the stationary state -lap V_hat / L.
 """
 from __future__ import annotations
 
@@ -261,6 +264,13 @@ class Simulation:
         self.P2 = kernel_etdrk2(self.L, p.dt) if p.scheme == "etdrk2" else None
         self.G = kernel_local(geom, w=2 * np.pi)
         self.field = np.zeros((geom.Ny, geom.Nx), dtype=np.float64)
+        self.lapV_hat = None
+
+    def set_external_potential(self, V):
+        """External potential V (or None): the forcing gains lap V_hat, without
+        Gamma (logpfc_extv/simulation.py:557-577); constant, so in ETDRK2 it
+        cancels from the second stage."""
+        self.lapV_hat = None if V is None else self.lap * self.fft.rfft2(np.asarray(V, dtype=np.float64))
 
     # -- state -------------------------------------------------------------
     def set_field(self, f):
@@ -287,7 +297,7 @@ class Simulation:
             u_hat = self.fft.rfft2(u)
             F_u = self._forcing_hat(u)
             # ETD1 (stdpfc_cpu/simulation.py:365-397)
-            a_hat = self.E * u_hat + self.P * F_u
+            a_hat = self.E * u_hat + self.P * (F_u if self.lapV_hat is None else F_u + self.lapV_hat)
             if self.p.scheme == "etd1":
                 self.field = self.fft.irfft2(self._pin(a_hat), s)
             elif self.p.scheme == "etdrk2":

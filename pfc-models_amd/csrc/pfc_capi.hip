@@ -375,6 +375,8 @@ struct Engine {
   DevBuf S, Tb, D, E1, Q1, Q2;       // [cap][...]
   DevBuf twN, twY, twN64, twY64, kx2, ky2, dts;  // kx2 [cap][Nc], ky2 [cap][Ny], dts [cap]
   DevBuf part, out, pin, status, eps, Ms, active, diag, dgi, dgo;  // dgi/dgo: diagnostics entries / results
+  DevBuf vhat, extc;  // external potential: rfft2(V) and its source term, [cap][Ny][Nc] complex FP64
+  std::vector<char> h_ext;  // slots with a potential
   std::vector<double> h_eps, h_M, h_Lx, h_Ly, h_dt;
   std::vector<int> h_active;
   std::vector<char> field_set;
@@ -386,6 +388,10 @@ struct Engine {
   }
   bool f64() const { return p.precision == 64; }
   int nact() const { return (int)h_active.size(); }
+  bool has_ext() const { return extc.p != nullptr; }
+  ColTables tables() const {
+    return ColTables{E1.p, Q1.p, Q2.p, table_rows(p.Ny), (const V2<double>*)extc.p, (long long)Nc * p.Ny};
+  }
   int pstride() const { return row_blocks(rg64) + col_blocks(cg64); }
   Slots slots_active() const {
     return Slots{(const int*)active.p, (long long)nc, (long long)ntab, p.pin_mean ? (const double*)pin.p : nullptr,
@@ -518,6 +524,12 @@ void engine_tables(Engine& e, int slot0, int n, hipStream_t st) {
   hipLaunchKernelGGL(k_tables<double>, dim3(blocks, n), dim3(256), 0, st, (double*)e.E1.p, (double*)e.Q1.p,
                      (double*)e.Q2.p, (long long)e.ntab, sg, tp, (const double*)e.eps.p, (const double*)e.Ms.p,
                      slot0);
+  if (e.has_ext()) {
+    const long long cs = (long long)e.Nc * e.p.Ny;
+    const int cb = (int)std::min<long long>((cs + 255) / 256, 1024);
+    hipLaunchKernelGGL(k_ext_table, dim3(cb, n), dim3(256), 0, st, (V2<double>*)e.extc.p,
+                       (const V2<double>*)e.vhat.p, cs, sg, tp, (const double*)e.eps.p, (const double*)e.Ms.p, slot0);
+  }
   hip_check(hipGetLastError(), "table launch");
 }
 
@@ -552,7 +564,7 @@ struct Steps {
                          Tb, e.rg, q, (const V*)e.twN.p, sl);
   }
   static void col(Engine& e, int mode, hipStream_t st, const Slots& sl, int ny, V* S, const V* Tb, V* D) {
-    ColTables tb{e.E1.p, e.Q1.p, e.Q2.p, table_rows(e.p.Ny)};
+    ColTables tb = e.tables();
     if (e.fcol.on) {
       FastLaunch<T>::col(e.p.Ny, mode, e.fcol, ny, st, S, Tb, D, e.Nc, e.lay, tb, (const V2<double>*)e.twY64.p, sl);
       return;
@@ -571,7 +583,7 @@ struct Steps {
   // of at most kResidentChunk steps
   static void resident(Engine& e, long long nsteps, hipStream_t st, const Slots& sl, int ny, V* S) {
     constexpr long long kResidentChunk = 4096;
-    ColTables tb{e.E1.p, e.Q1.p, e.Q2.p, table_rows(e.p.Ny)};
+    ColTables tb = e.tables();
     const size_t lds = resident_lds(e.p.Nx, e.p.Ny, sizeof(V));
     static const int threads = [] {
       const char* v = std::getenv("PFC_HIP_RESIDENT_THREADS");
@@ -649,6 +661,7 @@ void autotune(Engine& e, hipStream_t st) {
   e.tuned = true;
   const char* env = std::getenv("PFC_HIP_TUNE");
   if (e.resident || (env && env[0] == '0')) return;
+  if (e.has_ext()) e.fcol.on = false;  // the fast column kernel has no potential term
   using V = V2<T>;
   const int ny = e.nact();
   if (ny == 0) return;
@@ -707,7 +720,7 @@ void autotune(Engine& e, hipStream_t st) {
       D2.alloc(bytes);
       hip_check(hipMemcpyAsync(S2.p, e.S.p, bytes, hipMemcpyDeviceToDevice, st), "copy");
       hip_check(hipMemcpyAsync(T2.p, e.S.p, bytes, hipMemcpyDeviceToDevice, st), "copy");
-      ColTables tb{e.E1.p, e.Q1.p, e.Q2.p, table_rows(e.p.Ny)};
+      ColTables tb = e.tables();
       const FastShape pick = pick_fastest(cands, [&](const FastShape& f) {
         FastLaunch<T>::col(Ny, mode, f, ny, st, (V*)S2.p, (const V*)T2.p, (V*)D2.p, Nc, e.lay, tb,
                            (const V2<double>*)e.twY64.p, sl);
@@ -966,6 +979,77 @@ void engine_set_slot_box(Engine& e, int slot, double Lx, double Ly, double dt, h
   hip_check(hipStreamSynchronize(st), "sync");
 }
 
+// cache of transform-only twiddles/geometry for pfc_rfft2 / pfc_irfft2
+struct FftEntry {
+  RowGeom rg{};
+  ColGeom cg{};
+  DevBuf twN, twY, scratch;
+};
+std::mutex g_fft_mu;
+std::map<std::tuple<int, int, int, int>, std::unique_ptr<FftEntry>> g_fft;
+
+FftEntry* fft_entry(int Ny, int Nx, int dtype) {
+  int dev = 0;
+  hip_check(hipGetDevice(&dev), "hipGetDevice");
+  auto key = std::make_tuple(dev, Ny, Nx, dtype);
+  auto it = g_fft.find(key);
+  if (it != g_fft.end()) return it->second.get();
+  auto e = std::make_unique<FftEntry>();
+  const size_t cs = dtype == 64 ? sizeof(V2<double>) : sizeof(V2<float>);
+  e->rg = row_geom(Nx, Ny, cs);
+  e->cg = col_geom(Nx, Ny, cs);
+  auto tn = twiddles(Nx), ty = twiddles(Ny);
+  if (dtype == 64) {
+    e->twN.upload(tn);
+    e->twY.upload(ty);
+  } else {
+    e->twN.upload(to_f32(tn));
+    e->twY.upload(to_f32(ty));
+  }
+  FftEntry* raw = e.get();
+  g_fft[key] = std::move(e);
+  return raw;
+}
+
+// External potential of one slot: V_hat = rfft2(V) (FP64) and its source
+// term; V = null removes it.  The first potential switches the engine to the
+// generic column kernel, which carries the term (the resident stepper does too).
+void engine_set_external(Engine& e, int slot, const void* V, int dtype, hipStream_t st) {
+  e.check_slot(slot);
+  if (V && dtype != 64 && dtype != 32) fail(PFC_EINVAL, "dtype must be 64 or 32");
+  const size_t per = (size_t)e.Nc * e.p.Ny;
+  if (!V && !e.has_ext()) return;
+  if (!e.has_ext()) {
+    hip_check(hipStreamSynchronize(st), "sync before allocation");
+    e.vhat.alloc((size_t)e.cap * per * sizeof(V2<double>));
+    e.extc.alloc((size_t)e.cap * per * sizeof(V2<double>));
+    hip_check(hipMemset(e.vhat.p, 0, e.vhat.n), "memset");
+    hip_check(hipMemset(e.extc.p, 0, e.extc.n), "memset");
+    e.h_ext.assign(e.cap, 0);
+    e.fcol.on = false;
+  }
+  V2<double>* vh = (V2<double>*)e.vhat.p + (size_t)slot * per;
+  if (!V) {
+    hip_check(hipMemsetAsync(vh, 0, per * sizeof(V2<double>), st), "memset");
+  } else {
+    std::lock_guard<std::mutex> lk(g_fft_mu);
+    FftEntry* fe = fft_entry(e.p.Ny, e.p.Nx, 64);
+    const size_t nreal = (size_t)e.p.Ny * e.p.Nx;
+    if (dtype == 64)
+      hipLaunchKernelGGL((k_row_r2c<double, double>), dim3(row_blocks(fe->rg)), dim3(256), row_lds(fe->rg, 16), st,
+                         (const double*)V, vh, fe->rg, (const V2<double>*)fe->twN.p, 1.0, nreal, per);
+    else
+      hipLaunchKernelGGL((k_row_r2c<double, float>), dim3(row_blocks(fe->rg)), dim3(256), row_lds(fe->rg, 16), st,
+                         (const float*)V, vh, fe->rg, (const V2<double>*)fe->twN.p, 1.0, nreal, per);
+    hipLaunchKernelGGL((k_col_fft<double, false>), dim3(col_blocks(fe->cg)), dim3(256), col_lds(fe->cg, 16), st, vh,
+                       fe->cg, (const V2<double>*)fe->twY.p, 1.0, per);
+    hip_check(hipGetLastError(), "potential transform launch");
+  }
+  e.h_ext[slot] = V ? 1 : 0;
+  engine_tables(e, slot, 1, st);
+  hip_check(hipStreamSynchronize(st), "sync");
+}
+
 void engine_set_active(Engine& e, const int* slots, int n, hipStream_t st) {
   if (n < 0 || n > e.cap) fail(PFC_EINVAL, "active count out of range");
   if (n > 0 && !slots) fail(PFC_EINVAL, "NULL slot list");
@@ -1054,42 +1138,12 @@ void launch_filter(Engine& e, int slot, int mode, void* out0, void* out1, int dt
 size_t engine_bytes(const Engine& e) {
   size_t b = 0;
   for (const DevBuf* d : {&e.S, &e.Tb, &e.D, &e.E1, &e.Q1, &e.Q2, &e.twN, &e.twY, &e.twN64, &e.twY64, &e.kx2, &e.ky2,
-                          &e.dts, &e.part, &e.out, &e.pin, &e.status, &e.eps, &e.Ms, &e.active, &e.diag, &e.dgi, &e.dgo})
+                          &e.dts, &e.part, &e.out, &e.pin, &e.status, &e.eps, &e.Ms, &e.active, &e.diag, &e.dgi, &e.dgo, &e.vhat,
+                          &e.extc})
     b += d->n;
   return b;
 }
 
-// cache of transform-only twiddles/geometry for pfc_rfft2 / pfc_irfft2
-struct FftEntry {
-  RowGeom rg{};
-  ColGeom cg{};
-  DevBuf twN, twY, scratch;
-};
-std::mutex g_fft_mu;
-std::map<std::tuple<int, int, int, int>, std::unique_ptr<FftEntry>> g_fft;
-
-FftEntry* fft_entry(int Ny, int Nx, int dtype) {
-  int dev = 0;
-  hip_check(hipGetDevice(&dev), "hipGetDevice");
-  auto key = std::make_tuple(dev, Ny, Nx, dtype);
-  auto it = g_fft.find(key);
-  if (it != g_fft.end()) return it->second.get();
-  auto e = std::make_unique<FftEntry>();
-  const size_t cs = dtype == 64 ? sizeof(V2<double>) : sizeof(V2<float>);
-  e->rg = row_geom(Nx, Ny, cs);
-  e->cg = col_geom(Nx, Ny, cs);
-  auto tn = twiddles(Nx), ty = twiddles(Ny);
-  if (dtype == 64) {
-    e->twN.upload(tn);
-    e->twY.upload(ty);
-  } else {
-    e->twN.upload(to_f32(tn));
-    e->twY.upload(to_f32(ty));
-  }
-  FftEntry* raw = e.get();
-  g_fft[key] = std::move(e);
-  return raw;
-}
 
 }  // namespace
 
@@ -1195,6 +1249,13 @@ int pfc_local_field(pfc_plan* pl, void* out, int dtype, void* stream) {
   });
 }
 
+int pfc_set_external_potential(pfc_plan* pl, const void* V, int dtype, void* stream) {
+  return guarded([&] {
+    if (!pl) fail(PFC_EINVAL, "NULL plan");
+    engine_set_external(pl->e, 0, V, dtype, reinterpret_cast<hipStream_t>(stream));
+  });
+}
+
 int pfc_diagnose(pfc_plan* pl, const int* ky, const int* kx, int nent, double* out, void* stream) {
   return guarded([&] {
     if (!pl) fail(PFC_EINVAL, "NULL plan");
@@ -1284,6 +1345,13 @@ int pfc_batch_get_field(pfc_batch* b, int slot, void* field, int dtype, void* st
   return guarded([&] {
     if (!b) fail(PFC_EINVAL, "NULL batch");
     engine_get_field(b->e, slot, field, dtype, reinterpret_cast<hipStream_t>(stream));
+  });
+}
+
+int pfc_batch_set_external_potential(pfc_batch* b, int slot, const void* V, int dtype, void* stream) {
+  return guarded([&] {
+    if (!b) fail(PFC_EINVAL, "NULL batch");
+    engine_set_external(b->e, slot, V, dtype, reinterpret_cast<hipStream_t>(stream));
   });
 }
 

@@ -211,6 +211,10 @@ struct ColTables {
   const void* Q1;
   const void* Q2;
   int hy;          // rows stored per column (table_rows(Ny))
+  // external-potential source of the first stage, P lap V_hat / (Nx Ny), FP64
+  // [Ny][Nc] per slot (cstride apart), or null: no potential
+  const V2<double>* C;
+  long long cstride;
 };
 
 constexpr int kColRegs = 16;  // A values held per thread between the two forward FFTs
@@ -296,6 +300,11 @@ __global__ void __launch_bounds__(256) k_col_etd(V2<T>* __restrict__ S, const V2
         const double e1 = E1[ti], q1 = Q1[ti];
         ox = e1 * double(a[i].x) + q1 * bx;
         oy = e1 * double(a[i].y) + q1 * by;
+        if (tb.C) {
+          const V2<double> c = tb.C[(size_t)slot * tb.cstride + (size_t)ky * g.Nc + (c0 + line)];
+          ox += c.x;
+          oy += c.y;
+        }
         if (MODE == COL_RK2_STAGE1) {
           const double q2 = Q2[ti];
           D[di] = V{T(ox - q2 * bx), T(oy - q2 * by)};
@@ -497,6 +506,11 @@ __global__ void __launch_bounds__(BS) k_resident_etd1(V2<T>* __restrict__ S, Res
       const double e1 = tE[ti], q1 = tQ[ti];
       double ox = e1 * double(a.x) + q1 * double(b.x);
       double oy = e1 * double(a.y) + q1 * double(b.y);
+      if (tb.C) {
+        const V2<double> c = tb.C[(size_t)slot * tb.cstride + e];
+        ox += c.x;
+        oy += c.y;
+      }
       if (pinned && line == 0 && ky == 0) ox = pin, oy = 0.0;
       W[e] = V{T(ox), T(oy)};
     }
@@ -813,6 +827,24 @@ __device__ __forceinline__ double etd2_phi_dt(double z, double dt) {
   return dt * ((expm1(z) - z) / (z * z));
 }
 
+// E = exp(L dt) and P = (E - 1)/L (dt (1 + z/2 + z^2/6) for |z| <= smallZTol,
+// z = L dt) of L = Gamma lap (beta (q0^2 + lap)^2 + eps + M): the reference's
+// kernel_etd (discretization/spectral/kernels.py:55-71) in FP64.
+struct EtdCoef {
+  double L, z, E, P;
+};
+__device__ __forceinline__ EtdCoef etd_coef(double lap, const TabParams& p) {
+  const double sq = p.q0 * p.q0 + lap;
+  double lmu = p.beta * (sq * sq) + p.epsilon;
+  if (p.M != 0.0) lmu = lmu + p.M;
+  EtdCoef c;
+  c.L = p.Gamma * lap * lmu;
+  c.z = c.L * p.dt;
+  c.E = exp(c.z);
+  c.P = fabs(c.z) > p.smallZTol ? (c.E - 1) / c.L : p.dt * (1 + c.z / 2 + c.z * c.z / 6);
+  return c;
+}
+
 // blockIdx.y = slot - slot0; per-slot epsilon, M, box and dt come from
 // eps[], Ms[] and the SlotGrid.
 template <typename T>
@@ -834,16 +866,37 @@ __global__ void __launch_bounds__(256) k_tables(T* __restrict__ E1, T* __restric
     const int c = (int)(i / p.hy);
     const int j = (int)(i - (size_t)c * p.hy);
     const double lap = -(kx2[c] + ky2[j]);
-    const double sq = p.q0 * p.q0 + lap;
-    double lmu = p.beta * (sq * sq) + p.epsilon;
-    if (p.M != 0.0) lmu = lmu + p.M;
-    const double L = p.Gamma * lap * lmu;
-    const double z = L * p.dt;
-    const double E = exp(z);
-    const double P = fabs(z) > p.smallZTol ? (E - 1) / L : p.dt * (1 + z / 2 + z * z / 6);
-    E1[i] = T(E * p.inv_ny);
-    Q1[i] = T(P * (p.Gamma * lap) * p.inv_n);
-    if (Q2) Q2[i] = T(etd2_phi_dt(z, p.dt) * (p.Gamma * lap) * p.inv_n);
+    const EtdCoef k = etd_coef(lap, p);
+    E1[i] = T(k.E * p.inv_ny);
+    Q1[i] = T(k.P * (p.Gamma * lap) * p.inv_n);
+    if (Q2) Q2[i] = T(etd2_phi_dt(k.z, p.dt) * (p.Gamma * lap) * p.inv_n);
+  }
+}
+
+// External-potential source of slots [slot0, slot0 + gridDim.y):
+// C = P lap V_hat / (Nx Ny) over the full [Ny][Nc] half spectrum (V_hat = rfft2
+// of the slot's potential; logpfc_extv/simulation.py:557-577 adds
+// P lap V_hat to the update, without Gamma).
+static __global__ void __launch_bounds__(256) k_ext_table(V2<double>* __restrict__ C,
+                                                          const V2<double>* __restrict__ Vhat, long long cstride,
+                                                          SlotGrid sg, TabParams p, const double* __restrict__ eps,
+                                                          const double* __restrict__ Ms, int slot0) {
+  const int slot = slot0 + blockIdx.y;
+  p.epsilon = eps[slot];
+  p.M = Ms[slot];
+  p.dt = sg.dt[slot];
+  const double* __restrict__ kx2 = sg.kx2 + (size_t)slot * p.Nc;
+  const double* __restrict__ ky2 = sg.ky2 + (size_t)slot * p.Ny;
+  C += (size_t)slot * cstride;
+  Vhat += (size_t)slot * cstride;
+  const int n = p.Nc * p.Ny;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const int ky = i / p.Nc;
+    const int kx = i - ky * p.Nc;
+    const double lap = -(kx2[kx] + ky2[ky]);
+    const double w = etd_coef(lap, p).P * lap * p.inv_n;
+    const V2<double> v = Vhat[i];
+    C[i] = V2<double>{w * v.x, w * v.y};
   }
 }
 

@@ -70,6 +70,8 @@ _SIGNATURES = [
     ("pfc_batch_set_slot_box", ctypes.c_int, [_P, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                               _P]),
     ("pfc_batch_get_fields", ctypes.c_int, [_P, _P, ctypes.c_int, _P]),
+    ("pfc_batch_set_external_potential", ctypes.c_int, [_P, ctypes.c_int, _P, ctypes.c_int, _P]),
+    ("pfc_set_external_potential", ctypes.c_int, [_P, _P, ctypes.c_int, _P]),
     ("pfc_batch_diagnose", ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
                                           ctypes.c_int, ctypes.POINTER(ctypes.c_double), _P]),
     ("pfc_diagnose", ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int), ctypes.c_int,

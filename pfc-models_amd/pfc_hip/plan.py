@@ -123,6 +123,18 @@ class Plan:
         _lib.check(self._lib.pfc_step(self._h, int(nsteps), current_stream_handle()))
         return self
 
+    def set_external_potential(self, V):
+        """External potential V ((Ny, Nx) array; None removes it): the update
+        gains P lap V_hat (pfc_set_external_potential)."""
+        if V is None:
+            _lib.check(self._lib.pfc_set_external_potential(self._h, None, 64, current_stream_handle()))
+            return self
+        t = self._device_field(V)
+        dt = 64 if t.dtype == self.torch.float64 else 32
+        _lib.check(self._lib.pfc_set_external_potential(self._h, ctypes.c_void_p(t.data_ptr()), dt,
+                                                        current_stream_handle()))
+        return self
+
     def diagnose(self, ky, kx):
         """Spectral diagnostics of the field (pfc_diagnose): (mean, m0, m1, m2,
         n_hat at the requested (ky, kx) as a complex array)."""
@@ -252,6 +264,18 @@ class Batch:
         _lib.check(self._lib.pfc_batch_get_field(self._h, int(slot), ctypes.c_void_p(out.data_ptr()), dt,
                                                  current_stream_handle()))
         return out
+
+    def set_external_potential(self, slot, V):
+        """The slot's external potential ((Ny, Nx) array; None removes it)."""
+        if V is None:
+            _lib.check(self._lib.pfc_batch_set_external_potential(self._h, int(slot), None, 64,
+                                                                  current_stream_handle()))
+            return self
+        t = _field_tensor(self.torch, V, self.Ny, self.Nx)
+        dt = 64 if t.dtype == self.torch.float64 else 32
+        _lib.check(self._lib.pfc_batch_set_external_potential(self._h, int(slot), ctypes.c_void_p(t.data_ptr()), dt,
+                                                              current_stream_handle()))
+        return self
 
     def get_fields(self, dtype=None):
         """Fields of the active slots in active order, one device array (n_active, Ny, Nx)."""

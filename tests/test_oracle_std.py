@@ -206,3 +206,35 @@ def test_exact_mean_and_peaks():
     geom = O.build_geometry(16, 12, 16.0, 12.0)
     f = 0.3 * np.cos(2 * np.pi * 3 * geom.X / 16.0) + 0.2 * np.cos(2 * np.pi * 2 * geom.Y / 12.0)
     assert O.structure_factor_peaks(f, k=3) == sorted([(0, 3), (2, 0), (10, 0)])
+
+
+@pytest.mark.parametrize("scheme", ["etd1", "etdrk2"])
+def test_external_potential_forcing_is_exact_in_the_linear_regime(scheme):
+    # d_t n_hat = L n_hat + lap V_hat with N negligible (g = 0, |n| ~ 1e-9):
+    # n_hat(t) = e^{Lt} n_hat0 + (e^{Lt} - 1)/L lap V_hat, which both schemes
+    # reproduce exactly for constant forcing; the k = 0 mode is untouched
+    geom = O.build_geometry(32, 24, 32 * np.pi / 4, 24 * np.pi / 4)
+    # Gamma != 1, so a Gamma wrongly applied to the potential term would show
+    p = O.Params(model="std", Gamma=0.7, beta=1.0, q0=1.0, epsilon=0.2, g=0.0, dt=0.3, scheme=scheme)
+    sim = O.Simulation(p, geom)
+    rng = np.random.default_rng(8)
+    n0 = 1e-9 * rng.standard_normal((24, 32))
+    V = 1e-9 * (np.sin(2 * np.pi * 2 * geom.X / geom.Lx) + 0.5 * np.cos(2 * np.pi * 3 * geom.Y / geom.Ly)
+                + 0.2 * rng.standard_normal((24, 32)))
+    sim.set_field(n0)
+    sim.set_external_potential(V)
+    K = 40
+    sim.step(K)
+    lap = O.laplacian(geom)
+    L = p.Gamma * lap * O.linear_kernel_mu(p.beta, p.q0, p.epsilon, lap)
+    t = K * p.dt
+    safeL = np.where(L == 0, 1.0, L)
+    gain = np.where(L == 0, 0.0, np.expm1(L * t) / safeL)
+    exact = np.exp(L * t) * np.fft.rfft2(n0) + gain * lap * np.fft.rfft2(V)
+    got = np.fft.rfft2(sim.field)
+    assert np.abs(got - exact).max() <= 1e-12 * np.abs(exact).max()
+    assert sim.mean() == pytest.approx(n0.mean(), abs=1e-24)
+    # long runs settle on the stationary state -lap V_hat / L
+    sim.step(4000)
+    stat = np.where(L == 0, np.fft.rfft2(n0), -lap * np.fft.rfft2(V) / safeL)
+    assert np.abs(np.fft.rfft2(sim.field) - stat).max() <= 1e-10 * np.abs(stat).max()
