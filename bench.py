@@ -307,7 +307,8 @@ def run_phase_diagram(n_dens=12, serial_every=1):
     for k in range(len(sub)):
         j = k * serial_every
         same &= str(rb["status"][j]) == str(rs["status"][k])
-        same &= bool(np.isnan(rs["q0star"][k]) or (rb["q0star"][j] == rs["q0star"][k] and rb["fmin"][j] == rs["fmin"][k]))
+        same &= bool(np.isnan(rs["q0star"][k])
+                     or (rb["q0star"][j] == rs["q0star"][k] and rb["fmin"][j] == rs["fmin"][k]))
     n_eval = int(rb["qscan"]["q00"].size)
     return {
         "metric": "q0* density scans per second (logPFC triangle, 28 x 24 cell, eps -0.9, 3 x 21 q00 relaxations)",
