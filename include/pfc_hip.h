@@ -182,6 +182,13 @@ int pfc_batch_diagnose(pfc_batch* batch, const int* ky, const int* kx, int nent,
 /* Slots stepped and reduced by the next calls, in this order (synchronises `stream`). */
 int pfc_batch_set_active(pfc_batch* batch, const int* slots, int n, void* stream);
 int pfc_batch_step(pfc_batch* batch, long long nsteps, void* stream);
+/* counts[i] ETD1 steps for the i-th active slot (active order), in one
+ * launch per 4096 steps of the largest count: the slots of a batch advance by
+ * different numbers of steps, as the phase-diagram driver's adaptive time
+ * step (q0star.py:309-326) gives each evaluation its own dt per sub-block.
+ * Resident-stepper engines only (small ETD1 grids; PFC_EUNSUPPORTED
+ * otherwise).  Synchronises `stream`. */
+int pfc_batch_step_counts(pfc_batch* batch, const long long* counts, void* stream);
 /* FP64 reductions of the active slots: out[4 i + {0,1,2,3}] = free energy per
  * site, mean, min, max of the i-th active slot.  Synchronises `stream`. */
 int pfc_batch_reduce(pfc_batch* batch, double* out, void* stream);

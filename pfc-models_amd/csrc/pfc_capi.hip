@@ -375,6 +375,7 @@ struct Engine {
   DevBuf S, Tb, D, E1, Q1, Q2;       // [cap][...]
   DevBuf twN, twY, twN64, twY64, kx2, ky2, dts;  // kx2 [cap][Nc], ky2 [cap][Ny], dts [cap]
   DevBuf part, out, pin, status, eps, Ms, active, diag, dgi, dgo;  // dgi/dgo: diagnostics entries / results
+  DevBuf cnt;         // per-slot step counts (pfc_batch_step_counts)
   DevBuf vhat, extc;  // external potential: rfft2(V) and its source term, [cap][Ny][Nc] complex FP64
   std::vector<char> h_ext;  // slots with a potential
   std::vector<double> h_eps, h_M, h_Lx, h_Ly, h_dt;
@@ -581,7 +582,9 @@ struct Steps {
   }
   // nsteps ETD1 steps of every active slot, one workgroup each, in launches
   // of at most kResidentChunk steps
-  static void resident(Engine& e, long long nsteps, hipStream_t st, const Slots& sl, int ny, V* S) {
+  // counts (device, per active slot) or null: every slot nsteps
+  static void resident(Engine& e, long long nsteps, hipStream_t st, const Slots& sl, int ny, V* S,
+                       const long long* counts = nullptr) {
     constexpr long long kResidentChunk = 4096;
     ColTables tb = e.tables();
     const size_t lds = resident_lds(e.p.Nx, e.p.Ny, sizeof(V));
@@ -593,10 +596,10 @@ struct Steps {
       const long long n = std::min(kResidentChunk, nsteps - done);
       if (threads == 512)
         hipLaunchKernelGGL((k_resident_etd1<T, 512>), dim3(ny), dim3(512), lds, st, S, e.res, nonlin_of(e), tb,
-                           (const V*)e.twN.p, (const V*)e.twY.p, sl, n);
+                           (const V*)e.twN.p, (const V*)e.twY.p, sl, n, counts, done);
       else
         hipLaunchKernelGGL((k_resident_etd1<T, 256>), dim3(ny), dim3(256), lds, st, S, e.res, nonlin_of(e), tb,
-                           (const V*)e.twN.p, (const V*)e.twY.p, sl, n);
+                           (const V*)e.twN.p, (const V*)e.twY.p, sl, n, counts, done);
     }
   }
   static void step(Engine& e, long long nsteps, hipStream_t st) {
@@ -746,6 +749,33 @@ void engine_step(Engine& e, long long nsteps, hipStream_t st) {
     Steps<double>::step(e, nsteps, st);
   else
     Steps<float>::step(e, nsteps, st);
+}
+
+// counts[i] steps for the i-th active slot (resident engines only)
+void engine_step_counts(Engine& e, const long long* counts, hipStream_t st) {
+  const int n = e.nact();
+  if (n == 0) return;
+  if (!counts) fail(PFC_EINVAL, "NULL counts");
+  if (!e.resident) fail(PFC_EUNSUPPORTED, "per-slot step counts need the resident stepper (small ETD1 grids)");
+  for (int s : e.h_active)
+    if (!e.field_set[s]) fail(PFC_EINVAL, "a stepped slot has no field (pfc_set_field / pfc_batch_set_field)");
+  long long most = 0;
+  for (int i = 0; i < n; ++i) {
+    if (counts[i] < 0) fail(PFC_EINVAL, "negative step count");
+    most = std::max(most, counts[i]);
+  }
+  if (e.cnt.n < (size_t)n * sizeof(long long)) {
+    hip_check(hipStreamSynchronize(st), "sync before scratch growth");
+    e.cnt.alloc((size_t)e.cap * sizeof(long long));
+  }
+  hip_check(hipMemcpyAsync(e.cnt.p, counts, (size_t)n * sizeof(long long), hipMemcpyHostToDevice, st), "copy");
+  const Slots sl = e.slots_active();
+  if (e.f64())
+    Steps<double>::resident(e, most, st, sl, n, (V2<double>*)e.S.p, (const long long*)e.cnt.p);
+  else
+    Steps<float>::resident(e, most, st, sl, n, (V2<float>*)e.S.p, (const long long*)e.cnt.p);
+  hip_check(hipGetLastError(), "step launch");
+  hip_check(hipStreamSynchronize(st), "sync");  // the counts must outlive the copy
 }
 
 // Average launch time of each kernel kind of the engine's current schedule:
@@ -1139,7 +1169,7 @@ size_t engine_bytes(const Engine& e) {
   size_t b = 0;
   for (const DevBuf* d : {&e.S, &e.Tb, &e.D, &e.E1, &e.Q1, &e.Q2, &e.twN, &e.twY, &e.twN64, &e.twY64, &e.kx2, &e.ky2,
                           &e.dts, &e.part, &e.out, &e.pin, &e.status, &e.eps, &e.Ms, &e.active, &e.diag, &e.dgi, &e.dgo, &e.vhat,
-                          &e.extc})
+                          &e.extc, &e.cnt})
     b += d->n;
   return b;
 }
@@ -1359,6 +1389,13 @@ int pfc_batch_get_fields(pfc_batch* b, void* fields, int dtype, void* stream) {
   return guarded([&] {
     if (!b) fail(PFC_EINVAL, "NULL batch");
     engine_get_fields(b->e, fields, dtype, reinterpret_cast<hipStream_t>(stream));
+  });
+}
+
+int pfc_batch_step_counts(pfc_batch* b, const long long* counts, void* stream) {
+  return guarded([&] {
+    if (!b) fail(PFC_EINVAL, "NULL batch");
+    engine_step_counts(b->e, counts, reinterpret_cast<hipStream_t>(stream));
   });
 }
 

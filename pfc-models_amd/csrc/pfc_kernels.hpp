@@ -424,10 +424,16 @@ __host__ __device__ inline size_t resident_table_bytes(int Nc, int hy) {
   return ((size_t)2 * Nc * hy * sizeof(double) + 15) & ~(size_t)15;
 }
 
+// `nsteps` steps per workgroup, or, with `counts` (per active slot, in
+// active order), min(counts[i] - done, nsteps) steps for the i-th: each
+// instance can take its own number of steps in one launch.
 template <typename T, int BS>
 __global__ void __launch_bounds__(BS) k_resident_etd1(V2<T>* __restrict__ S, ResGeom g, Nonlin q, ColTables tb,
-                                                       const V2<T>* __restrict__ twN, const V2<T>* __restrict__ twY,
-                                                       Slots sl, long long nsteps) {
+                                                      const V2<T>* __restrict__ twN, const V2<T>* __restrict__ twY,
+                                                      Slots sl, long long nsteps, const long long* __restrict__ counts,
+                                                      long long done) {
+  if (counts) nsteps = max(0LL, min(nsteps, counts[blockIdx.x] - done));
+  if (nsteps == 0) return;
   using V = V2<T>;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int M = g.M, Nc = g.Nc, Ny = g.Ny, nsp = Ny * Nc, nt = Nc * g.hy;

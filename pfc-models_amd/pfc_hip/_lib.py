@@ -70,6 +70,7 @@ _SIGNATURES = [
     ("pfc_batch_set_slot_box", ctypes.c_int, [_P, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                               _P]),
     ("pfc_batch_get_fields", ctypes.c_int, [_P, _P, ctypes.c_int, _P]),
+    ("pfc_batch_step_counts", ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_longlong), _P]),
     ("pfc_batch_set_external_potential", ctypes.c_int, [_P, ctypes.c_int, _P, ctypes.c_int, _P]),
     ("pfc_set_external_potential", ctypes.c_int, [_P, _P, ctypes.c_int, _P]),
     ("pfc_batch_diagnose", ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
@@ -112,7 +113,15 @@ def load(path: str | None = None):
     return lib
 
 
+class PfcError(RuntimeError):
+    """A non-zero status of a library call; `code` is the PFC_E* value."""
+    INVALID, UNSUPPORTED, HIP, NOMEM = -1, -2, -3, -4
+
+    def __init__(self, code, message):
+        super().__init__(f"libpfc_hip: {ERRORS.get(code, code)}: {message}")
+        self.code = code
+
+
 def check(rc: int):
     if rc != 0:
-        msg = load().pfc_last_error().decode(errors="replace")
-        raise RuntimeError(f"libpfc_hip: {ERRORS.get(rc, rc)}: {msg}")
+        raise PfcError(rc, load().pfc_last_error().decode(errors="replace"))

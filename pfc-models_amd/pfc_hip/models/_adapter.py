@@ -239,9 +239,10 @@ class SlotBatch:
         self.batch = Batch(capacity, **plan_kwargs(kind, model_params, box(cell, 1.0), precision=precision))
         self._active = None
 
-    def place(self, slot, temp, meanDensity, q00, dt, state):
+    def place(self, slot, temp, meanDensity, q00, dt, state, t=0.0):
         """Slot `slot` at temperature `temp` in the box of q00 with time step dt
-        and field `state` (which carries the mean density)."""
+        and field `state` (which carries the mean density); the models are
+        autonomous, so the time `t` changes nothing here."""
         g = box(self.cell, q00)
         self.batch.set_slot(slot, temp)
         self.batch.set_slot_box(slot, g.Lx, g.Ly, dt)
@@ -261,6 +262,18 @@ class SlotBatch:
 
     def step(self, n):
         self.batch.step(n)
+
+    def step_each(self, counts):
+        """counts[i] steps for the i-th active slot; False if this batch cannot
+        (grids beyond the resident stepper), and nothing is stepped then."""
+        from .._lib import PfcError
+        try:
+            self.batch.step_counts(counts)
+        except PfcError as e:
+            if e.code == PfcError.UNSUPPORTED:
+                return False
+            raise
+        return True
 
     def free_energy(self):
         """FP64 free energy per site of each active slot, in active order."""

@@ -16,8 +16,13 @@ block.  The per-point semantics are the serial driver's
 * the same stop tests after every block, with the same diagnostics on the
   same fields; a stopped slot leaves the active list.
 
-Adaptive dt (relax.dtMax) changes each point's steps per block, so with it
-the points are evaluated one at a time.
+With adaptive dt (relax.dtMax) each block is `dtChecks` sub-blocks and
+every point keeps its own dt: the sub-blocks run in lockstep, each slot
+taking its own number of steps in the same launch (pfc_batch_step_counts),
+and a slot whose dt changes is rebuilt from its field as _adaptDt rebuilds
+the Simulation (q0star.py:309-326).  Batches whose grids are too large for
+the resident stepper cannot step slots by different counts; their points
+are then evaluated one at a time.
 
 `find_q0star_batched` runs one density's q00 scan with a whole pass per
 batch; `scan_density_batched` runs the passes of a whole density list
@@ -25,8 +30,10 @@ together (pass p of every density still scanning in one batch), with
 scan_density's results, statuses, records and checkpoint files.
 
 The adapter provides, besides the serial names: `open_batch(params, cell,
-capacity)`, an object with `place(slot, temp, meanDensity, q00, dt, state)`,
-`set_state(slot, state)`, `set_active(slots)`, `step(n)`, and, for the
+capacity)`, an object with `place(slot, temp, meanDensity, q00, dt, state,
+t=0.0)` (t: the simulation time the slot's model sees, if any),
+`set_state(slot, state)`, `set_active(slots)`, `step(n)`, `step_each(counts)`
+(False if the slots cannot take different step counts), and, for the
 active slots in active order, `states()`, `free_energy()` and
 `diagnose_active(structure, cell)` (the adapter's `diagnose` of each), and
 `close()`; and `ansatz_state(structure, cell, ansatz, meanDensity)`, the
@@ -45,8 +52,9 @@ from . import q0star as Q
 
 class _Point:
     """One evaluation's running values: time, f, A and diagnostics after the
-    last block, time steps, stop reason (None while running) and field."""
-    __slots__ = ("temp", "rho", "q00", "t", "f", "A", "diag", "steps", "stop", "state")
+    last block, time steps, stop reason (None while running), field, and
+    with adaptive dt its dt, largest dt and sub-block A."""
+    __slots__ = ("temp", "rho", "q00", "t", "f", "A", "diag", "steps", "stop", "state", "dt", "dt_max", "A_s")
 
     def __init__(self, temp, rho, q00):
         self.temp, self.rho, self.q00 = float(temp), float(rho), float(q00)
@@ -61,8 +69,8 @@ def evaluate_batched(minimizer, points, states=None, precision=64):
     points = [tuple(p) for p in points]
     if states is None:
         states = [None] * len(points)
-    if r.dtMax is not None or not points:
-        return [m.evaluate(t, rho, q, state=s) for (t, rho, q), s in zip(points, states)]
+    if not points:
+        return []
     dt0 = r.stepT / int(np.ceil(r.stepT / r.dt))
     n = int(round(r.stepT / dt0))
     blocks = int(r.totalT / r.stepT)
@@ -82,6 +90,12 @@ def evaluate_batched(minimizer, points, states=None, precision=64):
                     sb.set_state(j, Q.noisy_state(x, r.seed, r.noise))
         for p, x, f, d in zip(pts, sb.states(), sb.free_energy(), sb.diagnose_active(m.structure, m.cell)):
             p.state, p.f, p.diag, p.A = x, float(f), d, d.amplitudes[0]
+            p.dt = p.dt_max = dt0
+        if r.dtMax is not None:
+            if not _adaptive_blocks(m, ad, r, sb, pts, blocks):
+                return [m.evaluate(t, rho, q, state=s) for (t, rho, q), s in zip(points, states)]
+            return [Q.Evaluation(p.f, float(p.diag.fitness), tuple(float(a) for a in p.diag.amplitudes),
+                                 p.stop or Q.STOP_TOTALT, p.steps, p.dt_max, p.state) for p in pts]
         for _ in range(blocks):
             act = [j for j, p in enumerate(pts) if p.stop is None]
             if not act:
@@ -92,7 +106,7 @@ def evaluate_batched(minimizer, points, states=None, precision=64):
             fields = sb.states()
             df = {}
             for k, j in enumerate(act):
-                df[j] = _block(m, ad, r, sb, pts[j], j, float(fe[k]), fields[k], n, dt0)
+                df[j] = _block(m, ad, r, sb, pts[j], j, float(fe[k]), fields[k], n, dt0, r.stepT)
             for j, d in zip(act, sb.diagnose_active(m.structure, m.cell)):
                 p = pts[j]
                 p.diag = d
@@ -104,8 +118,8 @@ def evaluate_batched(minimizer, points, states=None, precision=64):
         sb.close()
 
 
-def _block(m, ad, r, sb, p, j, f, field, n, dt):
-    """Slot j after a block of n steps ending at free energy f and `field`:
+def _block(m, ad, r, sb, p, j, f, field, n, dt, stepT):
+    """Slot j after a block (of stepT) of n steps ending at free energy f and `field`:
     the serial driver's time and step bookkeeping, and its rollback of a
     rising or NaN block (on a Simulation of its own, from the block's
     starting state; the outcome goes back into the slot).  Returns the
@@ -122,7 +136,7 @@ def _block(m, ad, r, sb, p, j, f, field, n, dt):
     ad.set_time(sim, t_before)
     steps0 = m.last_steps
     try:
-        df = m._rollback(sim, p.temp, p.rho, p.q00, r.stepT, before, t_before, f_before, bool(np.isnan(f)),
+        df = m._rollback(sim, p.temp, p.rho, p.q00, stepT, before, t_before, f_before, bool(np.isnan(f)),
                          divs=r.divs)
     except Q.MaxDepthReached as e:
         p.stop = Q.STOP_NAN if e.nan else Q.STOP_MAX_DEPTH
@@ -131,6 +145,67 @@ def _block(m, ad, r, sb, p, j, f, field, n, dt):
     p.state, p.f, p.t = ad.get_state(sim), float(ad.free_energy(sim)), ad.get_time(sim)
     sb.set_state(j, p.state)
     return df
+
+
+def _adaptive_blocks(m, ad, r, sb, pts, blocks):
+    """Get_f_ABC's adaptive-dt blocks for every point (q0star.py:347-404):
+    each block is dtChecks sub-blocks of stepT/dtChecks, each slot at its own
+    dt, dt adapted after every sub-block but the last and between blocks.
+    Returns False, having stepped nothing, if the batch cannot step its slots
+    by different counts."""
+    sub_T = r.stepT / r.dtChecks
+    for _ in range(blocks):
+        act = [j for j, p in enumerate(pts) if p.stop is None]
+        if not act:
+            break
+        f0 = {j: pts[j].f for j in act}
+        for j in act:
+            pts[j].A_s = pts[j].A
+        for k in range(r.dtChecks):
+            live = [j for j in act if pts[j].stop is None]
+            if not live:
+                break
+            sb.set_active(live)
+            counts = [int(round(sub_T / pts[j].dt)) for j in live]
+            if not sb.step_each(counts):
+                return False
+            fe, fields = sb.free_energy(), sb.states()
+            d_sub = {j: _block(m, ad, r, sb, pts[j], j, float(fe[i]), fields[i], counts[i], pts[j].dt, sub_T)
+                     for i, j in enumerate(live)}
+            for j, d in zip(live, sb.diagnose_active(m.structure, m.cell)):
+                p = pts[j]
+                p.diag = d
+                if p.stop is not None:
+                    continue
+                A_new = d.amplitudes[0]
+                if k < r.dtChecks - 1:
+                    _adapt(m, r, sb, p, j, sub_T, d_sub[j], p.f, A_new, p.A_s)
+                p.A_s = A_new
+        for j in act:
+            p = pts[j]
+            if p.stop is not None:
+                continue
+            df = p.f - f0[j]
+            A = p.diag.amplitudes[0]
+            A_last = p.A
+            _stop_tests(r, p, df, p.diag)
+            if p.stop is None:
+                _adapt(m, r, sb, p, j, sub_T, df / r.dtChecks, p.f, A, A_last)
+    return True
+
+
+def _adapt(m, r, sb, p, j, sub_T, delta_f, f, A, A_last):
+    """_adaptDt for slot j: the next dt from the relative rate; on a change
+    the slot is rebuilt at the new dt from its field, its time kept."""
+    rate = max(abs(delta_f) / max(abs(f), 1e-300), abs(A - A_last) / max(A, 1e-300)) / sub_T
+    target = r.dtMax / np.sqrt(1.0 + (rate / r.dtRate) ** 2)
+    target = min(max(target, r.dt), r.dtMax, p.dt * r.dtGrow)
+    new = sub_T / int(np.ceil(sub_T / target - 1e-9))
+    if abs(new - p.dt) <= 1e-12 * p.dt:
+        return
+    sb.place(j, p.temp, p.rho, p.q00, new, p.state, t=p.t)
+    p.dt = new
+    p.dt_max = max(p.dt_max, new)
 
 
 def _stop_tests(r, p, df, diag):

@@ -227,3 +227,112 @@ def scan_density(minimizer, temp, densities, q_start=1.0, record=None, done=None
         for i, d in edge_walks(grid, pass1):
             edge += edge_walk(solve, grid, pass1, i, d, minimizer.settings.maxPast)
     return merge_edge(pass1, edge), pass1, edge
+
+
+# ---------------------------------------------------------------------------
+# many densities at once
+# ---------------------------------------------------------------------------
+def solve_batched(minimizer, temp, densities, q_start=1.0, precision=64):
+    """MinimizeFrozenQ.solve at every density, in lockstep: iteration `it`
+    relaxes every density still iterating as one batch (evaluate_batched,
+    fields carried, each at its own q and dt) at the stage every density is
+    in at that iteration, then jumps each to the q_opt of its relaxed field.
+    The records equal solve()'s (wall_s is the batch's elapsed time).  If a
+    batch raises, the densities still iterating are solved one at a time."""
+    from . import batched as _b
+    t0 = time.perf_counter()
+    st = minimizer.settings
+    last_stage = len(st.schedule) - 1
+    ad = minimizer.adapter
+    rhos = [float(p) for p in densities]
+    n = len(rhos)
+    q = [float(q_start)] * n
+    carry = [None] * n
+    steps = [0] * n
+    last = [None] * n
+    field = [None] * n
+    done = [False] * n
+    stage = 0
+    fallback = []
+    for it in range(st.maxIter):
+        run = [i for i in range(n) if not done[i]]
+        if not run:
+            break
+        m = minimizer.stage_minimizer(stage)
+        try:
+            evs = _b.evaluate_batched(m, [(temp, rhos[i], q[i]) for i in run], states=[carry[i] for i in run],
+                                      precision=precision)
+        except Exception:  # noqa: BLE001 - those densities are solved one at a time below
+            fallback = run
+            break
+        for i, ev in zip(run, evs):
+            steps[i] += ev.steps
+            field[i] = ev.state
+            frozen = ad.frozen_field(ev.state, minimizer.params, temp, minimizer.cell)
+            q_new = float(frozen.q_opt())
+            # 'stage': the stage solve() holds after this iteration (it
+            # advances before the next relaxation unless the loop stops here)
+            last[i] = {'q': q[i], 'q_opt': q_new, 'f': float(ev.f), 'f_frozen': float(frozen.f(q_new)),
+                       'fitness': float(ev.fitness), 'amps': [float(a) for a in ev.amplitudes], 'stop': ev.stop,
+                       'it': it, 'stage': stage}
+            if ev.stop not in (_q.STOP_CONVERGED, _q.STOP_TOTALT):
+                done[i] = True
+                continue
+            dq = q_new - q[i]
+            carry[i], q[i] = ev.state, q_new
+            if stage < last_stage:
+                last[i]['stage'] = stage + 1
+            elif abs(dq) < st.qTol and ev.stop == _q.STOP_CONVERGED:
+                done[i] = True
+        if stage < last_stage:
+            stage += 1
+    out = []
+    wall = time.perf_counter() - t0
+    for i in range(n):
+        if i in fallback:
+            out.append(minimizer.solve(temp, rhos[i], q_start))
+            continue
+        L = last[i]
+        rec = {'method': METHOD, 'q_start': float(q_start), 'message': ''}
+        settled = L['stop'] == _q.STOP_CONVERGED and L['stage'] == last_stage and abs(L['q_opt'] - L['q']) < st.qTol
+        if settled:
+            status = STATUS_OK
+        elif L['stop'] in (_q.STOP_CONVERGED, _q.STOP_TOTALT):
+            status = STATUS_NOT_CONVERGED
+        else:
+            status = L['stop']
+        rec.update(status=status, q=L['q_opt'], q_relaxed=L['q'], f=L['f'], f_frozen=L['f_frozen'],
+                   fitness=L['fitness'], stop=L['stop'], steps=steps[i], iters=L['it'] + 1)
+        for j, a in enumerate(L['amps']):
+            rec[f'A{j + 1}'] = a
+        defects = getattr(ad, 'defect_diagnostics', None)
+        if defects is not None and field[i] is not None:
+            rec.update(defects(field[i], minimizer.cell))
+        rec['wall_s'] = wall
+        out.append(rec)
+    return out
+
+
+def scan_density_batched(minimizer, temp, densities, q_start=1.0, record=None, precision=64):
+    """scan_density with the independent solves batched (solve_batched); the
+    edge walks, few and sequential by nature, run one solve at a time.
+    Returns (final, pass1, edge) as scan_density does."""
+    grid = [float(p) for p in densities]
+    if any(b <= a for a, b in zip(grid, grid[1:])):
+        raise ValueError('densities must be strictly increasing')
+    pass1 = {}
+    for p, r in zip(grid, solve_batched(minimizer, temp, grid, q_start, precision=precision)):
+        r.update(phi=p, pass_='indep')
+        pass1[p] = r
+        if record is not None:
+            record(r)
+    edge = []
+    if minimizer.settings.edge:
+        def solve(p, q):
+            r = minimizer.solve(temp, p, q)
+            if record is not None:
+                record({**r, 'phi': p, 'pass_': 'edge'})
+            return r
+        for i, d in edge_walks(grid, pass1):
+            edge += edge_walk(solve, grid, pass1, i, d, minimizer.settings.maxPast)
+    return merge_edge(pass1, edge), pass1, edge

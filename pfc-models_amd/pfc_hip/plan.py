@@ -306,6 +306,15 @@ class Batch:
         _lib.check(self._lib.pfc_batch_step(self._h, int(nsteps), current_stream_handle()))
         return self
 
+    def step_counts(self, counts):
+        """counts[i] steps for the i-th active slot (resident-stepper batches)."""
+        c = np.ascontiguousarray(np.asarray(counts, dtype=np.int64))
+        if c.size != len(self._active):
+            raise ValueError("one count per active slot")
+        _lib.check(self._lib.pfc_batch_step_counts(self._h, c.ctypes.data_as(ctypes.POINTER(ctypes.c_longlong)),
+                                                   current_stream_handle()))
+        return self
+
     def reduce(self):
         n = len(self._active)
         out = (ctypes.c_double * max(4, 4 * n))()

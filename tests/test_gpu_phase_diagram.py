@@ -342,3 +342,47 @@ def test_adapter_device_diagnose_equals_host(hip_device, slug, structure):
     assert d.fitness == pytest.approx(h.fitness, abs=1e-13)
     assert d.amplitudes == pytest.approx(h.amplitudes, abs=1e-14)
     assert d.details == h.details
+
+
+# -- adaptive dt batched; frozen-q solves batched across densities ------------------
+def test_batched_adaptive_dt_equals_serial(hip_device):
+    import time
+    from pfc_hip.models.logpfc_hip import phase_diagram as LOG
+    from pfc_hip.phase_diagram import batched as B
+    from pfc_hip.phase_diagram import q0star as Q
+    params = {"beta": 0.5, "g": -2.0, "v0": 1.0}
+    _, qscan = _settings()
+    m = Q.MinimizeFreeEnergy(LOG, params, "triangle", SMALL, H2L, _frozen_relax(), qscan)
+    pts = [(-0.9, 0.62, q) for q in (0.90, 0.95, 0.96, 1.0, 1.06)]
+    t0 = time.perf_counter()
+    serial = [m.evaluate(*p) for p in pts]
+    t1 = time.perf_counter()
+    batched = B.evaluate_batched(m, pts)
+    t2 = time.perf_counter()
+    print(f"\nadaptive dt, {len(pts)} points: serial {t1 - t0:.2f} s, batched {t2 - t1:.2f} s; "
+          f"dt_max {[round(e.dt_max, 4) for e in batched]} steps {[e.steps for e in batched]}", flush=True)
+    for s, b in zip(serial, batched):
+        assert (s.f, s.fitness, s.amplitudes, s.stop, s.steps, s.dt_max) == \
+            (b.f, b.fitness, b.amplitudes, b.stop, b.steps, b.dt_max)
+        assert np.array_equal(s.state, b.state)
+    assert max(e.dt_max for e in batched) > 0.01
+
+
+def test_frozen_q_batched_across_densities(hip_device):
+    import time
+    from pfc_hip.models.logpfc_hip import phase_diagram as LOG
+    from pfc_hip.phase_diagram import frozen_q as FQ
+    params = {"beta": 0.5, "g": -2.0, "v0": 1.0}
+    m = FQ.MinimizeFrozenQ(LOG, params, "triangle", SMALL, H2L, _frozen_relax())
+    rhos = [0.58, 0.60, 0.62, 0.64, 0.66, 0.68, 0.70]
+    t0 = time.perf_counter()
+    serial = [m.solve(-0.9, r) for r in rhos]
+    t1 = time.perf_counter()
+    batched = FQ.solve_batched(m, -0.9, rhos)
+    t2 = time.perf_counter()
+    print(f"\nfrozen-q {len(rhos)} densities: serial {t1 - t0:.2f} s, batched {t2 - t1:.2f} s; "
+          f"statuses {[r['status'] for r in batched]}", flush=True)
+    keys = ("status", "q", "q_relaxed", "f", "f_frozen", "fitness", "stop", "steps", "iters", "A1", "A2", "A3",
+            "peak_spread", "subq_ratio", "n_peaks", "defect_flag")
+    for a, b in zip(serial, batched):
+        assert {k: a.get(k) for k in keys} == {k: b.get(k) for k in keys}

@@ -60,8 +60,9 @@ class ToyBatch:
     def __init__(self, ad, capacity):
         self.ad, self.slots, self.active = ad, [None] * capacity, list(range(capacity))
 
-    def place(self, slot, temp, meanDensity, q00, dt, state):
+    def place(self, slot, temp, meanDensity, q00, dt, state, t=0.0):
         self.slots[slot] = self.ad.build({}, 'dots', {}, {}, temp, meanDensity, q00, dt, state=state)
+        self.slots[slot].t = t
 
     def states(self):
         return [self.ad.get_state(self.slots[j]) for j in self.active]
@@ -76,6 +77,12 @@ class ToyBatch:
         for j in self.active:
             for _ in range(n):
                 self.slots[j].step()
+
+    def step_each(self, counts):
+        for j, n in zip(self.active, counts):
+            for _ in range(n):
+                self.slots[j].step()
+        return True
 
     def free_energy(self):
         return np.array([self.slots[j].free_energy() for j in self.active])
@@ -268,11 +275,37 @@ def test_scan_density_batched_checkpoint_and_resume(tmp_path):
         np.testing.assert_array_equal(again[k], fresh[k])
 
 
-def test_adaptive_dt_falls_back_to_serial_evaluations():
-    relax = Q.RelaxSettings(**{**RELAX.__dict__, 'dtMax': 0.5, 'dtRate': 1e-3})
+ADAPTIVE = Q.RelaxSettings(**{**RELAX.__dict__, 'dtMax': 0.5, 'dtRate': 1e-3})
+
+
+def test_adaptive_dt_batched_equals_serial():
+    # every slot keeps its own dt: sub-blocks in lockstep with per-slot step
+    # counts, dt rebuilds, rollbacks (stiff), NaN stops (blowup), the seed
     ad = Toy()
-    serial = [_mini(ad, relax).evaluate(*p) for p in POINTS[:2]]
-    batched = B.evaluate_batched(_mini(ad, relax), POINTS[:2])
-    assert ad.batches_opened == 0
+    serial = [_mini(ad, ADAPTIVE).evaluate(*p) for p in POINTS]
+    batched = B.evaluate_batched(_mini(ad, ADAPTIVE), POINTS)
+    assert ad.batches_opened == 1
     for s, b in zip(serial, batched):
         _same(s, b)
+    assert max(e.dt_max for e in batched) > 0.15          # dt grew somewhere
+    assert len({e.steps for e in batched}) > 3
+
+
+def test_adaptive_dt_falls_back_when_slots_cannot_step_apart():
+    class Rigid(Toy):
+        def open_batch(self, *a, **k):
+            b = super().open_batch(*a, **k)
+            b.step_each = lambda counts: False
+            return b
+    ad = Rigid()
+    serial = [_mini(Toy(), ADAPTIVE).evaluate(*p) for p in POINTS[:3]]
+    batched = B.evaluate_batched(_mini(ad, ADAPTIVE), POINTS[:3])
+    for s, b in zip(serial, batched):
+        _same(s, b)
+
+
+def test_batched_find_q0star_with_adaptive_dt():
+    r1, r2 = [], []
+    q1, f1 = _mini(Toy(), ADAPTIVE).Find_q0Star(0.2, 0.5, record=r1.append)
+    q2, f2 = B.find_q0star_batched(_mini(Toy(), ADAPTIVE), 0.2, 0.5, record=r2.append)
+    assert (q1, float(f1)) == (q2, float(f2)) and r1 == r2

@@ -204,3 +204,53 @@ def test_scan_density_edge_pass_recovers_a_failed_density():
     assert len(seen) == 4 and len(edge) == 1
     with pytest.raises(ValueError):
         FQ.scan_density(m, 0.0, [0.2, 0.1])
+
+
+def test_solve_batched_equals_solve_on_a_batch_toy():
+    # the batched driver's toy from test_phase_diagram_batched, given a
+    # frozen field whose q_opt pulls toward QF: records equal solve()'s
+    from test_phase_diagram_batched import Toy as BToy
+
+    class FrozenToy(BToy):
+        def frozen_field(self, state, params, temp, cell):
+            x = float(np.mean(state[0]))
+
+            class F_:
+                def q_opt(self_):
+                    return 1.037 + 0.01 * x
+
+                def f(self_, qq):
+                    return (qq - 1.037) ** 2 + x
+            return F_()
+    relax = Q.RelaxSettings(dt=0.1, totalT=2e3, stepT=10.0, divs=10, maxDepth=3, requiredFitness=0.995,
+                            minimumA=1e-6, dtMax=0.5, dtRate=1e-3)
+    m = FQ.MinimizeFrozenQ(FrozenToy(), {}, "dots", {}, {}, relax, FQ.FrozenQSettings(maxIter=12))
+    rhos = [0.45, 0.5, 0.7, 2.5, 0.8]
+    serial = [m.solve(0.2, r, 1.0) for r in rhos]
+    batched = FQ.solve_batched(m, 0.2, rhos, 1.0)
+    keys = ("status", "q", "q_relaxed", "f", "f_frozen", "fitness", "stop", "steps", "iters", "A1", "A2")
+    for a, b in zip(serial, batched):
+        assert {k: a.get(k) for k in keys} == {k: b.get(k) for k in keys}
+    assert {r["status"] for r in batched} >= {FQ.STATUS_OK, Q.STOP_POOR_FIT}
+
+
+def test_solve_batched_when_max_iter_ends_before_the_last_stage():
+    from test_phase_diagram_batched import Toy as BToy
+
+    class FrozenToy(BToy):
+        def frozen_field(self, state, params, temp, cell):
+            class F_:
+                def q_opt(self_):
+                    return 1.0
+
+                def f(self_, qq):
+                    return 0.0
+            return F_()
+    relax = Q.RelaxSettings(dt=0.1, totalT=2e3, stepT=10.0, divs=10, maxDepth=3, requiredFitness=0.995,
+                            minimumA=1e-6)
+    for iters in (2, 3, 4, 6):
+        m = FQ.MinimizeFrozenQ(FrozenToy(), {}, "dots", {}, {}, relax, FQ.FrozenQSettings(maxIter=iters))
+        a = [m.solve(0.2, r, 1.0) for r in (0.5, 0.45)]
+        b = FQ.solve_batched(m, 0.2, [0.5, 0.45], 1.0)
+        assert [x["status"] for x in a] == [x["status"] for x in b], iters
+        assert [x["iters"] for x in a] == [x["iters"] for x in b], iters
