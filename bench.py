@@ -66,6 +66,16 @@ def run_gpu(precision, steps, warmup):
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     fe, mean, _, _ = pl.reduce()
+    graphs = pl.graph_stats()
+    # the same steps launched directly (calls of < 32 steps replay no graph),
+    # for the step-loop graph's gain; the state goes on from the timed run
+    t0 = time.perf_counter()
+    left = steps
+    while left > 0:
+        pl.step(min(left, 31))
+        left -= 31
+    torch.cuda.synchronize()
+    eager_wall = time.perf_counter() - t0
     # per-kernel device time: each kernel kind of the schedule in use, launched
     # back to back on scratch copies of the live state, one HIP event pair per
     # kind on the plan's stream; weighted by launches per ETDRK2 step (two row
@@ -88,6 +98,8 @@ def run_gpu(precision, steps, warmup):
     return {
         "value": N * N * steps / wall / 1e6,
         "ms_per_step": wall / steps * 1e3,
+        "eager_ms_per_step": eager_wall / steps * 1e3,
+        "graph_captures_replays": list(graphs),
         "free_energy": fe,
         "mean": mean,
         "kernels_us": {k: round(v[0] * 1e3, 3) for k, v in kinds.items()},
@@ -378,8 +390,11 @@ def main():
         "roofline": f64["roofline"],
         "step_roofline": f64["step_roofline"],
         "kernels_us": f64["kernels_us"],
+        "step_graph": {"eager_ms_per_step": round(f64["eager_ms_per_step"], 5),
+                       "captures_replays": f64["graph_captures_replays"]},
         "fp32": None if f32 is None else {
             "value": round(f32["value"], 1), "ms_per_step": round(f32["ms_per_step"], 5),
+            "eager_ms_per_step": round(f32["eager_ms_per_step"], 5),
             "dtype": "f32 bulk, f64 tables, combine and reductions", "roofline": f32["roofline"],
             "kernels_us": f32["kernels_us"],
             "rel_fe_vs_fp64": abs(f32["free_energy"] - f64["free_energy"]) / abs(f64["free_energy"])},

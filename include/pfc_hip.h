@@ -141,6 +141,15 @@ int pfc_step_profile(pfc_plan* plan, long long nsteps, void* stream, double ms[3
  * holds the whole row spectrum in LDS for all the steps of a call).
  * (No reference counterpart.) */
 int pfc_plan_schedule(const pfc_plan* plan, int info[8]);
+/* Step-loop graph counters: out[0] = graphs captured, out[1] = graph replays.
+ * A pfc_step / pfc_batch_step call of n >= 32 steps on the two-kernel path
+ * replays a HIP graph of 32 steps n / 32 times (captured once per launch
+ * configuration) and launches the remaining steps directly; results are
+ * bit-identical to direct launches.  PFC_HIP_GRAPH=0 disables the graphs.
+ * (No reference counterpart: the reference's callers loop TimeStep(),
+ * models/stdpfc/simulation.py:213-235, one kernel sequence per step.) */
+int pfc_plan_graph_stats(const pfc_plan* plan, long long out[2]);
+int pfc_batch_graph_stats(const struct pfc_batch* batch, long long out[2]);
 /* Device bytes held by a plan (state, tables, scratch). */
 int pfc_plan_bytes(const pfc_plan* plan, long long* bytes);
 

@@ -383,7 +383,18 @@ struct Engine {
   std::vector<char> field_set;
   double* host_out = nullptr;   // pinned [4 cap]
   int* host_status = nullptr;   // pinned [cap]
+  // step-loop graphs: kGraphSteps steps of the row/column launches, captured
+  // per launch configuration (see engine_step)
+  std::map<std::array<long long, 9>, hipGraphExec_t> graphs;
+  hipStream_t cap_st = nullptr;  // private stream the graphs are captured on
+  long long graph_replays = 0, graph_captures = 0;
+  void drop_graphs() {
+    for (auto& kv : graphs) (void)hipGraphExecDestroy(kv.second);
+    graphs.clear();
+  }
   ~Engine() {
+    drop_graphs();
+    if (cap_st) (void)hipStreamDestroy(cap_st);
     if (host_out) (void)hipHostFree(host_out);
     if (host_status) (void)hipHostFree(host_status);
   }
@@ -611,6 +622,12 @@ struct Steps {
       hip_check(hipGetLastError(), "step launch");
       return;
     }
+    loop(e, nsteps, st);
+  }
+  // nsteps of the row/column launch sequence (also what a step graph holds)
+  static void loop(Engine& e, long long nsteps, hipStream_t st) {
+    const int ny = e.nact();
+    const Slots sl = e.slots_active();
     for (long long s = 0; s < nsteps; ++s) {
       row(e, st, sl, ny);
       if (e.p.scheme == PFC_SCHEME_ETD1) {
@@ -735,6 +752,60 @@ void autotune(Engine& e, hipStream_t st) {
   }
 }
 
+// ---------------------------------------------------------------------------
+// step-loop graphs: a call of n >= kGraphSteps steps on the two-kernel path
+// replays a HIP graph of kGraphSteps steps n / kGraphSteps times and runs the
+// rest eagerly, so the host enqueues one graph instead of 2 (ETD1) or 4
+// (ETDRK2) kernels per step.  A graph is captured on a private stream the
+// first time a launch configuration is stepped and kept for the engine's
+// life: its nodes hold the engine's own arrays, the active-list pointer and
+// the schedule, and read slot ids, tables, pins and shifts from device
+// memory when they run, so it stays valid as those change.  What changes a
+// node (the active count, the schedule, an external potential's arrays) is in
+// the key.  The nodes are exactly the eager launches, so results are
+// bit-identical.  PFC_HIP_GRAPH=0 steps eagerly.
+// ---------------------------------------------------------------------------
+constexpr long long kGraphSteps = 32;
+constexpr size_t kGraphCacheMax = 16;
+
+bool graphs_enabled() {
+  static const bool on = [] {
+    const char* v = std::getenv("PFC_HIP_GRAPH");
+    return !(v && v[0] == '0');
+  }();
+  return on;
+}
+
+hipGraphExec_t step_graph(Engine& e) {
+  const std::array<long long, 9> key{e.nact(),          e.frow.on, e.frow.var, e.frow.lines,
+                                     e.fcol.on,         e.fcol.var, e.fcol.lines,
+                                     (long long)(intptr_t)e.extc.p, (long long)(intptr_t)e.D.p};
+  auto it = e.graphs.find(key);
+  if (it != e.graphs.end()) return it->second;
+  if (e.graphs.size() >= kGraphCacheMax) e.drop_graphs();
+  if (!e.cap_st) hip_check(hipStreamCreateWithFlags(&e.cap_st, hipStreamNonBlocking), "hipStreamCreate");
+  hip_check(hipStreamBeginCapture(e.cap_st, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+  hipGraph_t g = nullptr;
+  try {
+    if (e.f64())
+      Steps<double>::loop(e, kGraphSteps, e.cap_st);
+    else
+      Steps<float>::loop(e, kGraphSteps, e.cap_st);
+  } catch (...) {
+    (void)hipStreamEndCapture(e.cap_st, &g);
+    if (g) (void)hipGraphDestroy(g);
+    throw;
+  }
+  hip_check(hipStreamEndCapture(e.cap_st, &g), "hipStreamEndCapture");
+  hipGraphExec_t x = nullptr;
+  const hipError_t ie = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  hip_check(ie, "hipGraphInstantiate");
+  e.graphs[key] = x;
+  ++e.graph_captures;
+  return x;
+}
+
 void engine_step(Engine& e, long long nsteps, hipStream_t st) {
   if (nsteps < 0) fail(PFC_EINVAL, "nsteps < 0");
   for (int s : e.h_active)
@@ -744,6 +815,13 @@ void engine_step(Engine& e, long long nsteps, hipStream_t st) {
       autotune<double>(e, st);
     else
       autotune<float>(e, st);
+  }
+  if (!e.resident && e.nact() > 0 && nsteps >= kGraphSteps && graphs_enabled()) {
+    const hipGraphExec_t g = step_graph(e);
+    const long long reps = nsteps / kGraphSteps;
+    for (long long r = 0; r < reps; ++r) hip_check(hipGraphLaunch(g, st), "hipGraphLaunch");
+    e.graph_replays += reps;
+    nsteps -= reps * kGraphSteps;
   }
   if (e.f64())
     Steps<double>::step(e, nsteps, st);
@@ -1322,6 +1400,22 @@ int pfc_plan_schedule(const pfc_plan* pl, int info[8]) {
       info[4 * k + 2] = fs[k]->P;
       info[4 * k + 3] = fs[k]->lines;
     }
+  });
+}
+
+int pfc_plan_graph_stats(const pfc_plan* pl, long long out[2]) {
+  return guarded([&] {
+    if (!pl || !out) fail(PFC_EINVAL, "NULL argument");
+    out[0] = pl->e.graph_captures;
+    out[1] = pl->e.graph_replays;
+  });
+}
+
+int pfc_batch_graph_stats(const pfc_batch* b, long long out[2]) {
+  return guarded([&] {
+    if (!b || !out) fail(PFC_EINVAL, "NULL argument");
+    out[0] = b->e.graph_captures;
+    out[1] = b->e.graph_replays;
   });
 }
 

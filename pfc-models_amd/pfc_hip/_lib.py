@@ -63,6 +63,8 @@ _SIGNATURES = [
     ("pfc_step_profile", ctypes.c_int, [_P, ctypes.c_longlong, _P, ctypes.POINTER(ctypes.c_double),
                                         ctypes.POINTER(ctypes.c_longlong)]),
     ("pfc_plan_schedule", ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_int)]),
+    ("pfc_plan_graph_stats", ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_longlong)]),
+    ("pfc_batch_graph_stats", ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_longlong)]),
     ("pfc_plan_bytes", ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_longlong)]),
     ("pfc_batch_create", ctypes.c_int, [ctypes.POINTER(PfcParams), ctypes.c_int, ctypes.POINTER(_P)]),
     ("pfc_batch_destroy", ctypes.c_int, [_P]),

@@ -196,6 +196,12 @@ class Plan:
         _lib.check(self._lib.pfc_plan_schedule(self._h, info))
         return {"row": tuple(info[0:4]), "col": tuple(info[4:8])}
 
+    def graph_stats(self):
+        """(step-loop graphs captured, graph replays) so far."""
+        v = (ctypes.c_longlong * 2)()
+        _lib.check(self._lib.pfc_plan_graph_stats(self._h, v))
+        return int(v[0]), int(v[1])
+
     def device_bytes(self):
         v = ctypes.c_longlong()
         _lib.check(self._lib.pfc_plan_bytes(self._h, ctypes.byref(v)))
@@ -325,6 +331,12 @@ class Batch:
         v = (ctypes.c_int * self.capacity)()
         _lib.check(self._lib.pfc_batch_get_status(self._h, v, current_stream_handle()))
         return np.array(v[:], dtype=np.int32)
+
+    def graph_stats(self):
+        """(step-loop graphs captured, graph replays) so far."""
+        v = (ctypes.c_longlong * 2)()
+        _lib.check(self._lib.pfc_batch_graph_stats(self._h, v))
+        return int(v[0]), int(v[1])
 
     def device_bytes(self):
         v = ctypes.c_longlong()
