@@ -255,6 +255,9 @@ def run_sweep(plain_fraction=8):
     inst = SW.sweep_instances()
     b32 = SW.run_batched(inst, 5000, precision=32)
     b64 = SW.run_batched(inst, 5000, precision=64)
+    # the same sweep through one Group (a row and a column launch per
+    # register class per step over every size's tiles)
+    g32 = SW.run_batched(inst, 5000, precision=32, grouped=True)
     sub = inst[::plain_fraction]
     plain = SW.run_plain(sub, 5000, checkpoint=500)
     idx = [i.index for i in sub]
@@ -273,6 +276,7 @@ def run_sweep(plain_fraction=8):
         "batched_fp32_solves_per_s": round(b32.solves_per_s, 2),
         "batched_fp64_solves_per_s": round(b64.solves_per_s, 2),
         "plain_fp64_solves_per_s": round(plain.solves_per_s, 2),
+        "grouped_fp32_solves_per_s": round(g32.solves_per_s, 2),
         "speedup_fp32_vs_plain_fp64": round(b32.solves_per_s / plain.solves_per_s, 2),
         "plain_sample": f"every {plain_fraction}th instance ({len(sub)} solves)",
         "fp32_rel_fe_err": {"median": float(np.median(rel32)), "max": float(rel32.max())},

@@ -207,6 +207,31 @@ int pfc_batch_bytes(const pfc_batch* batch, long long* bytes);
 /* Device bytes one slot of these parameters needs (state + tables); for admission control. */
 int pfc_slot_bytes(const pfc_params* params, long long* bytes);
 
+/* ---- groups: batches of different grid sizes advanced by shared launches ----
+ * A group refers to (does not own) `n` batches that share precision and
+ * scheme; each has Nx/2 and Ny in {32, 48, 64, 96, 128, 192, 256, 384, 512},
+ * is not small enough for the resident stepper and has no external potential.
+ * pfc_group_step advances the active slots of every member by `nsteps`
+ * steps with, per nonlinear evaluation, one row launch and one column launch
+ * for each register class that has members (class 0: lengths 32, 64, 128,
+ * 256, 512; class 1: 48, 96, 192, 384), whatever the number of sizes: the
+ * launches' blocks are the members' row (column) tiles, largest instances
+ * first, each block finding its member, instance and tile from the members'
+ * block offsets.  The transforms are the variant-0
+ * schedules, so a slot's result equals its batch stepped alone with that
+ * schedule (PFC_HIP_TUNE=0), bit for bit.  Calls of >= 32 steps replay
+ * step-loop graphs as pfc_batch_step does.  The batches stay usable on their
+ * own; a member must outlive the group, and members' active lists may change
+ * between calls (the next call rebuilds the launch layout after draining
+ * `stream`).  (No reference counterpart: the reference relaxes one Simulation
+ * at a time, phase_diagram/q0star.py:409-477; this is the uneven-size sweep
+ * form, one launch per kernel kind per step over a work list of FFT tiles.) */
+typedef struct pfc_group pfc_group;
+int pfc_group_create(pfc_batch* const* batches, int n, pfc_group** out);
+int pfc_group_destroy(pfc_group* group);
+int pfc_group_step(pfc_group* group, long long nsteps, void* stream);
+int pfc_group_graph_stats(const pfc_group* group, long long out[2]);
+
 /* Batched 2D real transforms over (batch, Ny, Nx) device arrays.
  * pfc_rfft2:  out[b] = rfft2(in[b])             (unnormalised)
  * pfc_irfft2: out[b] = irfft2(in[b], s=(Ny,Nx)) (scaled 1/(Nx Ny); `in` is not modified) */

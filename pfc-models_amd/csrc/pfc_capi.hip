@@ -1264,6 +1264,178 @@ struct pfc_batch {
 };
 
 // ---------------------------------------------------------------------------
+// groups: batches of different grid sizes stepped by shared launches
+// (k_row_group / k_col_group), one per pass and register class with members.
+// The descriptor lists live in one device array allocated at creation; they
+// depend only on each member's active count (everything else they hold is
+// fixed for a batch's life), so they are rebuilt, after the stream has
+// drained, only when an active count changes.
+// ---------------------------------------------------------------------------
+struct GroupLaunch {
+  int nmem = 0, blocks = 0;
+  size_t lds = 0, off = 0;  // off: first descriptor of this launch in the array
+};
+
+struct pfc_group {
+  std::vector<pfc_batch*> mem;  // not owned
+  std::vector<int> order;       // members by decreasing instance size (launch order)
+  int precision = 0, scheme = 0;
+  DevBuf desc;                  // GroupMember<T>[2 passes][2 classes][members]
+  std::vector<long long> built; // active counts the descriptors hold
+  GroupLaunch rows[kGroupClasses], cols[kGroupClasses];
+  std::map<std::vector<long long>, hipGraphExec_t> graphs;
+  hipStream_t cap_st = nullptr;
+  long long captures = 0, replays = 0;
+  ~pfc_group() {
+    for (auto& kv : graphs) (void)hipGraphExecDestroy(kv.second);
+    if (cap_st) (void)hipStreamDestroy(cap_st);
+  }
+};
+
+namespace {
+
+void group_check_member(const Engine& e) {
+  int P = 0;
+  if (e.resident)
+    fail(PFC_EUNSUPPORTED, "a group member runs the resident stepper (small ETD1 grid): step it on its own");
+  if (e.has_ext()) fail(PFC_EUNSUPPORTED, "a group member has an external potential");
+  if (group_case(e.p.Nx / 2, &P) < 0 || group_case(e.p.Ny, &P) < 0)
+    fail(PFC_EUNSUPPORTED, "group members need Nx/2 and Ny in {32, 48, 64, 96, 128, 192, 256, 384, 512}");
+}
+
+std::vector<long long> group_counts(const pfc_group& g) {
+  std::vector<long long> k;
+  for (const pfc_batch* b : g.mem) k.push_back(b->e.nact());
+  return k;
+}
+
+template <typename T>
+void group_build(pfc_group& g, hipStream_t st) {
+  const size_t n = g.mem.size();
+  std::vector<GroupMember<T>> all(2 * kGroupClasses * n);
+  for (int pass = 0; pass < 2; ++pass)
+    for (int c = 0; c < kGroupClasses; ++c) {
+      GroupLaunch& L = pass == 0 ? g.rows[c] : g.cols[c];
+      L = GroupLaunch{};
+      L.off = (size_t)(pass * kGroupClasses + c) * n;
+      for (int i : g.order) {
+        Engine& e = g.mem[i]->e;
+        const int ny = e.nact();
+        if (ny == 0) continue;
+        const int len = pass == 0 ? e.p.Nx / 2 : e.p.Ny;
+        int P = 0, cls = 0;
+        const int kcase = group_case(len, &P, &cls);
+        if (cls != c) continue;
+        GroupMember<T> m{};
+        m.S = (V2<T>*)e.S.p;
+        m.Tb = (V2<T>*)e.Tb.p;
+        m.D = (V2<T>*)e.D.p;
+        m.tw = (const V2<double>*)(pass == 0 ? e.twN64.p : e.twY64.p);
+        m.lay = e.lay;
+        m.q = nonlin_of(e);
+        m.sl = e.slots_active();
+        m.tb = e.tables();
+        m.Ny = e.p.Ny;
+        m.Nc = e.Nc;
+        m.kcase = kcase;
+        const int per = kGroupThreads / (len / P);  // rows (columns) per block
+        m.ntile = pass == 0 ? (e.p.Ny + per - 1) / per : (e.Nc + per - 1) / per;
+        m.blk0 = L.blocks;
+        L.blocks += m.ntile * ny;
+        L.lds = std::max(L.lds, (size_t)per * (pass == 0 ? row_ld(len) : col_ld(len)) * sizeof(V2<T>));
+        all[L.off + L.nmem++] = m;
+      }
+    }
+  // launches (or graphs) still reading the old descriptors finish first
+  hip_check(hipStreamSynchronize(st), "sync before group rebuild");
+  hip_check(hipMemcpy(g.desc.p, all.data(), all.size() * sizeof(GroupMember<T>), hipMemcpyHostToDevice), "copy");
+  g.built = group_counts(g);
+}
+
+template <typename T>
+void group_launch(int kind, int cls, int blocks, size_t lds, hipStream_t st, const GroupMember<T>* mem, int nmem);
+template <>
+void group_launch<double>(int kind, int cls, int blocks, size_t lds, hipStream_t st, const GroupMember<double>* mem,
+                          int nmem) {
+  group_launch_f64(kind, cls, blocks, lds, st, mem, nmem);
+}
+template <>
+void group_launch<float>(int kind, int cls, int blocks, size_t lds, hipStream_t st, const GroupMember<float>* mem,
+                         int nmem) {
+  group_launch_f32(kind, cls, blocks, lds, st, mem, nmem);
+}
+
+// one pass (kind 0: rows, else the column mode kind - 1) over every class with members
+template <typename T>
+void group_pass(pfc_group& g, int kind, hipStream_t st) {
+  const auto* base = (const GroupMember<T>*)g.desc.p;
+  const GroupLaunch* L = kind == 0 ? g.rows : g.cols;
+  for (int c = 0; c < kGroupClasses; ++c)
+    if (L[c].nmem) group_launch<T>(kind, c, L[c].blocks, L[c].lds, st, base + L[c].off, L[c].nmem);
+}
+
+template <typename T>
+void group_loop(pfc_group& g, long long nsteps, hipStream_t st) {
+  for (long long s = 0; s < nsteps; ++s) {
+    group_pass<T>(g, 0, st);
+    if (g.scheme == PFC_SCHEME_ETD1) {
+      group_pass<T>(g, 1 + COL_ETD1, st);
+    } else {
+      group_pass<T>(g, 1 + COL_RK2_STAGE1, st);
+      group_pass<T>(g, 0, st);
+      group_pass<T>(g, 1 + COL_RK2_STAGE2, st);
+    }
+  }
+  hip_check(hipGetLastError(), "group step launch");
+}
+
+template <typename T>
+void group_step(pfc_group& g, long long nsteps, hipStream_t st) {
+  if (nsteps < 0) fail(PFC_EINVAL, "nsteps < 0");
+  for (const pfc_batch* b : g.mem) {
+    group_check_member(b->e);
+    for (int s : b->e.h_active)
+      if (!b->e.field_set[s]) fail(PFC_EINVAL, "a stepped slot has no field (pfc_batch_set_field)");
+  }
+  const std::vector<long long> counts = group_counts(g);
+  if (counts != g.built) group_build<T>(g, st);
+  bool any = false;
+  for (int c = 0; c < kGroupClasses; ++c) any |= g.rows[c].nmem > 0;
+  if (!any || nsteps == 0) return;
+  if (nsteps >= kGraphSteps && graphs_enabled()) {
+    hipGraphExec_t x = nullptr;
+    auto it = g.graphs.find(counts);
+    if (it != g.graphs.end()) {
+      x = it->second;
+    } else {
+      if (!g.cap_st) hip_check(hipStreamCreateWithFlags(&g.cap_st, hipStreamNonBlocking), "hipStreamCreate");
+      hip_check(hipStreamBeginCapture(g.cap_st, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+      hipGraph_t gr = nullptr;
+      try {
+        group_loop<T>(g, kGraphSteps, g.cap_st);
+      } catch (...) {
+        (void)hipStreamEndCapture(g.cap_st, &gr);
+        if (gr) (void)hipGraphDestroy(gr);
+        throw;
+      }
+      hip_check(hipStreamEndCapture(g.cap_st, &gr), "hipStreamEndCapture");
+      const hipError_t ie = hipGraphInstantiate(&x, gr, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(gr);
+      hip_check(ie, "hipGraphInstantiate");
+      g.graphs[counts] = x;
+      ++g.captures;
+    }
+    const long long reps = nsteps / kGraphSteps;
+    for (long long r = 0; r < reps; ++r) hip_check(hipGraphLaunch(x, st), "hipGraphLaunch");
+    g.replays += reps;
+    nsteps -= reps * kGraphSteps;
+  }
+  group_loop<T>(g, nsteps, st);
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
 // C ABI
 // ---------------------------------------------------------------------------
 extern "C" {
@@ -1532,6 +1704,64 @@ int pfc_batch_bytes(const pfc_batch* b, long long* bytes) {
   return guarded([&] {
     if (!b || !bytes) fail(PFC_EINVAL, "NULL batch or bytes");
     *bytes = (long long)engine_bytes(b->e);
+  });
+}
+
+int pfc_group_create(pfc_batch* const* batches, int n, pfc_group** out) {
+  return guarded([&] {
+    if (!out) fail(PFC_EINVAL, "out is NULL");
+    *out = nullptr;
+    if (n < 1 || !batches) fail(PFC_EINVAL, "a group needs at least one batch");
+    auto g = std::make_unique<pfc_group>();
+    for (int i = 0; i < n; ++i) {
+      pfc_batch* b = batches[i];
+      if (!b) fail(PFC_EINVAL, "NULL batch");
+      for (const pfc_batch* o : g->mem)
+        if (o == b) fail(PFC_EINVAL, "a batch appears twice in the group");
+      group_check_member(b->e);
+      if (i == 0) {
+        g->precision = b->e.p.precision;
+        g->scheme = b->e.p.scheme;
+      } else if (b->e.p.precision != g->precision || b->e.p.scheme != g->scheme) {
+        fail(PFC_EINVAL, "group members must share precision and scheme");
+      }
+      g->mem.push_back(b);
+    }
+    g->order.resize(n);
+    for (int i = 0; i < n; ++i) g->order[i] = i;
+    // largest instances first: their tiles are dispatched first, the small
+    // members' tiles fill in behind them
+    std::stable_sort(g->order.begin(), g->order.end(), [&](int a, int b) {
+      const Engine &ea = g->mem[a]->e, &eb = g->mem[b]->e;
+      return (long long)ea.p.Nx * ea.p.Ny > (long long)eb.p.Nx * eb.p.Ny;
+    });
+    const size_t dsize = g->precision == 64 ? sizeof(GroupMember<double>) : sizeof(GroupMember<float>);
+    g->desc.alloc((size_t)2 * kGroupClasses * n * dsize);
+    g->built.assign(n, -1);
+    *out = g.release();
+  });
+}
+
+int pfc_group_destroy(pfc_group* g) {
+  return guarded([&] { delete g; });
+}
+
+int pfc_group_step(pfc_group* g, long long nsteps, void* stream) {
+  return guarded([&] {
+    if (!g) fail(PFC_EINVAL, "NULL group");
+    const hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    if (g->precision == 64)
+      group_step<double>(*g, nsteps, st);
+    else
+      group_step<float>(*g, nsteps, st);
+  });
+}
+
+int pfc_group_graph_stats(const pfc_group* g, long long out[2]) {
+  return guarded([&] {
+    if (!g || !out) fail(PFC_EINVAL, "NULL argument");
+    out[0] = g->captures;
+    out[1] = g->replays;
   });
 }
 

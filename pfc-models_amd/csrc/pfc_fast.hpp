@@ -225,25 +225,21 @@ __device__ __forceinline__ void reg_fft(V2<T>* v, V2<T>* line, int lane, const V
 // LDS: W lines of M + 1 (padded).  Roots come from the FP64 table
 // exp(-2 pi i m / Nx), m < Nx, into registers.
 // ---------------------------------------------------------------------------
+// Row tile `tile` (blockDim.x / TPL rows) of active instance `yi`; every
+// thread of the block calls it.
 template <typename T, class S>
-__global__ void __launch_bounds__(256) k_row_fast(const V2<T>* __restrict__ Sg, V2<T>* __restrict__ Tout, int Ny,
-                                                  Lay lay, Nonlin q, const V2<double>* __restrict__ twN,
-                                                  Slots sl) {
+__device__ __forceinline__ void row_fast_tile(const V2<T>* __restrict__ Sg, V2<T>* __restrict__ Tout, int Ny,
+                                              const Lay& lay, Nonlin q, const V2<double>* __restrict__ twN,
+                                              const Slots& sl, int yi, int tile, unsigned char* smem_raw) {
   using V = V2<T>;
   constexpr int M = S::L, P = S::P, TPL = S::TPL, LD = row_ld(M);
-  extern __shared__ __align__(16) unsigned char smem_raw[];
-  // XCD-aware order over (instance, row block): the blocks that share a row
-  // group's 128-byte lines run on one XCD and meet in its L2
-  const int ntile = gridDim.x;
-  const int lin = xcd_remap(blockIdx.x + blockIdx.y * ntile, ntile * gridDim.y);
-  const int yi = lin / ntile;
   const int slot = slot_of(sl, yi);
   Sg += (size_t)slot * sl.sstride;
   Tout += (size_t)slot * sl.sstride;
   if (sl.M) q.M = sl.M[slot];
   const int wl = threadIdx.x / TPL;
   const int lane = threadIdx.x - wl * TPL;
-  const int row = (lin - yi * ntile) * (blockDim.x / TPL) + wl;
+  const int row = tile * (blockDim.x / TPL) + wl;
   const bool live = row < Ny;
   V* line = reinterpret_cast<V*>(smem_raw) + wl * LD;
 
@@ -315,19 +311,34 @@ __global__ void __launch_bounds__(256) k_row_fast(const V2<T>* __restrict__ Sg, 
   }
 }
 
+template <typename T, class S>
+__global__ void __launch_bounds__(256) k_row_fast(const V2<T>* __restrict__ Sg, V2<T>* __restrict__ Tout, int Ny,
+                                                  Lay lay, Nonlin q, const V2<double>* __restrict__ twN,
+                                                  Slots sl) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  // XCD-aware order over (instance, row block): the blocks that share a row
+  // group's 128-byte lines run on one XCD and meet in its L2
+  const int ntile = gridDim.x;
+  const int lin = xcd_remap(blockIdx.x + blockIdx.y * ntile, ntile * gridDim.y);
+  const int yi = lin / ntile;
+  row_fast_tile<T, S>(Sg, Tout, Ny, lay, q, twN, sl, yi, lin - yi * ntile, smem_raw);
+}
+
 // ---------------------------------------------------------------------------
 // fast column kernel: forward column FFTs of S~ and T, ETD combine in
 // registers, inverse column FFT, S~ written in place.  C columns per block.
 // LDS: C lines of Ny (padded); roots from the FP64 table exp(-2 pi i m / Ny)
 // in registers.
 // ---------------------------------------------------------------------------
+// Column tile `tile` (blockDim.x / TPL columns) of active instance `yi`;
+// every thread of the block calls it.
 template <typename T, class S, int MODE>
-__global__ void __launch_bounds__(256) k_col_fast(V2<T>* __restrict__ Sg, const V2<T>* __restrict__ Tin,
-                                                  V2<T>* __restrict__ D, int Nc, Lay lay, ColTables tb,
-                                                  const V2<double>* __restrict__ twY, Slots sl) {
+__device__ __forceinline__ void col_fast_tile(V2<T>* __restrict__ Sg, const V2<T>* __restrict__ Tin,
+                                              V2<T>* __restrict__ D, int Nc, const Lay& lay, const ColTables& tb,
+                                              const V2<double>* __restrict__ twY, const Slots& sl, int yi, int tile,
+                                              unsigned char* smem_raw) {
   using V = V2<T>;
   constexpr int NY = S::L, P = S::P, TPL = S::TPL, LD = col_ld(NY);
-  extern __shared__ __align__(16) unsigned char smem_raw[];
   V* lds = reinterpret_cast<V*>(smem_raw);
   // Thread t is transform lane t / C of column t % C of the tile, holding rows
   // lane + TPL q.  The lanes of one load or store then cover C columns of
@@ -338,13 +349,8 @@ __global__ void __launch_bounds__(256) k_col_fast(V2<T>* __restrict__ Sg, const 
   const int lgC = __ffs(C) - 1;
   const int wl = threadIdx.x & (C - 1);
   const int lane = threadIdx.x >> lgC;
-  // XCD-aware order over (instance, tile): consecutive tiles of an instance
-  // land on the same XCD
-  const int ntile = gridDim.x;
-  const int lin = xcd_remap(blockIdx.x + blockIdx.y * ntile, ntile * gridDim.y);
-  const int yi = lin / ntile;
   const int slot = slot_of(sl, yi);
-  const int c0 = (lin - yi * ntile) * C;
+  const int c0 = tile * C;
   Sg += (size_t)slot * sl.sstride;
   Tin += (size_t)slot * sl.sstride;
   if (D) D += (size_t)slot * sl.sstride;
@@ -426,6 +432,19 @@ __global__ void __launch_bounds__(256) k_col_fast(V2<T>* __restrict__ Sg, const 
   }
 }
 
+template <typename T, class S, int MODE>
+__global__ void __launch_bounds__(256) k_col_fast(V2<T>* __restrict__ Sg, const V2<T>* __restrict__ Tin,
+                                                  V2<T>* __restrict__ D, int Nc, Lay lay, ColTables tb,
+                                                  const V2<double>* __restrict__ twY, Slots sl) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  // XCD-aware order over (instance, tile): consecutive tiles of an instance
+  // land on the same XCD
+  const int ntile = gridDim.x;
+  const int lin = xcd_remap(blockIdx.x + blockIdx.y * ntile, ntile * gridDim.y);
+  const int yi = lin / ntile;
+  col_fast_tile<T, S, MODE>(Sg, Tin, D, Nc, lay, tb, twY, sl, yi, lin - yi * ntile, smem_raw);
+}
+
 // ---------------------------------------------------------------------------
 // schedule variants per length (the autotuner chooses among them)
 // X(L, VARIANT, P, radices...)
@@ -472,5 +491,119 @@ bool col_fast_launch_f64(int Ny, int var, int mode, dim3 grid, dim3 block, size_
 bool col_fast_launch_f32(int Ny, int var, int mode, dim3 grid, dim3 block, size_t lds, hipStream_t st, V2<float>* S,
                          const V2<float>* T, V2<float>* D, int Nc, const Lay& lay, const ColTables& tb,
                          const V2<double>* tw, const Slots& sl);
+
+// ---------------------------------------------------------------------------
+// multi-shape step launches: batches of different grid sizes ("members")
+// advance together.  A launch covers the row (or column) tiles of every
+// member whose transform length is in its register class: class 0 holds the
+// 8-points-per-lane schedules, class 1 the 12-points-per-lane ones (lengths
+// with a factor 3), which need more registers; one kernel over both would run
+// every block at the larger register count (a third fewer waves per SIMD for
+// class 0).  So a step takes at most two row and two column launches,
+// whatever the number of sizes.  Block b of a launch is tile (b - blk0) %
+// tiles of active instance (b - blk0) / tiles of the member whose block range
+// holds b; every block has 256 threads, so a member's rows (columns) per
+// block are 256 / TPL.  The transforms are the variant-0 schedules of each
+// length (the same radix sequence as variant 0 of PFC_SCHEDULES, so a
+// member's results equal its batch stepped alone with that variant).
+// X(CASE, CLASS, L, P, radices...)
+// ---------------------------------------------------------------------------
+#define PFC_GROUP_SCHEDULES(X)       \
+  X(0, 0, 32, 8, 8, 4)               \
+  X(1, 1, 48, 12, 4, 4, 3)           \
+  X(2, 0, 64, 8, 8, 8)               \
+  X(3, 1, 96, 12, 4, 4, 2, 3)        \
+  X(4, 0, 128, 8, 8, 8, 2)           \
+  X(5, 1, 192, 12, 4, 4, 4, 3)       \
+  X(6, 0, 256, 8, 8, 8, 4)           \
+  X(7, 1, 384, 12, 4, 4, 4, 2, 3)    \
+  X(8, 0, 512, 8, 8, 8, 8)
+
+constexpr int kGroupThreads = 256;
+constexpr int kGroupClasses = 2;
+
+// schedule case of a transform length, or -1; *P = points per lane, *cls = register class
+inline int group_case(int L, int* P, int* cls = nullptr) {
+#define PFC_X(CASE, CLS, LEN, PP, ...) \
+  if (L == LEN) {                      \
+    *P = PP;                           \
+    if (cls) *cls = CLS;               \
+    return CASE;                       \
+  }
+  PFC_GROUP_SCHEDULES(PFC_X)
+#undef PFC_X
+  return -1;
+}
+
+// one member's part of a launch: `kcase` is the schedule case of the pass's
+// transform (row: Nx/2, column: Ny), `ntile` its tiles per instance and
+// `blk0` its first block in the launch
+template <typename T> struct GroupMember {
+  V2<T>* S;   // row spectra S~, [cap][...] in the member's layout
+  V2<T>* Tb;  // row spectra of N
+  V2<T>* D;   // ETDRK2 carry, or null
+  const V2<double>* tw;  // FP64 roots: exp(-2 pi i m / Nx), m < Nx (rows) or exp(-2 pi i m / Ny) (columns)
+  Lay lay;
+  Nonlin q;
+  Slots sl;      // the member's active slots
+  ColTables tb;  // the member's ETD tables
+  int Ny, Nc;
+  int kcase, ntile, blk0;
+};
+
+template <typename T, int CLASS>
+__global__ void __launch_bounds__(kGroupThreads) k_row_group(const GroupMember<T>* __restrict__ mem, int nmem) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  // members are laid out in block order; XCD-aware order as in k_row_fast
+  const int lin = xcd_remap(blockIdx.x, gridDim.x);
+  int m = 0;
+  while (m + 1 < nmem && lin >= mem[m + 1].blk0) ++m;
+  const GroupMember<T> g = mem[m];
+  const int local = lin - g.blk0;
+  const int yi = local / g.ntile;
+  const int tile = local - yi * g.ntile;
+  switch (g.kcase) {
+#define PFC_X(CASE, CLS, LEN, PP, ...)                                                                           \
+  case CASE:                                                                                                     \
+    if constexpr (CLS == CLASS)                                                                                  \
+      row_fast_tile<T, Sch<LEN, PP, __VA_ARGS__>>(g.S, g.Tb, g.Ny, g.lay, g.q, g.tw, g.sl, yi, tile, smem_raw);  \
+    break;
+    PFC_GROUP_SCHEDULES(PFC_X)
+#undef PFC_X
+    default:
+      break;
+  }
+}
+
+template <typename T, int MODE, int CLASS>
+__global__ void __launch_bounds__(kGroupThreads) k_col_group(const GroupMember<T>* __restrict__ mem, int nmem) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  const int lin = xcd_remap(blockIdx.x, gridDim.x);
+  int m = 0;
+  while (m + 1 < nmem && lin >= mem[m + 1].blk0) ++m;
+  const GroupMember<T> g = mem[m];
+  const int local = lin - g.blk0;
+  const int yi = local / g.ntile;
+  const int tile = local - yi * g.ntile;
+  switch (g.kcase) {
+#define PFC_X(CASE, CLS, LEN, PP, ...)                                                                          \
+  case CASE:                                                                                                    \
+    if constexpr (CLS == CLASS)                                                                                 \
+      col_fast_tile<T, Sch<LEN, PP, __VA_ARGS__>, MODE>(g.S, g.Tb, g.D, g.Nc, g.lay, g.tb, g.tw, g.sl, yi, tile, \
+                                                        smem_raw);                                              \
+    break;
+    PFC_GROUP_SCHEDULES(PFC_X)
+#undef PFC_X
+    default:
+      break;
+  }
+}
+
+// Launchers (pfc_inst_group*.hip): kind 0 = row pass, else the column pass of
+// mode kind - 1 (COL_ETD1, COL_RK2_STAGE1, COL_RK2_STAGE2); cls = register class.
+void group_launch_f64(int kind, int cls, int blocks, size_t lds, hipStream_t st, const GroupMember<double>* mem,
+                      int nmem);
+void group_launch_f32(int kind, int cls, int blocks, size_t lds, hipStream_t st, const GroupMember<float>* mem,
+                      int nmem);
 
 }  // namespace pfc

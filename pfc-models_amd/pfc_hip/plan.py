@@ -344,6 +344,47 @@ class Batch:
         return v.value
 
 
+class Group:
+    """Batches of different grid sizes advanced by shared launches.
+
+    `step(n)` advances the active slots of every member batch with one row
+    launch and one column launch per nonlinear evaluation for the whole group
+    (pfc_group_step).  Members share precision and scheme and have Nx/2 and
+    Ny in {32, 48, 64, 96, 128, 192, 256, 384, 512}; they stay usable on
+    their own (set fields, reduce, read).  The group keeps its members alive."""
+
+    def __init__(self, batches):
+        self._lib = _lib.load()
+        self.torch = _torch()
+        self.batches = list(batches)
+        arr = (ctypes.c_void_p * len(self.batches))(*[b._h.value for b in self.batches])
+        h = ctypes.c_void_p()
+        _lib.check(self._lib.pfc_group_create(arr, len(self.batches), ctypes.byref(h)))
+        self._h = h
+
+    def step(self, nsteps=1):
+        _lib.check(self._lib.pfc_group_step(self._h, int(nsteps), current_stream_handle()))
+        return self
+
+    def graph_stats(self):
+        """(step-loop graphs captured, graph replays) so far."""
+        v = (ctypes.c_longlong * 2)()
+        _lib.check(self._lib.pfc_group_graph_stats(self._h, v))
+        return int(v[0]), int(v[1])
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            self.torch.cuda.synchronize()
+            _lib.check(self._lib.pfc_group_destroy(self._h))
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def _diagnose(fn, h, ky, kx, n):
     ky = np.asarray(ky, dtype=np.int32).ravel()
     kx = np.asarray(kx, dtype=np.int32).ravel()

@@ -110,9 +110,16 @@ def run_plain(instances, steps, dt=0.5, precision=64, scheme="etd1", checkpoint=
 
 
 class BatchedSweep:
-    """Instances grouped by grid size, one Batch (and one HIP stream) per group."""
+    """Instances grouped by grid size, one Batch per size.
 
-    def __init__(self, instances, dt=0.5, precision=32, scheme="etd1"):
+    By default each size's Batch steps with its own (autotuned) launches on
+    its own HIP stream, replaying 32-step graphs, so the sizes' launches
+    overlap on the device.  grouped=True advances every size through one
+    Group instead: per step one row and one column launch per register class
+    over every size's tiles (variant-0 schedules); on the 256-instance sweep
+    this measured slower, so it is not the default."""
+
+    def __init__(self, instances, dt=0.5, precision=32, scheme="etd1", grouped=False):
         import torch
         from .plan import Batch
         self.instances = list(instances)
@@ -133,15 +140,24 @@ class BatchedSweep:
                     b.set_field(slot, initial_field(inst))
             self.batches[n], self.streams[n] = b, s
         torch.cuda.synchronize()
+        self.group = None
+        if grouped:
+            from .plan import Group
+            self.group = Group([self.batches[n] for n in sorted(self.batches, reverse=True)])
 
     def step(self, steps):
         import torch
+        if self.group is not None:
+            self.group.step(steps)
+            return
         for n, b in self.batches.items():
             with torch.cuda.stream(self.streams[n]):
                 b.step(steps)
 
     def results(self):
         import torch
+        if self.group is not None:
+            torch.cuda.synchronize()  # the group stepped on the current stream
         m = len(self.instances)
         fe, mean, bad = np.zeros(m), np.zeros(m), np.zeros(m, dtype=np.int32)
         for n, b in self.batches.items():
@@ -154,17 +170,19 @@ class BatchedSweep:
         return fe, mean, bad
 
     def close(self):
+        if self.group is not None:
+            self.group.close()
         for b in self.batches.values():
             b.close()
 
 
-def run_batched(instances, steps, dt=0.5, precision=32, scheme="etd1", chunk=None):
+def run_batched(instances, steps, dt=0.5, precision=32, scheme="etd1", chunk=None, grouped=False):
     """All instances at once; the wall time covers setup (allocation, tables,
     upload, kernel tuning), stepping and the reductions, as run_plain's does."""
     import torch
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    sw = BatchedSweep(instances, dt=dt, precision=precision, scheme=scheme)
+    sw = BatchedSweep(instances, dt=dt, precision=precision, scheme=scheme, grouped=grouped)
     done = 0
     chunk = chunk or steps
     while done < steps:

@@ -49,3 +49,21 @@ def test_coexistence_densities_fp32_vs_fp64(hip_device):
         assert c32[r].status == "ok"
         assert abs(c32[r].phi_a - c64[r].phi_a) <= 1e-4, (r, c32[r], c64[r])
         assert abs(c32[r].phi_b - c64[r].phi_b) <= 1e-4, (r, c32[r], c64[r])
+
+
+def test_grouped_sweep_matches_per_size_batches(hip_device):
+    """The same sweep corner through one Group (shared launches over every
+    size's tiles) and through per-size Batches on their own streams: FP64 free
+    energies within 1e-12 relative (the per-size batches may run other
+    autotuned radix schedules than the group's variant-0 ones), FP32 means to
+    1e-6."""
+    from pfc_hip import sweep as SW
+    inst = SW.sweep_instances(n_r=4, n_psi=4, r_range=(-0.5, -0.2), psi_range=(-0.45, -0.15))
+    for precision in (64, 32):
+        a = SW.run_batched(inst, 200, precision=precision, grouped=True)
+        b = SW.run_batched(inst, 200, precision=precision, grouped=False)
+        assert (a.nonfinite == 0).all()
+        if precision == 64:
+            rel = np.abs(a.fe - b.fe) / np.abs(b.fe)
+            assert rel.max() <= 1e-12, rel
+        assert np.abs(a.mean - b.mean).max() <= 1e-6
