@@ -205,3 +205,30 @@ def coexistence_by_r(instances, fe):
     for inst, f in zip(instances, fe):
         groups.setdefault(inst.r, []).append((inst.psibar, f))
     return {r: std_sweep_coexistence(r, [p for p, _ in g], [f for _, f in g]) for r, g in sorted(groups.items())}
+
+
+def maxwell_lines_by_r(instances, fe, crystal_margin=1e-6, min_points=4):
+    """Maxwell lines of the sweep with r as the temperature: at every r the
+    lower envelope of the analytic liquid and the tabulated crystal branch
+    (the relaxed states at least `crystal_margin` below the liquid; r values
+    with fewer than `min_points` of them have the liquid alone), both over
+    the sweep's psibar range.  Returns a
+    MaxwellLines, whose write_csv gives the rows / metastable / windows tables."""
+    from .coexistence import std_liquid
+    from .phase_diagram.common_tangent import lower_envelope, maxwell_lines, tabulated_branch
+    groups = {}
+    for inst, f in zip(instances, fe):
+        groups.setdefault(inst.r, []).append((inst.psibar, f))
+    envs = {}
+    for r, g in sorted(groups.items()):
+        psi = np.array([p for p, _ in g], dtype=float)
+        f = np.array([v for _, v in g], dtype=float)
+        # the liquid over the sweep's own densities: beyond them no crystal
+        # data exist, and a longer liquid would add domain-limited bridges
+        liq = std_liquid(r, float(psi.min()), float(psi.max()))
+        crystal = f < np.asarray(liq.f(psi)) - crystal_margin
+        branches = [liq]
+        if crystal.sum() >= min_points:
+            branches.append(tabulated_branch("crystal", psi[crystal], f[crystal], min_points=min_points))
+        envs[float(r)] = lower_envelope(branches, temp=float(r))
+    return maxwell_lines(envs)

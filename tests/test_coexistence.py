@@ -87,3 +87,33 @@ def test_std_liquid_and_sweep_coexistence_on_a_model_crystal():
     tri = tabulated_branch("c", psi, fc)
     assert abs(float(liq.mu(t.phi_a)) - float(tri.mu(t.phi_b))) < 1e-12 or \
         abs(float(tri.mu(t.phi_a)) - float(liq.mu(t.phi_b))) < 1e-12
+
+
+def test_sweep_maxwell_lines_from_relaxed_states(tmp_path):
+    """Maxwell lines of a sweep-shaped table (16 psibar per r) whose crystal
+    branch crosses the liquid at psibar = -0.45 with a kink, f_c = f_l -
+    A (psibar + 0.45) above it, A growing with -r: every r gives one stable
+    liquid-crystal row that equals the pairwise tangent, and the CSV holds
+    exactly the rows."""
+    import csv
+    from pfc_hip.sweep import Instance, maxwell_lines_by_r
+    rs = [-0.4, -0.3, -0.2]
+    psis = np.linspace(-0.6, 0.0, 16)
+    inst, fe = [], []
+    for r in rs:
+        liq = std_liquid(r)
+        for p in psis:
+            inst.append(Instance(index=len(inst), r=r, psibar=p, N=64, Lx=1.0, Ly=1.0, seed=0))
+            fe.append(float(liq.f(p)) - 0.2 * (-r) * max(p + 0.45, 0.0))
+    ml = maxwell_lines_by_r(inst, np.array(fe))
+    assert ml.temps == tuple(rs)
+    assert [(row["temp"], row["a"], row["b"]) for row in ml.rows] == [(r, "liquid", "crystal") for r in rs]
+    for r in rs:
+        row = next(x for x in ml.rows if x["temp"] == r)
+        t = std_sweep_coexistence(r, psis, [f for i, f in zip(inst, fe) if i.r == r])
+        assert t.status == "ok" and row["residual"] < 1e-12
+        assert abs(row["phi_a"] - t.phi_a) < 1e-12 and abs(row["phi_b"] - t.phi_b) < 1e-12
+    path = tmp_path / "maxwell.csv"
+    ml.write_csv(str(path))
+    got = list(csv.DictReader(open(path)))
+    assert len(got) == len(ml.rows) and float(got[0]["mu"]) == ml.rows[0]["mu"]
