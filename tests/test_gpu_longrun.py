@@ -61,3 +61,62 @@ def test_std_128_fp32_bulk(hip_device):
     geom = O.build_geometry(128, 128, 32 * np.pi, 32 * np.pi)
     p = O.Params(model="std", epsilon=-0.25, g=0.0, dt=0.5)
     _compare(geom, p, F.uniform_noise((128, 128), -0.25, 0.05, seed=1), 2000, 100, precision=32, fe_tol=1e-6)
+
+
+def test_logpfc_2048_fp32_stabilised_10000_steps(hip_device):
+    """The logPFC 2048^2 workload: builder defaults (Gamma = beta = q0 = eps =
+    g = 1, dt = 0.01, minPhi = 1e-6, models/logpfc/simulation.py:51-75), dx =
+    pi/4, psibar = -0.2 (phi = 0.8) + U(-0.02, 0.02) clipped to psi > -1 +
+    1e-6, ETD1 with the stabilised split (M re-fitted when the phi range
+    drifts by > 10 %), FP32 bulk, 10 000 steps.
+    * The first 20 steps follow the FP64 oracle with the same shift to
+      max|dphi| <= 2e-6 and relative free energy <= 1e-6.
+    * After 10 000 steps: finite, phi > 0, the mean constant (k = 0 pin); the
+      FP64 device reduction of the final field (through an FP64 plan) equals
+      the oracle's free energy of that field to 1e-12; the free energy fell
+      and lies above the uniform liquid's closed form at the realised mean
+      (the linearly stable state this phi relaxes to,
+      models/logpfc_cpu/phase_diagram.py:324-333) by less than 1e-5 relative."""
+    import os
+    from pfc_hip.models.logpfc_hip import simulation as sim_mod
+    from pfc_hip.models.logpfc_hip import phase_diagram as LAD
+    from pfc_hip.plan import Plan
+    n = 2048
+    L = n * np.pi / 4
+    psi0 = np.clip(F.uniform_noise((n, n), -0.2, 0.02, seed=3), -1 + 1e-6, None)
+    phi0 = 1.0 + psi0
+    mean0 = O.exact_mean(phi0)
+    geometry = sim_mod.Build2DGeometry(Nx=n, Ny=n, Lx=L, Ly=L)
+    params = sim_mod.BuildLogPFCModelParameters(meanDensity=mean0)
+    sim = sim_mod.BuildHipGPU_Simulation(params, geometry, precision=32, stabilise=True)
+    sim.State.phi[...] = phi0
+    fe0 = float(sim.CalculateFreeEnergy())
+    sim.run(20)
+    # fitted once from the FP32 field's min and max (not re-fitted: the range shrinks)
+    M = sim.State.M
+    assert abs(M - O.optimal_shift(phi0.min(), phi0.max(), 1.0, 1e-6)) <= 1e-6 * abs(M)
+    geom = O.build_geometry(n, n, L, L)
+    ref = O.Simulation(O.Params(model="log", dt=0.01, clamp_log=True, M=M, pin_mean=mean0), geom, fft="scipy",
+                       workers=min(16, os.cpu_count() or 1))
+    ref.set_field(phi0)
+    ref.step(20)
+    got = sim.State.phi.get()
+    mean20 = sim._plan.reduce()[1]
+    assert np.abs(got - ref.field).max() <= 2e-6
+    assert abs(float(sim.CalculateFreeEnergy()) - ref.free_energy()) <= 1e-6 * abs(ref.free_energy())
+    sim.run(10000 - 20)
+    phi = sim.State.phi.get()
+    assert np.isfinite(phi).all() and phi.min() > 0.0
+    fe, mean, _, _ = sim._plan.reduce()
+    # the k = 0 coefficient is pinned every step to the FP32 rounding of the
+    # initial mean: bit-constant, within half an FP32 ulp of the FP64 mean
+    assert mean == mean20 and abs(mean - mean0) <= 0.5 * np.spacing(np.float32(mean0))
+    pl = Plan(model="log", Nx=n, Ny=n, Lx=L, Ly=L, dt=0.01, clamp_log=True).set_field(phi)
+    fe64 = pl.free_energy()
+    chk = O.Simulation(O.Params(model="log", dt=0.01, clamp_log=True), geom, fft="scipy",
+                       workers=min(16, os.cpu_count() or 1))
+    chk.set_field(phi)
+    assert abs(fe64 - chk.free_energy()) <= 1e-12 * abs(chk.free_energy())
+    liq = LAD.liquid_free_energy(mean, {"beta": 1.0, "g": 1.0, "q0": 1.0, "v0": 1.0}, 1.0)
+    assert fe < fe0
+    assert 0.0 <= fe - liq <= 1e-5 * abs(liq), (fe, liq)
