@@ -253,6 +253,11 @@ def run_sweep(plain_fraction=8):
     import numpy as np
     from pfc_hip import sweep as SW
     inst = SW.sweep_instances()
+    # untimed 10-step runs of each form first: the process loads the sweep
+    # sizes' kernel code and the autotuner caches their schedules; the timed
+    # runs still include allocation, tables, fields, graph capture and results
+    for prec, grouped in ((32, False), (64, False), (32, True)):
+        SW.run_batched(inst, 10, precision=prec, grouped=grouped)
     b32 = SW.run_batched(inst, 5000, precision=32)
     b64 = SW.run_batched(inst, 5000, precision=64)
     # the same sweep through one Group (a row and a column launch per
@@ -279,6 +284,8 @@ def run_sweep(plain_fraction=8):
         "grouped_fp32_solves_per_s": round(g32.solves_per_s, 2),
         "speedup_fp32_vs_plain_fp64": round(b32.solves_per_s / plain.solves_per_s, 2),
         "plain_sample": f"every {plain_fraction}th instance ({len(sub)} solves)",
+        "timing": "whole job (allocation, tables, fields, graph capture, stepping, results) after an untimed "
+                  "10-step run of the same sweep (kernel code loaded, schedules autotuned and cached)",
         "fp32_rel_fe_err": {"median": float(np.median(rel32)), "max": float(rel32.max())},
         "fp32_rel_fe_err_relaxed": {"n": int(relaxed.sum()), "median": float(np.median(rel32[relaxed])),
                                     "max": float(rel32[relaxed].max())},

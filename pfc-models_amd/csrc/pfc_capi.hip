@@ -1283,12 +1283,20 @@ struct pfc_group {
   DevBuf desc;                  // GroupMember<T>[2 passes][2 classes][members]
   std::vector<long long> built; // active counts the descriptors hold
   GroupLaunch rows[kGroupClasses], cols[kGroupClasses];
+  // every active member has its row and column transforms in one class, and
+  // both classes have members: the two classes' launch chains are
+  // independent, and a step graph runs them as two parallel branches
+  bool split = false;
   std::map<std::vector<long long>, hipGraphExec_t> graphs;
-  hipStream_t cap_st = nullptr;
+  hipStream_t cap_st = nullptr, cap_st2 = nullptr;
+  hipEvent_t fork = nullptr, join = nullptr;
   long long captures = 0, replays = 0;
   ~pfc_group() {
     for (auto& kv : graphs) (void)hipGraphExecDestroy(kv.second);
+    if (fork) (void)hipEventDestroy(fork);
+    if (join) (void)hipEventDestroy(join);
     if (cap_st) (void)hipStreamDestroy(cap_st);
+    if (cap_st2) (void)hipStreamDestroy(cap_st2);
   }
 };
 
@@ -1312,6 +1320,14 @@ std::vector<long long> group_counts(const pfc_group& g) {
 template <typename T>
 void group_build(pfc_group& g, hipStream_t st) {
   const size_t n = g.mem.size();
+  g.split = true;
+  for (const pfc_batch* b : g.mem) {
+    if (b->e.nact() == 0) continue;
+    int P = 0, rc = 0, cc = 0;
+    group_case(b->e.p.Nx / 2, &P, &rc);
+    group_case(b->e.p.Ny, &P, &cc);
+    g.split = g.split && rc == cc;
+  }
   std::vector<GroupMember<T>> all(2 * kGroupClasses * n);
   for (int pass = 0; pass < 2; ++pass)
     for (int c = 0; c < kGroupClasses; ++c) {
@@ -1350,6 +1366,7 @@ void group_build(pfc_group& g, hipStream_t st) {
   hip_check(hipStreamSynchronize(st), "sync before group rebuild");
   hip_check(hipMemcpy(g.desc.p, all.data(), all.size() * sizeof(GroupMember<T>), hipMemcpyHostToDevice), "copy");
   g.built = group_counts(g);
+  g.split = g.split && g.rows[0].nmem > 0 && g.rows[1].nmem > 0;
 }
 
 template <typename T>
@@ -1365,28 +1382,50 @@ void group_launch<float>(int kind, int cls, int blocks, size_t lds, hipStream_t 
   group_launch_f32(kind, cls, blocks, lds, st, mem, nmem);
 }
 
-// one pass (kind 0: rows, else the column mode kind - 1) over every class with members
+// one pass (kind 0: rows, else the column mode kind - 1) over class `cls`
+// (-1: every class with members)
 template <typename T>
-void group_pass(pfc_group& g, int kind, hipStream_t st) {
+void group_pass(pfc_group& g, int kind, hipStream_t st, int cls = -1) {
   const auto* base = (const GroupMember<T>*)g.desc.p;
   const GroupLaunch* L = kind == 0 ? g.rows : g.cols;
   for (int c = 0; c < kGroupClasses; ++c)
-    if (L[c].nmem) group_launch<T>(kind, c, L[c].blocks, L[c].lds, st, base + L[c].off, L[c].nmem);
+    if (L[c].nmem && (cls < 0 || cls == c))
+      group_launch<T>(kind, c, L[c].blocks, L[c].lds, st, base + L[c].off, L[c].nmem);
 }
 
 template <typename T>
-void group_loop(pfc_group& g, long long nsteps, hipStream_t st) {
+void group_loop(pfc_group& g, long long nsteps, hipStream_t st, int cls = -1) {
   for (long long s = 0; s < nsteps; ++s) {
-    group_pass<T>(g, 0, st);
+    group_pass<T>(g, 0, st, cls);
     if (g.scheme == PFC_SCHEME_ETD1) {
-      group_pass<T>(g, 1 + COL_ETD1, st);
+      group_pass<T>(g, 1 + COL_ETD1, st, cls);
     } else {
-      group_pass<T>(g, 1 + COL_RK2_STAGE1, st);
-      group_pass<T>(g, 0, st);
-      group_pass<T>(g, 1 + COL_RK2_STAGE2, st);
+      group_pass<T>(g, 1 + COL_RK2_STAGE1, st, cls);
+      group_pass<T>(g, 0, st, cls);
+      group_pass<T>(g, 1 + COL_RK2_STAGE2, st, cls);
     }
   }
   hip_check(hipGetLastError(), "group step launch");
+}
+
+// kGraphSteps steps captured on g.cap_st: with independent classes, class 1's
+// chain forks onto a second stream and joins at the end (two parallel graph
+// branches); else one chain
+template <typename T>
+void group_capture_steps(pfc_group& g) {
+  if (!g.split) {
+    group_loop<T>(g, kGraphSteps, g.cap_st);
+    return;
+  }
+  if (!g.cap_st2) hip_check(hipStreamCreateWithFlags(&g.cap_st2, hipStreamNonBlocking), "hipStreamCreate");
+  if (!g.fork) hip_check(hipEventCreateWithFlags(&g.fork, hipEventDisableTiming), "hipEventCreate");
+  if (!g.join) hip_check(hipEventCreateWithFlags(&g.join, hipEventDisableTiming), "hipEventCreate");
+  hip_check(hipEventRecord(g.fork, g.cap_st), "hipEventRecord");
+  hip_check(hipStreamWaitEvent(g.cap_st2, g.fork, 0), "hipStreamWaitEvent");
+  group_loop<T>(g, kGraphSteps, g.cap_st2, 1);
+  group_loop<T>(g, kGraphSteps, g.cap_st, 0);
+  hip_check(hipEventRecord(g.join, g.cap_st2), "hipEventRecord");
+  hip_check(hipStreamWaitEvent(g.cap_st, g.join, 0), "hipStreamWaitEvent");
 }
 
 template <typename T>
@@ -1412,7 +1451,7 @@ void group_step(pfc_group& g, long long nsteps, hipStream_t st) {
       hip_check(hipStreamBeginCapture(g.cap_st, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
       hipGraph_t gr = nullptr;
       try {
-        group_loop<T>(g, kGraphSteps, g.cap_st);
+        group_capture_steps<T>(g);
       } catch (...) {
         (void)hipStreamEndCapture(g.cap_st, &gr);
         if (gr) (void)hipGraphDestroy(gr);
