@@ -362,6 +362,8 @@ def main():
     ap.add_argument("--no-phase-diagram", action="store_true", help="skip the phase-diagram leg")
     ap.add_argument("--phase-diagram", action="store_true",
                     help="only run the phase-diagram leg and print its JSON line")
+    ap.add_argument("--sweep", action="store_true",
+                    help="only run the 256-instance sweep leg and print its JSON line")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain device copy of known size (counter calibration) and exit")
     a = ap.parse_args()
@@ -373,6 +375,9 @@ def main():
         return
     if a.phase_diagram:
         print(json.dumps({"phase_diagram": run_phase_diagram()}), flush=True)
+        return
+    if a.sweep:
+        print(json.dumps({"sweep": run_sweep()}), flush=True)
         return
     if a.gpus != 1:
         print("bench.py: this build measures one GPU; --gpus above 1 is not supported "
