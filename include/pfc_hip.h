@@ -143,9 +143,11 @@ int pfc_step_profile(pfc_plan* plan, long long nsteps, void* stream, double ms[3
 int pfc_plan_schedule(const pfc_plan* plan, int info[8]);
 /* Step-loop graph counters: out[0] = graphs captured, out[1] = graph replays.
  * A pfc_step / pfc_batch_step call of n >= 32 steps on the two-kernel path
- * replays a HIP graph of 32 steps n / 32 times (captured once per launch
- * configuration) and launches the remaining steps directly; results are
- * bit-identical to direct launches.  PFC_HIP_GRAPH=0 disables the graphs.
+ * replays the HIP graph of 32 steps of its launch configuration n / 32 times
+ * and launches the remaining steps directly; a configuration without a graph
+ * gets one captured by the first call of n >= 256 steps (shorter calls launch
+ * directly).  Results are bit-identical to direct launches.  PFC_HIP_GRAPH=0
+ * disables the graphs.
  * (No reference counterpart: the reference's callers loop TimeStep(),
  * models/stdpfc/simulation.py:213-235, one kernel sequence per step.) */
 int pfc_plan_graph_stats(const pfc_plan* plan, long long out[2]);

@@ -756,9 +756,13 @@ void autotune(Engine& e, hipStream_t st) {
 // step-loop graphs: a call of n >= kGraphSteps steps on the two-kernel path
 // replays a HIP graph of kGraphSteps steps n / kGraphSteps times and runs the
 // rest eagerly, so the host enqueues one graph instead of 2 (ETD1) or 4
-// (ETDRK2) kernels per step.  A graph is captured on a private stream the
-// first time a launch configuration is stepped and kept for the engine's
-// life: its nodes hold the engine's own arrays, the active-list pointer and
+// (ETDRK2) kernels per step.  A graph is captured on a private stream when a
+// call of n >= kGraphCaptureSteps steps meets a launch configuration without
+// one (at least eight replays to pay for the capture and instantiation; a
+// serving loop's short calls on changing active sets never capture), and is
+// replayed by any later call of n >= kGraphSteps with that configuration.
+// Graphs are kept for the engine's life: their nodes hold the engine's own
+// arrays, the active-list pointer and
 // the schedule, and read slot ids, tables, pins and shifts from device
 // memory when they run, so it stays valid as those change.  What changes a
 // node (the active count, the schedule, an external potential's arrays) is in
@@ -766,6 +770,7 @@ void autotune(Engine& e, hipStream_t st) {
 // bit-identical.  PFC_HIP_GRAPH=0 steps eagerly.
 // ---------------------------------------------------------------------------
 constexpr long long kGraphSteps = 32;
+constexpr long long kGraphCaptureSteps = 8 * kGraphSteps;
 constexpr size_t kGraphCacheMax = 16;
 
 bool graphs_enabled() {
@@ -776,12 +781,15 @@ bool graphs_enabled() {
   return on;
 }
 
-hipGraphExec_t step_graph(Engine& e) {
+// the cached graph of the engine's launch configuration; without one,
+// captured when `capture`, else null
+hipGraphExec_t step_graph(Engine& e, bool capture) {
   const std::array<long long, 9> key{e.nact(),          e.frow.on, e.frow.var, e.frow.lines,
                                      e.fcol.on,         e.fcol.var, e.fcol.lines,
                                      (long long)(intptr_t)e.extc.p, (long long)(intptr_t)e.D.p};
   auto it = e.graphs.find(key);
   if (it != e.graphs.end()) return it->second;
+  if (!capture) return nullptr;
   if (e.graphs.size() >= kGraphCacheMax) e.drop_graphs();
   if (!e.cap_st) hip_check(hipStreamCreateWithFlags(&e.cap_st, hipStreamNonBlocking), "hipStreamCreate");
   hip_check(hipStreamBeginCapture(e.cap_st, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
@@ -816,8 +824,10 @@ void engine_step(Engine& e, long long nsteps, hipStream_t st) {
     else
       autotune<float>(e, st);
   }
-  if (!e.resident && e.nact() > 0 && nsteps >= kGraphSteps && graphs_enabled()) {
-    const hipGraphExec_t g = step_graph(e);
+  const hipGraphExec_t g = !e.resident && e.nact() > 0 && nsteps >= kGraphSteps && graphs_enabled()
+                               ? step_graph(e, nsteps >= kGraphCaptureSteps)
+                               : nullptr;
+  if (g) {
     const long long reps = nsteps / kGraphSteps;
     for (long long r = 0; r < reps; ++r) hip_check(hipGraphLaunch(g, st), "hipGraphLaunch");
     e.graph_replays += reps;
@@ -1446,7 +1456,7 @@ void group_step(pfc_group& g, long long nsteps, hipStream_t st) {
     auto it = g.graphs.find(counts);
     if (it != g.graphs.end()) {
       x = it->second;
-    } else {
+    } else if (nsteps >= kGraphCaptureSteps) {
       if (!g.cap_st) hip_check(hipStreamCreateWithFlags(&g.cap_st, hipStreamNonBlocking), "hipStreamCreate");
       hip_check(hipStreamBeginCapture(g.cap_st, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
       hipGraph_t gr = nullptr;
@@ -1464,10 +1474,12 @@ void group_step(pfc_group& g, long long nsteps, hipStream_t st) {
       g.graphs[counts] = x;
       ++g.captures;
     }
-    const long long reps = nsteps / kGraphSteps;
-    for (long long r = 0; r < reps; ++r) hip_check(hipGraphLaunch(x, st), "hipGraphLaunch");
-    g.replays += reps;
-    nsteps -= reps * kGraphSteps;
+    if (x) {
+      const long long reps = nsteps / kGraphSteps;
+      for (long long r = 0; r < reps; ++r) hip_check(hipGraphLaunch(x, st), "hipGraphLaunch");
+      g.replays += reps;
+      nsteps -= reps * kGraphSteps;
+    }
   }
   group_loop<T>(g, nsteps, st);
 }

@@ -1,7 +1,9 @@
 """Step-loop graphs (needs a GPU).
 
-A step call of n >= 32 steps on the two-kernel path replays a captured HIP
-graph of 32 steps n // 32 times and launches the rest directly.  The graph's
+A step call of n >= 32 steps on the two-kernel path replays the captured HIP
+graph of 32 steps of its launch configuration n // 32 times and launches the
+rest directly; a configuration gets its graph from the first call of n >= 256
+steps (shorter calls on a new configuration launch directly).  The graph's
 nodes are the direct launches, so a plan stepped through graphs must equal,
 bit for bit, the same plan stepped in calls of fewer than 32 steps (no graph).
 Each comparison follows several replays with eager work and synchronisations
@@ -33,7 +35,7 @@ def test_plan_graph_bitwise_equal_eager(hip_device, precision, scheme, n):
     psi0 = F.uniform_noise((n, n), -0.3, 0.05, seed=11)
     a = Plan(**kw).set_field(psi0)
     b = Plan(**kw).set_field(psi0)
-    for call, nsteps in enumerate([64, 100, 32, 45]):
+    for call, nsteps in enumerate([100, 300, 100, 32, 45]):
         a.step(nsteps)
         _eager(b, nsteps)
         fa, fb = a.free_energy(), b.free_energy()
@@ -41,7 +43,7 @@ def test_plan_graph_bitwise_equal_eager(hip_device, precision, scheme, n):
         assert np.array_equal(xa, xb), (call, float(np.abs(xa - xb).max()))
         assert fa == fb, call
     cap, rep = a.graph_stats()
-    assert cap == 1 and rep == 2 + 3 + 1 + 1, (cap, rep)
+    assert cap == 1 and rep == 0 + 9 + 3 + 1 + 1, (cap, rep)
     assert b.graph_stats() == (0, 0)
 
 
@@ -68,8 +70,8 @@ def test_batch_graph_follows_tables_and_active_list(hip_device):
         assert np.array_equal(xa, xe), tag
         assert np.array_equal(ra, re_), tag
 
-    a.step(96)
-    _eager(e, 96)
+    a.step(288)
+    _eager(e, 288)
     same("all slots")
     # new tables for slot 2: the same graph reads them
     a.set_slot(2, -0.35)
@@ -78,11 +80,16 @@ def test_batch_graph_follows_tables_and_active_list(hip_device):
     _eager(e, 64)
     same("new epsilon")
     assert a.graph_stats()[0] == 1
-    # a shorter active list changes the launch grid: a second graph
+    # a shorter active list changes the launch grid: a short call launches
+    # directly, a long one captures a second graph
     a.set_active([3, 1])
     e.set_active([3, 1])
     a.step(70)
     _eager(e, 70)
+    same("two active, short call")
+    assert a.graph_stats() == (1, 9 + 2)
+    a.step(300)
+    _eager(e, 300)
     same("two active")
     a.set_active([0, 1, 2, 3])
     e.set_active([0, 1, 2, 3])
@@ -90,4 +97,4 @@ def test_batch_graph_follows_tables_and_active_list(hip_device):
     _eager(e, 32)
     same("all again")
     cap, rep = a.graph_stats()
-    assert cap == 2 and rep == 3 + 2 + 2 + 1, (cap, rep)
+    assert cap == 2 and rep == 9 + 2 + 9 + 1, (cap, rep)
