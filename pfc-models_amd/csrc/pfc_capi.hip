@@ -601,16 +601,18 @@ struct Steps {
     const size_t lds = resident_lds(e.p.Nx, e.p.Ny, sizeof(V));
     static const int threads = [] {
       const char* v = std::getenv("PFC_HIP_RESIDENT_THREADS");
-      return v && std::string(v) == "256" ? 256 : 512;
+      return v && std::atoi(v) == 256 ? 256 : 512;
     }();
     for (long long done = 0; done < nsteps; done += kResidentChunk) {
       const long long n = std::min(kResidentChunk, nsteps - done);
+#define PFC_RESIDENT(BS)                                                                                          \
+  hipLaunchKernelGGL((k_resident_etd1<T, BS>), dim3(ny), dim3(BS), lds, st, S, e.res, nonlin_of(e), tb,          \
+                     (const V*)e.twN.p, (const V*)e.twY.p, sl, n, counts, done)
       if (threads == 512)
-        hipLaunchKernelGGL((k_resident_etd1<T, 512>), dim3(ny), dim3(512), lds, st, S, e.res, nonlin_of(e), tb,
-                           (const V*)e.twN.p, (const V*)e.twY.p, sl, n, counts, done);
+        PFC_RESIDENT(512);
       else
-        hipLaunchKernelGGL((k_resident_etd1<T, 256>), dim3(ny), dim3(256), lds, st, S, e.res, nonlin_of(e), tb,
-                           (const V*)e.twN.p, (const V*)e.twY.p, sl, n, counts, done);
+        PFC_RESIDENT(256);
+#undef PFC_RESIDENT
     }
   }
   static void step(Engine& e, long long nsteps, hipStream_t st) {

@@ -354,6 +354,20 @@ struct ResGeom {
   Radices plM, plY;
 };
 
+// n / d for 0 <= n < 2^20 and 1 <= d < 2^20, from inv, a float reciprocal of
+// d: the estimate n * inv is within 0.25 of the quotient (relative error below
+// 2^-22), so its truncation is the quotient or one off either way, and one
+// correction each way makes it exact.  Index math of the resident stepper
+// runs on runtime lengths, where a 32-bit division is a ~20-instruction
+// sequence; this is five.
+__device__ __forceinline__ int qdiv(int n, int d, float inv) {
+  int q = __float2int_rz((float)n * inv);
+  const int r = n - q * d;
+  q += r >= d ? 1 : 0;
+  q -= r < 0 ? 1 : 0;
+  return q;
+}
+
 // One Stockham pass (sk_pass's arithmetic) over nl lines of length L in
 // LDS: line l starts at (l % ng) * ld + (l / ng) * gs, elements es apart.
 template <typename T, int R, bool INV>
@@ -365,14 +379,15 @@ __device__ __forceinline__ void rs_pass(const V2<T>* __restrict__ src, V2<T>* __
   const int total = nb * nl;
   const int twstep = (L / (Ns * R)) * tws;
   const int rootstep = nb * tws;
+  const float inb = 1.0f / (float)nb, ing = 1.0f / (float)ng, ins = 1.0f / (float)Ns;
   for (int w = threadIdx.x; w < total; w += blockDim.x) {
-    const int line = w / nb;
+    const int line = qdiv(w, nb, inb);
     const int j = w - line * nb;
-    const int grp = line / ng;
+    const int grp = qdiv(line, ng, ing);
     const int off = (line - grp * ng) * ld + grp * gs;
     const V* s = src + off;
     V* d = dst + off;
-    const int k = j % Ns;
+    const int k = j - qdiv(j, Ns, ins) * Ns;
     V v[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) v[r] = s[(j + r * nb) * es];
@@ -456,8 +471,9 @@ __global__ void __launch_bounds__(BS) k_resident_etd1(V2<T>* __restrict__ S, Res
   }
   for (int e = threadIdx.x; e < g.Nx; e += blockDim.x) twn[e] = twN[e];
   for (int e = threadIdx.x; e < Ny; e += blockDim.x) twy[e] = twY[e];
+  const float iM = 1.0f / (float)M, iNc = 1.0f / (float)Nc;
   for (int e = threadIdx.x; e < nsp; e += blockDim.x) {
-    const int y = e / Nc;
+    const int y = qdiv(e, Nc, iNc);
     P0[e] = S[g.lay.at(y, e - y * Nc)];
   }
   __syncthreads();
@@ -469,7 +485,7 @@ __global__ void __launch_bounds__(BS) k_resident_etd1(V2<T>* __restrict__ S, Res
     // projection of the kx = 0 and kx = Nx/2 columns (k_col_etd stores it
     // after its inverse transform; the column pass below reads it here).
     for (int e = threadIdx.x; e < Ny * M; e += blockDim.x) {
-      const int line = e / M;
+      const int line = qdiv(e, M, iM);
       const int k = e - line * M;
       V xk = X[line * Nc + k];
       V xm = X[line * Nc + (M - k)];
@@ -487,7 +503,7 @@ __global__ void __launch_bounds__(BS) k_resident_etd1(V2<T>* __restrict__ S, Res
     __syncthreads();
     V* z = rs_fft<T, true>(Y, Y + nsp, g.plM, Ny, Ny, Nc, 0, 1, twn, 2);
     for (int e = threadIdx.x; e < Ny * M; e += blockDim.x) {
-      const int line = e / M;
+      const int line = qdiv(e, M, iM);
       const int m = e - line * M;
       V v = z[line * Nc + m];
       bad |= !(isfinite(v.x) && isfinite(v.y));
@@ -496,7 +512,7 @@ __global__ void __launch_bounds__(BS) k_resident_etd1(V2<T>* __restrict__ S, Res
     __syncthreads();
     V* Z = rs_fft<T, false>(z, z == Y ? Y + nsp : Y, g.plM, Ny, Ny, Nc, 0, 1, twn, 2);
     for (int e = threadIdx.x; e < nsp; e += blockDim.x) {
-      const int line = e / Nc;
+      const int line = qdiv(e, Nc, iNc);
       X[nsp + e] = row_unpack<T>(Z + line * Nc, e - line * Nc, M, twn);
     }
     __syncthreads();
@@ -505,7 +521,7 @@ __global__ void __launch_bounds__(BS) k_resident_etd1(V2<T>* __restrict__ S, Res
     // place, inverse transform of those Nc columns
     V* W = rs_fft<T, false>(X, Y, g.plY, 2 * Nc, Nc, 1, nsp, Nc, twy, 1);
     for (int e = threadIdx.x; e < nsp; e += blockDim.x) {
-      const int ky = e / Nc;
+      const int ky = qdiv(e, Nc, iNc);
       const int line = e - ky * Nc;
       const int ti = line * g.hy + ky_half(ky, Ny);
       const V a = W[e], b = W[nsp + e];
@@ -525,7 +541,7 @@ __global__ void __launch_bounds__(BS) k_resident_etd1(V2<T>* __restrict__ S, Res
     Y = X == P0 ? P1 : P0;
   }
   for (int e = threadIdx.x; e < nsp; e += blockDim.x) {
-    const int y = e / Nc;
+    const int y = qdiv(e, Nc, iNc);
     const int k = e - y * Nc;
     V v = X[e];
     if (k == 0 || k == Nc - 1) v.y = T(0);
