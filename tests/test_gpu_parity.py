@@ -145,6 +145,39 @@ def test_fp64_1024_etdrk2_matches_oracle(hip_device):
     assert pl.free_energy() == pytest.approx(sim.free_energy(), rel=1e-12)
 
 
+@pytest.mark.parametrize("precision", [64, 32])
+def test_largest_grid_4096_matches_oracle(hip_device, precision):
+    """4096 x 4096, the largest column length the column kernel holds (one
+    4096-point column in LDS: 128 KiB FP64), stdPFC with g != 0, ETDRK2 dt =
+    0.5 from seeded noise: 5 steps against the oracle.  FP64: max|d psi| <=
+    1e-10 and free energy to 1e-12; FP32 bulk: max|d psi| <= 1e-5 and free
+    energy to 1e-6."""
+    import os
+    n = 4096
+    L = n * np.pi / 4
+    geom = O.build_geometry(n, n, L, L)
+    psi0 = F.uniform_noise((n, n), -0.3, 0.05, seed=41)
+    p = O.Params(model="std", epsilon=-0.3, g=0.5, dt=0.5, scheme="etdrk2")
+    sim = O.Simulation(p, geom, fft="scipy", workers=min(16, os.cpu_count() or 1))
+    sim.set_field(psi0)
+    sim.step(5)
+    pl = _plan(geom, precision, epsilon=-0.3, g=0.5, dt=0.5, scheme="etdrk2").set_field(psi0)
+    pl.step(5)
+    err = float(np.abs(pl.get_field(dtype=None).cpu().numpy() - sim.field).max())
+    assert err <= (1e-10 if precision == 64 else 1e-5), err
+    assert pl.free_energy() == pytest.approx(sim.free_energy(), rel=1e-12 if precision == 64 else 1e-6)
+
+
+def test_grid_beyond_the_column_kernel_is_refused(hip_device):
+    """A column longer than the column kernel holds fails at creation with
+    PFC_EUNSUPPORTED, before any allocation (no silent fallback)."""
+    from pfc_hip import _lib
+    n = 8192
+    with pytest.raises(_lib.PfcError) as e:
+        _plan(O.build_geometry(64, n, 16.0, n * 0.8), 64, epsilon=-0.3, g=0.0, dt=0.5)
+    assert e.value.code == _lib.PfcError.UNSUPPORTED
+
+
 @pytest.mark.parametrize("shape", [(128, 128), (96, 192), (256, 384), (64, 2048)])
 @pytest.mark.parametrize("precision", [64, 32])
 def test_fast_kernels_match_generic(hip_device, shape, precision, monkeypatch):
