@@ -88,6 +88,14 @@ int pfc_step(pfc_plan* plan, long long nsteps, void* stream);
  * out[0] = free energy per site (the box mean of f), out[1] = mean,
  * out[2] = min, out[3] = max. */
 int pfc_reduce(pfc_plan* plan, double out[4], void* stream);
+/* Tracker: advance `nsteps` steps and record pfc_reduce's four values after
+ * every `every` steps into host `out` ((nsteps / every) x 4 doubles), the
+ * records written on the device and copied back with one synchronisation at
+ * the end (steps past the last full block run unrecorded).  Replaces the
+ * TimeStep / CalculateFreeEnergy / State.n.mean() loop of the reference's
+ * callers (tests/test_smoke_models.py:104-113; the driver's per-block free
+ * energy, phase_diagram/q0star.py:261-307), which synchronises per record. */
+int pfc_run_tracked(pfc_plan* plan, long long nsteps, long long every, double* out, void* stream);
 /* Rebuild the ETD tables with the stabilising shift M (logPFC). */
 int pfc_set_shift(pfc_plan* plan, double M);
 /* Read and clear the PFC_STATUS_* flags (synchronises `stream`). */
@@ -203,6 +211,8 @@ int pfc_batch_step_counts(pfc_batch* batch, const long long* counts, void* strea
 /* FP64 reductions of the active slots: out[4 i + {0,1,2,3}] = free energy per
  * site, mean, min, max of the i-th active slot.  Synchronises `stream`. */
 int pfc_batch_reduce(pfc_batch* batch, double* out, void* stream);
+/* pfc_run_tracked for the active slots: out[rec][i][4] for the i-th active slot. */
+int pfc_batch_run_tracked(pfc_batch* batch, long long nsteps, long long every, double* out, void* stream);
 /* Read and clear every slot's PFC_STATUS_* flags: flags[capacity]. */
 int pfc_batch_get_status(pfc_batch* batch, int* flags, void* stream);
 int pfc_batch_bytes(const pfc_batch* batch, long long* bytes);

@@ -376,6 +376,7 @@ struct Engine {
   DevBuf twN, twY, twN64, twY64, kx2, ky2, dts;  // kx2 [cap][Nc], ky2 [cap][Ny], dts [cap]
   DevBuf part, out, pin, status, eps, Ms, active, diag, dgi, dgo;  // dgi/dgo: diagnostics entries / results
   DevBuf cnt;         // per-slot step counts (pfc_batch_step_counts)
+  DevBuf track;       // tracker series (engine_run_tracked)
   DevBuf vhat, extc;  // external potential: rfft2(V) and its source term, [cap][Ny][Nc] complex FP64
   std::vector<char> h_ext;  // slots with a potential
   std::vector<double> h_eps, h_M, h_Lx, h_Ly, h_dt;
@@ -816,7 +817,9 @@ hipGraphExec_t step_graph(Engine& e, bool capture) {
   return x;
 }
 
-void engine_step(Engine& e, long long nsteps, hipStream_t st) {
+// `capture`: a launch configuration without a graph may get one (by
+// default when this call alone has enough steps to pay for it)
+void engine_step(Engine& e, long long nsteps, hipStream_t st, int capture = -1) {
   if (nsteps < 0) fail(PFC_EINVAL, "nsteps < 0");
   for (int s : e.h_active)
     if (!e.field_set[s]) fail(PFC_EINVAL, "a stepped slot has no field (pfc_set_field / pfc_batch_set_field)");
@@ -827,7 +830,7 @@ void engine_step(Engine& e, long long nsteps, hipStream_t st) {
       autotune<float>(e, st);
   }
   const hipGraphExec_t g = !e.resident && e.nact() > 0 && nsteps >= kGraphSteps && graphs_enabled()
-                               ? step_graph(e, nsteps >= kGraphCaptureSteps)
+                               ? step_graph(e, capture < 0 ? nsteps >= kGraphCaptureSteps : capture != 0)
                                : nullptr;
   if (g) {
     const long long reps = nsteps / kGraphSteps;
@@ -913,7 +916,7 @@ void kernel_times(Engine& e, long long n, hipStream_t st, double ms[3], long lon
 // pin_out (device, per slot) receives the means.
 template <typename T>
 void launch_reduce(Engine& e, const void* Sbase, const double* eps, const double* kx2, const double* ky2,
-                   hipStream_t st, const Slots& sl, int ny, double* pin_out) {
+                   hipStream_t st, const Slots& sl, int ny, double* pin_out, double* out = nullptr) {
   using V = V2<T>;
   const V* S = (const V*)Sbase;
   FeParams fp{e.p.model, e.p.beta, e.p.q0, e.p.epsilon, e.p.g, kx2, ky2, eps};
@@ -926,8 +929,8 @@ void launch_reduce(Engine& e, const void* Sbase, const double* eps, const double
                      part, sl, ps);
   hipLaunchKernelGGL((k_fe_col<T>), dim3(cb, ny), dim3(256), cl, st, S, e.cg64, fp, e.p.Nx,
                      (const V2<double>*)e.twY64.p, part + rb, sl, ps);
-  hipLaunchKernelGGL(k_fe_final, dim3(ny), dim3(256), 0, st, part, rb, cb, e.p.Nx, e.p.Ny, (double*)e.out.p, pin_out,
-                     sl, ps);
+  hipLaunchKernelGGL(k_fe_final, dim3(ny), dim3(256), 0, st, part, rb, cb, e.p.Nx, e.p.Ny,
+                     out ? out : (double*)e.out.p, pin_out, sl, ps);
   hip_check(hipGetLastError(), "reduce launch");
 }
 
@@ -1207,6 +1210,41 @@ void engine_reduce_one(Engine& e, int slot, double out[4], hipStream_t st) {
   hip_check(hipMemcpyAsync(e.host_out, e.out.p, 4 * sizeof(double), hipMemcpyDeviceToHost, st), "copy");
   hip_check(hipStreamSynchronize(st), "sync");
   std::memcpy(out, e.host_out, 4 * sizeof(double));
+}
+
+// Tracker: `nsteps` steps of the active slots with the FP64 reductions
+// (free energy per site, mean, min, max) of every active slot written to a
+// device series after each `every` steps, then one copy to the host and one
+// synchronisation: out[rec][i][4] for rec < nsteps / every and the i-th
+// active slot.  Steps past the last full block run without a record.
+void engine_run_tracked(Engine& e, long long nsteps, long long every, double* out, hipStream_t st) {
+  if (nsteps < 0) fail(PFC_EINVAL, "nsteps < 0");
+  if (every < 1) fail(PFC_EINVAL, "every must be >= 1");
+  if (!out && nsteps >= every) fail(PFC_EINVAL, "NULL out");
+  const int ny = e.nact();
+  const long long nrec = nsteps / every;
+  const size_t bytes = (size_t)nrec * ny * 4 * sizeof(double);
+  if (e.track.n < bytes) {
+    hip_check(hipStreamSynchronize(st), "sync before tracker growth");
+    e.track.alloc(bytes);
+  }
+  // graphs pay for their capture over the whole call, not one block
+  const int capture = nsteps >= kGraphCaptureSteps ? 1 : 0;
+  const Slots sl = e.slots_active();
+  for (long long r = 0; r < nrec; ++r) {
+    engine_step(e, every, st, capture);
+    if (ny == 0) continue;
+    double* dst = (double*)e.track.p + (size_t)r * ny * 4;
+    if (e.f64())
+      launch_reduce<double>(e, e.S.p, (const double*)e.eps.p, (const double*)e.kx2.p, (const double*)e.ky2.p, st,
+                            sl, ny, nullptr, dst);
+    else
+      launch_reduce<float>(e, e.S.p, (const double*)e.eps.p, (const double*)e.kx2.p, (const double*)e.ky2.p, st,
+                           sl, ny, nullptr, dst);
+  }
+  engine_step(e, nsteps - nrec * every, st, capture);
+  if (bytes) hip_check(hipMemcpyAsync(out, e.track.p, bytes, hipMemcpyDeviceToHost, st), "copy");
+  hip_check(hipStreamSynchronize(st), "sync");
 }
 
 void engine_status(Engine& e, int* flags, hipStream_t st) {
@@ -1625,6 +1663,20 @@ int pfc_plan_schedule(const pfc_plan* pl, int info[8]) {
       info[4 * k + 2] = fs[k]->P;
       info[4 * k + 3] = fs[k]->lines;
     }
+  });
+}
+
+int pfc_run_tracked(pfc_plan* pl, long long nsteps, long long every, double* out, void* stream) {
+  return guarded([&] {
+    if (!pl) fail(PFC_EINVAL, "NULL plan");
+    engine_run_tracked(pl->e, nsteps, every, out, reinterpret_cast<hipStream_t>(stream));
+  });
+}
+
+int pfc_batch_run_tracked(pfc_batch* b, long long nsteps, long long every, double* out, void* stream) {
+  return guarded([&] {
+    if (!b) fail(PFC_EINVAL, "NULL batch");
+    engine_run_tracked(b->e, nsteps, every, out, reinterpret_cast<hipStream_t>(stream));
   });
 }
 

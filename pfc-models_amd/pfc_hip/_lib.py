@@ -64,6 +64,9 @@ _SIGNATURES = [
                                         ctypes.POINTER(ctypes.c_longlong)]),
     ("pfc_plan_schedule", ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_int)]),
     ("pfc_plan_graph_stats", ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_longlong)]),
+    ("pfc_run_tracked", ctypes.c_int, [_P, ctypes.c_longlong, ctypes.c_longlong, ctypes.POINTER(ctypes.c_double), _P]),
+    ("pfc_batch_run_tracked", ctypes.c_int, [_P, ctypes.c_longlong, ctypes.c_longlong, ctypes.POINTER(ctypes.c_double),
+                                             _P]),
     ("pfc_group_create", ctypes.c_int, [ctypes.POINTER(_P), ctypes.c_int, ctypes.POINTER(_P)]),
     ("pfc_group_destroy", ctypes.c_int, [_P]),
     ("pfc_group_step", ctypes.c_int, [_P, ctypes.c_longlong, _P]),

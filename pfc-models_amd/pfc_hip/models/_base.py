@@ -234,19 +234,35 @@ class Simulation:
     def TimeStep(self):
         self.run(1)
 
-    def run(self, nsteps):
-        """Advance `nsteps` steps in one asynchronous call (t advances by nsteps*dt)."""
+    def run(self, nsteps, record_every=None):
+        """Advance `nsteps` steps in one asynchronous call (t advances by nsteps*dt).
+
+        With `record_every`, returns an (nsteps // record_every, 4) FP64 array of
+        (free energy per site, mean, min, max) after every `record_every` steps,
+        recorded on the device and synchronised once (the stabilised logPFC,
+        which checks its phi range on the host every step, records block by
+        block)."""
         nsteps = int(nsteps)
+        rec = None
         if self.stabilise:
-            for _ in range(nsteps):
+            rows = []
+            for i in range(nsteps):
                 self.ValidatePhiRange()
                 self._plan.step(1)
                 self.State.t += self.Model.dt
+                if record_every and (i + 1) % int(record_every) == 0:
+                    rows.append(self._plan.reduce())
+            if record_every:
+                rec = np.array(rows, dtype=np.float64).reshape(-1, 4)
         else:
-            self._plan.step(nsteps)
+            if record_every:
+                rec = self._plan.run_tracked(nsteps, int(record_every))
+            else:
+                self._plan.step(nsteps)
             for _ in range(nsteps):  # t accumulates exactly as a TimeStep loop does
                 self.State.t += self.Model.dt
         self.State.mu = self.State.local_mu = self.mu = None
+        return rec
 
     # -- reference batches kept for callers that drive them directly ----------
     def _Batch1_CalcNonlinMu(self):

@@ -155,6 +155,17 @@ class Plan:
         _lib.check(self._lib.pfc_reduce(self._h, self._out, current_stream_handle()))
         return tuple(float(v) for v in self._out)
 
+    def run_tracked(self, nsteps, every):
+        """Advance `nsteps` steps, recording (free energy per site, mean, min,
+        max) in FP64 after every `every` steps: an (nsteps // every, 4) array,
+        synchronised once."""
+        nrec = int(nsteps) // int(every) if int(every) > 0 else 0
+        out = np.zeros((max(nrec, 1), 4), dtype=np.float64)
+        _lib.check(self._lib.pfc_run_tracked(self._h, int(nsteps), int(every),
+                                             out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                             current_stream_handle()))
+        return out[:nrec]
+
     def free_energy(self):
         return self.reduce()[0]
 
@@ -326,6 +337,18 @@ class Batch:
         out = (ctypes.c_double * max(4, 4 * n))()
         _lib.check(self._lib.pfc_batch_reduce(self._h, out, current_stream_handle()))
         return np.array(out[: 4 * n], dtype=np.float64).reshape(n, 4)
+
+    def run_tracked(self, nsteps, every):
+        """Advance the active slots `nsteps` steps, recording each one's (free
+        energy per site, mean, min, max) after every `every` steps: an
+        (nsteps // every, n_active, 4) FP64 array, synchronised once."""
+        nrec = int(nsteps) // int(every) if int(every) > 0 else 0
+        n = len(self._active)
+        out = np.zeros((max(nrec, 1), max(n, 1), 4), dtype=np.float64)
+        _lib.check(self._lib.pfc_batch_run_tracked(self._h, int(nsteps), int(every),
+                                                   out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                                   current_stream_handle()))
+        return out[:nrec, :n]
 
     def status(self):
         v = (ctypes.c_int * self.capacity)()
