@@ -974,11 +974,19 @@ __global__ void __launch_bounds__(256) k_fe_row(const V2<Tin>* __restrict__ S, R
   extern __shared__ __align__(16) unsigned char smem_raw[];
   V* b0 = reinterpret_cast<V*>(smem_raw);
   V* b1 = b0 + g.nl * g.ld;
-  const int slot = slot_of(sl, blockIdx.y);
+  // XCD-aware order over (instance, row block): the blocks sharing a
+  // line-tiled row group's 128-byte lines meet in one XCD's L2.  A block's
+  // partial sums still go to its row block's index, so the fixed reduction
+  // order (and the result) does not depend on which block ran it.
+  const int ntile = gridDim.x;
+  const int lin = xcd_remap(blockIdx.x + blockIdx.y * ntile, ntile * gridDim.y);
+  const int yi = lin / ntile;
+  const int tile = lin - yi * ntile;
+  const int slot = slot_of(sl, yi);
   S += (size_t)slot * sl.sstride;
   if (p.eps) p.epsilon = p.eps[slot];
-  part += (size_t)blockIdx.y * pstride;
-  const int row0 = blockIdx.x * g.nl;
+  part += (size_t)yi * pstride;
+  const int row0 = tile * g.nl;
   const int nrows = min(g.nl, g.Ny - row0);
   row_load_pack<double, Tin>(S, b1, b0, g, row0, nrows, twN);
   V* z = fft_lines<double, true>(b0, b1, g.plM, nrows, g.ld, twN, 2);
@@ -998,7 +1006,7 @@ __global__ void __launch_bounds__(256) k_fe_row(const V2<Tin>* __restrict__ S, R
     mine.mx = fmax(mine.mx, fmax(v.x, v.y));
   }
   __syncthreads();
-  block_reduce_partial(mine, reinterpret_cast<Partial*>(smem_raw), part + blockIdx.x);
+  block_reduce_partial(mine, reinterpret_cast<Partial*>(smem_raw), part + tile);
 }
 
 // columns of S~ -> A = n_hat/Nx in FP64 -> sum over the half spectrum of
@@ -1011,12 +1019,17 @@ __global__ void __launch_bounds__(256) k_fe_col(const V2<Tin>* __restrict__ S, C
   extern __shared__ __align__(16) unsigned char smem_raw[];
   V* b0 = reinterpret_cast<V*>(smem_raw);
   V* b1 = b0 + g.nc * g.ld;
-  const int slot = slot_of(sl, blockIdx.y);
+  // XCD-aware order, partials at the column block's index (as in k_fe_row)
+  const int ntile = gridDim.x;
+  const int lin = xcd_remap(blockIdx.x + blockIdx.y * ntile, ntile * gridDim.y);
+  const int yi = lin / ntile;
+  const int tile = lin - yi * ntile;
+  const int slot = slot_of(sl, yi);
   S += (size_t)slot * sl.sstride;
   const double* __restrict__ kx2 = p.kx2 + (size_t)slot * g.Nc;
   const double* __restrict__ ky2 = p.ky2 + (size_t)slot * g.Ny;
-  part += (size_t)blockIdx.y * pstride;
-  const int c0 = blockIdx.x * g.nc;
+  part += (size_t)yi * pstride;
+  const int c0 = tile * g.nc;
   const int ncols = min(g.nc, g.Nc - c0);
   for (int e = threadIdx.x; e < ncols * g.Ny; e += blockDim.x) {
     const int y = e / ncols;
@@ -1042,7 +1055,7 @@ __global__ void __launch_bounds__(256) k_fe_col(const V2<Tin>* __restrict__ S, C
     mine.s[2] = dd_add(mine.s[2], w * lf * (a.x * a.x + a.y * a.y));
   }
   __syncthreads();
-  block_reduce_partial(mine, reinterpret_cast<Partial*>(smem_raw), part + blockIdx.x);
+  block_reduce_partial(mine, reinterpret_cast<Partial*>(smem_raw), part + tile);
 }
 
 // one block per active instance (blockIdx.x indexes the active list):
