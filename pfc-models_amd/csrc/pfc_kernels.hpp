@@ -158,6 +158,14 @@ __device__ __forceinline__ V2<T> row_unpack_pair(V2<T> zk, V2<T> zm, V2<T> w) {
   return V{T(0.5) * (s.x + wd.y), T(0.5) * (s.y - wd.x)};
 }
 
+__device__ __forceinline__ int xcd_remap(int b, int nb) {
+  // Workgroups are dealt round-robin to the 8 XCDs; give each XCD a contiguous
+  // run of tiles so neighbouring columns share that XCD's L2.
+  const int q = nb >> 3, r = nb & 7;
+  const int x = b & 7, l = b >> 3;
+  return x < r ? x * (q + 1) + l : r * (q + 1) + (x - r) * q + l;
+}
+
 // ---------------------------------------------------------------------------
 // row kernel: S~ -> n -> N(n) -> T
 // ---------------------------------------------------------------------------
@@ -168,11 +176,16 @@ __global__ void __launch_bounds__(256) k_row_nonlin(const V2<T>* __restrict__ S,
   extern __shared__ __align__(16) unsigned char smem_raw[];
   V* b0 = reinterpret_cast<V*>(smem_raw);
   V* b1 = b0 + g.nl * g.ld;
-  const int slot = slot_of(sl, blockIdx.y);
+  // XCD-aware order over (instance, tile), as in the fast kernels
+  const int ntile = gridDim.x;
+  const int lin = xcd_remap(blockIdx.x + blockIdx.y * ntile, ntile * gridDim.y);
+  const int yi = lin / ntile;
+  const int tile = lin - yi * ntile;
+  const int slot = slot_of(sl, yi);
   S += (size_t)slot * sl.sstride;
   Tout += (size_t)slot * sl.sstride;
   if (sl.M) q.M = sl.M[slot];
-  const int row0 = blockIdx.x * g.nl;
+  const int row0 = tile * g.nl;
   const int nrows = min(g.nl, g.Ny - row0);
   row_load_pack<T, T>(S, b1, b0, g, row0, nrows, twN);
   V* z = fft_lines<T, true>(b0, b1, g.plM, nrows, g.ld, twN, 2);
@@ -219,13 +232,6 @@ struct ColTables {
 
 constexpr int kColRegs = 16;  // A values held per thread between the two forward FFTs
 
-__device__ __forceinline__ int xcd_remap(int b, int nb) {
-  // Workgroups are dealt round-robin to the 8 XCDs; give each XCD a contiguous
-  // run of tiles so neighbouring columns share that XCD's L2.
-  const int q = nb >> 3, r = nb & 7;
-  const int x = b & 7, l = b >> 3;
-  return x < r ? x * (q + 1) + l : r * (q + 1) + (x - r) * q + l;
-}
 
 template <typename T>
 __device__ __forceinline__ void col_load(const V2<T>* __restrict__ src, V2<T>* buf, const ColGeom& g, int c0,
@@ -643,9 +649,14 @@ __global__ void __launch_bounds__(256) k_row_r2c(const Tin* __restrict__ x, V2<T
   extern __shared__ __align__(16) unsigned char smem_raw[];
   V* b0 = reinterpret_cast<V*>(smem_raw);
   V* b1 = b0 + g.nl * g.ld;
-  x += blockIdx.y * in_bstride;
-  S += blockIdx.y * out_bstride;
-  const int row0 = blockIdx.x * g.nl;
+  // XCD-aware order over (instance, tile), as in the fast kernels
+  const int ntile = gridDim.x;
+  const int lin = xcd_remap(blockIdx.x + blockIdx.y * ntile, ntile * gridDim.y);
+  const int yi = lin / ntile;
+  const int tile = lin - yi * ntile;
+  x += yi * in_bstride;
+  S += yi * out_bstride;
+  const int row0 = tile * g.nl;
   const int nrows = min(g.nl, g.Ny - row0);
   for (int e = threadIdx.x; e < nrows * g.M; e += blockDim.x) {
     const int line = e / g.M;
@@ -672,9 +683,14 @@ __global__ void __launch_bounds__(256) k_row_c2r(const V2<T>* __restrict__ S, To
   extern __shared__ __align__(16) unsigned char smem_raw[];
   V* b0 = reinterpret_cast<V*>(smem_raw);
   V* b1 = b0 + g.nl * g.ld;
-  S += (size_t)(list ? list[blockIdx.y] : (int)blockIdx.y) * in_bstride;  // input instance: list entry or index
-  x += blockIdx.y * out_bstride;
-  const int row0 = blockIdx.x * g.nl;
+  // XCD-aware order over (instance, tile), as in the fast kernels
+  const int ntile = gridDim.x;
+  const int lin = xcd_remap(blockIdx.x + blockIdx.y * ntile, ntile * gridDim.y);
+  const int yi = lin / ntile;
+  const int tile = lin - yi * ntile;
+  S += (size_t)(list ? list[yi] : yi) * in_bstride;  // input instance: list entry or index
+  x += yi * out_bstride;
+  const int row0 = tile * g.nl;
   const int nrows = min(g.nl, g.Ny - row0);
   row_load_pack<T, T>(S, b1, b0, g, row0, nrows, twN);
   V* z = fft_lines<T, true>(b0, b1, g.plM, nrows, g.ld, twN, 2);
@@ -696,8 +712,13 @@ __global__ void __launch_bounds__(256) k_col_fft(V2<T>* __restrict__ S, ColGeom 
   extern __shared__ __align__(16) unsigned char smem_raw[];
   V* b0 = reinterpret_cast<V*>(smem_raw);
   V* b1 = b0 + g.nc * g.ld;
-  S += blockIdx.y * bstride;
-  const int c0 = blockIdx.x * g.nc;
+  // XCD-aware order over (instance, tile), as in the fast kernels
+  const int ntile = gridDim.x;
+  const int lin = xcd_remap(blockIdx.x + blockIdx.y * ntile, ntile * gridDim.y);
+  const int yi = lin / ntile;
+  const int tile = lin - yi * ntile;
+  S += yi * bstride;
+  const int c0 = tile * g.nc;
   const int ncols = min(g.nc, g.Nc - c0);
   col_load<T>(S, b0, g, c0, ncols);
   __syncthreads();
@@ -735,7 +756,12 @@ __global__ void __launch_bounds__(256) k_col_filter(const V2<T>* __restrict__ S,
   extern __shared__ __align__(16) unsigned char smem_raw[];
   V* b0 = reinterpret_cast<V*>(smem_raw);
   V* b1 = b0 + g.nc * g.ld;
-  const int c0 = blockIdx.x * g.nc;
+  // XCD-aware order over (instance, tile), as in the fast kernels
+  const int ntile = gridDim.x;
+  const int lin = xcd_remap(blockIdx.x + blockIdx.y * ntile, ntile * gridDim.y);
+  const int yi = lin / ntile;
+  const int tile = lin - yi * ntile;
+  const int c0 = tile * g.nc;
   const int ncols = min(g.nc, g.Nc - c0);
   const int nel = ncols * g.Ny;
   V a[kColRegs];
