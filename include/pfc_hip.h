@@ -98,6 +98,21 @@ int pfc_reduce(pfc_plan* plan, double out[4], void* stream);
 int pfc_run_tracked(pfc_plan* plan, long long nsteps, long long every, double* out, void* stream);
 /* Rebuild the ETD tables with the stabilising shift M (logPFC). */
 int pfc_set_shift(pfc_plan* plan, double M);
+/* Stabilising shift managed on the device (logPFC; on = 1): before every
+ * step the library takes the field's min and max (pfc_reduce's), and when the
+ * slot has no fit yet, or the min fell below the fitted floor lo by more than
+ * floor_tol (relative) or the max rose above the fitted ceiling hi by more than
+ * ceil_tol, refits lo = (min > 0 ? min : minPhi), hi = max,
+ * M = (N(hi) - N(lo)) / (hi - lo), N(phi) = g phi^2 + phi^3 + ln phi, and
+ * rebuilds that slot's ETD tables, all on the stream without a host
+ * synchronisation.  Replaces logpfc_extv's per-step ValidatePhiRange /
+ * OptimizeM (models/logpfc_extv/simulation.py:332-405: two device-to-host
+ * copies and a host table rebuild per step; tolerances 0.1 there).  A new
+ * field or slot parameters make the next step refit.  on = 0 hands M back to
+ * pfc_set_shift.  Not with groups or per-slot step counts. */
+int pfc_set_auto_shift(pfc_plan* plan, int on, double floor_tol, double ceil_tol, void* stream);
+/* out = {M, lo, hi} (lo = hi = NaN before the first fit); synchronises. */
+int pfc_get_shift(pfc_plan* plan, double out[3], void* stream);
 /* Read and clear the PFC_STATUS_* flags (synchronises `stream`). */
 int pfc_get_status(pfc_plan* plan, int* flags, void* stream);
 /* Chemical potential of the current field, real (Ny, Nx) device outputs:
@@ -211,6 +226,9 @@ int pfc_batch_step_counts(pfc_batch* batch, const long long* counts, void* strea
 /* FP64 reductions of the active slots: out[4 i + {0,1,2,3}] = free energy per
  * site, mean, min, max of the i-th active slot.  Synchronises `stream`. */
 int pfc_batch_reduce(pfc_batch* batch, double* out, void* stream);
+/* pfc_set_auto_shift / pfc_get_shift for every slot of a batch / one slot. */
+int pfc_batch_set_auto_shift(pfc_batch* batch, int on, double floor_tol, double ceil_tol, void* stream);
+int pfc_batch_get_shift(pfc_batch* batch, int slot, double out[3], void* stream);
 /* pfc_run_tracked for the active slots: out[rec][i][4] for the i-th active slot. */
 int pfc_batch_run_tracked(pfc_batch* batch, long long nsteps, long long every, double* out, void* stream);
 /* Read and clear every slot's PFC_STATUS_* flags: flags[capacity]. */

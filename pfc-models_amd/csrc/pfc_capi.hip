@@ -377,6 +377,11 @@ struct Engine {
   DevBuf part, out, pin, status, eps, Ms, active, diag, dgi, dgo;  // dgi/dgo: diagnostics entries / results
   DevBuf cnt;         // per-slot step counts (pfc_batch_step_counts)
   DevBuf track;       // tracker series (engine_run_tracked)
+  // stabilising shift managed on the device (pfc_set_auto_shift): per slot
+  // ShiftState and a table-rebuild flag, checked before every step
+  bool auto_shift = false;
+  ShiftParams shp{};
+  DevBuf shst, rebuildf;
   DevBuf vhat, extc;  // external potential: rfft2(V) and its source term, [cap][Ny][Nc] complex FP64
   std::vector<char> h_ext;  // slots with a potential
   std::vector<double> h_eps, h_M, h_Lx, h_Ly, h_dt;
@@ -386,7 +391,7 @@ struct Engine {
   int* host_status = nullptr;   // pinned [cap]
   // step-loop graphs: kGraphSteps steps of the row/column launches, captured
   // per launch configuration (see engine_step)
-  std::map<std::array<long long, 9>, hipGraphExec_t> graphs;
+  std::map<std::array<long long, 10>, hipGraphExec_t> graphs;
   hipStream_t cap_st = nullptr;  // private stream the graphs are captured on
   long long graph_replays = 0, graph_captures = 0;
   void drop_graphs() {
@@ -557,6 +562,41 @@ Nonlin nonlin_of(const Engine& e) {
   return q;
 }
 
+// The device-managed stabilising shift, before a step: the min and max of
+// every active slot's field (k_fe_row<MINMAX>, the arithmetic of
+// pfc_reduce's), the range check and refit (k_shift_update), and the ETD
+// tables (and external-potential source) of the refitted slots only.  No host
+// synchronisation; the launches can be recorded in a step graph.
+template <typename T>
+void shift_check(Engine& e, hipStream_t st) {
+  const int ny = e.nact();
+  if (ny == 0) return;
+  const Slots sl = e.slots_active();
+  const FeParams fp{e.p.model, e.p.beta, e.p.q0, e.p.epsilon, e.p.g, (const double*)e.kx2.p,
+                    (const double*)e.ky2.p, (const double*)e.eps.p};
+  Partial* part = (Partial*)e.part.p;
+  const int rb = row_blocks(e.rg64), ps = e.pstride();
+  const size_t rl = std::max(row_lds(e.rg64, sizeof(V2<double>)), 256 * sizeof(Partial));
+  hipLaunchKernelGGL((k_fe_row<T, true>), dim3(rb, ny), dim3(256), rl, st, (const V2<T>*)e.S.p, e.rg64, fp,
+                     (const V2<double>*)e.twN64.p, part, sl, ps);
+  hipLaunchKernelGGL(k_shift_update, dim3(ny), dim3(256), 0, st, (const Partial*)part, rb, sl, ps, e.shp,
+                     (ShiftState*)e.shst.p, (double*)e.Ms.p, (int*)e.rebuildf.p);
+  TabParams tp{e.p.Gamma, e.p.beta, e.p.q0, 0.0, 0.0, e.p.dt, e.p.smallZTol,
+               1.0 / e.p.Ny, 1.0 / ((double)e.p.Nx * (double)e.p.Ny), e.Nc, e.p.Ny, table_rows(e.p.Ny)};
+  const SlotGrid sg{(const double*)e.kx2.p, (const double*)e.ky2.p, (const double*)e.dts.p};
+  const int blocks = (int)std::min<size_t>((e.ntab + 255) / 256, 1024);
+  hipLaunchKernelGGL(k_tables<double>, dim3(blocks, ny), dim3(256), 0, st, (double*)e.E1.p, (double*)e.Q1.p,
+                     (double*)e.Q2.p, (long long)e.ntab, sg, tp, (const double*)e.eps.p, (const double*)e.Ms.p, 0,
+                     (const int*)e.active.p, (const int*)e.rebuildf.p);
+  if (e.has_ext()) {
+    const long long cs = (long long)e.Nc * e.p.Ny;
+    const int cb = (int)std::min<long long>((cs + 255) / 256, 1024);
+    hipLaunchKernelGGL(k_ext_table, dim3(cb, ny), dim3(256), 0, st, (V2<double>*)e.extc.p,
+                       (const V2<double>*)e.vhat.p, cs, sg, tp, (const double*)e.eps.p, (const double*)e.Ms.p, 0,
+                       (const int*)e.active.p, (const int*)e.rebuildf.p);
+  }
+}
+
 // one nonlinear row pass and one column pass (MODE) over `ny` instances, on
 // the engine's arrays or on caller-given ones of the same shape
 template <typename T>
@@ -621,7 +661,14 @@ struct Steps {
     if (ny == 0) return;
     const Slots sl = e.slots_active();
     if (e.resident) {
-      resident(e, nsteps, st, sl, ny, (V*)e.S.p);
+      if (e.auto_shift) {
+        for (long long s = 0; s < nsteps; ++s) {
+          shift_check<T>(e, st);
+          resident(e, 1, st, sl, ny, (V*)e.S.p);
+        }
+      } else {
+        resident(e, nsteps, st, sl, ny, (V*)e.S.p);
+      }
       hip_check(hipGetLastError(), "step launch");
       return;
     }
@@ -632,6 +679,7 @@ struct Steps {
     const int ny = e.nact();
     const Slots sl = e.slots_active();
     for (long long s = 0; s < nsteps; ++s) {
+      if (e.auto_shift) shift_check<T>(e, st);
       row(e, st, sl, ny);
       if (e.p.scheme == PFC_SCHEME_ETD1) {
         col(e, COL_ETD1, st, sl, ny);
@@ -787,9 +835,9 @@ bool graphs_enabled() {
 // the cached graph of the engine's launch configuration; without one,
 // captured when `capture`, else null
 hipGraphExec_t step_graph(Engine& e, bool capture) {
-  const std::array<long long, 9> key{e.nact(),          e.frow.on, e.frow.var, e.frow.lines,
-                                     e.fcol.on,         e.fcol.var, e.fcol.lines,
-                                     (long long)(intptr_t)e.extc.p, (long long)(intptr_t)e.D.p};
+  const std::array<long long, 10> key{e.nact(),          e.frow.on, e.frow.var, e.frow.lines,
+                                      e.fcol.on,         e.fcol.var, e.fcol.lines,
+                                      (long long)(intptr_t)e.extc.p, (long long)(intptr_t)e.D.p, e.auto_shift};
   auto it = e.graphs.find(key);
   if (it != e.graphs.end()) return it->second;
   if (!capture) return nullptr;
@@ -850,6 +898,7 @@ void engine_step_counts(Engine& e, const long long* counts, hipStream_t st) {
   if (n == 0) return;
   if (!counts) fail(PFC_EINVAL, "NULL counts");
   if (!e.resident) fail(PFC_EUNSUPPORTED, "per-slot step counts need the resident stepper (small ETD1 grids)");
+  if (e.auto_shift) fail(PFC_EUNSUPPORTED, "per-slot step counts with a device-managed shift");
   for (int s : e.h_active)
     if (!e.field_set[s]) fail(PFC_EINVAL, "a stepped slot has no field (pfc_set_field / pfc_batch_set_field)");
   long long most = 0;
@@ -948,6 +997,13 @@ void reduce_slot(Engine& e, int slot, hipStream_t st, bool set_pin) {
     launch_reduce<float>(e, S, eps, kx2, ky2, st, one, 1, pin_out);
 }
 
+// a new field or new slot parameters: the device-managed shift refits at the
+// next step (the host-driven split resets its window on a new field too)
+void reset_shift_fit(Engine& e, int slot, hipStream_t st) {
+  if (!e.auto_shift) return;
+  hip_check(hipMemsetAsync((ShiftState*)e.shst.p + slot, 0, sizeof(ShiftState), st), "memset");
+}
+
 void engine_set_field(Engine& e, int slot, const void* field, int dtype, hipStream_t st) {
   e.check_slot(slot);
   if (!field) fail(PFC_EINVAL, "NULL field");
@@ -977,6 +1033,7 @@ void engine_set_field(Engine& e, int slot, const void* field, int dtype, hipStre
   hip_check(hipGetLastError(), "set_field launch");
   reduce_slot(e, slot, st, true);
   e.field_set[slot] = 1;
+  reset_shift_fit(e, slot, st);
 }
 
 void engine_get_field(Engine& e, int slot, void* field, int dtype, hipStream_t st) {
@@ -1080,8 +1137,46 @@ void engine_set_slot(Engine& e, int slot, double epsilon, double M, hipStream_t 
   hip_check(hipMemcpyAsync((double*)e.eps.p + slot, &e.h_eps[slot], sizeof(double), hipMemcpyHostToDevice, st), "copy");
   hip_check(hipMemcpyAsync((double*)e.Ms.p + slot, &e.h_M[slot], sizeof(double), hipMemcpyHostToDevice, st), "copy");
   engine_tables(e, slot, 1, st);
+  reset_shift_fit(e, slot, st);
   // the host values must outlive the asynchronous copies
   hip_check(hipStreamSynchronize(st), "sync");
+}
+
+// Device-managed stabilising shift of every slot (logPFC), checked before
+// each step; `on` = 0 returns M to the host (pfc_set_shift / set_slot).
+void engine_set_auto_shift(Engine& e, int on, double floor_tol, double ceil_tol, hipStream_t st) {
+  if (on && e.p.model != PFC_MODEL_LOG) fail(PFC_EINVAL, "the stabilising shift is a logPFC option");
+  if (on && !(floor_tol >= 0 && ceil_tol >= 0)) fail(PFC_EINVAL, "tolerances must be >= 0");
+  if (on && !(e.p.minPhi > 0)) fail(PFC_EINVAL, "minPhi must be positive");
+  hip_check(hipStreamSynchronize(st), "sync");
+  if (on && !e.shst.p) {
+    e.shst.alloc((size_t)e.cap * sizeof(ShiftState));
+    e.rebuildf.alloc((size_t)e.cap * sizeof(int));
+  }
+  if (on) {
+    hip_check(hipMemset(e.shst.p, 0, e.shst.n), "memset");
+    hip_check(hipMemset(e.rebuildf.p, 0, e.rebuildf.n), "memset");
+  } else if (e.auto_shift) {
+    // the host takes M back: its copy follows the device's
+    hip_check(hipMemcpy(e.h_M.data(), e.Ms.p, (size_t)e.cap * sizeof(double), hipMemcpyDeviceToHost), "copy");
+  }
+  e.shp = ShiftParams{e.p.g, e.p.minPhi, floor_tol, ceil_tol};
+  e.auto_shift = on != 0;
+}
+
+// out = {M, lo, hi} of `slot` (lo = hi = NaN before the first fit); synchronises
+void engine_get_shift(Engine& e, int slot, double out[3], hipStream_t st) {
+  e.check_slot(slot);
+  double M = 0.0;
+  hip_check(hipMemcpyAsync(&M, (const double*)e.Ms.p + slot, sizeof(double), hipMemcpyDeviceToHost, st), "copy");
+  ShiftState w{NAN, NAN, 0};
+  if (e.shst.p)
+    hip_check(hipMemcpyAsync(&w, (const ShiftState*)e.shst.p + slot, sizeof(ShiftState), hipMemcpyDeviceToHost, st),
+              "copy");
+  hip_check(hipStreamSynchronize(st), "sync");
+  out[0] = M;
+  out[1] = w.fitted ? w.lo : NAN;
+  out[2] = w.fitted ? w.hi : NAN;
 }
 
 void engine_set_slot_box(Engine& e, int slot, double Lx, double Ly, double dt, hipStream_t st) {
@@ -1357,6 +1452,7 @@ void group_check_member(const Engine& e) {
   if (e.resident)
     fail(PFC_EUNSUPPORTED, "a group member runs the resident stepper (small ETD1 grid): step it on its own");
   if (e.has_ext()) fail(PFC_EUNSUPPORTED, "a group member has an external potential");
+  if (e.auto_shift) fail(PFC_EUNSUPPORTED, "a group member manages its stabilising shift on the device");
   if (group_case(e.p.Nx / 2, &P) < 0 || group_case(e.p.Ny, &P) < 0)
     fail(PFC_EUNSUPPORTED, "group members need Nx/2 and Ny in {32, 48, 64, 96, 128, 192, 256, 384, 512}");
 }
@@ -1584,6 +1680,34 @@ int pfc_set_shift(pfc_plan* pl, double M) {
     if (!pl) fail(PFC_EINVAL, "NULL plan");
     hip_check(hipDeviceSynchronize(), "sync before table rebuild");
     engine_set_slot(pl->e, 0, pl->e.h_eps[0], M, nullptr);
+  });
+}
+
+int pfc_set_auto_shift(pfc_plan* pl, int on, double floor_tol, double ceil_tol, void* stream) {
+  return guarded([&] {
+    if (!pl) fail(PFC_EINVAL, "NULL plan");
+    engine_set_auto_shift(pl->e, on, floor_tol, ceil_tol, reinterpret_cast<hipStream_t>(stream));
+  });
+}
+
+int pfc_get_shift(pfc_plan* pl, double out[3], void* stream) {
+  return guarded([&] {
+    if (!pl || !out) fail(PFC_EINVAL, "NULL plan or out");
+    engine_get_shift(pl->e, 0, out, reinterpret_cast<hipStream_t>(stream));
+  });
+}
+
+int pfc_batch_set_auto_shift(pfc_batch* b, int on, double floor_tol, double ceil_tol, void* stream) {
+  return guarded([&] {
+    if (!b) fail(PFC_EINVAL, "NULL batch");
+    engine_set_auto_shift(b->e, on, floor_tol, ceil_tol, reinterpret_cast<hipStream_t>(stream));
+  });
+}
+
+int pfc_batch_get_shift(pfc_batch* b, int slot, double out[3], void* stream) {
+  return guarded([&] {
+    if (!b || !out) fail(PFC_EINVAL, "NULL batch or out");
+    engine_get_shift(b->e, slot, out, reinterpret_cast<hipStream_t>(stream));
   });
 }
 

@@ -899,8 +899,12 @@ template <typename T>
 __global__ void __launch_bounds__(256) k_tables(T* __restrict__ E1, T* __restrict__ Q1, T* __restrict__ Q2,
                                                 long long tstride, SlotGrid sg, TabParams p,
                                                 const double* __restrict__ eps, const double* __restrict__ Ms,
-                                                int slot0) {
-  const int slot = slot0 + blockIdx.y;
+                                                int slot0, const int* __restrict__ list = nullptr,
+                                                const int* __restrict__ rebuild = nullptr) {
+  // slots slot0 + blockIdx.y, or list[blockIdx.y]; with `rebuild`, only the
+  // slots whose flag is set
+  const int slot = list ? list[blockIdx.y] : slot0 + (int)blockIdx.y;
+  if (rebuild && !rebuild[slot]) return;
   p.epsilon = eps[slot];
   p.M = Ms[slot];
   p.dt = sg.dt[slot];
@@ -928,8 +932,11 @@ __global__ void __launch_bounds__(256) k_tables(T* __restrict__ E1, T* __restric
 static __global__ void __launch_bounds__(256) k_ext_table(V2<double>* __restrict__ C,
                                                           const V2<double>* __restrict__ Vhat, long long cstride,
                                                           SlotGrid sg, TabParams p, const double* __restrict__ eps,
-                                                          const double* __restrict__ Ms, int slot0) {
-  const int slot = slot0 + blockIdx.y;
+                                                          const double* __restrict__ Ms, int slot0,
+                                                          const int* __restrict__ list = nullptr,
+                                                          const int* __restrict__ rebuild = nullptr) {
+  const int slot = list ? list[blockIdx.y] : slot0 + (int)blockIdx.y;  // as in k_tables
+  if (rebuild && !rebuild[slot]) return;
   p.epsilon = eps[slot];
   p.M = Ms[slot];
   p.dt = sg.dt[slot];
@@ -991,8 +998,10 @@ __device__ __forceinline__ void block_reduce_partial(Partial& mine, Partial* red
   if (threadIdx.x == 0) *out = red[0];
 }
 
-// rows of S~ (storage Tin) -> n in FP64 -> sums of n and of the local FE density
-template <typename Tin>
+// rows of S~ (storage Tin) -> n in FP64 -> sums of n and of the local FE
+// density, min and max; MINMAX: min and max only (the stabilised split's
+// range check), from the same FP64 row transforms
+template <typename Tin, bool MINMAX = false>
 __global__ void __launch_bounds__(256) k_fe_row(const V2<Tin>* __restrict__ S, RowGeom g, FeParams p,
                                                 const V2<double>* __restrict__ twN, Partial* __restrict__ part,
                                                 Slots sl, int pstride) {
@@ -1024,10 +1033,12 @@ __global__ void __launch_bounds__(256) k_fe_row(const V2<Tin>* __restrict__ S, R
     const int line = e / g.M;
     const int m = e - line * g.M;
     V v = z[line * g.ld + m];
-    mine.s[0] = dd_add(mine.s[0], v.x);
-    mine.s[0] = dd_add(mine.s[0], v.y);
-    mine.s[1] = dd_add(mine.s[1], local_fe(v.x, p));
-    mine.s[1] = dd_add(mine.s[1], local_fe(v.y, p));
+    if constexpr (!MINMAX) {
+      mine.s[0] = dd_add(mine.s[0], v.x);
+      mine.s[0] = dd_add(mine.s[0], v.y);
+      mine.s[1] = dd_add(mine.s[1], local_fe(v.x, p));
+      mine.s[1] = dd_add(mine.s[1], local_fe(v.y, p));
+    }
     mine.mn = fmin(mine.mn, fmin(v.x, v.y));
     mine.mx = fmax(mine.mx, fmax(v.x, v.y));
   }
@@ -1118,6 +1129,66 @@ static __global__ void __launch_bounds__(256) k_fe_final(const Partial* __restri
     out[2] = res.mn;
     out[3] = res.mx;
     if (pin_out) *pin_out = mean;
+  }
+}
+
+// The stabilised split's range check on the device, once per step before it
+// (logpfc_extv/simulation.py:375-405 with OptimizeM :332-373): from the row
+// partials of k_fe_row<MINMAX> (the min and max pfc_reduce reports), a slot
+// that has no fit yet, or whose minimum fell below its fitted floor or whose
+// maximum rose above its fitted ceiling by more than the relative
+// tolerances, is refitted: lo = min if min > 0 else minPhi, hi = max,
+// M = (N(hi) - N(lo)) / (hi - lo) with N(phi) = g phi^2 + phi^3 + ln phi,
+// written to the slot's shift and flagged for a table rebuild.  One block per
+// active slot.
+struct ShiftState {
+  double lo, hi;  // fitted window
+  int fitted;
+};
+
+struct ShiftParams {
+  double g, minPhi, floor_tol, ceil_tol;
+};
+
+static __global__ void __launch_bounds__(256) k_shift_update(const Partial* __restrict__ part, int nrow, Slots sl,
+                                                             int pstride, ShiftParams sp, ShiftState* __restrict__ st,
+                                                             double* __restrict__ Ms, int* __restrict__ rebuild) {
+  part += (size_t)blockIdx.x * pstride;
+  const int slot = slot_of(sl, blockIdx.x);
+  double mn = INFINITY, mx = -INFINITY;
+  for (int b = threadIdx.x; b < nrow; b += blockDim.x) {
+    mn = fmin(mn, part[b].mn);
+    mx = fmax(mx, part[b].mx);
+  }
+  __shared__ double rmn[256], rmx[256];
+  rmn[threadIdx.x] = mn;
+  rmx[threadIdx.x] = mx;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) {
+      rmn[threadIdx.x] = fmin(rmn[threadIdx.x], rmn[threadIdx.x + s]);
+      rmx[threadIdx.x] = fmax(rmx[threadIdx.x], rmx[threadIdx.x + s]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    mn = rmn[0];
+    mx = rmx[0];
+    ShiftState w = st[slot];
+    bool refit = !w.fitted;
+    if (!refit) {
+      const bool floor_breached = (w.lo - mn) / w.lo > sp.floor_tol;
+      const bool ceil_breached = (mx - w.hi) / w.hi > sp.ceil_tol;
+      refit = floor_breached || ceil_breached;
+    }
+    if (refit) {
+      const double lo = mn > 0.0 ? mn : sp.minPhi;
+      const double n_hi = sp.g * mx * mx + mx * mx * mx + log(mx);
+      const double n_lo = sp.g * lo * lo + lo * lo * lo + log(lo);
+      Ms[slot] = (n_hi - n_lo) / (mx - lo);
+      st[slot] = ShiftState{lo, mx, 1};
+    }
+    rebuild[slot] = refit ? 1 : 0;
   }
 }
 

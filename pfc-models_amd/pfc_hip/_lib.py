@@ -64,6 +64,10 @@ _SIGNATURES = [
                                         ctypes.POINTER(ctypes.c_longlong)]),
     ("pfc_plan_schedule", ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_int)]),
     ("pfc_plan_graph_stats", ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_longlong)]),
+    ("pfc_set_auto_shift", ctypes.c_int, [_P, ctypes.c_int, ctypes.c_double, ctypes.c_double, _P]),
+    ("pfc_get_shift", ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_double), _P]),
+    ("pfc_batch_set_auto_shift", ctypes.c_int, [_P, ctypes.c_int, ctypes.c_double, ctypes.c_double, _P]),
+    ("pfc_batch_get_shift", ctypes.c_int, [_P, ctypes.c_int, ctypes.POINTER(ctypes.c_double), _P]),
     ("pfc_run_tracked", ctypes.c_int, [_P, ctypes.c_longlong, ctypes.c_longlong, ctypes.POINTER(ctypes.c_double), _P]),
     ("pfc_batch_run_tracked", ctypes.c_int, [_P, ctypes.c_longlong, ctypes.c_longlong, ctypes.POINTER(ctypes.c_double),
                                              _P]),

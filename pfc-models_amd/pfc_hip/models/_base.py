@@ -174,6 +174,16 @@ class _SimulationState:
         self.local_mu = None
         setattr(self, field_name, DeviceField(sim))
 
+    def __getattr__(self, name):
+        # the stabilising shift when the device manages it (logPFC)
+        if name in ("M", "phi_min", "phi_max"):
+            sim = self.__dict__.get("_sim")
+            if sim is not None and getattr(sim, "_device_shift", False):
+                M, lo, hi = sim._plan.shift()
+                v = {"M": M, "phi_min": lo, "phi_max": hi}[name]
+                return v if name == "M" or v == v else None
+        raise AttributeError(name)
+
     @property
     def kSpaceShape(self):
         g = self._sim.Geometry
@@ -239,12 +249,11 @@ class Simulation:
 
         With `record_every`, returns an (nsteps // record_every, 4) FP64 array of
         (free energy per site, mean, min, max) after every `record_every` steps,
-        recorded on the device and synchronised once (the stabilised logPFC,
-        which checks its phi range on the host every step, records block by
-        block)."""
+        recorded on the device and synchronised once (the host-checked
+        stabilised logPFC, stabilise="host", records block by block)."""
         nsteps = int(nsteps)
         rec = None
-        if self.stabilise:
+        if self.stabilise and getattr(self, "_host_shift", False):
             rows = []
             for i in range(nsteps):
                 self.ValidatePhiRange()

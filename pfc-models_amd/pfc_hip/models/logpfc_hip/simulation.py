@@ -61,14 +61,24 @@ class Simulation(_Simulation):
     _model_kind = "log"
     _field_name = "phi"
 
-    def __init__(self, modelParameters, geometry, **kw):
-        super().__init__(modelParameters, geometry, **kw)
+    def __init__(self, modelParameters, geometry, *, stabilise=False, **kw):
+        host = stabilise == "host"
+        super().__init__(modelParameters, geometry, stabilise=bool(stabilise), **kw)
         s = self.State
-        s.M = 0.0
-        s.phi_min = None
-        s.phi_max = None
         s.phi_floor_tol = 0.1
         s.phi_ceiling_tol = 0.1
+        self._host_shift = self.stabilise and host
+        # stabilise=True: the library checks the phi range and refits M on the
+        # device before every step (pfc_set_auto_shift), with no host round
+        # trip; State.M / phi_min / phi_max then read the device's values.
+        # stabilise="host": the reference's per-step host check (below).
+        self._device_shift = self.stabilise and not host
+        if self._device_shift:
+            self._plan.set_auto_shift(True, s.phi_floor_tol, s.phi_ceiling_tol)
+        else:
+            s.M = 0.0
+            s.phi_min = None
+            s.phi_max = None
 
     @property
     def phi(self):
@@ -76,7 +86,9 @@ class Simulation(_Simulation):
 
     def _upload(self, value):
         super()._upload(value)
-        if getattr(self.State, "phi_min", None) is not None:
+        # a new field refits M at the next step (on the device, the library
+        # resets the fit itself)
+        if not getattr(self, "_device_shift", False) and getattr(self.State, "phi_min", None) is not None:
             self.State.phi_min = self.State.phi_max = None
 
     def Get_local_phi(self):
@@ -88,7 +100,12 @@ class Simulation(_Simulation):
     def OptimizeM(self):
         """M = (N(phi_max) - N(phi_min)) / (phi_max - phi_min), N = ln phi + g phi^2 + phi^3,
         phi_min replaced by minPhi when the field is not positive; rebuilds the
-        device ETD tables (logpfc_extv/simulation.py:332-373)."""
+        device ETD tables (logpfc_extv/simulation.py:332-373).  With the
+        device-managed shift, makes the next step refit."""
+        if getattr(self, "_device_shift", False):
+            s = self.State
+            self._plan.set_auto_shift(True, s.phi_floor_tol, s.phi_ceiling_tol)
+            return
         _, _, mn, mx = self._plan.reduce()
         g = float(self.Model.g)
         lo = mn if mn > 0 else float(self.Model.minPhi)
@@ -100,7 +117,10 @@ class Simulation(_Simulation):
 
     def ValidatePhiRange(self):
         """Re-fit M when phi leaves the fitted window by more than the tolerances
-        (logpfc_extv/simulation.py:375-405)."""
+        (logpfc_extv/simulation.py:375-405).  With the device-managed shift the
+        library does this before every step, so there is nothing to do here."""
+        if getattr(self, "_device_shift", False):
+            return
         s = self.State
         if s.phi_min is None or s.phi_max is None:
             self.OptimizeM()
@@ -114,6 +134,9 @@ class Simulation(_Simulation):
 
 def BuildHipGPU_Simulation(modelParameters, geometry, *, precision=64, scheme="etd1", pin_mean=True,
                            clamp_log=None, stabilise=False):
+    """stabilise: False, True (the M-shifted split with the range check and refit
+    on the device before every step) or "host" (the same check on the host,
+    as logpfc_extv does, with a synchronisation per step)."""
     """Counterpart of BuildCupyGPU_Simulation (models/logpfc/simulation.py:265-281).
     FP32 bulk always clamps the log (the guard near phi = 0)."""
     if clamp_log is None:

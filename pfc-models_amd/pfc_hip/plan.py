@@ -196,6 +196,19 @@ class Plan:
     def set_shift(self, M):
         _lib.check(self._lib.pfc_set_shift(self._h, float(M)))
 
+    def set_auto_shift(self, on=True, floor_tol=0.1, ceil_tol=0.1):
+        """Let the device refit the stabilising shift M before every step
+        (pfc_set_auto_shift); on=False hands M back to set_shift."""
+        _lib.check(self._lib.pfc_set_auto_shift(self._h, int(bool(on)), float(floor_tol), float(ceil_tol),
+                                                current_stream_handle()))
+        return self
+
+    def shift(self):
+        """(M, lo, hi) of the device-managed shift (lo = hi = nan before the first fit)."""
+        v = (ctypes.c_double * 3)()
+        _lib.check(self._lib.pfc_get_shift(self._h, v, current_stream_handle()))
+        return float(v[0]), float(v[1]), float(v[2])
+
     def status(self):
         v = ctypes.c_int()
         _lib.check(self._lib.pfc_get_status(self._h, ctypes.byref(v), current_stream_handle()))
@@ -337,6 +350,18 @@ class Batch:
         out = (ctypes.c_double * max(4, 4 * n))()
         _lib.check(self._lib.pfc_batch_reduce(self._h, out, current_stream_handle()))
         return np.array(out[: 4 * n], dtype=np.float64).reshape(n, 4)
+
+    def set_auto_shift(self, on=True, floor_tol=0.1, ceil_tol=0.1):
+        """Device-managed stabilising shift for every slot (see Plan.set_auto_shift)."""
+        _lib.check(self._lib.pfc_batch_set_auto_shift(self._h, int(bool(on)), float(floor_tol), float(ceil_tol),
+                                                      current_stream_handle()))
+        return self
+
+    def shift(self, slot):
+        """(M, lo, hi) of `slot`'s device-managed shift."""
+        v = (ctypes.c_double * 3)()
+        _lib.check(self._lib.pfc_batch_get_shift(self._h, int(slot), v, current_stream_handle()))
+        return float(v[0]), float(v[1]), float(v[2])
 
     def run_tracked(self, nsteps, every):
         """Advance the active slots `nsteps` steps, recording each one's (free
