@@ -253,7 +253,14 @@ int pfc_slot_bytes(const pfc_params* params, long long* bytes);
  * step-loop graphs as pfc_batch_step does.  The batches stay usable on their
  * own; a member must outlive the group, and members' active lists may change
  * between calls (the next call rebuilds the launch layout after draining
- * `stream`).  (No reference counterpart: the reference relaxes one Simulation
+ * `stream`).  With PFC_HIP_GROUP_CACHE_MB set to a budget in megabytes
+ * (default 0: one chunk), members are stepped in chunks, largest first,
+ * whose state arrays and tables fit it: a step graph runs 32 steps of one
+ * chunk, then of the next, so a chunk's working set can stay in the 256 MB
+ * Infinity Cache across its steps; chunks are independent, so the chunking
+ * changes no result.  pfc_group_chunks gives the number of chunks of the
+ * last call.
+ * (No reference counterpart: the reference relaxes one Simulation
  * at a time, phase_diagram/q0star.py:409-477; this is the uneven-size sweep
  * form, one launch per kernel kind per step over a work list of FFT tiles.) */
 typedef struct pfc_group pfc_group;
@@ -261,6 +268,7 @@ int pfc_group_create(pfc_batch* const* batches, int n, pfc_group** out);
 int pfc_group_destroy(pfc_group* group);
 int pfc_group_step(pfc_group* group, long long nsteps, void* stream);
 int pfc_group_graph_stats(const pfc_group* group, long long out[2]);
+int pfc_group_chunks(const pfc_group* group, int* nchunks);
 
 /* Batched 2D real transforms over (batch, Ny, Nx) device arrays.
  * pfc_rfft2:  out[b] = rfft2(in[b])             (unnormalised)

@@ -1425,13 +1425,21 @@ struct pfc_group {
   std::vector<pfc_batch*> mem;  // not owned
   std::vector<int> order;       // members by decreasing instance size (launch order)
   int precision = 0, scheme = 0;
-  DevBuf desc;                  // GroupMember<T>[2 passes][2 classes][members]
+  DevBuf desc;                  // GroupMember<T>[chunks][2 passes][2 classes][members]
   std::vector<long long> built; // active counts the descriptors hold
-  GroupLaunch rows[kGroupClasses], cols[kGroupClasses];
-  // every active member has its row and column transforms in one class, and
-  // both classes have members: the two classes' launch chains are
-  // independent, and a step graph runs them as two parallel branches
-  bool split = false;
+  // Members are stepped in chunks whose arrays and tables fit `budget` bytes
+  // (sized to the Infinity Cache): a step graph runs each chunk's 32 steps in
+  // turn, so a chunk's working set can stay cached across its steps (chunks
+  // are independent, so the order changes no result); 0 = one chunk
+  size_t budget = 0;
+  struct Chunk {
+    GroupLaunch rows[kGroupClasses], cols[kGroupClasses];
+    // every member has its row and column transforms in one class, and both
+    // classes have members: the two classes' launch chains are independent,
+    // and a step graph runs them as two parallel branches
+    bool split = false;
+  };
+  std::vector<Chunk> chunks;
   std::map<std::vector<long long>, hipGraphExec_t> graphs;
   hipStream_t cap_st = nullptr, cap_st2 = nullptr;
   hipEvent_t fork = nullptr, join = nullptr;
@@ -1463,56 +1471,81 @@ std::vector<long long> group_counts(const pfc_group& g) {
   return k;
 }
 
+// bytes one step of member `e` works on: its active slots' state arrays
+// (S~, T and the ETDRK2 carry) and FP64 ETD tables
+size_t group_working_set(const Engine& e) {
+  const size_t k = e.p.scheme == PFC_SCHEME_ETDRK2 ? 3 : 2;
+  return (size_t)e.nact() * k * (e.nc * e.csize + e.ntab * sizeof(double));
+}
+
 template <typename T>
 void group_build(pfc_group& g, hipStream_t st) {
   const size_t n = g.mem.size();
-  g.split = true;
-  for (const pfc_batch* b : g.mem) {
-    if (b->e.nact() == 0) continue;
-    int P = 0, rc = 0, cc = 0;
-    group_case(b->e.p.Nx / 2, &P, &rc);
-    group_case(b->e.p.Ny, &P, &cc);
-    g.split = g.split && rc == cc;
-  }
-  std::vector<GroupMember<T>> all(2 * kGroupClasses * n);
-  for (int pass = 0; pass < 2; ++pass)
-    for (int c = 0; c < kGroupClasses; ++c) {
-      GroupLaunch& L = pass == 0 ? g.rows[c] : g.cols[c];
-      L = GroupLaunch{};
-      L.off = (size_t)(pass * kGroupClasses + c) * n;
-      for (int i : g.order) {
-        Engine& e = g.mem[i]->e;
-        const int ny = e.nact();
-        if (ny == 0) continue;
-        const int len = pass == 0 ? e.p.Nx / 2 : e.p.Ny;
-        int P = 0, cls = 0;
-        const int kcase = group_case(len, &P, &cls);
-        if (cls != c) continue;
-        GroupMember<T> m{};
-        m.S = (V2<T>*)e.S.p;
-        m.Tb = (V2<T>*)e.Tb.p;
-        m.D = (V2<T>*)e.D.p;
-        m.tw = (const V2<double>*)(pass == 0 ? e.twN64.p : e.twY64.p);
-        m.lay = e.lay;
-        m.q = nonlin_of(e);
-        m.sl = e.slots_active();
-        m.tb = e.tables();
-        m.Ny = e.p.Ny;
-        m.Nc = e.Nc;
-        m.kcase = kcase;
-        const int per = kGroupThreads / (len / P);  // rows (columns) per block
-        m.ntile = pass == 0 ? (e.p.Ny + per - 1) / per : (e.Nc + per - 1) / per;
-        m.blk0 = L.blocks;
-        L.blocks += m.ntile * ny;
-        L.lds = std::max(L.lds, (size_t)per * (pass == 0 ? row_ld(len) : col_ld(len)) * sizeof(V2<T>));
-        all[L.off + L.nmem++] = m;
-      }
+  // chunks in launch order (largest instances first), greedily up to the budget
+  std::vector<std::vector<int>> parts;
+  size_t cur = 0;
+  for (int i : g.order) {
+    const Engine& e = g.mem[i]->e;
+    if (e.nact() == 0) continue;
+    const size_t ws = group_working_set(e);
+    if (parts.empty() || (g.budget && cur > 0 && cur + ws > g.budget)) {
+      parts.emplace_back();
+      cur = 0;
     }
+    parts.back().push_back(i);
+    cur += ws;
+  }
+  g.chunks.assign(parts.size(), pfc_group::Chunk{});
+  std::vector<GroupMember<T>> all(parts.size() * 2 * kGroupClasses * n);
+  for (size_t k = 0; k < parts.size(); ++k) {
+    pfc_group::Chunk& ch = g.chunks[k];
+    ch.split = true;
+    for (int i : parts[k]) {
+      const Engine& e = g.mem[i]->e;
+      int P = 0, rc = 0, cc = 0;
+      group_case(e.p.Nx / 2, &P, &rc);
+      group_case(e.p.Ny, &P, &cc);
+      ch.split = ch.split && rc == cc;
+    }
+    for (int pass = 0; pass < 2; ++pass)
+      for (int c = 0; c < kGroupClasses; ++c) {
+        GroupLaunch& L = pass == 0 ? ch.rows[c] : ch.cols[c];
+        L = GroupLaunch{};
+        L.off = ((k * 2 + pass) * kGroupClasses + c) * n;
+        for (int i : parts[k]) {
+          Engine& e = g.mem[i]->e;
+          const int ny = e.nact();
+          const int len = pass == 0 ? e.p.Nx / 2 : e.p.Ny;
+          int P = 0, cls = 0;
+          const int kcase = group_case(len, &P, &cls);
+          if (cls != c) continue;
+          GroupMember<T> m{};
+          m.S = (V2<T>*)e.S.p;
+          m.Tb = (V2<T>*)e.Tb.p;
+          m.D = (V2<T>*)e.D.p;
+          m.tw = (const V2<double>*)(pass == 0 ? e.twN64.p : e.twY64.p);
+          m.lay = e.lay;
+          m.q = nonlin_of(e);
+          m.sl = e.slots_active();
+          m.tb = e.tables();
+          m.Ny = e.p.Ny;
+          m.Nc = e.Nc;
+          m.kcase = kcase;
+          const int per = kGroupThreads / (len / P);  // rows (columns) per block
+          m.ntile = pass == 0 ? (e.p.Ny + per - 1) / per : (e.Nc + per - 1) / per;
+          m.blk0 = L.blocks;
+          L.blocks += m.ntile * ny;
+          L.lds = std::max(L.lds, (size_t)per * (pass == 0 ? row_ld(len) : col_ld(len)) * sizeof(V2<T>));
+          all[L.off + L.nmem++] = m;
+        }
+      }
+    ch.split = ch.split && ch.rows[0].nmem > 0 && ch.rows[1].nmem > 0;
+  }
   // launches (or graphs) still reading the old descriptors finish first
   hip_check(hipStreamSynchronize(st), "sync before group rebuild");
-  hip_check(hipMemcpy(g.desc.p, all.data(), all.size() * sizeof(GroupMember<T>), hipMemcpyHostToDevice), "copy");
+  if (!all.empty())
+    hip_check(hipMemcpy(g.desc.p, all.data(), all.size() * sizeof(GroupMember<T>), hipMemcpyHostToDevice), "copy");
   g.built = group_counts(g);
-  g.split = g.split && g.rows[0].nmem > 0 && g.rows[1].nmem > 0;
 }
 
 template <typename T>
@@ -1528,50 +1561,58 @@ void group_launch<float>(int kind, int cls, int blocks, size_t lds, hipStream_t 
   group_launch_f32(kind, cls, blocks, lds, st, mem, nmem);
 }
 
-// one pass (kind 0: rows, else the column mode kind - 1) over class `cls`
-// (-1: every class with members)
+// one pass (kind 0: rows, else the column mode kind - 1) of chunk `k` over
+// class `cls` (-1: every class with members)
 template <typename T>
-void group_pass(pfc_group& g, int kind, hipStream_t st, int cls = -1) {
+void group_pass(pfc_group& g, size_t k, int kind, hipStream_t st, int cls = -1) {
   const auto* base = (const GroupMember<T>*)g.desc.p;
-  const GroupLaunch* L = kind == 0 ? g.rows : g.cols;
+  const GroupLaunch* L = kind == 0 ? g.chunks[k].rows : g.chunks[k].cols;
   for (int c = 0; c < kGroupClasses; ++c)
     if (L[c].nmem && (cls < 0 || cls == c))
       group_launch<T>(kind, c, L[c].blocks, L[c].lds, st, base + L[c].off, L[c].nmem);
 }
 
 template <typename T>
-void group_loop(pfc_group& g, long long nsteps, hipStream_t st, int cls = -1) {
+void group_loop(pfc_group& g, size_t k, long long nsteps, hipStream_t st, int cls = -1) {
   for (long long s = 0; s < nsteps; ++s) {
-    group_pass<T>(g, 0, st, cls);
+    group_pass<T>(g, k, 0, st, cls);
     if (g.scheme == PFC_SCHEME_ETD1) {
-      group_pass<T>(g, 1 + COL_ETD1, st, cls);
+      group_pass<T>(g, k, 1 + COL_ETD1, st, cls);
     } else {
-      group_pass<T>(g, 1 + COL_RK2_STAGE1, st, cls);
-      group_pass<T>(g, 0, st, cls);
-      group_pass<T>(g, 1 + COL_RK2_STAGE2, st, cls);
+      group_pass<T>(g, k, 1 + COL_RK2_STAGE1, st, cls);
+      group_pass<T>(g, k, 0, st, cls);
+      group_pass<T>(g, k, 1 + COL_RK2_STAGE2, st, cls);
     }
   }
   hip_check(hipGetLastError(), "group step launch");
 }
 
-// kGraphSteps steps captured on g.cap_st: with independent classes, class 1's
-// chain forks onto a second stream and joins at the end (two parallel graph
-// branches); else one chain
+// nsteps steps of every chunk, chunk after chunk
+template <typename T>
+void group_loop_chunks(pfc_group& g, long long nsteps, hipStream_t st) {
+  for (size_t k = 0; k < g.chunks.size(); ++k) group_loop<T>(g, k, nsteps, st);
+}
+
+// kGraphSteps steps of every chunk captured on g.cap_st, chunk after chunk;
+// within a chunk with independent classes, class 1's chain forks onto a
+// second stream and joins at the chunk's end (two parallel graph branches)
 template <typename T>
 void group_capture_steps(pfc_group& g) {
-  if (!g.split) {
-    group_loop<T>(g, kGraphSteps, g.cap_st);
-    return;
+  for (size_t k = 0; k < g.chunks.size(); ++k) {
+    if (!g.chunks[k].split) {
+      group_loop<T>(g, k, kGraphSteps, g.cap_st);
+      continue;
+    }
+    if (!g.cap_st2) hip_check(hipStreamCreateWithFlags(&g.cap_st2, hipStreamNonBlocking), "hipStreamCreate");
+    if (!g.fork) hip_check(hipEventCreateWithFlags(&g.fork, hipEventDisableTiming), "hipEventCreate");
+    if (!g.join) hip_check(hipEventCreateWithFlags(&g.join, hipEventDisableTiming), "hipEventCreate");
+    hip_check(hipEventRecord(g.fork, g.cap_st), "hipEventRecord");
+    hip_check(hipStreamWaitEvent(g.cap_st2, g.fork, 0), "hipStreamWaitEvent");
+    group_loop<T>(g, k, kGraphSteps, g.cap_st2, 1);
+    group_loop<T>(g, k, kGraphSteps, g.cap_st, 0);
+    hip_check(hipEventRecord(g.join, g.cap_st2), "hipEventRecord");
+    hip_check(hipStreamWaitEvent(g.cap_st, g.join, 0), "hipStreamWaitEvent");
   }
-  if (!g.cap_st2) hip_check(hipStreamCreateWithFlags(&g.cap_st2, hipStreamNonBlocking), "hipStreamCreate");
-  if (!g.fork) hip_check(hipEventCreateWithFlags(&g.fork, hipEventDisableTiming), "hipEventCreate");
-  if (!g.join) hip_check(hipEventCreateWithFlags(&g.join, hipEventDisableTiming), "hipEventCreate");
-  hip_check(hipEventRecord(g.fork, g.cap_st), "hipEventRecord");
-  hip_check(hipStreamWaitEvent(g.cap_st2, g.fork, 0), "hipStreamWaitEvent");
-  group_loop<T>(g, kGraphSteps, g.cap_st2, 1);
-  group_loop<T>(g, kGraphSteps, g.cap_st, 0);
-  hip_check(hipEventRecord(g.join, g.cap_st2), "hipEventRecord");
-  hip_check(hipStreamWaitEvent(g.cap_st, g.join, 0), "hipStreamWaitEvent");
 }
 
 template <typename T>
@@ -1584,9 +1625,7 @@ void group_step(pfc_group& g, long long nsteps, hipStream_t st) {
   }
   const std::vector<long long> counts = group_counts(g);
   if (counts != g.built) group_build<T>(g, st);
-  bool any = false;
-  for (int c = 0; c < kGroupClasses; ++c) any |= g.rows[c].nmem > 0;
-  if (!any || nsteps == 0) return;
+  if (g.chunks.empty() || nsteps == 0) return;
   if (nsteps >= kGraphSteps && graphs_enabled()) {
     hipGraphExec_t x = nullptr;
     auto it = g.graphs.find(counts);
@@ -1617,7 +1656,7 @@ void group_step(pfc_group& g, long long nsteps, hipStream_t st) {
       nsteps -= reps * kGraphSteps;
     }
   }
-  group_loop<T>(g, nsteps, st);
+  group_loop_chunks<T>(g, nsteps, st);
 }
 
 }  // namespace
@@ -1965,7 +2004,13 @@ int pfc_group_create(pfc_batch* const* batches, int n, pfc_group** out) {
       return (long long)ea.p.Nx * ea.p.Ny > (long long)eb.p.Nx * eb.p.Ny;
     });
     const size_t dsize = g->precision == 64 ? sizeof(GroupMember<double>) : sizeof(GroupMember<float>);
-    g->desc.alloc((size_t)2 * kGroupClasses * n * dsize);
+    g->desc.alloc((size_t)n * 2 * kGroupClasses * n * dsize);  // at most one chunk per member
+    // cache budget per chunk in MB (PFC_HIP_GROUP_CACHE_MB; default 0 = one
+    // chunk: on the 256-instance sweep, 100 and 160 MB chunks ran 8 % slower
+    // than one chunk, the lost overlap of the two class chains outweighing
+    // the cached working set)
+    const char* cm = std::getenv("PFC_HIP_GROUP_CACHE_MB");
+    g->budget = (size_t)((cm ? std::max(0.0, std::atof(cm)) : 0.0) * 1024 * 1024);
     g->built.assign(n, -1);
     *out = g.release();
   });
@@ -1991,6 +2036,13 @@ int pfc_group_graph_stats(const pfc_group* g, long long out[2]) {
     if (!g || !out) fail(PFC_EINVAL, "NULL argument");
     out[0] = g->captures;
     out[1] = g->replays;
+  });
+}
+
+int pfc_group_chunks(const pfc_group* g, int* nchunks) {
+  return guarded([&] {
+    if (!g || !nchunks) fail(PFC_EINVAL, "NULL argument");
+    *nchunks = (int)g->chunks.size();
   });
 }
 

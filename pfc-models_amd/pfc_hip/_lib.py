@@ -75,6 +75,7 @@ _SIGNATURES = [
     ("pfc_group_destroy", ctypes.c_int, [_P]),
     ("pfc_group_step", ctypes.c_int, [_P, ctypes.c_longlong, _P]),
     ("pfc_group_graph_stats", ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_longlong)]),
+    ("pfc_group_chunks", ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_int)]),
     ("pfc_batch_graph_stats", ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_longlong)]),
     ("pfc_plan_bytes", ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_longlong)]),
     ("pfc_batch_create", ctypes.c_int, [ctypes.POINTER(PfcParams), ctypes.c_int, ctypes.POINTER(_P)]),

@@ -420,6 +420,12 @@ class Group:
         _lib.check(self._lib.pfc_group_graph_stats(self._h, v))
         return int(v[0]), int(v[1])
 
+    def chunks(self):
+        """Number of cache-sized chunks the last step call ran the members in."""
+        v = ctypes.c_int()
+        _lib.check(self._lib.pfc_group_chunks(self._h, ctypes.byref(v)))
+        return int(v.value)
+
     def close(self):
         if getattr(self, "_h", None) is not None and self._h.value:
             self.torch.cuda.synchronize()

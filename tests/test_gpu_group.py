@@ -57,6 +57,28 @@ def test_group_bitwise_equal_batches_alone(hip_device, monkeypatch, precision, s
     assert not any(b.status().any() for b in grouped)
 
 
+@pytest.mark.parametrize("precision,scheme", [(32, "etd1"), (64, "etdrk2")])
+def test_group_chunks_change_no_result(hip_device, monkeypatch, precision, scheme):
+    # a 2 MB cache budget puts the 512^2 and 384^2 members in chunks of their
+    # own and the small ones together; each chunk runs its 32 steps in turn
+    from pfc_hip.plan import Group
+    monkeypatch.setenv("PFC_HIP_TUNE", "0")
+    monkeypatch.setenv("PFC_HIP_GROUP_CACHE_MB", "2")
+    grouped, alone = _pairs(SHAPES, 2, precision, scheme)
+    grp = Group(grouped)
+    for call, n in enumerate([256, 7, 64]):
+        grp.step(n)
+        for a in alone:
+            a.step(n)
+        _same(grouped, alone, call)
+    assert grp.chunks() >= 3
+    assert grp.graph_stats() == (1, 8 + 2)
+    monkeypatch.delenv("PFC_HIP_GROUP_CACHE_MB")
+    one = Group(grouped)  # default: one chunk
+    one.step(1)
+    assert one.chunks() == 1
+
+
 def test_group_follows_active_lists(hip_device, monkeypatch):
     from pfc_hip.plan import Group
     monkeypatch.setenv("PFC_HIP_TUNE", "0")
