@@ -13,7 +13,10 @@ Prints ONE JSON line.  `value` is Mpts*steps/s of the plain FP64 step;
 `roofline` is measured for the dominant kernel (largest total time) with a HIP
 event recorded after every launch on the stream the kernels run on, launches
 back to back; `cpu_baseline` times the FP64 NumPy/SciPy CPU restatement of the
-same step on this host's cores over a bounded sample.
+same step on this host's cores over a bounded sample.  `sweep`,
+`phase_diagram` and `logpfc` carry the 256-instance sweep, the batched q0*
+scans and the logPFC 2048^2 stabilised workload (`--sweep`, `--phase-diagram`,
+`--logpfc` run one of them alone).
 """
 from __future__ import annotations
 
@@ -319,6 +322,49 @@ def run_sweep(plain_fraction=8, steps=5000):
     }
 
 
+def run_logpfc(steps=10000, n=2048):
+    """The logPFC 2048^2 workload: builder defaults (Gamma = beta = q0 = eps =
+    g = 1, dt = 0.01, minPhi = 1e-6), dx = pi/4, psibar = -0.2 + U(-0.02,
+    0.02) (seed 3) clipped to psi > -1 + 1e-6, ETD1 with the stabilised split
+    (range check and M refit on the device before every step), `steps` steps
+    in FP32 bulk and in FP64.  Mpts*steps/s over the steps after an untimed
+    20-step start (tuning, first fit), and the final FP64 free energy against
+    the uniform liquid's closed form at the same mean (the state this
+    linearly stable phi = 0.8 relaxes to)."""
+    import numpy as np
+    import torch
+    from pfc_hip import fields as F
+    from pfc_hip.models.logpfc_hip import phase_diagram as LAD
+    from pfc_hip.models.logpfc_hip import simulation as S
+    L = n * math.pi / 4
+    phi0 = 1.0 + np.clip(F.uniform_noise((n, n), -0.2, 0.02, seed=3), -1 + 1e-6, None)
+    out = {"workload": f"logPFC {n}^2, dx=pi/4, builder defaults (dt=0.01), phi=0.8+U(-0.02,0.02), ETD1, "
+                       f"stabilised split managed on the device, {steps} steps"}
+    fes = {}
+    for prec in (32, 64):
+        geometry = S.Build2DGeometry(Nx=n, Ny=n, Lx=L, Ly=L)
+        sim = S.BuildHipGPU_Simulation(S.BuildLogPFCModelParameters(meanDensity=float(phi0.mean())), geometry,
+                                       precision=prec, stabilise=True)
+        sim.State.phi[...] = phi0
+        fe0 = float(sim.CalculateFreeEnergy())
+        sim.run(20)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        sim.run(steps - 20)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        fe, mean, lo, hi = sim._plan.reduce()
+        liq = LAD.liquid_free_energy(mean, {"beta": 1.0, "g": 1.0, "q0": 1.0, "v0": 1.0}, 1.0)
+        fes[prec] = fe
+        out[f"fp{prec}"] = {"value": n * n * (steps - 20) / wall / 1e6, "unit": "Mpts*steps/s",
+                            "ms_per_step": 1e3 * wall / (steps - 20), "free_energy_start": fe0, "free_energy": fe,
+                            "rel_above_liquid": (fe - liq) / abs(liq), "phi_min": lo, "phi_max": hi,
+                            "M": float(sim.State.M), "status": int(sim.status())}
+        sim.close()
+    out["fp32_rel_fe_vs_fp64"] = abs(fes[32] - fes[64]) / abs(fes[64])
+    return out
+
+
 def run_phase_diagram(n_dens=12, serial_every=1):
     """q0* over a density list with the phase-diagram driver: logPFC triangle,
     beta 0.5, g -2, eps -0.9 on the 28 x 24 cell with the driver tests'
@@ -385,6 +431,10 @@ def main():
     ap.add_argument("--sweep", action="store_true",
                     help="only run the 256-instance sweep leg and print its JSON line")
     ap.add_argument("--sweep-steps", type=int, default=5000, help="steps per sweep instance (short profiling runs)")
+    ap.add_argument("--no-logpfc", action="store_true", help="skip the logPFC 2048^2 leg")
+    ap.add_argument("--logpfc", action="store_true",
+                    help="only run the logPFC 2048^2 stabilised leg and print its JSON line")
+    ap.add_argument("--logpfc-steps", type=int, default=10000, help="steps of the logPFC leg")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain device copy of known size (counter calibration) and exit")
     a = ap.parse_args()
@@ -400,6 +450,9 @@ def main():
     if a.sweep:
         print(json.dumps({"sweep": run_sweep(steps=a.sweep_steps)}), flush=True)
         return
+    if a.logpfc:
+        print(json.dumps({"logpfc": run_logpfc(steps=a.logpfc_steps)}), flush=True)
+        return
     if a.gpus != 1:
         print("bench.py: this build measures one GPU; --gpus above 1 is not supported "
               "(independent instances would shard one scheduler per GPU)", file=sys.stderr)
@@ -408,6 +461,7 @@ def main():
     f32 = None if a.no_fp32 else run_gpu(32, a.steps, a.warmup)
     sweep = None if a.no_sweep else run_sweep()
     pd = None if a.no_phase_diagram else run_phase_diagram()
+    logpfc = None if a.no_logpfc else run_logpfc(steps=a.logpfc_steps)
     cpu = None if a.no_cpu else run_cpu(a.cpu_seconds)
     line = {
         "metric": "Mpts*steps/s at 1024^2 (stdPFC ETDRK2 step, FP64; FP32 in 'fp32')",
@@ -438,6 +492,7 @@ def main():
         "cpu_baseline": cpu,
         "sweep": sweep,
         "phase_diagram": pd,
+        "logpfc": logpfc,
     }
     print(json.dumps(line), flush=True)
 

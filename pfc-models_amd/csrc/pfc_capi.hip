@@ -286,6 +286,15 @@ FastShape pick_default(const std::vector<FastShape>& cands, int target_blocks) {
   return best;
 }
 
+inline bool row_range_launch(int M, int var, dim3 g, dim3 b, size_t lds, hipStream_t st, const V2<double>* S, int Ny,
+                             const Lay& lay, const V2<double>* tw, const Slots& sl, RangePart* part) {
+  return row_range_launch_f64(M, var, g, b, lds, st, S, Ny, lay, tw, sl, part);
+}
+inline bool row_range_launch(int M, int var, dim3 g, dim3 b, size_t lds, hipStream_t st, const V2<float>* S, int Ny,
+                             const Lay& lay, const V2<double>* tw, const Slots& sl, RangePart* part) {
+  return row_range_launch_f32(M, var, g, b, lds, st, S, Ny, lay, tw, sl, part);
+}
+
 template <typename T> struct FastLaunch;
 template <> struct FastLaunch<double> {
   static void row(int M, const FastShape& f, int ny_inst, hipStream_t st, const V2<double>* S, V2<double>* T, int Ny,
@@ -382,6 +391,7 @@ struct Engine {
   bool auto_shift = false;
   ShiftParams shp{};
   DevBuf shst, rebuildf;
+  DevBuf rpart;  // k_row_range: per-tile (min, max)
   DevBuf vhat, extc;  // external potential: rfft2(V) and its source term, [cap][Ny][Nc] complex FP64
   std::vector<char> h_ext;  // slots with a potential
   std::vector<double> h_eps, h_M, h_Lx, h_Ly, h_dt;
@@ -563,10 +573,12 @@ Nonlin nonlin_of(const Engine& e) {
 }
 
 // The device-managed stabilising shift, before a step: the min and max of
-// every active slot's field (k_fe_row<MINMAX>, the arithmetic of
-// pfc_reduce's), the range check and refit (k_shift_update), and the ETD
-// tables (and external-potential source) of the refitted slots only.  No host
-// synchronisation; the launches can be recorded in a step graph.
+// every active slot's field (with the fast row shape k_row_range, the step's
+// own inverse row transforms, so the range is that of the values the
+// nonlinearity sees; else k_fe_row<MINMAX>, the arithmetic of pfc_reduce's),
+// the range check and refit (k_shift_update), and the ETD tables (and
+// external-potential source) of the refitted slots only.  No host synchronisation; the launches can be
+// recorded in a step graph.
 template <typename T>
 void shift_check(Engine& e, hipStream_t st) {
   const int ny = e.nact();
@@ -574,13 +586,23 @@ void shift_check(Engine& e, hipStream_t st) {
   const Slots sl = e.slots_active();
   const FeParams fp{e.p.model, e.p.beta, e.p.q0, e.p.epsilon, e.p.g, (const double*)e.kx2.p,
                     (const double*)e.ky2.p, (const double*)e.eps.p};
-  Partial* part = (Partial*)e.part.p;
-  const int rb = row_blocks(e.rg64), ps = e.pstride();
-  const size_t rl = std::max(row_lds(e.rg64, sizeof(V2<double>)), 256 * sizeof(Partial));
-  hipLaunchKernelGGL((k_fe_row<T, true>), dim3(rb, ny), dim3(256), rl, st, (const V2<T>*)e.S.p, e.rg64, fp,
-                     (const V2<double>*)e.twN64.p, part, sl, ps);
-  hipLaunchKernelGGL(k_shift_update, dim3(ny), dim3(256), 0, st, (const Partial*)part, rb, sl, ps, e.shp,
-                     (ShiftState*)e.shst.p, (double*)e.Ms.p, (int*)e.rebuildf.p);
+  if (e.frow.on) {
+    // the fast row shape's inverse transforms
+    const FastShape& f = e.frow;
+    if (!row_range_launch(e.p.Nx / 2, f.var, dim3(f.blocks, ny), dim3(f.lines * f.tpl), f.lds, st,
+                          (const V2<T>*)e.S.p, e.p.Ny, e.lay, (const V2<double>*)e.twN64.p, sl, (RangePart*)e.rpart.p))
+      fail(PFC_EUNSUPPORTED, "no fast row schedule");
+    hipLaunchKernelGGL(k_shift_update<RangePart>, dim3(ny), dim3(256), 0, st, (const RangePart*)e.rpart.p, f.blocks,
+                       sl, f.blocks, e.shp, (ShiftState*)e.shst.p, (double*)e.Ms.p, (int*)e.rebuildf.p);
+  } else {
+    Partial* part = (Partial*)e.part.p;
+    const int rb = row_blocks(e.rg64), ps = e.pstride();
+    const size_t rl = std::max(row_lds(e.rg64, sizeof(V2<double>)), 256 * sizeof(Partial));
+    hipLaunchKernelGGL((k_fe_row<T, true>), dim3(rb, ny), dim3(256), rl, st, (const V2<T>*)e.S.p, e.rg64, fp,
+                       (const V2<double>*)e.twN64.p, part, sl, ps);
+    hipLaunchKernelGGL(k_shift_update<Partial>, dim3(ny), dim3(256), 0, st, (const Partial*)part, rb, sl, ps, e.shp,
+                       (ShiftState*)e.shst.p, (double*)e.Ms.p, (int*)e.rebuildf.p);
+  }
   TabParams tp{e.p.Gamma, e.p.beta, e.p.q0, 0.0, 0.0, e.p.dt, e.p.smallZTol,
                1.0 / e.p.Ny, 1.0 / ((double)e.p.Nx * (double)e.p.Ny), e.Nc, e.p.Ny, table_rows(e.p.Ny)};
   const SlotGrid sg{(const double*)e.kx2.p, (const double*)e.ky2.p, (const double*)e.dts.p};
@@ -1152,6 +1174,7 @@ void engine_set_auto_shift(Engine& e, int on, double floor_tol, double ceil_tol,
   if (on && !e.shst.p) {
     e.shst.alloc((size_t)e.cap * sizeof(ShiftState));
     e.rebuildf.alloc((size_t)e.cap * sizeof(int));
+    e.rpart.alloc((size_t)e.cap * e.p.Ny * sizeof(RangePart));  // at most Ny row tiles per instance
   }
   if (on) {
     hip_check(hipMemset(e.shst.p, 0, e.shst.n), "memset");

@@ -225,6 +225,55 @@ __device__ __forceinline__ void reg_fft(V2<T>* v, V2<T>* line, int lane, const V
 // LDS: W lines of M + 1 (padded).  Roots come from the FP64 table
 // exp(-2 pi i m / Nx), m < Nx, into registers.
 // ---------------------------------------------------------------------------
+// The C2R half of the row pass for one line: loads row `rowc` of S~ (the
+// lane's P points and the kx = M point), folds the half spectrum into the
+// M-point complex sequence whose inverse FFT interleaves the row's even and
+// odd real samples, and runs that FFT.  On return v[q] = (n[2k], n[2k+1]),
+// k = lane + TPL q, and wk, wM, wt hold the roots the forward half needs.
+template <typename T, class S>
+__device__ __forceinline__ void row_c2r(const V2<T>* __restrict__ Sg, const Lay& lay, int rowc, int lane, V2<T>* line,
+                                        const V2<double>* __restrict__ twN, V2<T> (&v)[S::P], V2<T> (&wk)[S::P],
+                                        V2<T>& wM, V2<T> (&wt)[tw_count<S>()]) {
+  using V = V2<T>;
+  constexpr int M = S::L, P = S::P, TPL = S::TPL;
+  V x[P];
+#pragma unroll
+  for (int qq = 0; qq < P; ++qq) x[qq] = Sg[lay.at(rowc, lane + TPL * qq)];
+  const V xM = Sg[lay.at(rowc, M)];
+  // roots: w_Nx^k for the (un)packing of k = lane + TPL q, as w_Nx^lane times
+  // the wave-uniform w_Nx^(TPL q) (scalar loads), in FP64; and the FFT roots
+  const V2<double> wl0 = twN[lane];
+#pragma unroll
+  for (int qq = 0; qq < P; ++qq) {
+    const V2<double> c = cmul(wl0, twN[TPL * qq]);
+    wk[qq] = V{T(c.x), T(c.y)};
+  }
+  wM = V{T(twN[M].x), T(twN[M].y)};
+  tw_load<T, S>(wt, lane, twN, 2);
+  // keep every load above in flight together: no arithmetic is scheduled
+  // ahead of the last one
+  __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+  for (int qq = 0; qq < P; ++qq) line[lpad(lane + TPL * qq)] = x[qq];
+  line[lpad(M)] = xM;  // every lane stores the same value: no branch for the load to sink into
+  line_sync<!S::WAVE>();
+#pragma unroll
+  for (int qq = 0; qq < P; ++qq) {
+    const int k = lane + TPL * qq;
+    V xk = x[qq];
+    V xm = line[lpad(M - k)];
+    if (k == 0) {
+      xk.y = T(0);
+      xm.y = T(0);
+    }
+    V s = V{xk.x + xm.x, xk.y - xm.y};
+    V d = V{xk.x - xm.x, xk.y + xm.y};
+    V wd = cmul(cconj(wk[qq]), d);
+    v[qq] = V{s.x - wd.y, s.y + wd.x};
+  }
+  reg_fft<T, S, true, !S::WAVE>(v, line, lane, wt);
+}
+
 // Row tile `tile` (blockDim.x / TPL rows) of active instance `yi`; every
 // thread of the block calls it.
 template <typename T, class S>
@@ -245,47 +294,10 @@ __device__ __forceinline__ void row_fast_tile(const V2<T>* __restrict__ Sg, V2<T
 
   // Loads are unconditional (rows past Ny read row Ny - 1; results are not
   // stored): a load under a branch makes the join wait for it, which would
-  // serialise it with the root loads below.
-  const int rowc = live ? row : Ny - 1;
-  V x[P];
-#pragma unroll
-  for (int qq = 0; qq < P; ++qq) x[qq] = Sg[lay.at(rowc, lane + TPL * qq)];
-  const V xM = Sg[lay.at(rowc, M)];
-  // roots: w_Nx^k for the (un)packing of k = lane + TPL q, as w_Nx^lane times
-  // the wave-uniform w_Nx^(TPL q) (scalar loads), in FP64; and the FFT roots
-  const V2<double> wl0 = twN[lane];
-  V wk[P];
-#pragma unroll
-  for (int qq = 0; qq < P; ++qq) {
-    const V2<double> c = cmul(wl0, twN[TPL * qq]);
-    wk[qq] = V{T(c.x), T(c.y)};
-  }
-  const V wM = V{T(twN[M].x), T(twN[M].y)};
+  // serialise it with the root loads.
+  V v[P], wk[P], wM;
   V wt[tw_count<S>()];
-  tw_load<T, S>(wt, lane, twN, 2);
-  // keep every load above in flight together: no arithmetic is scheduled
-  // ahead of the last one
-  __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-  for (int qq = 0; qq < P; ++qq) line[lpad(lane + TPL * qq)] = x[qq];
-  line[lpad(M)] = xM;  // every lane stores the same value: no branch for the load to sink into
-  line_sync<!S::WAVE>();
-  V v[P];
-#pragma unroll
-  for (int qq = 0; qq < P; ++qq) {
-    const int k = lane + TPL * qq;
-    V xk = x[qq];
-    V xm = line[lpad(M - k)];
-    if (k == 0) {
-      xk.y = T(0);
-      xm.y = T(0);
-    }
-    V s = V{xk.x + xm.x, xk.y - xm.y};
-    V d = V{xk.x - xm.x, xk.y + xm.y};
-    V wd = cmul(cconj(wk[qq]), d);
-    v[qq] = V{s.x - wd.y, s.y + wd.x};
-  }
-  reg_fft<T, S, true, !S::WAVE>(v, line, lane, wt);
+  row_c2r<T, S>(Sg, lay, live ? row : Ny - 1, lane, line, twN, v, wk, wM, wt);
   bool bad = false;
 #pragma unroll
   for (int qq = 0; qq < P; ++qq) {
@@ -322,6 +334,43 @@ __global__ void __launch_bounds__(256) k_row_fast(const V2<T>* __restrict__ Sg, 
   const int lin = xcd_remap(blockIdx.x + blockIdx.y * ntile, ntile * gridDim.y);
   const int yi = lin / ntile;
   row_fast_tile<T, S>(Sg, Tout, Ny, lay, q, twN, sl, yi, lin - yi * ntile, smem_raw);
+}
+
+// ---------------------------------------------------------------------------
+// range of the field for the stabilised split's check, one launch before each
+// step: the row pass's inverse transforms (row_c2r, the values the step's
+// nonlinearity will see) and the min and max of each row tile, written to
+// part[instance][tile]; k_shift_update<RangePart> then decides per slot.
+// ---------------------------------------------------------------------------
+template <typename T, class S>
+__global__ void __launch_bounds__(256) k_row_range(const V2<T>* __restrict__ Sg, int Ny, Lay lay,
+                                                   const V2<double>* __restrict__ twN, Slots sl,
+                                                   RangePart* __restrict__ part) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  using V = V2<T>;
+  constexpr int P = S::P, TPL = S::TPL, LD = row_ld(S::L);
+  const int ntile = gridDim.x;
+  const int lin = xcd_remap(blockIdx.x + blockIdx.y * ntile, ntile * gridDim.y);
+  const int yi = lin / ntile, tile = lin - yi * ntile;
+  Sg += (size_t)slot_of(sl, yi) * sl.sstride;
+  const int wl = threadIdx.x / TPL;
+  const int lane = threadIdx.x - wl * TPL;
+  const int row = tile * (blockDim.x / TPL) + wl;
+  const bool live = row < Ny;
+  V* line = reinterpret_cast<V*>(smem_raw) + wl * LD;
+  V v[P], wk[P], wM;
+  V wt[tw_count<S>()];
+  row_c2r<T, S>(Sg, lay, live ? row : Ny - 1, lane, line, twN, v, wk, wM, wt);
+  double mn = INFINITY, mx = -INFINITY;
+  if (live) {
+#pragma unroll
+    for (int qq = 0; qq < P; ++qq) {
+      mn = fmin(mn, fmin((double)v[qq].x, (double)v[qq].y));
+      mx = fmax(mx, fmax((double)v[qq].x, (double)v[qq].y));
+    }
+  }
+  block_minmax(mn, mx);
+  if (threadIdx.x == 0) part[(size_t)yi * ntile + tile] = RangePart{mn, mx};
 }
 
 // ---------------------------------------------------------------------------
@@ -485,6 +534,10 @@ bool row_fast_launch_f64(int M, int var, dim3 grid, dim3 block, size_t lds, hipS
 bool row_fast_launch_f32(int M, int var, dim3 grid, dim3 block, size_t lds, hipStream_t st, const V2<float>* S,
                          V2<float>* T, int Ny, const Lay& lay, const Nonlin& q, const V2<double>* tw,
                          const Slots& sl);
+bool row_range_launch_f64(int M, int var, dim3 grid, dim3 block, size_t lds, hipStream_t st, const V2<double>* S,
+                          int Ny, const Lay& lay, const V2<double>* tw, const Slots& sl, RangePart* part);
+bool row_range_launch_f32(int M, int var, dim3 grid, dim3 block, size_t lds, hipStream_t st, const V2<float>* S,
+                          int Ny, const Lay& lay, const V2<double>* tw, const Slots& sl, RangePart* part);
 bool col_fast_launch_f64(int Ny, int var, int mode, dim3 grid, dim3 block, size_t lds, hipStream_t st, V2<double>* S,
                          const V2<double>* T, V2<double>* D, int Nc, const Lay& lay, const ColTables& tb,
                          const V2<double>* tw, const Slots& sl);

@@ -22,4 +22,16 @@ bool row_fast_launch_f32(int M, int var, dim3 grid, dim3 block, size_t lds, hipS
   return false;
 }
 
+bool row_range_launch_f32(int M, int var, dim3 grid, dim3 block, size_t lds, hipStream_t st, const V2<float>* S,
+                          int Ny, const Lay& lay, const V2<double>* tw, const Slots& sl, RangePart* part) {
+#define PFC_X(LEN, VAR, PP, ...)                                                                             \
+  if (M == LEN && var == VAR) {                                                                              \
+    hipLaunchKernelGGL((k_row_range<float, Sch<LEN, PP, __VA_ARGS__>>), grid, block, lds, st, S, Ny, lay, tw, sl, part); \
+    return true;                                                                                             \
+  }
+  PFC_SCHEDULES(PFC_X)
+#undef PFC_X
+  return false;
+}
+
 }  // namespace pfc

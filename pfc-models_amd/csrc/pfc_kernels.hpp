@@ -1133,14 +1133,15 @@ static __global__ void __launch_bounds__(256) k_fe_final(const Partial* __restri
 }
 
 // The stabilised split's range check on the device, once per step before it
-// (logpfc_extv/simulation.py:375-405 with OptimizeM :332-373): from the row
-// partials of k_fe_row<MINMAX> (the min and max pfc_reduce reports), a slot
-// that has no fit yet, or whose minimum fell below its fitted floor or whose
-// maximum rose above its fitted ceiling by more than the relative
-// tolerances, is refitted: lo = min if min > 0 else minPhi, hi = max,
-// M = (N(hi) - N(lo)) / (hi - lo) with N(phi) = g phi^2 + phi^3 + ln phi,
-// written to the slot's shift and flagged for a table rebuild.  One block per
-// active slot.
+// (logpfc_extv/simulation.py:375-405 with OptimizeM :332-373): from the
+// field's min and max, a slot that has no fit yet, or whose minimum fell
+// below its fitted floor or whose maximum rose above its fitted ceiling by
+// more than the relative tolerances, is refitted: lo = min if min > 0 else
+// minPhi, hi = max, M = (N(hi) - N(lo)) / (hi - lo) with N(phi) = g phi^2 +
+// phi^3 + ln phi, written to the slot's shift and flagged for a table
+// rebuild: k_shift_update, one block per active slot, over row partials of
+// the field's min and max (from k_fe_row<MINMAX>, as pfc_reduce reports them,
+// or from k_row_range, the step's own inverse row transforms).
 struct ShiftState {
   double lo, hi;  // fitted window
   int fitted;
@@ -1150,46 +1151,71 @@ struct ShiftParams {
   double g, minPhi, floor_tol, ceil_tol;
 };
 
-static __global__ void __launch_bounds__(256) k_shift_update(const Partial* __restrict__ part, int nrow, Slots sl,
-                                                             int pstride, ShiftParams sp, ShiftState* __restrict__ st,
-                                                             double* __restrict__ Ms, int* __restrict__ rebuild) {
+// min and max over a block of at most 256 threads (any count: the fast row
+// shapes' blocks hold lines x lanes threads); every thread passes its own and
+// receives the block's
+__device__ __forceinline__ void block_minmax(double& mn, double& mx) {
+  __shared__ double rmn[256], rmx[256];
+  __syncthreads();  // earlier readers of the arrays are done
+  for (int i = threadIdx.x; i < 256; i += blockDim.x) {
+    rmn[i] = INFINITY;
+    rmx[i] = -INFINITY;
+  }
+  __syncthreads();
+  rmn[threadIdx.x] = mn;
+  rmx[threadIdx.x] = mx;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    for (int i = threadIdx.x; i < s; i += blockDim.x) {
+      rmn[i] = fmin(rmn[i], rmn[i + s]);
+      rmx[i] = fmax(rmx[i], rmx[i + s]);
+    }
+    __syncthreads();
+  }
+  mn = rmn[0];
+  mx = rmx[0];
+}
+
+// the refit decision for one slot from its field's min and max (one thread)
+__device__ __forceinline__ void shift_refit(double mn, double mx, int slot, const ShiftParams& sp,
+                                            ShiftState* __restrict__ st, double* __restrict__ Ms,
+                                            int* __restrict__ rebuild) {
+  ShiftState w = st[slot];
+  bool refit = !w.fitted;
+  if (!refit) {
+    const bool floor_breached = (w.lo - mn) / w.lo > sp.floor_tol;
+    const bool ceil_breached = (mx - w.hi) / w.hi > sp.ceil_tol;
+    refit = floor_breached || ceil_breached;
+  }
+  if (refit) {
+    const double lo = mn > 0.0 ? mn : sp.minPhi;
+    const double n_hi = sp.g * mx * mx + mx * mx * mx + log(mx);
+    const double n_lo = sp.g * lo * lo + lo * lo * lo + log(lo);
+    Ms[slot] = (n_hi - n_lo) / (mx - lo);
+    st[slot] = ShiftState{lo, mx, 1};
+  }
+  rebuild[slot] = refit ? 1 : 0;
+}
+
+// per-tile min and max of the field (k_row_range)
+struct RangePart {
+  double mn, mx;
+};
+
+// PartT: Partial (k_fe_row<MINMAX>) or RangePart (k_row_range), `nrow` per
+// instance, `pstride` apart
+template <class PartT>
+__global__ void __launch_bounds__(256) k_shift_update(const PartT* __restrict__ part, int nrow, Slots sl, int pstride,
+                                                      ShiftParams sp, ShiftState* __restrict__ st,
+                                                      double* __restrict__ Ms, int* __restrict__ rebuild) {
   part += (size_t)blockIdx.x * pstride;
-  const int slot = slot_of(sl, blockIdx.x);
   double mn = INFINITY, mx = -INFINITY;
   for (int b = threadIdx.x; b < nrow; b += blockDim.x) {
     mn = fmin(mn, part[b].mn);
     mx = fmax(mx, part[b].mx);
   }
-  __shared__ double rmn[256], rmx[256];
-  rmn[threadIdx.x] = mn;
-  rmx[threadIdx.x] = mx;
-  __syncthreads();
-  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
-    if (threadIdx.x < s) {
-      rmn[threadIdx.x] = fmin(rmn[threadIdx.x], rmn[threadIdx.x + s]);
-      rmx[threadIdx.x] = fmax(rmx[threadIdx.x], rmx[threadIdx.x + s]);
-    }
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) {
-    mn = rmn[0];
-    mx = rmx[0];
-    ShiftState w = st[slot];
-    bool refit = !w.fitted;
-    if (!refit) {
-      const bool floor_breached = (w.lo - mn) / w.lo > sp.floor_tol;
-      const bool ceil_breached = (mx - w.hi) / w.hi > sp.ceil_tol;
-      refit = floor_breached || ceil_breached;
-    }
-    if (refit) {
-      const double lo = mn > 0.0 ? mn : sp.minPhi;
-      const double n_hi = sp.g * mx * mx + mx * mx * mx + log(mx);
-      const double n_lo = sp.g * lo * lo + lo * lo * lo + log(lo);
-      Ms[slot] = (n_hi - n_lo) / (mx - lo);
-      st[slot] = ShiftState{lo, mx, 1};
-    }
-    rebuild[slot] = refit ? 1 : 0;
-  }
+  block_minmax(mn, mx);
+  if (threadIdx.x == 0) shift_refit(mn, mx, slot_of(sl, blockIdx.x), sp, st, Ms, rebuild);
 }
 
 }  // namespace pfc

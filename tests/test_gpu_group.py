@@ -104,6 +104,15 @@ def test_group_follows_active_lists(hip_device, monkeypatch):
     for a in alone:
         a.step(260)
     _same(grouped, alone, "changed again")
+    # same counts, other slots and order: the cached graph replays over the
+    # lists, which the members update in place
+    for lst in (grouped, alone):
+        lst[0].set_active([2, 3, 0, 1])
+        lst[1].set_active([0])
+    grp.step(64)
+    for a in alone:
+        a.step(64)
+    _same(grouped, alone, "same counts")
     assert grp.graph_stats()[0] == 2  # the 50-step call on the subset launched directly
 
 
