@@ -23,6 +23,7 @@ applies it (`coexistence.py`) at every r of the sweep.
 from __future__ import annotations
 
 import math
+import os
 import time
 from dataclasses import dataclass
 
@@ -126,6 +127,13 @@ class BatchedSweep:
         self.groups = {}
         for j, inst in enumerate(self.instances):
             self.groups.setdefault(inst.N, []).append(j)
+        # the seeded initial fields, drawn on host threads (NumPy's generators
+        # release the GIL; each field has its own seed, so the values do not
+        # depend on the thread count)
+        from concurrent.futures import ThreadPoolExecutor
+        workers = max(1, min(16, len(os.sched_getaffinity(0))))
+        with ThreadPoolExecutor(workers) as ex:
+            fields = list(ex.map(initial_field, self.instances))
         self.batches, self.streams = {}, {}
         for n, idx in self.groups.items():
             first = self.instances[idx[0]]
@@ -137,7 +145,7 @@ class BatchedSweep:
                     if (inst.Lx, inst.Ly) != (first.Lx, first.Ly):
                         raise ValueError("instances of one grid size must share the box")
                     b.set_slot(slot, inst.r)
-                    b.set_field(slot, initial_field(inst))
+                    b.set_field(slot, fields[j])
             self.batches[n], self.streams[n] = b, s
         torch.cuda.synchronize()
         self.group = None
