@@ -298,9 +298,9 @@ inline bool row_range_launch(int M, int var, dim3 g, dim3 b, size_t lds, hipStre
 template <typename T> struct FastLaunch;
 template <> struct FastLaunch<double> {
   static void row(int M, const FastShape& f, int ny_inst, hipStream_t st, const V2<double>* S, V2<double>* T, int Ny,
-                  const Lay& lay, const Nonlin& q, const V2<double>* tw, const Slots& sl) {
+                  const Lay& lay, const Nonlin& q, const V2<double>* tw, const Slots& sl, RangePart* rpart = nullptr) {
     if (!row_fast_launch_f64(M, f.var, dim3(f.blocks, ny_inst), dim3(f.lines * f.tpl), f.lds, st, S, T, Ny, lay, q,
-                             tw, sl))
+                             tw, sl, rpart))
       fail(PFC_EUNSUPPORTED, "no fast row schedule");
   }
   static void col(int Ny, int mode, const FastShape& f, int ny_inst, hipStream_t st, V2<double>* S,
@@ -313,9 +313,9 @@ template <> struct FastLaunch<double> {
 };
 template <> struct FastLaunch<float> {
   static void row(int M, const FastShape& f, int ny_inst, hipStream_t st, const V2<float>* S, V2<float>* T, int Ny,
-                  const Lay& lay, const Nonlin& q, const V2<double>* tw, const Slots& sl) {
+                  const Lay& lay, const Nonlin& q, const V2<double>* tw, const Slots& sl, RangePart* rpart = nullptr) {
     if (!row_fast_launch_f32(M, f.var, dim3(f.blocks, ny_inst), dim3(f.lines * f.tpl), f.lds, st, S, T, Ny, lay, q,
-                             tw, sl))
+                             tw, sl, rpart))
       fail(PFC_EUNSUPPORTED, "no fast row schedule");
   }
   static void col(int Ny, int mode, const FastShape& f, int ny_inst, hipStream_t st, V2<float>* S,
@@ -421,6 +421,12 @@ struct Engine {
     return ColTables{E1.p, Q1.p, Q2.p, table_rows(p.Ny), (const V2<double>*)extc.p, (long long)Nc * p.Ny};
   }
   int pstride() const { return row_blocks(rg64) + col_blocks(cg64); }
+  // ETD1 under the device-managed shift folds -M phi into the E1 table: the
+  // row pass computes N without it, so the range check can follow the step's
+  // own row pass (with the fast row shape, whose blocks are whole wavefronts,
+  // the row kernel writes the range partials itself)
+  bool fold_shift() const { return auto_shift && p.scheme == PFC_SCHEME_ETD1 && !resident; }
+  bool fused_range() const { return fold_shift() && frow.on && (frow.lines * frow.tpl) % 64 == 0; }
   Slots slots_active() const {
     return Slots{(const int*)active.p, (long long)nc, (long long)ntab, p.pin_mean ? (const double*)pin.p : nullptr,
                  (int*)status.p, (const double*)Ms.p};
@@ -543,10 +549,15 @@ void engine_init(Engine& e, const pfc_params* params, int cap, bool batched) {
   e.active.upload(e.h_active);
 }
 
+TabParams tab_params(const Engine& e) {
+  return TabParams{e.p.Gamma, e.p.beta, e.p.q0, 0.0, 0.0, e.p.dt, e.p.smallZTol,
+                   1.0 / e.p.Ny, 1.0 / ((double)e.p.Nx * (double)e.p.Ny), e.Nc, e.p.Ny, table_rows(e.p.Ny),
+                   e.fold_shift() ? 1 : 0};
+}
+
 // ETD tables of slots [slot0, slot0 + n), built in FP64 on the device
 void engine_tables(Engine& e, int slot0, int n, hipStream_t st) {
-  TabParams tp{e.p.Gamma, e.p.beta, e.p.q0, 0.0, 0.0, e.p.dt, e.p.smallZTol,
-               1.0 / e.p.Ny, 1.0 / ((double)e.p.Nx * (double)e.p.Ny), e.Nc, e.p.Ny, table_rows(e.p.Ny)};
+  const TabParams tp = tab_params(e);
   const int blocks = (int)std::min<size_t>((e.ntab + 255) / 256, 1024);
   const SlotGrid sg{(const double*)e.kx2.p, (const double*)e.ky2.p, (const double*)e.dts.p};
   hipLaunchKernelGGL(k_tables<double>, dim3(blocks, n), dim3(256), 0, st, (double*)e.E1.p, (double*)e.Q1.p,
@@ -580,13 +591,18 @@ Nonlin nonlin_of(const Engine& e) {
 // external-potential source) of the refitted slots only.  No host synchronisation; the launches can be
 // recorded in a step graph.
 template <typename T>
-void shift_check(Engine& e, hipStream_t st) {
+void shift_check(Engine& e, hipStream_t st, bool row_partials = false) {
   const int ny = e.nact();
   if (ny == 0) return;
   const Slots sl = e.slots_active();
   const FeParams fp{e.p.model, e.p.beta, e.p.q0, e.p.epsilon, e.p.g, (const double*)e.kx2.p,
                     (const double*)e.ky2.p, (const double*)e.eps.p};
-  if (e.frow.on) {
+  if (row_partials) {
+    // the step's row kernel wrote one range pair per row tile (fused_range)
+    hipLaunchKernelGGL(k_shift_update<RangePart>, dim3(ny), dim3(256), 0, st, (const RangePart*)e.rpart.p,
+                       e.frow.blocks, sl, e.frow.blocks, e.shp, (ShiftState*)e.shst.p, (double*)e.Ms.p,
+                       (int*)e.rebuildf.p);
+  } else if (e.frow.on) {
     // the fast row shape's inverse transforms
     const FastShape& f = e.frow;
     if (!row_range_launch(e.p.Nx / 2, f.var, dim3(f.blocks, ny), dim3(f.lines * f.tpl), f.lds, st,
@@ -603,8 +619,7 @@ void shift_check(Engine& e, hipStream_t st) {
     hipLaunchKernelGGL(k_shift_update<Partial>, dim3(ny), dim3(256), 0, st, (const Partial*)part, rb, sl, ps, e.shp,
                        (ShiftState*)e.shst.p, (double*)e.Ms.p, (int*)e.rebuildf.p);
   }
-  TabParams tp{e.p.Gamma, e.p.beta, e.p.q0, 0.0, 0.0, e.p.dt, e.p.smallZTol,
-               1.0 / e.p.Ny, 1.0 / ((double)e.p.Nx * (double)e.p.Ny), e.Nc, e.p.Ny, table_rows(e.p.Ny)};
+  const TabParams tp = tab_params(e);
   const SlotGrid sg{(const double*)e.kx2.p, (const double*)e.ky2.p, (const double*)e.dts.p};
   const int blocks = (int)std::min<size_t>((e.ntab + 255) / 256, 1024);
   hipLaunchKernelGGL(k_tables<double>, dim3(blocks, ny), dim3(256), 0, st, (double*)e.E1.p, (double*)e.Q1.p,
@@ -624,19 +639,23 @@ void shift_check(Engine& e, hipStream_t st) {
 template <typename T>
 struct Steps {
   using V = V2<T>;
-  static void row(Engine& e, hipStream_t st, const Slots& sl, int ny) {
-    row(e, st, sl, ny, (const V*)e.S.p, (V*)e.Tb.p);
+  static void row(Engine& e, hipStream_t st, const Slots& sl, int ny, RangePart* rpart = nullptr) {
+    row(e, st, sl, ny, (const V*)e.S.p, (V*)e.Tb.p, rpart);
   }
   static void col(Engine& e, int mode, hipStream_t st, const Slots& sl, int ny) {
     col(e, mode, st, sl, ny, (V*)e.S.p, (const V*)e.Tb.p, (V*)e.D.p);
   }
-  static void row(Engine& e, hipStream_t st, const Slots& sl, int ny, const V* S, V* Tb) {
+  static void row(Engine& e, hipStream_t st, const Slots& sl, int ny, const V* S, V* Tb,
+                  RangePart* rpart = nullptr) {
     const Nonlin q = nonlin_of(e);
+    Slots sr = sl;
+    if (e.fold_shift()) sr.M = nullptr;  // the shift acts through E1
     if (e.frow.on)
-      FastLaunch<T>::row(e.p.Nx / 2, e.frow, ny, st, S, Tb, e.p.Ny, e.lay, q, (const V2<double>*)e.twN64.p, sl);
+      FastLaunch<T>::row(e.p.Nx / 2, e.frow, ny, st, S, Tb, e.p.Ny, e.lay, q, (const V2<double>*)e.twN64.p, sr,
+                         rpart);
     else
       hipLaunchKernelGGL((k_row_nonlin<T>), dim3(row_blocks(e.rg), ny), dim3(256), row_lds(e.rg, sizeof(V)), st, S,
-                         Tb, e.rg, q, (const V*)e.twN.p, sl);
+                         Tb, e.rg, q, (const V*)e.twN.p, sr);
   }
   static void col(Engine& e, int mode, hipStream_t st, const Slots& sl, int ny, V* S, const V* Tb, V* D) {
     ColTables tb = e.tables();
@@ -700,7 +719,16 @@ struct Steps {
   static void loop(Engine& e, long long nsteps, hipStream_t st) {
     const int ny = e.nact();
     const Slots sl = e.slots_active();
+    const bool fused = e.fused_range();
     for (long long s = 0; s < nsteps; ++s) {
+      if (fused) {
+        // row pass (N without the shift, and the field's range), decision and
+        // rebuild, then the column pass with the refitted E1
+        row(e, st, sl, ny, (RangePart*)e.rpart.p);
+        shift_check<T>(e, st, true);
+        col(e, COL_ETD1, st, sl, ny);
+        continue;
+      }
       if (e.auto_shift) shift_check<T>(e, st);
       row(e, st, sl, ny);
       if (e.p.scheme == PFC_SCHEME_ETD1) {
@@ -1184,7 +1212,13 @@ void engine_set_auto_shift(Engine& e, int on, double floor_tol, double ceil_tol,
     hip_check(hipMemcpy(e.h_M.data(), e.Ms.p, (size_t)e.cap * sizeof(double), hipMemcpyDeviceToHost), "copy");
   }
   e.shp = ShiftParams{e.p.g, e.p.minPhi, floor_tol, ceil_tol};
+  const bool folded = e.fold_shift();
   e.auto_shift = on != 0;
+  if (e.fold_shift() != folded) {
+    // E1 with or without the folded shift, for every slot
+    engine_tables(e, 0, e.cap, st);
+    hip_check(hipStreamSynchronize(st), "sync");
+  }
 }
 
 // out = {M, lo, hi} of `slot` (lo = hi = NaN before the first fit); synchronises

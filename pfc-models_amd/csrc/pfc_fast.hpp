@@ -276,10 +276,15 @@ __device__ __forceinline__ void row_c2r(const V2<T>* __restrict__ Sg, const Lay&
 
 // Row tile `tile` (blockDim.x / TPL rows) of active instance `yi`; every
 // thread of the block calls it.
-template <typename T, class S>
+// RANGE (a block of whole wavefronts): the min and max of the tile's field,
+// the values the nonlinearity sees, are also written to `rtile`: the range
+// check of the stabilised split folded into the ETD1 tables.  A compile-time
+// choice: the ordinary kernels keep exactly the code the group kernels share.
+template <typename T, class S, bool RANGE = false>
 __device__ __forceinline__ void row_fast_tile(const V2<T>* __restrict__ Sg, V2<T>* __restrict__ Tout, int Ny,
                                               const Lay& lay, Nonlin q, const V2<double>* __restrict__ twN,
-                                              const Slots& sl, int yi, int tile, unsigned char* smem_raw) {
+                                              const Slots& sl, int yi, int tile, unsigned char* smem_raw,
+                                              RangePart* rtile = nullptr) {
   using V = V2<T>;
   constexpr int M = S::L, P = S::P, TPL = S::TPL, LD = row_ld(M);
   const int slot = slot_of(sl, yi);
@@ -298,6 +303,18 @@ __device__ __forceinline__ void row_fast_tile(const V2<T>* __restrict__ Sg, V2<T
   V v[P], wk[P], wM;
   V wt[tw_count<S>()];
   row_c2r<T, S>(Sg, lay, live ? row : Ny - 1, lane, line, twN, v, wk, wM, wt);
+  // the lane's min and max now; the block's after the stores, where the
+  // reduction overlaps their drain
+  double rmn = INFINITY, rmx = -INFINITY;
+  if constexpr (RANGE) {
+    if (live) {
+#pragma unroll
+      for (int qq = 0; qq < P; ++qq) {
+        rmn = fmin(rmn, fmin((double)v[qq].x, (double)v[qq].y));
+        rmx = fmax(rmx, fmax((double)v[qq].x, (double)v[qq].y));
+      }
+    }
+  }
   bool bad = false;
 #pragma unroll
   for (int qq = 0; qq < P; ++qq) {
@@ -321,19 +338,25 @@ __device__ __forceinline__ void row_fast_tile(const V2<T>* __restrict__ Sg, V2<T
     }
     if (lane == 0) store_stream(Tout + lay.at(row, M), row_unpack_pair<T>(line[0], line[0], wM));
   }
+  if constexpr (RANGE) {
+    block_minmax_waves(rmn, rmx);
+    if (threadIdx.x == 0) *rtile = RangePart{rmn, rmx};
+  }
 }
 
-template <typename T, class S>
+// RANGE: rpart holds [instance][tile] range pairs (row_fast_tile)
+template <typename T, class S, bool RANGE>
 __global__ void __launch_bounds__(256) k_row_fast(const V2<T>* __restrict__ Sg, V2<T>* __restrict__ Tout, int Ny,
                                                   Lay lay, Nonlin q, const V2<double>* __restrict__ twN,
-                                                  Slots sl) {
+                                                  Slots sl, RangePart* rpart) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   // XCD-aware order over (instance, row block): the blocks that share a row
   // group's 128-byte lines run on one XCD and meet in its L2
   const int ntile = gridDim.x;
   const int lin = xcd_remap(blockIdx.x + blockIdx.y * ntile, ntile * gridDim.y);
-  const int yi = lin / ntile;
-  row_fast_tile<T, S>(Sg, Tout, Ny, lay, q, twN, sl, yi, lin - yi * ntile, smem_raw);
+  const int yi = lin / ntile, tile = lin - yi * ntile;
+  row_fast_tile<T, S, RANGE>(Sg, Tout, Ny, lay, q, twN, sl, yi, tile, smem_raw,
+                             RANGE ? rpart + (size_t)yi * ntile + tile : nullptr);
 }
 
 // ---------------------------------------------------------------------------
@@ -530,10 +553,10 @@ struct SchedInfo {
 // returns false if (L, variant) has no schedule.
 bool row_fast_launch_f64(int M, int var, dim3 grid, dim3 block, size_t lds, hipStream_t st, const V2<double>* S,
                          V2<double>* T, int Ny, const Lay& lay, const Nonlin& q, const V2<double>* tw,
-                         const Slots& sl);
+                         const Slots& sl, RangePart* rpart);
 bool row_fast_launch_f32(int M, int var, dim3 grid, dim3 block, size_t lds, hipStream_t st, const V2<float>* S,
                          V2<float>* T, int Ny, const Lay& lay, const Nonlin& q, const V2<double>* tw,
-                         const Slots& sl);
+                         const Slots& sl, RangePart* rpart);
 bool row_range_launch_f64(int M, int var, dim3 grid, dim3 block, size_t lds, hipStream_t st, const V2<double>* S,
                           int Ny, const Lay& lay, const V2<double>* tw, const Slots& sl, RangePart* part);
 bool row_range_launch_f32(int M, int var, dim3 grid, dim3 block, size_t lds, hipStream_t st, const V2<float>* S,

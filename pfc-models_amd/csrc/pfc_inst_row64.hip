@@ -10,11 +10,15 @@ PFC_SCHEDULES(PFC_X)
 
 bool row_fast_launch_f64(int M, int var, dim3 grid, dim3 block, size_t lds, hipStream_t st, const V2<double>* S,
                          V2<double>* T, int Ny, const Lay& lay, const Nonlin& q, const V2<double>* tw,
-                         const Slots& sl) {
+                         const Slots& sl, RangePart* rpart) {
 #define PFC_X(LEN, VAR, PP, ...)                                                                          \
   if (M == LEN && var == VAR) {                                                                           \
-    hipLaunchKernelGGL((k_row_fast<double, Sch<LEN, PP, __VA_ARGS__>>), grid, block, lds, st, S, T, Ny, lay, q, tw, \
-                       sl);                                                                           \
+    if (rpart)                                                                                            \
+      hipLaunchKernelGGL((k_row_fast<double, Sch<LEN, PP, __VA_ARGS__>, true>), grid, block, lds, st, S, T, Ny, lay, \
+                         q, tw, sl, rpart);                                                               \
+    else                                                                                                  \
+      hipLaunchKernelGGL((k_row_fast<double, Sch<LEN, PP, __VA_ARGS__>, false>), grid, block, lds, st, S, T, Ny, lay, \
+                         q, tw, sl, rpart);                                                               \
     return true;                                                                                          \
   }
   PFC_SCHEDULES(PFC_X)

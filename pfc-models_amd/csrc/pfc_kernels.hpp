@@ -847,12 +847,15 @@ __global__ void __launch_bounds__(256) k_col_filter(const V2<T>* __restrict__ S,
 // (the ETD1 coefficients of kernels.py:55-71), L = Gamma lap (beta (q0^2 + lap)^2
 // + eps + M), lap = -(kx^2 + ky^2); P2 = dt (e^z - 1 - z)/z^2 (ETDRK2).
 // Stored column-major [Nc][Ny], pre-scaled for the row-spectrum state:
-// E1 = E/Ny, Q1 = P Gamma lap/(Nx Ny), Q2 = P2 Gamma lap/(Nx Ny).
+// E1 = E/Ny, Q1 = P Gamma lap/(Nx Ny), Q2 = P2 Gamma lap/(Nx Ny); with `fold`
+// (ETD1 under the device-managed shift) E1 = (E - M P Gamma lap)/Ny, which
+// takes the split's -M phi term out of the row pass.
 struct TabParams {
   double Gamma, beta, q0, epsilon, M, dt, smallZTol;
   double inv_ny, inv_n;
   int Nc, Ny;
-  int hy;  // stored rows per column: Ny/2 + 1 (see ky_half)
+  int hy;    // stored rows per column: Ny/2 + 1 (see ky_half)
+  int fold;  // ETD1 with the shift folded into E1 (the row pass computes N without -M phi)
 };
 
 // Per-slot box and time step: slot s reads kx^2 at kx2 + s*Nc, ky^2 at
@@ -919,7 +922,9 @@ __global__ void __launch_bounds__(256) k_tables(T* __restrict__ E1, T* __restric
     const int j = (int)(i - (size_t)c * p.hy);
     const double lap = -(kx2[c] + ky2[j]);
     const EtdCoef k = etd_coef(lap, p);
-    E1[i] = T(k.E * p.inv_ny);
+    // folded shift: E phi + P Gamma lap (N - M phi) = (E - M P Gamma lap) phi + P Gamma lap N
+    const double e = p.fold ? k.E - p.M * k.P * (p.Gamma * lap) : k.E;
+    E1[i] = T(e * p.inv_ny);
     Q1[i] = T(k.P * (p.Gamma * lap) * p.inv_n);
     if (Q2) Q2[i] = T(etd2_phi_dt(k.z, p.dt) * (p.Gamma * lap) * p.inv_n);
   }
@@ -1174,6 +1179,28 @@ __device__ __forceinline__ void block_minmax(double& mn, double& mx) {
   }
   mn = rmn[0];
   mx = rmx[0];
+}
+
+// min and max over a block of whole wavefronts (at most four): shuffles
+// within each wavefront, then one LDS exchange; the result is thread 0's
+__device__ __forceinline__ void block_minmax_waves(double& mn, double& mx) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    mn = fmin(mn, __shfl_xor(mn, o, 64));
+    mx = fmax(mx, __shfl_xor(mx, o, 64));
+  }
+  __shared__ double wmn[4], wmx[4];
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) {
+    wmn[w] = mn;
+    wmx[w] = mx;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0)
+    for (int i = 1; i < (int)(blockDim.x >> 6); ++i) {
+      mn = fmin(mn, wmn[i]);
+      mx = fmax(mx, wmx[i]);
+    }
 }
 
 // the refit decision for one slot from its field's min and max (one thread)

@@ -6,8 +6,10 @@ without a host round trip.  Against the host-driven check (the reference's
 per-step ValidatePhiRange / OptimizeM, `stabilise="host"`): the same fits
 (M to 1e-13 relative: the device's log and cube may round differently from the
 host's in the last place) and the same fields to 1e-11, on a crystallising
-instance whose growing range forces refits, on the two-kernel path (step
-graphs included) and on the resident small-grid path.  Batch slots equal
+instance whose growing range forces refits, on the fast two-kernel path (step
+graphs included; the ETD1 step takes the shift through its E1 table and its
+row kernel reports the range), on the generic kernels and on the resident
+small-grid path.  Batch slots equal
 plans bit for bit; a new field refits at the next step.
 """
 import numpy as np
@@ -30,8 +32,13 @@ def _sim(nx, ny, stabilise, precision=64, seed=5, mean=0.55):
     return sim
 
 
-@pytest.mark.parametrize("shape", [(96, 64), (28, 24)])
-def test_device_shift_follows_the_host_check(hip_device, shape):
+@pytest.mark.parametrize("shape,generic", [((96, 64), False), ((96, 64), True), ((28, 24), False)])
+def test_device_shift_follows_the_host_check(hip_device, monkeypatch, shape, generic):
+    # (96, 64): the fast row kernel reports the range in the step itself, the
+    # shift folded into the ETD1 table; generic kernels: the same fold after a
+    # separate range pass; (28, 24): the resident stepper, shift in the row pass
+    if generic:
+        monkeypatch.setenv("PFC_HIP_GENERIC", "1")
     nx, ny = shape
     dev, host = _sim(nx, ny, True), _sim(nx, ny, "host")
     assert dev._device_shift and host._host_shift
