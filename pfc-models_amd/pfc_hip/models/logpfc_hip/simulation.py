@@ -134,11 +134,11 @@ class Simulation(_Simulation):
 
 def BuildHipGPU_Simulation(modelParameters, geometry, *, precision=64, scheme="etd1", pin_mean=True,
                            clamp_log=None, stabilise=False):
-    """stabilise: False, True (the M-shifted split with the range check and refit
-    on the device before every step) or "host" (the same check on the host,
-    as logpfc_extv does, with a synchronisation per step)."""
     """Counterpart of BuildCupyGPU_Simulation (models/logpfc/simulation.py:265-281).
-    FP32 bulk always clamps the log (the guard near phi = 0)."""
+    FP32 bulk always clamps the log (the guard near phi = 0).  stabilise:
+    False, True (the M-shifted split with the range check and refit on the
+    device every step) or "host" (the same check on the host, as logpfc_extv
+    does, with a synchronisation per step)."""
     if clamp_log is None:
         clamp_log = precision == 32 or stabilise
     return Simulation(modelParameters, geometry, precision=precision, scheme=scheme, pin_mean=pin_mean,

@@ -22,25 +22,27 @@ pytestmark = pytest.mark.gpu
 Q0 = 1.0
 
 
-def _sim(nx, ny, stabilise, precision=64, seed=5, mean=0.55):
+def _sim(nx, ny, stabilise, precision=64, seed=5, mean=0.55, scheme="etd1"):
     from pfc_hip.models.logpfc_hip import simulation as S
     geom = S.Build2DGeometry(Nx=nx, Ny=ny, Lx=nx * np.pi / 4, Ly=ny * np.pi / 4)
     params = S.BuildLogPFCModelParameters(Gamma=1.0, beta=0.5, q0=Q0, epsilon=-0.9, g=-2.0, dt=0.1,
                                           meanDensity=mean)
-    sim = S.BuildHipGPU_Simulation(params, geom, precision=precision, stabilise=stabilise)
+    sim = S.BuildHipGPU_Simulation(params, geom, precision=precision, stabilise=stabilise, scheme=scheme)
     sim.State.phi[...] = F.uniform_noise((ny, nx), mean, 0.05, seed=seed)
     return sim
 
 
-@pytest.mark.parametrize("shape,generic", [((96, 64), False), ((96, 64), True), ((28, 24), False)])
-def test_device_shift_follows_the_host_check(hip_device, monkeypatch, shape, generic):
-    # (96, 64): the fast row kernel reports the range in the step itself, the
-    # shift folded into the ETD1 table; generic kernels: the same fold after a
-    # separate range pass; (28, 24): the resident stepper, shift in the row pass
+@pytest.mark.parametrize("shape,generic,scheme", [((96, 64), False, "etd1"), ((96, 64), True, "etd1"),
+                                                  ((28, 24), False, "etd1"), ((96, 64), False, "etdrk2")])
+def test_device_shift_follows_the_host_check(hip_device, monkeypatch, shape, generic, scheme):
+    # (96, 64) ETD1: the fast row kernel reports the range in the step itself,
+    # the shift folded into the ETD1 table; generic kernels: the same fold
+    # after a separate range pass; (28, 24): the resident stepper; ETDRK2: a
+    # range pass before the step, the shift in the row pass
     if generic:
         monkeypatch.setenv("PFC_HIP_GENERIC", "1")
     nx, ny = shape
-    dev, host = _sim(nx, ny, True), _sim(nx, ny, "host")
+    dev, host = _sim(nx, ny, True, scheme=scheme), _sim(nx, ny, "host", scheme=scheme)
     assert dev._device_shift and host._host_shift
     fits = set()
     # the range grows from [0.50, 0.60] to ~[0.11, 1.86] in the first ~300
