@@ -1241,7 +1241,7 @@ __global__ void __launch_bounds__(256) k_shift_update(const PartT* __restrict__ 
     mn = fmin(mn, part[b].mn);
     mx = fmax(mx, part[b].mx);
   }
-  block_minmax(mn, mx);
+  block_minmax_waves(mn, mx);  // 256 threads: whole wavefronts
   if (threadIdx.x == 0) shift_refit(mn, mx, slot_of(sl, blockIdx.x), sp, st, Ms, rebuild);
 }
 
