@@ -98,9 +98,11 @@ int pfc_reduce(pfc_plan* plan, double out[4], void* stream);
 int pfc_run_tracked(pfc_plan* plan, long long nsteps, long long every, double* out, void* stream);
 /* Rebuild the ETD tables with the stabilising shift M (logPFC). */
 int pfc_set_shift(pfc_plan* plan, double M);
-/* Stabilising shift managed on the device (logPFC; on = 1): before every
- * step the library takes the field's min and max (pfc_reduce's), and when the
- * slot has no fit yet, or the min fell below the fitted floor lo by more than
+/* Stabilising shift managed on the device (logPFC; on = 1): every step, before
+ * the shift acts, the library takes the field's min and max (the values the
+ * step's inverse row transforms give; pfc_reduce's where no fast row shape
+ * exists), and when the slot has no fit yet, or the min fell below the fitted
+ * floor lo by more than
  * floor_tol (relative) or the max rose above the fitted ceiling hi by more than
  * ceil_tol, refits lo = (min > 0 ? min : minPhi), hi = max,
  * M = (N(hi) - N(lo)) / (hi - lo), N(phi) = g phi^2 + phi^3 + ln phi, and
@@ -108,8 +110,11 @@ int pfc_set_shift(pfc_plan* plan, double M);
  * synchronisation.  Replaces logpfc_extv's per-step ValidatePhiRange /
  * OptimizeM (models/logpfc_extv/simulation.py:332-405: two device-to-host
  * copies and a host table rebuild per step; tolerances 0.1 there).  A new
- * field or slot parameters make the next step refit.  on = 0 hands M back to
- * pfc_set_shift.  Not with groups or per-slot step counts. */
+ * field or slot parameters make the next step refit.  Under ETD1 the shift
+ * is folded into the ETD table, (E - M P Gamma lap) n_hat + P Gamma lap N_hat
+ * with N computed without -M phi, so the check sits between the step's row and
+ * column passes (equal to the unfolded update up to rounding).  on = 0 hands M
+ * back to pfc_set_shift.  Not with groups or per-slot step counts. */
 int pfc_set_auto_shift(pfc_plan* plan, int on, double floor_tol, double ceil_tol, void* stream);
 /* out = {M, lo, hi} (lo = hi = NaN before the first fit); synchronises. */
 int pfc_get_shift(pfc_plan* plan, double out[3], void* stream);
