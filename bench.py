@@ -454,7 +454,7 @@ def main():
         print(json.dumps({"logpfc": run_logpfc(steps=a.logpfc_steps)}), flush=True)
         return
     if a.gpus != 1:
-        print("bench.py: this build measures one GPU; --gpus above 1 is not supported "
+        print("bench.py: the benchmark runs on one GPU; --gpus above 1 is not supported "
               "(independent instances would shard one scheduler per GPU)", file=sys.stderr)
         sys.exit(2)
     f64 = run_gpu(64, a.steps, a.warmup)
