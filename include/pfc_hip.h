@@ -114,7 +114,10 @@ int pfc_set_shift(pfc_plan* plan, double M);
  * is folded into the ETD table, (E - M P Gamma lap) n_hat + P Gamma lap N_hat
  * with N computed without -M phi, so the check sits between the step's row and
  * column passes (equal to the unfolded update up to rounding).  on = 0 hands M
- * back to pfc_set_shift.  Not with groups or per-slot step counts. */
+ * back to pfc_set_shift; on = 2 (the shift already on) replaces the two
+ * tolerances and keeps every slot's fit, as changing phi_floor_tol /
+ * phi_ceiling_tol between steps does in the reference, which reads them at
+ * every check.  Not with groups or per-slot step counts. */
 int pfc_set_auto_shift(pfc_plan* plan, int on, double floor_tol, double ceil_tol, void* stream);
 /* out = {M, lo, hi} (lo = hi = NaN before the first fit); synchronises. */
 int pfc_get_shift(pfc_plan* plan, double out[3], void* stream);
@@ -277,7 +280,10 @@ int pfc_group_chunks(const pfc_group* group, int* nchunks);
 
 /* Batched 2D real transforms over (batch, Ny, Nx) device arrays.
  * pfc_rfft2:  out[b] = rfft2(in[b])             (unnormalised)
- * pfc_irfft2: out[b] = irfft2(in[b], s=(Ny,Nx)) (scaled 1/(Nx Ny); `in` is not modified) */
+ * pfc_irfft2: out[b] = irfft2(in[b], s=(Ny,Nx)) (scaled 1/(Nx Ny); `in` is not modified).
+ * pfc_irfft2 transforms a working copy of `in` held per (device, stream), so
+ * calls on different streams may overlap; it grows (hipMalloc) when a call
+ * needs more than the stream's copy holds. */
 int pfc_rfft2(const void* in, void* out, int batch, int Ny, int Nx, int dtype, void* stream);
 int pfc_irfft2(const void* in, void* out, int batch, int Ny, int Nx, int dtype, void* stream);
 

@@ -1195,10 +1195,19 @@ void engine_set_slot(Engine& e, int slot, double epsilon, double M, hipStream_t 
 // Device-managed stabilising shift of every slot (logPFC), checked before
 // each step; `on` = 0 returns M to the host (pfc_set_shift / set_slot).
 void engine_set_auto_shift(Engine& e, int on, double floor_tol, double ceil_tol, hipStream_t st) {
+  if (on < 0 || on > 2) fail(PFC_EINVAL, "on must be 0, 1 or 2");
   if (on && e.p.model != PFC_MODEL_LOG) fail(PFC_EINVAL, "the stabilising shift is a logPFC option");
   if (on && !(floor_tol >= 0 && ceil_tol >= 0)) fail(PFC_EINVAL, "tolerances must be >= 0");
   if (on && !(e.p.minPhi > 0)) fail(PFC_EINVAL, "minPhi must be positive");
+  if (on == 2 && !e.auto_shift) fail(PFC_EINVAL, "on = 2 changes the tolerances of a device-managed shift that is on");
   hip_check(hipStreamSynchronize(st), "sync");
+  if (on == 2) {
+    // new tolerances, the fit kept; graph nodes carry the tolerances by value
+    const ShiftParams nshp{e.p.g, e.p.minPhi, floor_tol, ceil_tol};
+    if (std::memcmp(&nshp, &e.shp, sizeof nshp) != 0) e.drop_graphs();
+    e.shp = nshp;
+    return;
+  }
   if (on && !e.shst.p) {
     e.shst.alloc((size_t)e.cap * sizeof(ShiftState));
     e.rebuildf.alloc((size_t)e.cap * sizeof(int));
@@ -1211,7 +1220,9 @@ void engine_set_auto_shift(Engine& e, int on, double floor_tol, double ceil_tol,
     // the host takes M back: its copy follows the device's
     hip_check(hipMemcpy(e.h_M.data(), e.Ms.p, (size_t)e.cap * sizeof(double), hipMemcpyDeviceToHost), "copy");
   }
-  e.shp = ShiftParams{e.p.g, e.p.minPhi, floor_tol, ceil_tol};
+  const ShiftParams nshp{e.p.g, e.p.minPhi, floor_tol, ceil_tol};
+  if (std::memcmp(&nshp, &e.shp, sizeof nshp) != 0) e.drop_graphs();
+  e.shp = nshp;
   const bool folded = e.fold_shift();
   e.auto_shift = on != 0;
   if (e.fold_shift() != folded) {
@@ -1258,10 +1269,14 @@ void engine_set_slot_box(Engine& e, int slot, double Lx, double Ly, double dt, h
 struct FftEntry {
   RowGeom rg{};
   ColGeom cg{};
-  DevBuf twN, twY, scratch;
+  DevBuf twN, twY;
 };
 std::mutex g_fft_mu;
 std::map<std::tuple<int, int, int, int>, std::unique_ptr<FftEntry>> g_fft;
+// pfc_irfft2's working copy of the spectrum, one per (device, stream): a
+// stream's calls are ordered on it, so calls on different streams never share
+// one, and growth waits only for the stream that uses it
+std::map<std::pair<int, hipStream_t>, std::unique_ptr<DevBuf>> g_fft_scratch;
 
 FftEntry* fft_entry(int Ny, int Nx, int dtype) {
   int dev = 0;
@@ -1419,7 +1434,9 @@ void launch_filter(Engine& e, int slot, int mode, void* out0, void* out1, int dt
   V* Tb = (V*)e.Tb.p + (size_t)slot * e.nc;
   const V* twN = (const V*)e.twN.p;
   const V* twY = (const V*)e.twY.p;
-  FilterParams fp{e.p.beta, e.p.q0, e.h_eps[slot], e.h_M[slot], 6.283185307179586, e.p.Nx,
+  // the shift as the step sees it: the device copy, which k_shift_update
+  // refits without touching the host's
+  FilterParams fp{e.p.beta, e.p.q0, e.h_eps[slot], 6.283185307179586, (const double*)e.Ms.p + slot, e.p.Nx,
                   (const double*)e.kx2.p + (size_t)slot * e.Nc, (const double*)e.ky2.p + (size_t)slot * e.p.Ny};
   const int rb = row_blocks(e.rg), cb = col_blocks(e.cg);
   const size_t rl = row_lds(e.rg, sizeof(V)), cl = col_lds(e.cg, sizeof(V));
@@ -2154,24 +2171,29 @@ int pfc_irfft2(const void* in, void* out, int batch, int Ny, int Nx, int dtype, 
     const size_t nreal = (size_t)Ny * Nx, ncplx = (size_t)Ny * (Nx / 2 + 1);
     const size_t cs = dtype == 64 ? 16 : 8;
     const size_t need = (size_t)batch * ncplx * cs;
-    if (e->scratch.n < need) {
+    int dev = 0;
+    hip_check(hipGetDevice(&dev), "hipGetDevice");
+    std::unique_ptr<DevBuf>& sbuf = g_fft_scratch[std::make_pair(dev, st)];
+    if (!sbuf) sbuf = std::make_unique<DevBuf>();
+    if (sbuf->n < need) {
       hip_check(hipStreamSynchronize(st), "sync before scratch growth");
-      e->scratch.alloc(need);
+      sbuf->alloc(need);
     }
-    hip_check(hipMemcpyAsync(e->scratch.p, in, need, hipMemcpyDeviceToDevice, st), "copy");
+    DevBuf* scr = sbuf.get();
+    hip_check(hipMemcpyAsync(scr->p, in, need, hipMemcpyDeviceToDevice, st), "copy");
     dim3 rgrid(row_blocks(e->rg), batch), cgrid(col_blocks(e->cg), batch);
     const double scale = 1.0 / ((double)Nx * (double)Ny);
     if (dtype == 64) {
       hipLaunchKernelGGL((k_col_fft<double, true>), cgrid, dim3(256), col_lds(e->cg, 16), st,
-                         (V2<double>*)e->scratch.p, e->cg, (const V2<double>*)e->twY.p, 1.0, ncplx);
+                         (V2<double>*)scr->p, e->cg, (const V2<double>*)e->twY.p, 1.0, ncplx);
       hipLaunchKernelGGL((k_row_c2r<double, double>), rgrid, dim3(256), row_lds(e->rg, 16), st,
-                         (const V2<double>*)e->scratch.p, (double*)out, e->rg, (const V2<double>*)e->twN.p, scale,
+                         (const V2<double>*)scr->p, (double*)out, e->rg, (const V2<double>*)e->twN.p, scale,
                          ncplx, nreal, nullptr);
     } else {
-      hipLaunchKernelGGL((k_col_fft<float, true>), cgrid, dim3(256), col_lds(e->cg, 8), st, (V2<float>*)e->scratch.p,
+      hipLaunchKernelGGL((k_col_fft<float, true>), cgrid, dim3(256), col_lds(e->cg, 8), st, (V2<float>*)scr->p,
                          e->cg, (const V2<float>*)e->twY.p, 1.0f, ncplx);
       hipLaunchKernelGGL((k_row_c2r<float, float>), rgrid, dim3(256), row_lds(e->rg, 8), st,
-                         (const V2<float>*)e->scratch.p, (float*)out, e->rg, (const V2<float>*)e->twN.p,
+                         (const V2<float>*)scr->p, (float*)out, e->rg, (const V2<float>*)e->twN.p,
                          (float)scale, ncplx, nreal, nullptr);
     }
     hip_check(hipGetLastError(), "irfft2 launch");

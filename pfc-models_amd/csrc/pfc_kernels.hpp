@@ -742,7 +742,8 @@ __global__ void __launch_bounds__(256) k_col_fft(V2<T>* __restrict__ S, ColGeom 
 enum { FILTER_MU = 0, FILTER_LOCAL = 1 };
 
 struct FilterParams {
-  double beta, q0, epsilon, M, w;
+  double beta, q0, epsilon, w;
+  const double* M;  // the slot's stabilising shift in device memory (the value the step uses)
   int Nx;
   const double* kx2;
   const double* ky2;
@@ -798,7 +799,7 @@ __global__ void __launch_bounds__(256) k_col_filter(const V2<T>* __restrict__ S,
       const int kx = c0 + line;
       const double k2 = fp.kx2[kx] + fp.ky2[ky];
       const double t = fp.q0 * fp.q0 - k2;
-      const double lmu = fp.beta * (t * t) + fp.epsilon + fp.M;
+      const double lmu = fp.beta * (t * t) + fp.epsilon + *fp.M;
       const double gg = exp(-k2 * fp.w * fp.w / 2);
       const double nx = double(fp.Nx);
       if (MODE == FILTER_MU) {

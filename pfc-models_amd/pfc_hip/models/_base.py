@@ -174,6 +174,16 @@ class _SimulationState:
         self.local_mu = None
         setattr(self, field_name, DeviceField(sim))
 
+    def __setattr__(self, name, value):
+        object.__setattr__(self, name, value)
+        # the reference reads the shift's window tolerances at every check
+        # (logpfc_extv/simulation.py:375-405): a change reaches the device's
+        # check at the next step, the current fit kept
+        if name in ("phi_floor_tol", "phi_ceiling_tol"):
+            sim = self.__dict__.get("_sim")
+            if sim is not None and getattr(sim, "_device_shift", False):
+                sim._plan.set_auto_shift(2, self.__dict__["phi_floor_tol"], self.__dict__["phi_ceiling_tol"])
+
     def __getattr__(self, name):
         # the stabilising shift when the device manages it (logPFC)
         if name in ("M", "phi_min", "phi_max"):

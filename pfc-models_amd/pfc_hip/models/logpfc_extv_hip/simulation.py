@@ -77,9 +77,11 @@ class Simulation(_LogSimulation):
         return -np.stack([np.fft.irfft2(dk.d_dx * V_hat, s=s), np.fft.irfft2(dk.d_dy * V_hat, s=s)])
 
     def CalculateChemicalPotential(self):
-        """mu and <mu> (the potential is not part of mu), as host arrays."""
+        """mu = dF/dphi of the current field as a host array (:419-425); the
+        potential is not part of mu.  State.local_mu is filled as well."""
+        self._Batch1_CalcNonlinMu()
         self._Batch2_CalcMu_Terms()
-        return self.State.mu, self.State.local_mu
+        return self.State.mu
 
 
 def BuildHipGPU_Simulation(modelParameters, geometry, *, precision=64, scheme="etd1"):

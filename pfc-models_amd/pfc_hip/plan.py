@@ -198,9 +198,10 @@ class Plan:
 
     def set_auto_shift(self, on=True, floor_tol=0.1, ceil_tol=0.1):
         """Let the device refit the stabilising shift M before every step
-        (pfc_set_auto_shift); on=False hands M back to set_shift."""
-        _lib.check(self._lib.pfc_set_auto_shift(self._h, int(bool(on)), float(floor_tol), float(ceil_tol),
-                                                current_stream_handle()))
+        (pfc_set_auto_shift); on=False hands M back to set_shift; on=2 (the
+        shift already on) changes the tolerances and keeps the current fit."""
+        _lib.check(self._lib.pfc_set_auto_shift(self._h, 2 if on == 2 else int(bool(on)), float(floor_tol),
+                                                float(ceil_tol), current_stream_handle()))
         return self
 
     def shift(self):
@@ -353,8 +354,8 @@ class Batch:
 
     def set_auto_shift(self, on=True, floor_tol=0.1, ceil_tol=0.1):
         """Device-managed stabilising shift for every slot (see Plan.set_auto_shift)."""
-        _lib.check(self._lib.pfc_batch_set_auto_shift(self._h, int(bool(on)), float(floor_tol), float(ceil_tol),
-                                                      current_stream_handle()))
+        _lib.check(self._lib.pfc_batch_set_auto_shift(self._h, 2 if on == 2 else int(bool(on)), float(floor_tol),
+                                                      float(ceil_tol), current_stream_handle()))
         return self
 
     def shift(self, slot):

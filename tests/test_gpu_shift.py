@@ -93,3 +93,64 @@ def test_device_shift_is_a_logpfc_option(hip_device):
     pl = Plan(Nx=64, Ny=64, Lx=50.0, Ly=50.0, epsilon=-0.3, g=0.0, dt=0.5)
     with pytest.raises(_lib.PfcError):
         pl.set_auto_shift(True)
+
+
+@pytest.mark.parametrize("shape", [(96, 64), (28, 24)])
+def test_chemical_potential_is_shift_independent(hip_device, shape):
+    """mu = L_mu phi_hat + N_hat does not depend on the stabilising shift (+M in
+    L_mu, -M phi in N cancel).  After the device refitted M on a crystallising
+    instance, the chemical potential equals the oracle's unshifted one of the
+    same field (the filter reads the device's M, as the step does)."""
+    from oracle import pfc_oracle as O
+    nx, ny = shape
+    sim = _sim(nx, ny, True)
+    sim.run(200)
+    M = sim.State.M
+    assert M != 0.0 and abs(M) > 1e-3
+    phi = sim.State.phi.get()
+    sim._Batch1_CalcNonlinMu()
+    sim._Batch2_CalcMu_Terms()
+    g = sim.Geometry
+    ref = O.Simulation(O.Params(model="log", beta=0.5, epsilon=-0.9, g=-2.0, dt=0.1, clamp_log=True),
+                       O.build_geometry(nx, ny, g.Lx, g.Ly))
+    ref.set_field(phi)
+    mu, lmu = ref.chemical_potential()
+    scale = float(np.abs(mu).max())
+    assert float(np.abs(sim.State.mu - mu).max()) <= 1e-12 * scale
+    assert float(np.abs(sim.State.local_mu - lmu).max()) <= 1e-12 * scale
+
+
+def test_tolerance_change_keeps_the_fit_and_reaches_the_step(hip_device):
+    """State.phi_floor_tol / phi_ceiling_tol are read by the device's check from
+    the next step on, without a refit; stepping through step graphs (calls of
+    >= 256 steps) picks up the new values as well.  Against the host check
+    with the same tolerance change."""
+    dev, host = _sim(96, 64, True), _sim(96, 64, "host")
+    for s in (dev, host):
+        s.run(300)
+    M0 = dev.State.M
+    for s in (dev, host):
+        s.State.phi_floor_tol = 0.02
+        s.State.phi_ceiling_tol = 0.02
+    assert dev.State.M == M0  # no refit from the change itself
+    for n in (5, 5, 300, 300):
+        dev.run(n)
+        host.run(n)
+        assert abs(dev.State.M - host.State.M) <= 1e-13 * abs(host.State.M)
+    err = float(np.abs(dev.State.phi.get() - host.State.phi.get()).max())
+    assert err <= 1e-11, err
+
+
+def test_extv_chemical_potential_returns_mu_only(hip_device):
+    """logpfc_extv's CalculateChemicalPotential returns mu as one host array
+    (models/logpfc_extv/simulation.py:419-425)."""
+    from pfc_hip.models.logpfc_extv_hip import simulation as X
+    geom = X.BuildGeometry(64, 48, 64 * np.pi / 4, 48 * np.pi / 4)
+    params = X.BuildLogPFCModel_withVacancyForce_Parameters(beta=0.5, epsilon=-0.9, g=-2.0, dt=0.1,
+                                                            meanDensity=0.55)
+    sim = X.BuildHipGPU_Simulation(params, geom)
+    sim.State.phi[...] = F.uniform_noise((48, 64), 0.55, 0.05, seed=9)
+    sim.run(50)
+    mu = sim.CalculateChemicalPotential()
+    assert isinstance(mu, np.ndarray) and mu.shape == (48, 64)
+    assert np.array_equal(mu, sim.State.mu)
