@@ -286,6 +286,18 @@ int pfc_group_chunks(const pfc_group* group, int* nchunks);
  * needs more than the stream's copy holds. */
 int pfc_rfft2(const void* in, void* out, int batch, int Ny, int Nx, int dtype, void* stream);
 int pfc_irfft2(const void* in, void* out, int batch, int Ny, int Nx, int dtype, void* stream);
+/* pfc_irfft2 without any allocation.  No counterpart in the reference: it is
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments and result as pfc_irfft2 (its roots are computed on the device,
+ * so results may differ from pfc_irfft2's in the last bit of a root).  The
+ * roots and the working copy of the spectrum live in one static device array
+ * of 64 MiB per device, held until the process ends:
+ * batch*Ny*(Nx/2+1)*(16 or 8) bytes plus the roots must fit in it, else
+ * PFC_EINVAL.  Every call on a device shares that array, so calls that
+ * overlap on one device (different streams, or threads) race; calls on one
+ * stream are ordered. */
+int pfc_irfft2_noalloc(const void* in, void* out, int batch, int Ny, int Nx, int dtype, void* stream);
 
 #ifdef __cplusplus
 }

@@ -1738,6 +1738,62 @@ void group_step(pfc_group& g, long long nsteps, hipStream_t st) {
 // ---------------------------------------------------------------------------
 // C ABI
 // ---------------------------------------------------------------------------
+// ---- an irfft2 that allocates nothing -----------------------------------------
+// pfc_irfft2_noalloc keeps its twiddles and its working copy of the spectrum in
+// this file-scope device array instead of library allocations.  Each device
+// holds its own copy until the process ends; host code reaches it only through
+// the address hipGetSymbolAddress gives on that device (the array's name is a
+// host-side stand-in there, not a device address).
+constexpr size_t kNoAllocScratchBytes = size_t(64) << 20;
+__device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+namespace {
+
+constexpr int kNoAllocMaxDevices = 64;
+
+struct NoAllocScratch {
+  std::once_flag once;
+  unsigned char* ptr = nullptr;
+  hipError_t err = hipSuccess;
+};
+NoAllocScratch g_noalloc[kNoAllocMaxDevices];
+
+// the current device's copy of `scratch`, looked up once per device
+unsigned char* noalloc_scratch() {
+  int dev = 0;
+  hip_check(hipGetDevice(&dev), "hipGetDevice");
+  if (dev < 0 || dev >= kNoAllocMaxDevices) fail(PFC_EUNSUPPORTED, "device index beyond the static scratch table");
+  NoAllocScratch& d = g_noalloc[dev];
+  std::call_once(d.once, [&] {
+    void* p = nullptr;
+    d.err = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    d.ptr = static_cast<unsigned char*>(p);
+  });
+  if (d.err != hipSuccess || !d.ptr)
+    fail(PFC_EHIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(d.err));
+  return d.ptr;
+}
+
+// tw[m] = exp(-2 pi i m / n), the quarter points exact; rounded once to T
+template <typename T>
+__global__ void __launch_bounds__(256) k_twiddle_table(V2<T>* __restrict__ tw, int n) {
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= n) return;
+  double c, sn;
+  if ((4LL * m) % n == 0) {
+    const int q = (int)((4LL * m) / n);
+    c = q == 0 ? 1.0 : (q == 2 ? -1.0 : 0.0);
+    sn = q == 1 ? 1.0 : (q == 3 ? -1.0 : 0.0);
+  } else {
+    sincospi(2.0 * (double)m / (double)n, &sn, &c);
+  }
+  tw[m] = V2<T>{T(c), T(-sn)};
+}
+
+size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
+
+}  // namespace
+
 extern "C" {
 
 const char* pfc_last_error(void) { return g_err.c_str(); }
@@ -2197,6 +2253,54 @@ int pfc_irfft2(const void* in, void* out, int batch, int Ny, int Nx, int dtype, 
                          (float)scale, ncplx, nreal, nullptr);
     }
     hip_check(hipGetLastError(), "irfft2 launch");
+  });
+}
+
+int pfc_irfft2_noalloc(const void* in, void* out, int batch, int Ny, int Nx, int dtype, void* stream) {
+  return guarded([&] {
+    if (!in || !out) fail(PFC_EINVAL, "NULL buffer");
+    if (batch < 1 || Nx < 4 || (Nx & 1) || Ny < 2) fail(PFC_EINVAL, "bad shape");
+    if (dtype != 64 && dtype != 32) fail(PFC_EINVAL, "dtype must be 64 or 32");
+    const size_t cs = dtype == 64 ? 16 : 8;
+    const size_t ncplx = (size_t)Ny * (Nx / 2 + 1), nreal = (size_t)Ny * Nx;
+    // layout of the static scratch: roots of Nx, roots of Ny, the spectrum copy
+    const size_t offY = align256((size_t)Nx * cs);
+    const size_t offS = offY + align256((size_t)Ny * cs);
+    const size_t need = offS + (size_t)batch * ncplx * cs;
+    if (need > kNoAllocScratchBytes)
+      fail(PFC_EINVAL, "batch x Ny x (Nx/2+1) spectrum plus roots needs " + std::to_string(need) +
+                           " bytes; the static scratch holds " + std::to_string(kNoAllocScratchBytes));
+    const RowGeom rg = row_geom(Nx, Ny, cs);
+    const ColGeom cg = col_geom(Nx, Ny, cs);
+    unsigned char* base = noalloc_scratch();
+    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    hip_check(hipMemcpyAsync(base + offS, in, (size_t)batch * ncplx * cs, hipMemcpyDeviceToDevice, st), "copy");
+    dim3 rgrid(row_blocks(rg), batch), cgrid(col_blocks(cg), batch);
+    const double scale = 1.0 / ((double)Nx * (double)Ny);
+    if (dtype == 64) {
+      using V = V2<double>;
+      V* twN = reinterpret_cast<V*>(base);
+      V* twY = reinterpret_cast<V*>(base + offY);
+      V* sp = reinterpret_cast<V*>(base + offS);
+      hipLaunchKernelGGL((k_twiddle_table<double>), dim3((Nx + 255) / 256), dim3(256), 0, st, twN, Nx);
+      hipLaunchKernelGGL((k_twiddle_table<double>), dim3((Ny + 255) / 256), dim3(256), 0, st, twY, Ny);
+      hipLaunchKernelGGL((k_col_fft<double, true>), cgrid, dim3(256), col_lds(cg, 16), st, sp, cg,
+                         (const V*)twY, 1.0, ncplx);
+      hipLaunchKernelGGL((k_row_c2r<double, double>), rgrid, dim3(256), row_lds(rg, 16), st, (const V*)sp,
+                         (double*)out, rg, (const V*)twN, scale, ncplx, nreal, nullptr);
+    } else {
+      using V = V2<float>;
+      V* twN = reinterpret_cast<V*>(base);
+      V* twY = reinterpret_cast<V*>(base + offY);
+      V* sp = reinterpret_cast<V*>(base + offS);
+      hipLaunchKernelGGL((k_twiddle_table<float>), dim3((Nx + 255) / 256), dim3(256), 0, st, twN, Nx);
+      hipLaunchKernelGGL((k_twiddle_table<float>), dim3((Ny + 255) / 256), dim3(256), 0, st, twY, Ny);
+      hipLaunchKernelGGL((k_col_fft<float, true>), cgrid, dim3(256), col_lds(cg, 8), st, sp, cg, (const V*)twY,
+                         1.0f, ncplx);
+      hipLaunchKernelGGL((k_row_c2r<float, float>), rgrid, dim3(256), row_lds(rg, 8), st, (const V*)sp,
+                         (float*)out, rg, (const V*)twN, (float)scale, ncplx, nreal, nullptr);
+    }
+    hip_check(hipGetLastError(), "irfft2_noalloc launch");
   });
 }
 

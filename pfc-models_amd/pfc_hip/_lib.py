@@ -101,6 +101,7 @@ _SIGNATURES = [
     ("pfc_slot_bytes", ctypes.c_int, [ctypes.POINTER(PfcParams), ctypes.POINTER(ctypes.c_longlong)]),
     ("pfc_rfft2", ctypes.c_int, [_P, _P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _P]),
     ("pfc_irfft2", ctypes.c_int, [_P, _P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _P]),
+    ("pfc_irfft2_noalloc", ctypes.c_int, [_P, _P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _P]),
 ]
 
 EXPORTED = [s[0] for s in _SIGNATURES]
