@@ -56,7 +56,7 @@ except Exception:  # pragma: no cover - scipy is present in this image
 # ---------------------------------------------------------------------------
 @dataclass
 class Geometry:
-    """2D periodic spectral geometry (discretization/spectral/geometry_2d.py:109-159)."""
+    """2D periodic spectral geometry (discretization/spectral/geometry_2d.py:38-88)."""
     Nx: int
     Ny: int
     Lx: float
@@ -75,7 +75,7 @@ class Geometry:
 
 
 def build_geometry(Nx, Ny, Lx, Ly):
-    """Restates Build2DGeometry (geometry_2d.py:141-159): kx from rfftfreq, ky from
+    """Restates Build2DGeometry (geometry_2d.py:70-88): kx from rfftfreq, ky from
     fftfreq, both times 2 pi; meshgrids in C order (Ny, Nx) / (Ny, Nx//2+1)."""
     g = Geometry(Nx=int(Nx), Ny=int(Ny), Lx=Lx, Ly=Ly)
     g.dx = g.Lx / g.Nx

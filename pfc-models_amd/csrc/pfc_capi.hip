@@ -330,7 +330,7 @@ template <> struct FastLaunch<float> {
 // ---------------------------------------------------------------------------
 // FP64 k-grids (the reference's host arithmetic, restated)
 // ---------------------------------------------------------------------------
-// kx = 2 pi * rfftfreq(Nx, dx), ky = 2 pi * fftfreq(Ny, dy)  (geometry_2d.py:154-155)
+// kx = 2 pi * rfftfreq(Nx, dx), ky = 2 pi * fftfreq(Ny, dy)  (geometry_2d.py:83-84)
 std::vector<double> rfft_k(int n, double d) {
   const double val = 1.0 / (n * d);
   std::vector<double> k(n / 2 + 1);

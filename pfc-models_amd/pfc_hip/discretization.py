@@ -1,7 +1,7 @@
 """Host-side 2D periodic spectral geometry and Fourier-operator symbols.
 
 The names and the contract are the reference substrate's
-(discretization/spectral/geometry_2d.py:109-159, kernels.py:33-71), so a
+(discretization/spectral/geometry_2d.py:38-88, kernels.py:33-71), so a
 caller written against them works unchanged with the `hip` models.  These
 FP64 NumPy arrays describe the grid for callers (coordinates, k-grids, the
 Gaussian symbol); the device ETD tables are built by libpfc_hip.so itself
@@ -14,7 +14,7 @@ import numpy as np
 
 class Geometry:
     """Nx, Ny, Lx, Ly, dx, dy, x, y, X, Y, kx, ky, KX, KY, K2 and the
-    coordinate/index conversions (geometry_2d.py:109-138)."""
+    coordinate/index conversions (geometry_2d.py:38-67)."""
 
     def __init__(self):
         self.Nx = None

@@ -1,41 +1,49 @@
-"""Coexistence between free-energy branches of one conserved density.
+"""Coexistence of free-energy branches of one conserved density.
 
-Host-side consumer of relaxed free energies (q0* scans, sweeps); restates
-`phase_diagram/common_tangent.py` of the reference (API at :98-113, branches
-:127-403, pairwise tangent :410-600, lower envelope :603-845, Maxwell lines
-:847-989) so that tables, envelopes and Maxwell-line CSVs produced from this
-framework's fmin curves match the reference's.
+The drop-in counterpart of the reference's phase_diagram/common_tangent.py: the
+same public names (Segment, Piece, Branch, TangentResult, Window, Envelope,
+TripleBracket, MaxwellLines, analytic_branch, tabulated_branch,
+common_tangent, lower_envelope, maxwell_lines), result fields, status names
+and CSV columns, so tables written by either read the same.  The
+construction is this module's own.
 
-A branch is the free energy per site f(phi) of one phase against the mean
-density phi: analytic (a uniform state's closed form) or tabulated (fmin at a
-list of densities, NaN where a relaxation failed).  A tabulated branch is one
-not-a-knot cubic spline per run of finite points ("segment"); the NaN runs
-between segments are its gaps.  Every segment is split at the zeros of f''
-into convex pieces (f'' > 0: mu = f' strictly increasing, so phi(mu) is single
-valued) and nonconvex intervals.
+A branch f(phi) is the free energy per site of one phase as a function of the
+mean density: analytic (callables, e.g. a uniform state) or tabulated (the
+relaxed fmin(phi) of a q0* scan, a not-a-knot cubic spline per NaN-free run).
+It is cut into convex pieces (f'' > 0, so mu = f' is increasing and has an
+inverse) and concave intervals.  Two phases coexist where one line touches
+both: equal mu and equal grand potential omega = f - mu phi.
 
-Common tangent of pieces A and B: a chemical potential mu with equal grand
-potential omega = f - mu phi on both, i.e. a root of
-omega_A(mu) - omega_B(mu) on the overlap of their mu ranges.  The difference
-is scanned at `n_scan` points and every sign change refined by brentq in mu
-(xtol 1e-15, rtol 4 eps), each omega evaluation inverting mu(phi) on its piece
-by brentq.  Statuses: ok, no_overlap (mu ranges disjoint), no_root (convex mu
-ranges overlap, no sign change; also a self-pair of a branch convex
-throughout), non_convex (no convex piece, or overlap only through concave
-parts).  Among several tangents the primary one is a supporting one (no branch
-of the pair below its line by more than atol), then the one nearest
-`mu_guess`, else the lowest mu.
+Tangents.  For convex pieces P (of branch a) and Q (of b) with overlapping mu
+ranges, let x run over the densities of P whose slope Q also takes, y(x) the
+density on Q with mu_b(y) = mu_a(x), and
 
-Lower envelope at one temperature: the lower convex hull of every branch
-sampled on a density grid gives the single-phase stretches in density order;
-neighbouring stretches are bridged by the ordered tangent of their pieces;
-then, until nothing changes, a window whose two bridges cross is removed and a
-branch dipping below a bridge's line is inserted as a window.  Every other
-tangent is metastable, labelled with a branch below its line.  Maxwell lines:
-envelopes over temperatures as row tables (CSV), with triple points bracketed
-where a window between L and R at one temperature is gone (L and R adjacent)
-at the next, its signed width interpolated to zero from the metastable L-X and
-X-R tangents when both exist.
+    h(x) = f_a(x) - f_b(y(x)) - mu_a(x) (x - y(x)),
+
+the height of P's tangent line at x above Q's parallel tangent line.  A root
+of h is a common tangent; h' = -f_a''(x) (x - y), so h is monotone while the
+two contact points keep their order.  h is sampled at `n_scan` densities,
+every sign change is refined by Brent's method, and y(x) is found by a
+safeguarded Newton iteration on mu_b (f'' where the branch has it).
+
+Envelope.  The lower convex hull of the convex pieces of all branches is
+built exactly by gift wrapping, from the leftmost density: on the current
+piece, the hull leaves at the tangent of smallest slope towards any piece to
+the right (tangent slopes grow along a convex piece, so this is the first
+contact of a line rotated about the curve); with no such tangent it runs to
+the piece's end and jumps by the chord of smallest slope.  Every other
+pairwise tangent is metastable and names the branch that dips deepest below
+its line.  No sampling grid decides the hull; `phi_grid` (or `n` points) only
+tabulates the envelope in `Envelope.hull_phi/hull_f`.
+
+Statuses of a pair without a tangent (the reference's contract):
+``no_overlap`` (mu ranges disjoint, concave parts included), ``no_root``
+(convex pieces overlap in mu but never touch one line; also a self pair of a
+branch with one convex piece) and ``non_convex`` (no convex piece, or
+overlap through concave parts only).
+
+phi: mean density; mu = df/dphi; omega = f - mu phi; `temp`: the model's
+temperature-like parameter (epsilon for the PFC models).
 """
 from __future__ import annotations
 
@@ -48,28 +56,27 @@ import numpy as np
 from scipy.interpolate import CubicSpline
 from scipy.optimize import brentq
 
-__all__ = [
-    "STATUSES", "Segment", "Piece", "Branch", "TangentResult", "Window", "Envelope", "TripleBracket",
-    "MaxwellLines", "analytic_branch", "tabulated_branch", "common_tangent", "lower_envelope", "maxwell_lines",
-]
+__all__ = ["STATUSES", "Segment", "Piece", "Branch", "TangentResult", "Window", "Envelope", "TripleBracket",
+           "MaxwellLines", "analytic_branch", "tabulated_branch", "common_tangent", "lower_envelope",
+           "maxwell_lines"]
 
 STATUSES = ("ok", "no_overlap", "no_root", "non_convex")
 
-_RTOL = 4.0 * np.finfo(float).eps
-_XTOL = 1e-15
+_EPS = np.finfo(float).eps
 
 
-def _val(fn, x) -> float:
-    """fn at one density, as a Python float (fn takes numpy arrays)."""
-    return float(np.asarray(fn(np.array([float(x)])), dtype=float).ravel()[0])
+def _scalar(fn, x):
+    """fn at one density, as a float (callables are vectorised)."""
+    return float(np.asarray(fn(np.array([float(x)])), dtype=float).reshape(-1)[0])
 
 
-# ----------------------------------------------------------------------------- branches
+# ---------------------------------------------------------------------------
+# branch data
+# ---------------------------------------------------------------------------
 @dataclass(frozen=True)
 class Segment:
-    """A contiguous density interval of a branch with its own f, mu (= f'),
-    f'' (or None) and q0* (or None); `nodes` are a tabulated segment's table
-    densities."""
+    """A density interval [lo, hi] with one representation: vectorised `f`,
+    `mu` (= f'), optional `d2f` and `q0star`; `nodes` are a table's densities."""
     lo: float
     hi: float
     f: Callable
@@ -81,7 +88,7 @@ class Segment:
 
 @dataclass(frozen=True)
 class Piece:
-    """Convex part (f'' > 0) of segment `segment`, with mu at its ends."""
+    """A convex part of segment `segment`: mu rises from mu_lo to mu_hi on [lo, hi]."""
     segment: int
     lo: float
     hi: float
@@ -91,12 +98,9 @@ class Piece:
 
 @dataclass(frozen=True)
 class Branch:
-    """Free energy of one phase over one or more density segments.
-
-    `pieces` (convex parts, density order) are what tangents use; `nonconvex`
-    lists the (lo, hi) intervals with f'' <= 0; `gaps` the density gaps between
-    segments (NaN runs of a table); `dropped` the densities of finite runs too
-    short to spline."""
+    """f(phi) of one phase: `segments` (one per NaN-free run of a table),
+    their convex `pieces` and concave intervals `nonconvex` in density order,
+    `gaps` between segments and `dropped` table densities (runs too short)."""
     name: str
     kind: str
     segments: tuple
@@ -104,7 +108,7 @@ class Branch:
     nonconvex: tuple
     gaps: tuple = ()
     dropped: tuple = ()
-    _samples: tuple = field(default=(), repr=False, compare=False)
+    _dense: tuple = field(default=(), repr=False, compare=False)  # (phi, f, mu) samples
 
     @property
     def lo(self) -> float:
@@ -120,72 +124,66 @@ class Branch:
 
     @property
     def mu_range(self) -> tuple:
-        """(min, max) of mu over the dense samples, concave parts included."""
-        m = self._samples[2]
-        return float(np.min(m)), float(np.max(m))
+        """(min, max) of mu over the sampled segments, concave parts included."""
+        mu = self._dense[2]
+        return float(mu.min()), float(mu.max())
 
-    def _where(self, which, phi):
+    def _apply(self, which, phi):
         x = np.asarray(phi, dtype=float)
-        flat = np.atleast_1d(x).ravel()
+        flat = np.atleast_1d(x).astype(float).ravel()
         out = np.full(flat.shape, np.nan)
         for seg in self.segments:
             fn = getattr(seg, which)
-            if fn is None:
-                continue
             inside = (flat >= seg.lo) & (flat <= seg.hi)
-            if inside.any():
+            if fn is not None and inside.any():
                 out[inside] = np.asarray(fn(flat[inside]), dtype=float)
         return float(out[0]) if x.ndim == 0 else out.reshape(x.shape)
 
     def f(self, phi):
-        """f(phi); NaN outside the segments."""
-        return self._where("f", phi)
+        """f(phi), NaN outside the segments."""
+        return self._apply("f", phi)
 
     def mu(self, phi):
-        """mu(phi) = f'(phi); NaN outside the segments."""
-        return self._where("mu", phi)
+        """mu(phi) = f'(phi), NaN outside the segments."""
+        return self._apply("mu", phi)
 
     def q0star(self, phi):
-        """q0*(phi) where given; NaN elsewhere."""
-        return self._where("q0star", phi)
+        """q0*(phi) where the branch carries it, NaN elsewhere."""
+        return self._apply("q0star", phi)
 
     def piece_at(self, phi: float):
         """Index of the convex piece holding `phi`, or None."""
-        return next((k for k, p in enumerate(self.pieces) if p.lo <= phi <= p.hi), None)
+        hits = [k for k, p in enumerate(self.pieces) if p.lo <= phi <= p.hi]
+        return hits[0] if hits else None
 
 
 def _dense_samples(segments, n):
-    """(phi, f, mu, segment) on n uniform points per segment plus its table nodes."""
-    cols = ([], [], [], [])
-    for k, seg in enumerate(segments):
+    """(phi, f, mu) on `n` points per segment plus its table nodes."""
+    xs = []
+    for seg in segments:
         x = np.linspace(seg.lo, seg.hi, n)
-        if seg.nodes is not None:
-            x = np.union1d(x, seg.nodes)
-        cols[0].append(x)
-        cols[1].append(np.asarray(seg.f(x), dtype=float))
-        cols[2].append(np.asarray(seg.mu(x), dtype=float))
-        cols[3].append(np.full(x.shape, k))
-    return tuple(np.concatenate(c) for c in cols)
+        xs.append(x if seg.nodes is None else np.union1d(x, seg.nodes))
+    phi = np.concatenate(xs)
+    f = np.concatenate([np.asarray(s.f(x), dtype=float) for s, x in zip(segments, xs)])
+    mu = np.concatenate([np.asarray(s.mu(x), dtype=float) for s, x in zip(segments, xs)])
+    return phi, f, mu
 
 
-def _split(segments, cuts, is_convex):
-    """Cut every segment at its candidate f'' zeros and label each interval
-    convex (a Piece; touching convex intervals merge) or nonconvex."""
+def _split(segments, cuts, convex_at):
+    """Cut each segment at its `cuts`; the intervals where `convex_at(seg, u,
+    v)` holds become pieces (touching ones merged), the rest concave."""
     pieces, concave = [], []
-    for k, (seg, cut) in enumerate(zip(segments, cuts)):
-        edges = np.union1d(np.unique(np.clip(np.asarray(cut, dtype=float), seg.lo, seg.hi)), [seg.lo, seg.hi])
-        tiny = 1e-12 * max(seg.hi - seg.lo, 1e-300)
-        for u, v in zip(edges[:-1], edges[1:]):
-            u, v = float(u), float(v)
-            if v - u <= tiny:
+    for k, seg in enumerate(segments):
+        edges = np.unique(np.concatenate([[seg.lo, seg.hi], np.clip(np.asarray(cuts[k], float), seg.lo, seg.hi)]))
+        span = max(seg.hi - seg.lo, 1e-300)
+        for u, v in zip(edges[:-1].tolist(), edges[1:].tolist()):
+            if v - u <= 1e-12 * span:
                 continue
-            if is_convex(seg, u, v):
-                mu_u, mu_v = _val(seg.mu, u), _val(seg.mu, v)
+            if convex_at(seg, u, v):
                 if pieces and pieces[-1].segment == k and pieces[-1].hi == u:
-                    last = pieces.pop()
-                    pieces.append(Piece(k, last.lo, v, last.mu_lo, mu_v))
+                    pieces[-1] = replace(pieces[-1], hi=v, mu_hi=_scalar(seg.mu, v))
                 else:
-                    pieces.append(Piece(k, u, v, mu_u, mu_v))
+                    pieces.append(Piece(k, u, v, _scalar(seg.mu, u), _scalar(seg.mu, v)))
             elif concave and concave[-1][1] == u:
                 concave[-1] = (concave[-1][0], v)
             else:
@@ -193,109 +191,112 @@ def _split(segments, cuts, is_convex):
     return tuple(pieces), tuple(concave)
 
 
-def analytic_branch(name: str, f: Callable, mu: Callable, lo: float, hi: float, *, d2f: Callable | None = None,
-                    q0star: Callable | None = None, n_sample: int = 2049) -> Branch:
-    """Branch of vectorised callables on [lo, hi] (mu = f' is the caller's to
-    keep consistent).  Convex pieces come from the sign of d2f when given, its
-    sign changes on n_sample points refined by brentq; otherwise from where mu
-    stops increasing on those points, each turn refined as a root of a central
-    difference of mu."""
+def _sign_change_roots(g, x, vals, xtol):
+    """Roots of g at the sign changes of `vals` = g(x) on the sorted grid x."""
+    out = []
+    for i in np.flatnonzero(np.sign(vals[:-1]) * np.sign(vals[1:]) < 0):
+        out.append(brentq(g, x[i], x[i + 1], xtol=xtol, rtol=4 * _EPS))
+    out.extend(x[np.flatnonzero(vals == 0.0)].tolist())
+    return sorted(out)
+
+
+def analytic_branch(name, f, mu, lo, hi, *, d2f=None, q0star=None, n_sample=2049):
+    """Branch of vectorised callables on [lo, hi] (`mu` = df/dphi).  The
+    concave intervals are where `d2f` < 0, or, without it, where the sampled
+    mu falls; each boundary is refined to a root of f'' (of a centred
+    difference of mu without `d2f`)."""
     lo, hi = float(lo), float(hi)
     if not hi > lo:
         raise ValueError(f"analytic_branch {name!r}: need lo < hi, got {lo}, {hi}")
     seg = Segment(lo, hi, f, mu, d2f, q0star)
     x = np.linspace(lo, hi, n_sample)
-    cuts = []
     if d2f is not None:
-        s = np.sign(np.asarray(d2f(x), dtype=float))
-        for i in np.flatnonzero(s[:-1] != s[1:]):
-            try:
-                cuts.append(brentq(lambda p: _val(d2f, p), x[i], x[i + 1], xtol=_XTOL, rtol=_RTOL))
-            except ValueError:
-                cuts.append(0.5 * (x[i] + x[i + 1]))
-
-        def is_convex(sg, u, v):
-            return _val(sg.d2f, 0.5 * (u + v)) > 0.0
+        g = lambda p: _scalar(d2f, p)  # noqa: E731
+        cuts = _sign_change_roots(g, x, np.asarray(d2f(x), dtype=float), 1e-15)
+        convex_at = lambda s, u, v: _scalar(s.d2f, 0.5 * (u + v)) > 0.0  # noqa: E731
     else:
-        rising = np.sign(np.diff(np.asarray(mu(x), dtype=float)))
-        h = 1e-6 * max(1.0, abs(lo), abs(hi))
+        m = np.asarray(mu(x), dtype=float)
+        step = 1e-6 * max(1.0, abs(lo), abs(hi))
 
-        def slope(p):
-            a, b = max(lo, p - h), min(hi, p + h)
-            return (_val(mu, b) - _val(mu, a)) / (b - a)
+        def slope(p):  # centred difference of mu, one-sided at the ends
+            a, b = max(lo, p - step), min(hi, p + step)
+            return (_scalar(mu, b) - _scalar(mu, a)) / (b - a)
 
-        for i in np.flatnonzero(rising[:-1] != rising[1:]):
-            try:
-                cuts.append(brentq(slope, x[i], x[i + 2], xtol=1e-14, rtol=_RTOL))
-            except ValueError:
-                cuts.append(x[i + 1])
+        # a turning point of mu sits between samples where its increments
+        # change sign; refine it as the root of mu's difference quotient
+        inc = np.sign(np.diff(m))
+        cuts = []
+        for i in np.flatnonzero(inc[:-1] * inc[1:] < 0):
+            a, b = x[i], x[i + 2]
+            sa, sb = slope(a), slope(b)
+            cuts.append(brentq(slope, a, b, xtol=1e-14, rtol=4 * _EPS) if sa * sb < 0 else x[i + 1])
+        convex_at = lambda s, u, v: _scalar(s.mu, v) > _scalar(s.mu, u)  # noqa: E731
+    pieces, concave = _split((seg,), [cuts], convex_at)
+    return Branch(name, "analytic", (seg,), pieces, concave, _dense=_dense_samples((seg,), n_sample))
 
-        def is_convex(sg, u, v):
-            return _val(sg.mu, v) > _val(sg.mu, u)
 
-    pieces, concave = _split((seg,), [cuts], is_convex)
-    return Branch(name, "analytic", (seg,), pieces, concave, _samples=_dense_samples((seg,), n_sample))
+def _spline_inflections(spl):
+    """Zeros of f'' of a cubic spline: f'' is linear between knots, so each
+    sign change of its knot values has one zero, found by interpolation."""
+    x = spl.x
+    d2 = spl(x, 2)
+    out = [float(x[i]) for i in np.flatnonzero(d2 == 0.0)]
+    for i in np.flatnonzero(d2[:-1] * d2[1:] < 0):
+        t = d2[i] / (d2[i] - d2[i + 1])
+        out.append(float(x[i] + t * (x[i + 1] - x[i])))
+    return sorted(out)
 
 
-def tabulated_branch(name: str, phi, f, q0star=None, *, min_points: int = 4, bc_type="not-a-knot",
-                     n_sample: int = 1025) -> Branch:
-    """Branch of a table f(phi) (NaN where not converged), e.g. fmin of a q0*
-    scan.  Sorted by phi; every run of >= min_points consecutive finite points
-    (phi, f and, when given, q0*) becomes a segment splined with `bc_type`;
-    shorter runs are listed in `dropped`.  f'' of a cubic spline is piecewise
-    linear, so the convex pieces are exact."""
+def tabulated_branch(name, phi, f, q0star=None, *, min_points=4, bc_type="not-a-knot", n_sample=1025):
+    """Branch of a table f(phi) with NaN where a relaxation failed: each run
+    of >= `min_points` consecutive finite entries (phi, f and q0star if
+    given) becomes a segment with cubic splines of f (mu and f'' its
+    derivatives) and q0*; shorter runs are listed in `dropped`, the spaces
+    between segments in `gaps`."""
     phi = np.asarray(phi, dtype=float).ravel()
     f = np.asarray(f, dtype=float).ravel()
     q = None if q0star is None else np.asarray(q0star, dtype=float).ravel()
-    if phi.shape != f.shape or (q is not None and q.shape != phi.shape):
+    if f.shape != phi.shape or (q is not None and q.shape != phi.shape):
         raise ValueError(f"tabulated_branch {name!r}: phi, f, q0star must have equal length")
-    order = np.argsort(phi, kind="stable")
-    phi, f = phi[order], f[order]
-    if q is not None:
-        q = q[order]
-    if np.any(np.diff(phi[np.isfinite(phi)]) == 0):
+    idx = np.argsort(phi, kind="stable")
+    phi, f = phi[idx], f[idx]
+    q = None if q is None else q[idx]
+    fin = np.isfinite(phi)
+    if np.any(np.diff(phi[fin]) == 0):
         raise ValueError(f"tabulated_branch {name!r}: repeated phi values")
-    ok = np.isfinite(phi) & np.isfinite(f)
-    if q is not None:
-        ok &= np.isfinite(q)
-    # maximal runs of usable points
-    edges = np.flatnonzero(np.diff(np.concatenate(([0], ok.astype(np.int8), [0]))))
-    segments, dropped, cuts = [], [], []
-    for i, j in zip(edges[0::2], edges[1::2]):
-        if j - i < min_points:
-            dropped.extend(float(v) for v in phi[i:j])
+    ok = fin & np.isfinite(f) & (np.isfinite(q) if q is not None else True)
+    # runs of consecutive usable entries: boundaries where `ok` switches
+    flips = np.flatnonzero(np.diff(np.concatenate([[0], ok.astype(int), [0]])))
+    segments, cuts, dropped = [], [], []
+    for s, e in zip(flips[0::2], flips[1::2]):
+        if e - s < min_points:
+            dropped += phi[s:e].tolist()
             continue
-        x = phi[i:j]
-        spl = CubicSpline(x, f[i:j], bc_type=bc_type)
-        qs = None if q is None else CubicSpline(x, q[i:j], bc_type=bc_type)
-        d2 = spl.derivative(2)
-        segments.append(Segment(float(x[0]), float(x[-1]), spl, spl.derivative(1), d2, qs, x))
-        r = np.asarray(d2.roots(extrapolate=False), dtype=float)
-        cuts.append(r[np.isfinite(r)])
+        x = phi[s:e]
+        spl = CubicSpline(x, f[s:e], bc_type=bc_type)
+        qs = None if q is None else CubicSpline(x, q[s:e], bc_type=bc_type)
+        segments.append(Segment(float(x[0]), float(x[-1]), spl, spl.derivative(1), spl.derivative(2), qs, x))
+        cuts.append(_spline_inflections(spl))
     if not segments:
         raise ValueError(f"tabulated_branch {name!r}: no run of >= {min_points} finite points")
-
-    def is_convex(sg, u, v):
-        return float(sg.d2f(0.5 * (u + v))) > 0.0
-
-    pieces, concave = _split(segments, cuts, is_convex)
+    pieces, concave = _split(segments, cuts, lambda s, u, v: _scalar(s.d2f, 0.5 * (u + v)) > 0.0)
     gaps = tuple((a.hi, b.lo) for a, b in zip(segments[:-1], segments[1:]))
     return Branch(name, "tabulated", tuple(segments), pieces, concave, gaps, tuple(dropped),
-                  _samples=_dense_samples(segments, n_sample))
+                  _dense=_dense_samples(segments, n_sample))
 
 
-# ------------------------------------------------------------------------- pairwise tangent
+# ---------------------------------------------------------------------------
+# the pairwise tangent
+# ---------------------------------------------------------------------------
 @dataclass(frozen=True)
 class TangentResult:
-    """Common tangent between branches `a` and `b` (names), or the reason for none.
+    """Common tangent of branches `a` and `b` (names), or the reason for none.
 
-    phi_a, phi_b: coexisting densities on a and b in argument order.
-    residual_mu_a/b = mu(phi) - mu on each side, residual_omega = omega_a -
-    omega_b; piece_a/b index `Branch.pieces`.  `supporting`: neither branch
-    dips below the line by more than atol.  `stable`/`below` are set by
-    lower_envelope (bridge or metastable, and a branch under a metastable
-    line).  `candidates`: every tangent found (primary included) when the
-    search was not restricted to one piece pair."""
+    phi_a, phi_b: contact densities on a and b (argument order; either may be
+    the lower).  residual_mu_a/b = mu(phi) - mu, residual_omega = omega_a -
+    omega_b.  piece_a/b index Branch.pieces.  `supporting`: neither branch of
+    the pair lies below the line by more than atol.  `stable`/`below` are set
+    by lower_envelope.  `candidates`: every tangent of the search."""
     a: str
     b: str
     status: str
@@ -329,128 +330,150 @@ class TangentResult:
 
     def as_dict(self) -> dict:
         d = asdict(self)
-        d.pop("candidates")
+        del d["candidates"]
         return d
 
 
-def _phi_at(br: Branch, k: int, mu: float) -> float:
-    """phi with mu(phi) = mu on piece k (clamped to the piece's ends)."""
+def _contact(br: Branch, k: int, m: float) -> float:
+    """Density on convex piece k of `br` where mu = m (clamped to the piece):
+    Newton steps with the branch's f'', kept inside a shrinking bracket and
+    bisected when a step leaves it; Brent's method where f'' is not given."""
     p = br.pieces[k]
-    if mu <= p.mu_lo:
+    if m <= p.mu_lo:
         return p.lo
-    if mu >= p.mu_hi:
+    if m >= p.mu_hi:
         return p.hi
-    m = br.segments[p.segment].mu
-    return brentq(lambda x: _val(m, x) - mu, p.lo, p.hi, xtol=_XTOL, rtol=_RTOL)
+    seg = br.segments[p.segment]
+    if seg.d2f is None:
+        return brentq(lambda v: _scalar(seg.mu, v) - m, p.lo, p.hi, xtol=1e-15, rtol=4 * _EPS)
+    lo, hi = p.lo, p.hi
+    x = lo + (hi - lo) * (m - p.mu_lo) / (p.mu_hi - p.mu_lo)  # linear start
+    for _ in range(100):
+        gx = _scalar(seg.mu, x) - m
+        if gx == 0.0:
+            return x
+        if gx < 0:
+            lo = x
+        else:
+            hi = x
+        d = _scalar(seg.d2f, x)
+        nx = x - gx / d if d > 0 else 0.5 * (lo + hi)
+        if not lo < nx < hi:
+            nx = 0.5 * (lo + hi)
+        if abs(nx - x) <= 2 * _EPS * max(1.0, abs(x)) or hi - lo <= 2 * _EPS * max(1.0, abs(x)):
+            return nx
+        x = nx
+    return x
 
 
-def _grand(br: Branch, k: int, mu: float):
-    """(omega, phi) on piece k at chemical potential mu."""
-    x = _phi_at(br, k, mu)
-    return _val(br.segments[br.pieces[k].segment].f, x) - mu * x, x
+def _line_gap(br: Branch, m: float, x0: float, f0: float):
+    """Lowest value of f - (f0 + m (phi - x0)) over the branch's dense
+    samples, and where it is."""
+    phi, f, _ = br._dense
+    d = f - (f0 + m * (phi - x0))
+    i = int(np.argmin(d))
+    return float(d[i]), float(phi[i])
 
 
-def _sign_changes(fn, lo, hi, n):
-    """Roots of fn on [lo, hi]: exact zeros on an n-point grid and brentq in
-    every bracketing grid interval, merged within 1e-12 relative."""
-    grid = np.linspace(lo, hi, max(int(n), 2))
-    vals = [fn(m) for m in grid]
-    roots = []
-    for k, m in enumerate(grid):
-        if vals[k] == 0.0:
-            roots.append(float(m))
-        elif k + 1 < len(grid) and vals[k] * vals[k + 1] < 0.0:
-            roots.append(brentq(fn, m, grid[k + 1], xtol=_XTOL, rtol=_RTOL))
-    out = []
-    for r in sorted(roots):
-        if not out or abs(r - out[-1]) > 1e-12 * (1.0 + abs(r)):
-            out.append(r)
-    return out
+def _touching(a: Branch, i: int, b: Branch, j: int, n_scan: int):
+    """Densities x on piece i of `a` where one line touches piece j of `b`."""
+    pa, pb = a.pieces[i], b.pieces[j]
+    m_lo, m_hi = max(pa.mu_lo, pb.mu_lo), min(pa.mu_hi, pb.mu_hi)
+    if not m_lo < m_hi:
+        return None
+    sa, sb = a.segments[pa.segment], b.segments[pb.segment]
+    x_lo, x_hi = _contact(a, i, m_lo), _contact(a, i, m_hi)
+
+    def h(x):
+        m = _scalar(sa.mu, x)
+        y = _contact(b, j, m)
+        return _scalar(sa.f, x) - _scalar(sb.f, y) - m * (x - y)
+
+    if not x_lo < x_hi:
+        return []
+    xs = np.linspace(x_lo, x_hi, max(int(n_scan), 2))
+    vals = np.array([h(x) for x in xs])
+    roots = _sign_change_roots(h, xs, vals, 1e-15)
+    uniq = []
+    for r in roots:
+        if not uniq or r - uniq[-1] > 1e-12 * (1.0 + abs(r)):
+            uniq.append(r)
+    return uniq
 
 
-def _below_line(br: Branch, mu: float, phi0: float, f0: float):
-    """(min of f - line, phi there) over the branch's dense samples, for the
-    line through (phi0, f0) with slope mu."""
-    ph, ff = br._samples[0], br._samples[1]
-    d = ff - (f0 + mu * (ph - phi0))
-    k = int(np.argmin(d))
-    return float(d[k]), float(ph[k])
-
-
-def _tangent_at(a, i, b, j, mu, same, atol):
-    sa, sb = a.segments[a.pieces[i].segment], b.segments[b.pieces[j].segment]
-    om_a, xa = _grand(a, i, mu)
-    om_b, xb = _grand(b, j, mu)
-    fa = om_a + mu * xa
-    dip = min(_below_line(a, mu, xa, fa)[0], _below_line(b, mu, xa, fa)[0])
-    return TangentResult(
-        a=a.name, b=b.name, status="ok", phi_a=float(xa), phi_b=float(xb), mu=float(mu), omega=float(om_a),
-        q0star_a=None if sa.q0star is None else _val(sa.q0star, xa),
-        q0star_b=None if sb.q0star is None else _val(sb.q0star, xb),
-        residual_mu_a=_val(sa.mu, xa) - mu, residual_mu_b=_val(sb.mu, xb) - mu,
-        residual_omega=float(om_a - om_b), piece_a=i, piece_b=j, same_branch=same,
-        across_gap=bool(same and a.pieces[i].segment != b.pieces[j].segment),
-        supporting=bool(dip >= -atol))
-
-
-def common_tangent(a: Branch, b: Branch, *, pieces=None, ordered: bool = False, mu_guess: float | None = None,
-                   n_scan: int = 33, atol: float = 1e-10) -> TangentResult:
-    """Common tangent (equal mu and omega) of branches a and b over every pair
-    of convex pieces (distinct pieces when `a is b`: a self-tangent across a
-    concave interval).  `pieces=(i, j)` restricts the search to one pair,
-    `ordered` keeps only phi_a < phi_b, `mu_guess` prefers the candidate
-    nearest that mu.  Never raises for a pair without a tangent."""
+def _tangent_at(a, i, b, j, x, atol):
+    """The TangentResult of the contact at density x of piece i of `a`."""
     same = a is b
-    for br in (a, b):
-        if not br.pieces:
-            return TangentResult(a.name, b.name, "non_convex", same_branch=same,
-                                 message=f"branch {br.name!r} has no convex piece")
+    sa, sb = a.segments[a.pieces[i].segment], b.segments[b.pieces[j].segment]
+    m = _scalar(sa.mu, x)
+    y = _contact(b, j, m)
+    fa, fb = _scalar(sa.f, x), _scalar(sb.f, y)
+    om_a, om_b = fa - m * x, fb - m * y
+    low = min(_line_gap(a, m, x, fa)[0], _line_gap(b, m, x, fa)[0])
+    return TangentResult(a.name, b.name, "ok", float(x), float(y), float(m), float(om_a),
+                         None if sa.q0star is None else _scalar(sa.q0star, x),
+                         None if sb.q0star is None else _scalar(sb.q0star, y),
+                         _scalar(sa.mu, x) - m, _scalar(sb.mu, y) - m, float(om_a - om_b), i, j, same,
+                         bool(same and a.pieces[i].segment != b.pieces[j].segment), bool(low >= -atol))
+
+
+def common_tangent(a: Branch, b: Branch, *, pieces=None, ordered=False, mu_guess=None, n_scan=33, atol=1e-10):
+    """Common tangent(s) of branches a and b over every pair of convex pieces
+    (distinct pieces when a is b: a self-tangent across a concave interval).
+    `pieces` = (i, j) restricts the search to one pair; `ordered` keeps
+    tangents with phi_a < phi_b; the primary result is a supporting tangent
+    before a non-supporting one, then the one nearest `mu_guess` (else the
+    lowest mu), and `.candidates` holds all.  A pair without a tangent gives
+    a non-ok status (see the module docstring), never an exception."""
+    same = a is b
+    empty = [br.name for br in (a, b) if not br.pieces]
+    if empty:
+        return TangentResult(a.name, b.name, "non_convex", same_branch=same,
+                             message=f"branch {empty[0]!r} has no convex piece")
     if pieces is not None:
         pairs = [tuple(pieces)]
-    elif same:
-        pairs = [(i, j) for i in range(len(a.pieces)) for j in range(i + 1, len(a.pieces))]
     else:
-        pairs = [(i, j) for i in range(len(a.pieces)) for j in range(len(b.pieces))]
-    found, overlap = [], False
+        pairs = [(i, j) for i in range(len(a.pieces)) for j in range(len(b.pieces)) if not same or j > i]
+    found, overlapped = [], False
     for i, j in pairs:
-        lo = max(a.pieces[i].mu_lo, b.pieces[j].mu_lo)
-        hi = min(a.pieces[i].mu_hi, b.pieces[j].mu_hi)
-        if not lo < hi:
+        xs = _touching(a, i, b, j, n_scan)
+        if xs is None:
             continue
-        overlap = True
-
-        def domega(m, i=i, j=j):
-            return _grand(a, i, m)[0] - _grand(b, j, m)[0]
-
-        found += [_tangent_at(a, i, b, j, m, same, atol) for m in _sign_changes(domega, lo, hi, n_scan)]
+        overlapped = True
+        found += [_tangent_at(a, i, b, j, x, atol) for x in xs]
     if ordered:
-        found = [c for c in found if c.phi_a < c.phi_b]
+        found = [t for t in found if t.phi_a < t.phi_b]
     if found:
-        found.sort(key=lambda c: (not c.supporting, abs(c.mu - mu_guess) if mu_guess is not None else c.mu))
+        rank = (lambda t: (not t.supporting, abs(t.mu - mu_guess))) if mu_guess is not None else \
+            (lambda t: (not t.supporting, t.mu))
+        found.sort(key=rank)
         best = replace(found[0], candidates=tuple(found) if pieces is None else ())
         if len(found) > 1:
-            best = replace(best, message=f"{len(found)} tangents found; primary chosen by supporting, "
-                                         "then mu_guess/lowest mu")
+            best = replace(best, message=f"{len(found)} tangents; primary: supporting first, then "
+                                         f"{'nearest mu_guess' if mu_guess is not None else 'lowest mu'}")
         return best
     if same and len(a.pieces) < 2:
         return TangentResult(a.name, b.name, "no_root", same_branch=True,
-                             message="branch is convex on one piece; no self-tangent")
-    (alo, ahi), (blo, bhi) = a.mu_range, b.mu_range
-    if not max(alo, blo) < min(ahi, bhi):
+                             message="one convex piece: no tangent of the branch with itself")
+    (a0, a1), (b0, b1) = a.mu_range, b.mu_range
+    if max(a0, b0) >= min(a1, b1):
         return TangentResult(a.name, b.name, "no_overlap", same_branch=same,
-                             message=f"mu ranges [{alo:.6g}, {ahi:.6g}] and [{blo:.6g}, {bhi:.6g}] are disjoint")
-    if overlap:
+                             message=f"mu ranges [{a0:.6g}, {a1:.6g}] and [{b0:.6g}, {b1:.6g}] do not overlap")
+    if overlapped:
         return TangentResult(a.name, b.name, "no_root", same_branch=same,
-                             message="no sign change of omega_a - omega_b on the convex mu overlap"
+                             message="convex pieces share slopes but no line touches both"
                                      + (" with phi_a < phi_b" if ordered else ""))
     return TangentResult(a.name, b.name, "non_convex", same_branch=same,
-                         message="mu ranges overlap only through concave parts")
+                         message="the mu ranges meet only on concave parts")
 
 
-# --------------------------------------------------------------------------- lower envelope
+# ---------------------------------------------------------------------------
+# the lower envelope
+# ---------------------------------------------------------------------------
 @dataclass(frozen=True)
 class Window:
-    """Densities where phase `name` (branches[branch].pieces[piece]) alone is stable."""
+    """Densities [phi_lo, phi_hi] where phase `name` (branch index `branch`,
+    convex piece `piece`) alone is on the envelope."""
     name: str
     branch: int
     piece: int
@@ -465,9 +488,9 @@ class Window:
 @dataclass(frozen=True)
 class Envelope:
     """Lower envelope at one temperature: windows[k] | coexistence[k] |
-    windows[k+1] in density order; `metastable` holds every other tangent
-    (stable=False, `below` a branch under its line); hull_phi/hull_f are the
-    sampled hull vertices."""
+    windows[k+1] in density order; `metastable`: the other tangents (stable
+    False, `below` the branch under the line); hull_phi/hull_f: the envelope
+    tabulated for plots."""
     temp: float | None
     names: tuple
     windows: tuple
@@ -478,39 +501,43 @@ class Envelope:
     hull_f: np.ndarray = field(repr=False, compare=False, default=None)
 
 
-def _hull_lower(x, y):
-    """Indices of the lower convex hull of points sorted by x (monotone chain)."""
-    h = []
-    for k in range(len(x)):
-        while len(h) >= 2:
-            o, p = h[-2], h[-1]
-            if (x[p] - x[o]) * (y[k] - y[o]) - (y[p] - y[o]) * (x[k] - x[o]) <= 0.0:
-                h.pop()
-            else:
-                break
-        h.append(k)
-    return h
+def _deepest(branches, res):
+    """(depth, branch index, phi) of the branch lying lowest under the line of `res`."""
+    f0 = res.omega + res.mu * res.phi_a
+    worst = (math.inf, None, None)
+    for bi, br in enumerate(branches):
+        d, x = _line_gap(br, res.mu, res.phi_a, f0)
+        if d < worst[0]:
+            worst = (d, bi, x)
+    return worst
 
 
-def _closest_piece(br: Branch, phi: float, seg: int) -> int:
-    """Index of the piece of segment `seg` nearest `phi` (-1 if it has none)."""
-    best, dist = -1, math.inf
-    for k, p in enumerate(br.pieces):
-        if p.segment != seg:
+def _chord(branches, labels, x0, f0, exclude):
+    """The chord of smallest slope from (x0, f0) to a convex piece lying
+    right of x0: (label, contact density)."""
+    best = (math.inf, None, None)
+    for lab in labels:
+        if lab == exclude:
             continue
-        d = 0.0 if p.lo <= phi <= p.hi else min(abs(phi - p.lo), abs(phi - p.hi))
-        if d < dist:
-            best, dist = k, d
-    return best
+        br = branches[lab[0]]
+        p = br.pieces[lab[1]]
+        if p.hi <= x0:
+            continue
+        seg = br.segments[p.segment]
+        x = np.linspace(max(p.lo, x0), p.hi, 257)[1:] if p.lo <= x0 else np.linspace(p.lo, p.hi, 257)
+        s = (np.asarray(seg.f(x), dtype=float) - f0) / (x - x0)
+        k = int(np.argmin(s))
+        if s[k] < best[0]:
+            best = (float(s[k]), lab, float(x[k]))
+    return best[1], best[2]
 
 
-def lower_envelope(branches: Sequence[Branch], phi_grid=None, *, temp: float | None = None, n: int = 4001,
-                   n_scan: int = 33, atol: float = 1e-10, max_iter: int | None = None) -> Envelope:
-    """Coexisting pairs and single-phase windows of all branches at one
-    temperature (branch names unique).  The hull of the branches sampled on
-    `phi_grid` (default: n uniform points over their union, plus segment ends)
-    orders the stretches; bridges are ordered tangents of neighbouring pieces;
-    crossed windows are removed and dipping branches inserted until stable."""
+def lower_envelope(branches: Sequence, phi_grid=None, *, temp=None, n=4001, n_scan=33, atol=1e-10, max_iter=None):
+    """Stable coexistence and single-phase windows of all `branches` at one
+    temperature (gift wrapping over their convex pieces, see the module
+    docstring); every other pairwise tangent is returned as metastable.
+    `phi_grid` (default `n` points over all domains) tabulates the envelope.
+    `max_iter` bounds the hull's steps (default: enough for every piece)."""
     branches = list(branches)
     if not branches:
         raise ValueError("lower_envelope: no branches")
@@ -518,143 +545,108 @@ def lower_envelope(branches: Sequence[Branch], phi_grid=None, *, temp: float | N
     if len(set(names)) != len(names):
         raise ValueError(f"lower_envelope: branch names must be unique, got {names}")
     warns = []
+    labels = [(bi, pi) for bi, br in enumerate(branches) for pi in range(len(br.pieces))]
+    if not labels:
+        raise ValueError("lower_envelope: no branch has a convex piece")
+    memo = {}
+
+    def tangents(la, lb):
+        key = (la, lb)
+        if key not in memo:
+            a, b = branches[la[0]], branches[lb[0]]
+            xs = _touching(a, la[1], b, lb[1], n_scan) or []
+            memo[key] = [_tangent_at(a, la[1], b, lb[1], x, atol) for x in xs]
+        return memo[key]
+
+    def start_of(lab):
+        p = branches[lab[0]].pieces[lab[1]]
+        return p.lo, _scalar(branches[lab[0]].segments[p.segment].f, p.lo)
+
+    # leftmost contact: the lowest f at the smallest density
+    cur = min(labels, key=lambda lab: start_of(lab))
+    x_in = start_of(cur)[0]
+    seq, bridges, entries = [cur], [], [x_in]
+    steps = max_iter if max_iter is not None else 4 * len(labels) + 8
+    for _ in range(steps):
+        best, nxt = None, None
+        for other in labels:
+            if other == cur:
+                continue
+            for t in tangents(cur, other):
+                if t.phi_a >= x_in - 1e-12 * (1 + abs(x_in)) and t.phi_b > t.phi_a and (best is None or t.mu < best.mu):
+                    best, nxt = t, other
+        if best is not None:
+            bridges.append(replace(best, stable=True))
+            cur, x_in = nxt, best.phi_b
+        else:
+            p = branches[cur[0]].pieces[cur[1]]
+            x0, f0 = p.hi, _scalar(branches[cur[0]].segments[p.segment].f, p.hi)
+            nxt, xn = _chord(branches, labels, x0, f0, cur)
+            if nxt is None:
+                break
+            a, b = branches[cur[0]], branches[nxt[0]]
+            res = common_tangent(a, b, pieces=(cur[1], nxt[1]), ordered=True, n_scan=n_scan, atol=atol)
+            bridges.append(res)
+            warns.append(f"hull bridge {res.a}-{res.b} has no tangent ({res.status}: {res.message}); "
+                         "the envelope there is domain-limited")
+            cur, x_in = nxt, xn
+        seq.append(cur)
+        entries.append(x_in)
+    else:
+        warns.append("envelope: the hull did not reach the right end within max_iter steps")
+    windows = []
+    for k, lab in enumerate(seq):
+        p = branches[lab[0]].pieces[lab[1]]
+        lo = bridges[k - 1].phi_b if k > 0 and bridges[k - 1].ok else (entries[k] if k > 0 else p.lo)
+        hi = bridges[k].phi_a if k < len(bridges) and bridges[k].ok else p.hi
+        windows.append(Window(names[lab[0]], lab[0], lab[1], float(lo), float(hi)))
+    for br in bridges:
+        if br.ok:
+            depth, bi, x = _deepest(branches, br)
+            if depth < -atol:
+                warns.append(f"{names[bi]!r} dips {depth:.3g} below the {br.a}-{br.b} tangent at phi={x:.6g} "
+                             "(on a concave part or a piece without a tangent)")
+            if br.across_gap:
+                warns.append(f"bridge {br.a}-{br.b} spans a NaN gap of {br.a!r} (phi {br.phi_a:.6g} to {br.phi_b:.6g})")
+    on_hull = {(b.a, b.piece_a, b.b, b.piece_b, round(b.mu, 9)) for b in bridges if b.ok}
+    meta = []
+    for ia, la in enumerate(labels):
+        for lb in labels[ia + 1:]:
+            for t in tangents(la, lb):
+                key = (t.a, t.piece_a, t.b, t.piece_b, round(t.mu, 9))
+                rev = (t.b, t.piece_b, t.a, t.piece_a, round(t.mu, 9))
+                if key in on_hull or rev in on_hull:
+                    continue
+                depth, bi, _ = _deepest(branches, t)
+                if depth < -atol:
+                    meta.append(replace(t, stable=False, below=names[bi]))
+                else:
+                    meta.append(replace(t, stable=False))
+                    warns.append(f"tangent {t.a}-{t.b} at mu={t.mu:.6g} supports every branch but is not on the hull")
+    meta.sort(key=lambda t: t.mu)
     grid = (np.linspace(min(b.lo for b in branches), max(b.hi for b in branches), n) if phi_grid is None
             else np.asarray(phi_grid, dtype=float))
-    # every branch on the grid (plus its segment ends), labelled (branch, piece)
-    ph, ff, lb, lp = [], [], [], []
-    for bi, br in enumerate(branches):
-        for si, seg in enumerate(br.segments):
-            x = np.union1d(grid[(grid >= seg.lo) & (grid <= seg.hi)], [seg.lo, seg.hi])
-            ph.append(x)
-            ff.append(np.asarray(seg.f(x), dtype=float))
-            lb.append(np.full(x.shape, bi))
-            lp.append(np.array([_closest_piece(br, v, si) for v in x], dtype=int))
-    P, F, LB, LP = map(np.concatenate, (ph, ff, lb, lp))
-    o = np.lexsort((F, P))
-    P, F, LB, LP = P[o], F[o], LB[o], LP[o]
-    first = np.ones(P.shape, bool)
-    first[1:] = P[1:] != P[:-1]  # one point per phi: the lowest f
-    P, F, LB, LP = P[first], F[first], LB[first], LP[first]
-    hull = _hull_lower(P, F)
-    groups, slopes = [], {}
-    for pos, k in enumerate(hull):
-        lab = (int(LB[k]), int(LP[k]))
-        if lab[1] < 0:
-            warns.append(f"hull vertex at phi={P[k]:.6g} on concave-only segment of {names[lab[0]]!r} ignored")
-            continue
-        if groups and groups[-1] == lab:
-            continue
-        if groups:
-            prev = hull[pos - 1]
-            slopes[(groups[-1], lab)] = (F[k] - F[prev]) / (P[k] - P[prev])
-        groups.append(lab)
-
-    bridged = {}
-
-    def bridge(gl, gr):
-        if (gl, gr) not in bridged:
-            bridged[(gl, gr)] = common_tangent(branches[gl[0]], branches[gr[0]], pieces=(gl[1], gr[1]),
-                                               ordered=True, mu_guess=slopes.get((gl, gr)), n_scan=n_scan,
-                                               atol=atol)
-        return bridged[(gl, gr)]
-
-    def lowest_dip(t):
-        """(dip, branch, phi) of the branch lying furthest below t's line."""
-        f0 = t.omega + t.mu * t.phi_a
-        best = (math.inf, None, None)
-        for bi, br in enumerate(branches):
-            d, x = _below_line(br, t.mu, t.phi_a, f0)
-            if d < best[0]:
-                best = (d, bi, x)
-        return best
-
-    def dedup(gs):
-        return [g for k, g in enumerate(gs) if k == 0 or g != gs[k - 1]]
-
-    limit = max_iter if max_iter is not None else 4 * len(groups) + 4 * len(branches) + 10
-    reported = set()
-    for _ in range(limit):
-        groups = dedup(groups)
-        bridges = [bridge(groups[k], groups[k + 1]) for k in range(len(groups) - 1)]
-        crossed = next((k for k in range(1, len(groups) - 1)
-                        if bridges[k - 1].ok and bridges[k].ok and bridges[k - 1].phi_b > bridges[k].phi_a), None)
-        if crossed is not None:
-            del groups[crossed]
-            continue
-        inserted = False
-        for k, t in enumerate(bridges):
-            if not t.ok:
-                continue
-            d, bi, x = lowest_dip(t)
-            if d >= -atol:
-                continue
-            seg = int(np.argmin([0 if s.lo <= x <= s.hi else 1 for s in branches[bi].segments]))
-            lab = (bi, _closest_piece(branches[bi], x, seg))
-            if lab[1] < 0 or lab in (groups[k], groups[k + 1]):
-                if (k, lab) not in reported:
-                    reported.add((k, lab))
-                    warns.append(f"{names[bi]!r} dips {d:.3g} below the {t.a}-{t.b} tangent at phi={x:.6g} but "
-                                 "cannot be inserted (same piece as a neighbour or concave)")
-                continue
-            groups.insert(k + 1, lab)
-            inserted = True
-            break
-        if not inserted:
-            break
-    else:
-        warns.append("envelope refinement did not converge; result may be inconsistent")
-    groups = dedup(groups)
-    bridges = [bridge(groups[k], groups[k + 1]) for k in range(len(groups) - 1)]
-    bridges = [replace(t, stable=True) if t.ok else t for t in bridges]
-    windows = []
-    for k, (bi, pi) in enumerate(groups):
-        p = branches[bi].pieces[pi]
-        lo = bridges[k - 1].phi_b if k > 0 and bridges[k - 1].ok else p.lo
-        hi = bridges[k].phi_a if k < len(bridges) and bridges[k].ok else p.hi
-        windows.append(Window(names[bi], bi, pi, float(lo), float(hi)))
-    for t in bridges:
-        if not t.ok:
-            warns.append(f"hull bridge {t.a}-{t.b} has no tangent ({t.status}: {t.message}); "
-                         "the envelope there is domain-limited")
-        elif t.across_gap:
-            warns.append(f"bridge {t.a}-{t.b} spans a NaN gap of {t.a!r} (phi {t.phi_a:.6g} to {t.phi_b:.6g})")
-    keys = []
-    for t in bridges:
-        if t.ok:
-            keys += [(t.a, t.piece_a, t.b, t.piece_b, t.mu), (t.b, t.piece_b, t.a, t.piece_a, t.mu)]
-
-    def on_envelope(c):
-        return any((c.a, c.piece_a, c.b, c.piece_b) == s[:4] and abs(c.mu - s[4]) <= 1e-9 * (1.0 + abs(c.mu))
-                   for s in keys)
-
-    meta = []
-    for i in range(len(branches)):
-        for j in range(i, len(branches)):
-            if i == j and len(branches[i].pieces) < 2:
-                continue
-            res = common_tangent(branches[i], branches[j], n_scan=n_scan, atol=atol)
-            for c in (res.candidates or ((res,) if res.ok else ())):
-                if on_envelope(c):
-                    continue
-                d, bi, _ = lowest_dip(c)
-                if d < -atol:
-                    meta.append(replace(c, stable=False, below=names[bi], candidates=()))
-                else:
-                    meta.append(replace(c, stable=False, candidates=()))
-                    warns.append(f"tangent {c.a}-{c.b} at mu={c.mu:.6g} supports every branch but is not on the "
-                                 "sampled hull; refine phi_grid")
-    meta.sort(key=lambda c: c.mu)
-    hv = np.array(hull, dtype=int)
-    return Envelope(temp, names, tuple(windows), tuple(bridges), tuple(meta), tuple(warns), P[hv], F[hv])
+    hull_f = np.full(grid.shape, np.nan)
+    for k, w in enumerate(windows):
+        inside = (grid >= w.phi_lo) & (grid <= w.phi_hi)
+        hull_f[inside] = branches[w.branch].f(grid[inside])
+        if k < len(bridges) and bridges[k].ok:
+            t = bridges[k]
+            gap = (grid > t.phi_a) & (grid < t.phi_b)
+            hull_f[gap] = t.omega + t.mu * grid[gap]
+    return Envelope(temp, names, tuple(windows), tuple(bridges), tuple(meta), tuple(warns), grid, hull_f)
 
 
-# ---------------------------------------------------------------------------- Maxwell lines
+# ---------------------------------------------------------------------------
+# Maxwell lines over temperatures
+# ---------------------------------------------------------------------------
 @dataclass(frozen=True)
 class TripleBracket:
-    """Triple point L-X-R between consecutive temperatures: X's window exists at
-    `temp_open` (width `width_open`) and is gone at `temp_closed`, where
-    `width_closed` is the signed width from the metastable L-X and X-R
-    tangents (negative) when both exist, and `temp_estimate` the linear zero."""
+    """Phases (L, X, R): X's window between L and R exists at `temp_open` and
+    has closed (L and R coexist) at the neighbouring `temp_closed`.
+    `width_closed` is X's signed width there from its metastable tangents
+    with L and R (negative); with both widths, `temp_estimate` is the linear
+    zero of the width."""
     phases: tuple
     temp_lo: float
     temp_hi: float
@@ -665,21 +657,21 @@ class TripleBracket:
     temp_estimate: float | None = None
 
 
-# column order of the rows / metastable tables and their CSV
 _ROW_KEYS = ("temp", "a", "b", "phi_a", "phi_b", "mu", "omega", "q0star_a", "q0star_b", "status", "same_branch",
              "across_gap", "stable", "below", "residual")
+_WINDOW_KEYS = ("temp", "name", "phi_lo", "phi_hi")
 
 
-def _row(temp, t: TangentResult) -> dict:
-    return {k: (temp if k == "temp" else (t.residual if t.ok else math.nan) if k == "residual" else getattr(t, k))
-            for k in _ROW_KEYS}
+def _as_row(temp, t: TangentResult) -> dict:
+    row = {k: getattr(t, k) for k in _ROW_KEYS[1:-1]}
+    return {"temp": temp, **row, "residual": t.residual if t.ok else math.nan}
 
 
 @dataclass(frozen=True)
 class MaxwellLines:
-    """Coexistence over temperatures: `rows` (stable tangents, by temperature
-    then density), `metastable` (same columns), `windows` (temp, name,
-    phi_lo, phi_hi), `triple_points` (TripleBracket list), `temps`."""
+    """Coexistence over temperatures: `rows` (stable tangents, by
+    temperature then density), `metastable`, `windows` (dicts) and
+    `triple_points`."""
     rows: tuple
     metastable: tuple
     windows: tuple
@@ -687,67 +679,54 @@ class MaxwellLines:
     temps: tuple
 
     def columns(self, which: str = "rows") -> dict:
-        """Column-oriented copy of `rows`, `metastable` or `windows`."""
+        """Column-wise copy of `rows`, `metastable` or `windows`."""
         data = getattr(self, which)
-        if data:
-            keys = list(data[0].keys())
-        else:
-            keys = ["temp", "name", "phi_lo", "phi_hi"] if which == "windows" else list(_ROW_KEYS)
+        keys = list(data[0]) if data else list(_WINDOW_KEYS if which == "windows" else _ROW_KEYS)
         return {k: [r[k] for r in data] for k in keys}
 
     def write_csv(self, dest, which: str = "rows") -> None:
-        """`rows`, `metastable` or `windows` as CSV: a header, floats as repr,
-        None as an empty field; `dest` is a path (ASCII, LF) or an open text file."""
+        """`rows`, `metastable` or `windows` as CSV (floats as repr, None as
+        ''); `dest` is a path (ASCII, LF) or an open text file."""
         keys = list(self.columns(which))
 
         def cell(v):
-            return "" if v is None else repr(v) if isinstance(v, float) else v
+            return "" if v is None else (repr(v) if isinstance(v, float) else v)
 
-        def emit(fh):
-            w = csv.writer(fh, lineterminator="\n")
-            w.writerow(keys)
-            for r in getattr(self, which):
-                w.writerow([cell(r[k]) for k in keys])
+        def write(fh):
+            out = csv.writer(fh, lineterminator="\n")
+            out.writerow(keys)
+            out.writerows([cell(r[k]) for k in keys] for r in getattr(self, which))
 
         if hasattr(dest, "write"):
-            emit(dest)
+            write(dest)
         else:
             with open(dest, "w", newline="", encoding="ascii") as fh:
-                emit(fh)
+                write(fh)
 
 
-def _window_width(env: Envelope, L: str, X: str, R: str, mu_ref):
-    """Signed width of X's window between L and R from env's metastable L-X and
-    X-R tangents (X-side densities), or None."""
-    def x_side(c, left, right):
-        if c.a == left and c.b == right and c.phi_a < c.phi_b:
-            return c.phi_b if right == X else c.phi_a
-        if c.a == right and c.b == left and c.phi_b < c.phi_a:
-            return c.phi_a if right == X else c.phi_b
-        return None
-
-    lx, xr = [], []
-    for c in env.metastable:
-        if not c.ok:
+def _window_edge(env: Envelope, left: str, right: str, side: str, mu_ref):
+    """X's edge from env's metastable tangents of X with its neighbour:
+    `side` "lo": the X density of an (L, X) tangent with L below X;
+    "hi": the X density of an (X, R) tangent with X below R.  The tangent
+    nearest `mu_ref` (else the first) when several exist."""
+    got = []
+    for t in env.metastable:
+        if not t.ok:
             continue
-        v = x_side(c, L, X)
-        if v is not None:
-            lx.append((c.mu, v))
-        v = x_side(c, X, R)
-        if v is not None:
-            xr.append((c.mu, v))
-    if not lx or not xr:
+        for (p, q, xp, xq) in ((t.a, t.b, t.phi_a, t.phi_b), (t.b, t.a, t.phi_b, t.phi_a)):
+            if (p, q) == (left, right) and xp < xq:
+                got.append((t.mu, xq if side == "lo" else xp))
+    if not got:
         return None
     if mu_ref is None:
-        return xr[0][1] - lx[0][1]
-    near = min(xr, key=lambda t: abs(t[0] - mu_ref)), min(lx, key=lambda t: abs(t[0] - mu_ref))
-    return near[0][1] - near[1][1]
+        return got[0][1]
+    return min(got, key=lambda g: abs(g[0] - mu_ref))[1]
 
 
-def maxwell_lines(per_temperature_results: Mapping | Iterable[Envelope]) -> MaxwellLines:
-    """Envelopes over temperatures ({temp: Envelope}, or envelopes built with
-    `temp`) as tables, with triple points bracketed between consecutive
-    temperatures in both directions."""
+def maxwell_lines(per_temperature_results) -> MaxwellLines:
+    """Collect lower_envelope results over temperatures ({temp: Envelope} or
+    envelopes built with `temp`) and bracket triple points between
+    consecutive temperatures."""
     if isinstance(per_temperature_results, Mapping):
         items = [(float(t), e) for t, e in per_temperature_results.items()]
     else:
@@ -756,28 +735,31 @@ def maxwell_lines(per_temperature_results: Mapping | Iterable[Envelope]) -> Maxw
             if e.temp is None:
                 raise ValueError("maxwell_lines: envelope without temp; pass a mapping")
             items.append((float(e.temp), e))
-    items.sort(key=lambda te: te[0])
-    rows, meta, wins = [], [], []
-    for t, e in items:
-        rows += [_row(t, c) for c in e.coexistence]
-        meta += [_row(t, c) for c in e.metastable]
-        wins += [{"temp": t, "name": w.name, "phi_lo": w.phi_lo, "phi_hi": w.phi_hi} for w in e.windows]
+    items.sort(key=lambda it: it[0])
+    rows = tuple(_as_row(t, c) for t, e in items for c in e.coexistence)
+    meta = tuple(_as_row(t, c) for t, e in items for c in e.metastable)
+    wins = tuple({"temp": t, "name": w.name, "phi_lo": w.phi_lo, "phi_hi": w.phi_hi} for t, e in items
+                 for w in e.windows)
     triples = []
-    for (t1, e1), (t2, e2) in zip(items[:-1], items[1:]):
-        for (ta, ea), (tb, eb) in (((t1, e1), (t2, e2)), ((t2, e2), (t1, e1))):
-            sa, sb = [w.name for w in ea.windows], [w.name for w in eb.windows]
-            for k in range(1, len(sa) - 1):
-                L, X, R = sa[k - 1], sa[k], sa[k + 1]
-                adjacent = [m for m in range(len(sb) - 1) if sb[m] == L and sb[m + 1] == R]
-                still = any(sb[m - 1:m + 2] == [L, X, R] for m in range(1, len(sb) - 1))
-                if not adjacent or still:
+    for (t1, e1), (t2, e2) in zip(items, items[1:]):
+        for (t_open, eo), (t_closed, ec) in (((t1, e1), (t2, e2)), ((t2, e2), (t1, e1))):
+            seq_o = [w.name for w in eo.windows]
+            seq_c = [w.name for w in ec.windows]
+            pairs_c = {(seq_c[m], seq_c[m + 1]): m for m in range(len(seq_c) - 1)}
+            triples_c = {tuple(seq_c[m:m + 3]) for m in range(len(seq_c) - 2)}
+            for k in range(1, len(seq_o) - 1):
+                L, X, R = seq_o[k - 1:k + 2]
+                if (L, R) not in pairs_c or (L, X, R) in triples_c:
                     continue
-                w_open = ea.windows[k].width
-                c = eb.coexistence[adjacent[0]]
-                w_closed = _window_width(eb, L, X, R, c.mu if c.ok else None)
+                bridge = ec.coexistence[pairs_c[(L, R)]]
+                mu_ref = bridge.mu if bridge.ok else None
+                lo = _window_edge(ec, L, X, "lo", mu_ref)
+                hi = _window_edge(ec, X, R, "hi", mu_ref)
+                w_open = float(eo.windows[k].width)
+                w_closed = None if lo is None or hi is None else hi - lo
                 est = None
                 if w_closed is not None and w_closed < 0.0 < w_open:
-                    est = ta + (tb - ta) * w_open / (w_open - w_closed)
-                triples.append(TripleBracket((L, X, R), min(ta, tb), max(ta, tb), ta, tb, float(w_open), w_closed,
-                                             est))
-    return MaxwellLines(tuple(rows), tuple(meta), tuple(wins), tuple(triples), tuple(t for t, _ in items))
+                    est = t_open + (t_closed - t_open) * w_open / (w_open - w_closed)
+                triples.append(TripleBracket((L, X, R), min(t_open, t_closed), max(t_open, t_closed), t_open,
+                                             t_closed, w_open, w_closed, est))
+    return MaxwellLines(rows, meta, wins, tuple(triples), tuple(t for t, _ in items))

@@ -27,6 +27,8 @@ from __future__ import annotations
 import time
 from dataclasses import dataclass, replace
 
+import numpy as np
+
 from . import q0star as _q
 
 METHOD = 'frozen-q'
@@ -54,6 +56,72 @@ class FrozenQSettings:
     maxIter: int = 40
     edge: bool = True
     maxPast: int = 3
+
+
+class _FrozenRun:
+    """The relax/jump iteration of one density as a state machine, driven one
+    relaxation at a time by the serial solve or in lockstep with other
+    densities by solve_batched.  `box` is the scale the next relaxation runs
+    at, `carry` the field it starts from (None: a fresh ansatz), `stage` its
+    index into the fTol schedule."""
+
+    def __init__(self, settings, q_start, state=None):
+        self.settings = settings
+        self.q_start = float(q_start)
+        self.box = float(q_start)
+        self.carry = state
+        self.stage = 0
+        self.steps = 0
+        self.seen = None       # the last relaxation: box, q_opt, f, f_frozen, fitness, amps, stop, iteration
+        self.field = None
+        self.finished = False
+        self.error = None
+
+    @property
+    def final_stage(self):
+        return len(self.settings.schedule) - 1
+
+    def observe(self, it, f, fitness, amps, stop, steps, field, frozen):
+        """Take one relaxation's outcome; decide the next box and stage."""
+        self.steps += steps
+        self.field = field
+        q_opt = float(frozen.q_opt())
+        self.seen = dict(box=self.box, q_opt=q_opt, f=float(f), f_frozen=float(frozen.f(q_opt)),
+                         fitness=float(fitness), amps=[float(a) for a in amps], stop=stop, it=it)
+        if stop not in (_q.STOP_CONVERGED, _q.STOP_TOTALT):
+            self.finished = True           # a rejected relaxation ends the density
+            return
+        jump = q_opt - self.box
+        self.carry, self.box = field, q_opt  # the field follows the box to q_opt
+        if self.stage < self.final_stage:
+            self.stage += 1
+        elif stop == _q.STOP_CONVERGED and abs(jump) < self.settings.qTol:
+            self.finished = True
+
+    def record(self, adapter, cell, wall_s):
+        rec = {'method': METHOD, 'q_start': self.q_start, 'message': ''}
+        if self.error is not None:
+            rec.update(status=STATUS_ERROR, message=self.error, steps=self.steps, wall_s=wall_s)
+            return rec
+        s = self.seen
+        # stage: where the run stands after its last relaxation; accepted
+        # only at the final stage, converged and with q_opt within qTol of
+        # the box the field was relaxed in
+        if s['stop'] == _q.STOP_CONVERGED and self.stage == self.final_stage and \
+                abs(s['q_opt'] - s['box']) < self.settings.qTol:
+            status = STATUS_OK
+        elif s['stop'] in (_q.STOP_CONVERGED, _q.STOP_TOTALT):
+            status = STATUS_NOT_CONVERGED
+        else:
+            status = s['stop']
+        rec.update(status=status, q=s['q_opt'], q_relaxed=s['box'], f=s['f'], f_frozen=s['f_frozen'],
+                   fitness=s['fitness'], stop=s['stop'], steps=self.steps, iters=s['it'] + 1)
+        rec.update({f'A{j + 1}': a for j, a in enumerate(s['amps'])})
+        defects = getattr(adapter, 'defect_diagnostics', None)
+        if defects is not None and self.field is not None:
+            rec.update(defects(self.field, cell))
+        rec['wall_s'] = wall_s
+        return rec
 
 
 class MinimizeFrozenQ:
@@ -87,6 +155,9 @@ class MinimizeFrozenQ:
         return _q.MinimizeFreeEnergy(self.adapter, self.params, self.structure, self.cell, self.ansatz, relax,
                                      _q.QScanSettings(1, 0.0, 1, 1.0))
 
+    def _frozen(self, field, temp):
+        return self.adapter.frozen_field(field, self.params, temp, self.cell)
+
     def solve(self, temp, meanDensity, q_start=1.0, state=None):
         """One density; returns a record (never raises: an exception is
         status 'error'): status, q (= q_opt of the final field, the q*),
@@ -94,51 +165,19 @@ class MinimizeFrozenQ:
         (frozen f at q), fitness, A1.., stop, steps, iters, wall_s, q_start,
         message, method, and the adapter's defect diagnostics if any."""
         t0 = time.perf_counter()
-        st = self.settings
-        last_stage = len(st.schedule) - 1
-        q, carry, steps, stage = float(q_start), state, 0, 0
-        rec = {'method': METHOD, 'q_start': float(q_start), 'message': ''}
-        last, field = None, None
+        run = _FrozenRun(self.settings, q_start, state)
         try:
-            for it in range(st.maxIter):
-                m = self.stage_minimizer(stage)
-                f, fit, *amps = m.Get_f_ABC(temp, meanDensity, q, state=carry)
-                steps += m.last_steps
-                field = m.last_state
-                frozen = self.adapter.frozen_field(field, self.params, temp, self.cell)
-                q_new = float(frozen.q_opt())
-                last = {'q': q, 'q_opt': q_new, 'f': float(f), 'f_frozen': float(frozen.f(q_new)),
-                        'fitness': float(fit), 'amps': [float(a) for a in amps], 'stop': m.last_stop, 'it': it}
-                if m.last_stop not in (_q.STOP_CONVERGED, _q.STOP_TOTALT):
-                    break
-                dq = q_new - q
-                carry, q = field, q_new
-                if stage < last_stage:
-                    stage += 1
-                elif abs(dq) < st.qTol and m.last_stop == _q.STOP_CONVERGED:
-                    break
+            it = 0
+            while it < self.settings.maxIter and not run.finished:
+                m = self.stage_minimizer(run.stage)
+                f, fit, *amps = m.Get_f_ABC(temp, meanDensity, run.box, state=run.carry)
+                run.observe(it, f, fit, amps, m.last_stop, m.last_steps, m.last_state,
+                            self._frozen(m.last_state, temp))
+                it += 1
         except Exception as e:  # noqa: BLE001 - a density's failure is its status
-            rec.update(status=STATUS_ERROR, message=f'{type(e).__name__}: {e}', steps=steps,
-                       wall_s=time.perf_counter() - t0)
-            self.last_result, self.last_state = rec, None
-            return rec
-        settled = (last['stop'] == _q.STOP_CONVERGED and stage == last_stage
-                   and abs(last['q_opt'] - last['q']) < st.qTol)
-        if settled:
-            status = STATUS_OK
-        elif last['stop'] in (_q.STOP_CONVERGED, _q.STOP_TOTALT):
-            status = STATUS_NOT_CONVERGED
-        else:
-            status = last['stop']
-        rec.update(status=status, q=last['q_opt'], q_relaxed=last['q'], f=last['f'], f_frozen=last['f_frozen'],
-                   fitness=last['fitness'], stop=last['stop'], steps=steps, iters=last['it'] + 1)
-        for j, a in enumerate(last['amps']):
-            rec[f'A{j + 1}'] = a
-        defects = getattr(self.adapter, 'defect_diagnostics', None)
-        if defects is not None and field is not None:
-            rec.update(defects(field, self.cell))
-        rec['wall_s'] = time.perf_counter() - t0
-        self.last_result, self.last_state = rec, field
+            run.error = f'{type(e).__name__}: {e}'
+        rec = run.record(self.adapter, self.cell, time.perf_counter() - t0)
+        self.last_result, self.last_state = rec, (None if run.error else run.field)
         return rec
 
     def Find_q0Star(self, temp, meanDensity, q_start=1.0, record=None):
@@ -156,18 +195,22 @@ class MinimizeFrozenQ:
 # ---------------------------------------------------------------------------
 # the edge pass
 # ---------------------------------------------------------------------------
-def _ok(r):
-    return r is not None and r.get('status') == STATUS_OK
+def _accepted(rec):
+    return rec is not None and rec.get('status') == STATUS_OK
+
+
+def _ok_mask(grid, results):
+    return np.array([_accepted(results.get(p)) for p in grid], dtype=bool)
 
 
 def edge_walks(grid, results):
     """(i, d) for every ok index i of the sorted density `grid` whose
     neighbour i + d (d = -1, +1) is not ok; `results` maps density -> record."""
-    out = []
-    for i, p in enumerate(grid):
-        if _ok(results.get(p)):
-            out += [(i, d) for d in (-1, 1) if 0 <= i + d < len(grid) and not _ok(results.get(grid[i + d]))]
-    return out
+    ok = _ok_mask(grid, results)
+    n = len(ok)
+    left = {int(i) for i in np.flatnonzero(ok[1:] & ~ok[:-1]) + 1} if n > 1 else set()    # i ok, i-1 not
+    right = {int(i) for i in np.flatnonzero(ok[:-1] & ~ok[1:])} if n > 1 else set()       # i ok, i+1 not
+    return sorted([(i, -1) for i in left] + [(i, 1) for i in right])
 
 
 def edge_walk(solve, grid, results, i, d, maxPast=3):
@@ -175,28 +218,53 @@ def edge_walk(solve, grid, results, i, d, maxPast=3):
     again by solve(phi, q_start) with q_start the q* of grid[i]; stop at an
     originally ok density, the grid's end or after maxPast consecutive
     non-ok results.  Returns the records, with q_start, edge_from, edge_dir."""
-    q_seed = float(results[grid[i]]['q'])
-    out, fails, j = [], 0, i + d
-    while 0 <= j < len(grid) and fails < maxPast and not _ok(results.get(grid[j])):
-        r = dict(solve(grid[j], q_seed))
-        r.update(phi=grid[j], q_start=q_seed, edge_from=grid[i], edge_dir=d, pass_='edge')
-        out.append(r)
-        fails = 0 if _ok(r) else fails + 1
-        j += d
+    seed = float(results[grid[i]]['q'])
+    stop = len(grid) if d > 0 else -1
+    out, misses = [], 0
+    for j in range(i + d, stop, d):
+        if misses >= maxPast or _accepted(results.get(grid[j])):
+            break
+        rec = {**solve(grid[j], seed), 'phi': grid[j], 'q_start': seed, 'edge_from': grid[i], 'edge_dir': d,
+               'pass_': 'edge'}
+        out.append(rec)
+        misses = 0 if _accepted(rec) else misses + 1
     return out
 
 
 def merge_edge(results, edge_records):
     """`results` with each non-ok original replaced by an ok edge record (the
-    lowest f when several edge records are ok)."""
-    out = dict(results)
+    lowest f when several edge records are ok, the first on a tie)."""
+    best = {}
     for r in edge_records:
-        if not _ok(r):
-            continue
-        cur = out.get(r['phi'])
-        if not _ok(cur) or (cur.get('pass_') == 'edge' and r['f'] < cur['f']):
-            out[r['phi']] = r
+        if _accepted(r) and (r['phi'] not in best or r['f'] < best[r['phi']]['f']):
+            best[r['phi']] = r
+    out = dict(results)
+    out.update({p: r for p, r in best.items() if not _accepted(results.get(p))})
     return out
+
+
+def _edge_pass(minimizer, temp, grid, pass1, record):
+    """The edge walks of scan_density (one solve at a time)."""
+    if not minimizer.settings.edge:
+        return []
+
+    def solve(p, q):
+        r = minimizer.solve(temp, p, q)
+        if record is not None:
+            record({**r, 'phi': p, 'pass_': 'edge'})
+        return r
+
+    edge = []
+    for i, d in edge_walks(grid, pass1):
+        edge += edge_walk(solve, grid, pass1, i, d, minimizer.settings.maxPast)
+    return edge
+
+
+def _strict_grid(densities):
+    grid = [float(p) for p in densities]
+    if any(b <= a for a, b in zip(grid, grid[1:])):
+        raise ValueError('densities must be strictly increasing')
+    return grid
 
 
 def scan_density(minimizer, temp, densities, q_start=1.0, record=None, done=None):
@@ -205,27 +273,13 @@ def scan_density(minimizer, temp, densities, q_start=1.0, record=None, done=None
     checkpoint), then the edge pass if the settings ask for it.  `record`
     receives every record as it is made.  Returns (final, pass1, edge):
     density -> merged record, density -> independent record, edge records."""
-    grid = [float(p) for p in densities]
-    if any(b <= a for a, b in zip(grid, grid[1:])):
-        raise ValueError('densities must be strictly increasing')
+    grid = _strict_grid(densities)
     pass1 = dict(done or {})
-    for p in grid:
-        if p in pass1:
-            continue
-        r = minimizer.solve(temp, p, q_start)
-        r.update(phi=p, pass_='indep')
-        pass1[p] = r
+    for p in (p for p in grid if p not in pass1):
+        pass1[p] = {**minimizer.solve(temp, p, q_start), 'phi': p, 'pass_': 'indep'}
         if record is not None:
-            record(r)
-    edge = []
-    if minimizer.settings.edge:
-        def solve(p, q):
-            r = minimizer.solve(temp, p, q)
-            if record is not None:
-                record({**r, 'phi': p, 'pass_': 'edge'})
-            return r
-        for i, d in edge_walks(grid, pass1):
-            edge += edge_walk(solve, grid, pass1, i, d, minimizer.settings.maxPast)
+            record(pass1[p])
+    edge = _edge_pass(minimizer, temp, grid, pass1, record)
     return merge_edge(pass1, edge), pass1, edge
 
 
@@ -235,104 +289,45 @@ def scan_density(minimizer, temp, densities, q_start=1.0, record=None, done=None
 def solve_batched(minimizer, temp, densities, q_start=1.0, precision=64):
     """MinimizeFrozenQ.solve at every density, in lockstep: iteration `it`
     relaxes every density still iterating as one batch (evaluate_batched,
-    fields carried, each at its own q and dt) at the stage every density is
-    in at that iteration, then jumps each to the q_opt of its relaxed field.
-    The records equal solve()'s (wall_s is the batch's elapsed time).  If a
-    batch raises, the densities still iterating are solved one at a time."""
+    fields carried, each at its own q and dt), then each density takes its
+    outcome as solve() does (the same _FrozenRun).  Every density is at the
+    same stage at the same iteration, so the batch shares one stage's
+    settings.  The records equal solve()'s (wall_s is the batch's elapsed
+    time).  If a batch raises, the densities still iterating are solved one
+    at a time."""
     from . import batched as _b
     t0 = time.perf_counter()
-    st = minimizer.settings
-    last_stage = len(st.schedule) - 1
-    ad = minimizer.adapter
     rhos = [float(p) for p in densities]
-    n = len(rhos)
-    q = [float(q_start)] * n
-    carry = [None] * n
-    steps = [0] * n
-    last = [None] * n
-    field = [None] * n
-    done = [False] * n
-    stage = 0
+    runs = [_FrozenRun(minimizer.settings, q_start) for _ in rhos]
     fallback = []
-    for it in range(st.maxIter):
-        run = [i for i in range(n) if not done[i]]
-        if not run:
+    for it in range(minimizer.settings.maxIter):
+        live = [k for k, r in enumerate(runs) if not r.finished]
+        if not live:
             break
-        m = minimizer.stage_minimizer(stage)
+        m = minimizer.stage_minimizer(runs[live[0]].stage)
         try:
-            evs = _b.evaluate_batched(m, [(temp, rhos[i], q[i]) for i in run], states=[carry[i] for i in run],
-                                      precision=precision)
+            evs = _b.evaluate_batched(m, [(temp, rhos[k], runs[k].box) for k in live],
+                                      states=[runs[k].carry for k in live], precision=precision)
         except Exception:  # noqa: BLE001 - those densities are solved one at a time below
-            fallback = run
+            fallback = live
             break
-        for i, ev in zip(run, evs):
-            steps[i] += ev.steps
-            field[i] = ev.state
-            frozen = ad.frozen_field(ev.state, minimizer.params, temp, minimizer.cell)
-            q_new = float(frozen.q_opt())
-            # 'stage': the stage solve() holds after this iteration (it
-            # advances before the next relaxation unless the loop stops here)
-            last[i] = {'q': q[i], 'q_opt': q_new, 'f': float(ev.f), 'f_frozen': float(frozen.f(q_new)),
-                       'fitness': float(ev.fitness), 'amps': [float(a) for a in ev.amplitudes], 'stop': ev.stop,
-                       'it': it, 'stage': stage}
-            if ev.stop not in (_q.STOP_CONVERGED, _q.STOP_TOTALT):
-                done[i] = True
-                continue
-            dq = q_new - q[i]
-            carry[i], q[i] = ev.state, q_new
-            if stage < last_stage:
-                last[i]['stage'] = stage + 1
-            elif abs(dq) < st.qTol and ev.stop == _q.STOP_CONVERGED:
-                done[i] = True
-        if stage < last_stage:
-            stage += 1
-    out = []
+        for k, ev in zip(live, evs):
+            runs[k].observe(it, ev.f, ev.fitness, ev.amplitudes, ev.stop, ev.steps, ev.state,
+                            minimizer._frozen(ev.state, temp))
     wall = time.perf_counter() - t0
-    for i in range(n):
-        if i in fallback:
-            out.append(minimizer.solve(temp, rhos[i], q_start))
-            continue
-        L = last[i]
-        rec = {'method': METHOD, 'q_start': float(q_start), 'message': ''}
-        settled = L['stop'] == _q.STOP_CONVERGED and L['stage'] == last_stage and abs(L['q_opt'] - L['q']) < st.qTol
-        if settled:
-            status = STATUS_OK
-        elif L['stop'] in (_q.STOP_CONVERGED, _q.STOP_TOTALT):
-            status = STATUS_NOT_CONVERGED
-        else:
-            status = L['stop']
-        rec.update(status=status, q=L['q_opt'], q_relaxed=L['q'], f=L['f'], f_frozen=L['f_frozen'],
-                   fitness=L['fitness'], stop=L['stop'], steps=steps[i], iters=L['it'] + 1)
-        for j, a in enumerate(L['amps']):
-            rec[f'A{j + 1}'] = a
-        defects = getattr(ad, 'defect_diagnostics', None)
-        if defects is not None and field[i] is not None:
-            rec.update(defects(field[i], minimizer.cell))
-        rec['wall_s'] = wall
-        out.append(rec)
-    return out
+    return [minimizer.solve(temp, rhos[k], q_start) if k in fallback
+            else runs[k].record(minimizer.adapter, minimizer.cell, wall) for k in range(len(rhos))]
 
 
 def scan_density_batched(minimizer, temp, densities, q_start=1.0, record=None, precision=64):
     """scan_density with the independent solves batched (solve_batched); the
     edge walks, few and sequential by nature, run one solve at a time.
     Returns (final, pass1, edge) as scan_density does."""
-    grid = [float(p) for p in densities]
-    if any(b <= a for a, b in zip(grid, grid[1:])):
-        raise ValueError('densities must be strictly increasing')
+    grid = _strict_grid(densities)
     pass1 = {}
     for p, r in zip(grid, solve_batched(minimizer, temp, grid, q_start, precision=precision)):
-        r.update(phi=p, pass_='indep')
-        pass1[p] = r
+        pass1[p] = {**r, 'phi': p, 'pass_': 'indep'}
         if record is not None:
-            record(r)
-    edge = []
-    if minimizer.settings.edge:
-        def solve(p, q):
-            r = minimizer.solve(temp, p, q)
-            if record is not None:
-                record({**r, 'phi': p, 'pass_': 'edge'})
-            return r
-        for i, d in edge_walks(grid, pass1):
-            edge += edge_walk(solve, grid, pass1, i, d, minimizer.settings.maxPast)
+            record(pass1[p])
+    edge = _edge_pass(minimizer, temp, grid, pass1, record)
     return merge_edge(pass1, edge), pass1, edge

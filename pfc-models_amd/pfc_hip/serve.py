@@ -1,16 +1,36 @@
 """Serving: a continuous-batching loop for PFC relaxation requests.
 
 A request names a model (std or log), a grid side, (r, psibar) and a step
-count.  Requests of one shape (model, grid, scheme, dt, precision) share a
-Batch ("pool") with its own HIP stream; each request occupies one slot,
-keeps its field and ETD tables resident in HBM, and is admitted as soon as a
-slot is free and the HBM budget allows the pool.  Every tick, each pool with
-work advances all of its running requests by the same number of steps -- the
-smallest remaining count among them, capped at `tick_steps`, so every request
-gets exactly the steps it asked for -- and finished requests are retired
-(free energy, mean and optionally the field read back) while the others keep
-running.  Pools step on their own streams, so their launches overlap on the
-device.
+count.  Requests of one shape (model, grid, scheme, dt, precision) run as
+slots of a Batch ("slab") with its own HIP stream; each keeps its field and
+ETD tables resident in HBM from admission to retirement.  Every tick, each
+slab with work advances all of its running requests by the same number of
+steps -- the smallest remaining count among them, capped at `tick_steps`, so
+every request gets exactly the steps it asked for -- and finished requests
+are retired (free energy, mean and optionally the field read back) while the
+others keep running.  Slabs step on their own streams, so their launches
+overlap on the device.
+
+Memory admission.  The HBM budget is spent slab by slab: a slab of k slots
+reserves k times the per-slot footprint (state and FP64 tables, as the
+library reports it) when it is created and returns it when it is released.
+Admission is first come, first served:
+
+* a request takes a free slot of a slab of its shape that is not draining;
+* otherwise a new slab of its shape is created, as large as the shape's
+  slabs together (at least `first_slots` and the requests of that shape now
+  waiting; at most `max_slots` and what the budget leaves);
+* if the budget is spent, idle slabs (no running request) are released,
+  least recently used first, until the new slab fits;
+* if it still does not fit, the oldest waiting request marks slabs
+  "draining" (they admit nothing new) until their footprint covers its
+  need; no later request may create a slab before it, so the memory freed
+  when the drained slabs empty goes to it;
+* a request whose single slot exceeds the whole budget is rejected at once.
+
+Running requests always finish, so draining slabs empty, are released, and
+the oldest request is admitted: the loop cannot livelock, and it sleeps
+instead of spinning whenever no slab stepped.
 
 A slot's arithmetic does not depend on its neighbours (the batch contract),
 so a request's result is the same alone and in the mix; with the kernel
@@ -52,6 +72,7 @@ class Result:
     done: float
     nonfinite: bool
     field: object = None
+    rejected: str = ""   # non-empty: the request was refused (the reason)
 
     @property
     def latency(self):
@@ -93,46 +114,86 @@ def poisson_stream(n=2000, rate=100.0, seed=5, sides=(128, 256, 512, 1024, 2048)
     return out
 
 
-class _Pool:
-    def __init__(self, key, params, capacity):
+class TorchDevice:
+    """The device side of the server: Batches of libpfc_hip on torch streams."""
+
+    def __init__(self):
         import torch
+        self.torch = torch
+
+    def slot_bytes(self, params):
+        from .plan import slot_bytes
+        return slot_bytes(**params)
+
+    def batch(self, capacity, params):
         from .plan import Batch
+        return Batch(capacity, **params)
+
+    def stream(self):
+        return self.torch.cuda.Stream()
+
+    def using(self, stream):
+        return self.torch.cuda.stream(stream)
+
+    def record(self, stream):
+        ev = self.torch.cuda.Event()
+        ev.record(stream)
+        return ev
+
+    def to_host(self, t):
+        return t.cpu().numpy()
+
+    def synchronize(self):
+        self.torch.cuda.synchronize()
+
+
+class _Slab:
+    """`capacity` slots of one shape on one stream."""
+
+    def __init__(self, dev, key, params, capacity, per_slot, serial):
+        self.dev = dev
         self.key = key
         self.params = params
-        self.batch = Batch(capacity, **params)
-        self.stream = torch.cuda.Stream()
+        self.batch = dev.batch(capacity, params)
+        self.bytes = int(self.batch.device_bytes())
+        self.stream = dev.stream()
         self.capacity = capacity
+        self.per_slot = per_slot
+        self.serial = serial       # creation order (ties in least-recently-used)
+        self.last_use = 0.0        # last admission time
         self.free = deque(range(capacity))
-        self.running = {}      # slot -> [request, remaining steps]
+        self.running = {}          # slot -> [request, remaining steps]
         self.flags = np.zeros(capacity, dtype=np.int32)  # sticky non-finite flags per slot
-        self.inflight = deque()  # events of enqueued ticks (bounded: backpressure)
-        self.dirty = True      # active list needs re-upload
+        self.inflight = deque()    # events of enqueued ticks (bounded: backpressure)
+        self.dirty = True          # active list needs re-upload
+        self.draining = False
+
+    @property
+    def idle(self):
+        return not self.running
 
     def admit(self, req, now):
-        import torch
         slot = self.free.popleft()
         self.flags[slot] = 0
-        with torch.cuda.stream(self.stream):
+        with self.dev.using(self.stream):
             self.batch.set_slot(slot, req.r)
             self.batch.set_field(slot, request_field(req))
         self.running[slot] = [req, req.steps]
         self.dirty = True
+        self.last_use = now
 
     def tick(self, tick_steps):
         """Advance every running request; returns the number of steps taken."""
-        import torch
         if not self.running:
             return 0
         k = min(tick_steps, min(rem for _, rem in self.running.values()))
-        with torch.cuda.stream(self.stream):
+        with self.dev.using(self.stream):
             if self.dirty:
                 self.batch.set_active(sorted(self.running))
                 self.dirty = False
             self.batch.step(k)
-            ev = torch.cuda.Event()
-            ev.record(self.stream)
-        self.inflight.append(ev)
-        # keep at most two ticks queued per pool so newly admitted requests
+            self.inflight.append(self.dev.record(self.stream))
+        # keep at most two ticks queued per slab so newly admitted requests
         # do not wait behind a long backlog
         while len(self.inflight) > 2:
             self.inflight.popleft().synchronize()
@@ -142,16 +203,15 @@ class _Pool:
 
     def retire(self, t0, keep_fields):
         """Read back and free every finished request."""
-        import torch
         done = [s for s, (_, rem) in self.running.items() if rem == 0]
         if not done:
             return []
         out = []
-        with torch.cuda.stream(self.stream):
+        with self.dev.using(self.stream):
             self.batch.set_active(done)
             red = self.batch.reduce()
-            self.flags |= self.batch.status()
-            fields = [self.batch.get_field(s).cpu().numpy() if keep_fields else None for s in done]
+            self.flags |= np.asarray(self.batch.status(), dtype=np.int32)
+            fields = [self.dev.to_host(self.batch.get_field(s)) if keep_fields else None for s in done]
         self.inflight.clear()  # the reductions synchronised the stream
         now = time.perf_counter() - t0
         for j, s in enumerate(done):
@@ -161,6 +221,9 @@ class _Pool:
             self.free.append(s)
         self.dirty = True
         return out
+
+    def close(self):
+        self.batch.close()
 
 
 @dataclass
@@ -172,83 +235,197 @@ class ServeStats:
     p99_s: float
     peak_bytes: int
     pools: dict = field(default_factory=dict)
+    slabs_created: int = 0
+    slabs_released: int = 0
+    rejected: int = 0
+    slabs_drained: int = 0
+    occupied_fraction: float = 0.0   # share of the wall time with at least one request on the device
 
 
 class Server:
-    """Continuous batching over per-shape pools under an HBM budget."""
+    """Continuous batching over per-shape slabs under an HBM budget."""
 
     def __init__(self, hbm_budget=200e9, tick_steps=200, precision=32, scheme="etd1", max_slots=64,
-                 keep_fields=False):
+                 keep_fields=False, device=None, first_slots=8):
         self.budget = float(hbm_budget)
         self.tick_steps = int(tick_steps)
         self.precision = precision
         self.scheme = scheme
-        self.max_slots = max_slots
+        self.max_slots = int(max_slots)
+        self.first_slots = int(first_slots)
         self.keep_fields = keep_fields
-        self.pools = {}
-        self.used = 0
+        self.dev = device if device is not None else TorchDevice()
+        self.slabs = {}        # key -> [slab, ...]
+        self.used = 0          # bytes reserved by live slabs
+        self.peak = 0
+        self.created = 0
+        self.released = 0
+        self.drained = 0       # slabs marked draining for an older request's memory
+        self._per = {}         # key -> bytes per slot
+        self._fixed = {}       # key -> bytes of a slab beyond its slots
 
-    def _pool_for(self, req):
-        from .plan import slot_bytes
+    # -- shapes and memory ----------------------------------------------------
+    def _shape(self, req):
         params = request_params(req, self.precision, self.scheme)
         key = (req.model, req.N, self.scheme, params["dt"], self.precision)
-        pool = self.pools.get(key)
-        if pool is None:
-            per = slot_bytes(**params)
-            cap = int(max(1, min(self.max_slots, (self.budget - self.used) // (2 * per))))
-            if per * cap > self.budget - self.used:
-                return None, key
-            pool = _Pool(key, params, cap)
-            self.pools[key] = pool
-            self.used += pool.batch.device_bytes()
-        return pool, key
+        if key not in self._per:
+            self._per[key] = int(self.dev.slot_bytes(params))
+        return key, params, self._per[key]
 
+    def _all_slabs(self):
+        return [s for lst in self.slabs.values() for s in lst]
+
+    def _release(self, slab):
+        slab.close()
+        self.slabs[slab.key].remove(slab)
+        if not self.slabs[slab.key]:
+            del self.slabs[slab.key]
+        self.used -= slab.bytes
+        self.released += 1
+
+    def _release_idle(self, need, keep=None):
+        """Release idle slabs, least recently used first, until `need` bytes
+        fit in the budget (or none is left)."""
+        for slab in sorted((s for s in self._all_slabs() if s.idle and s is not keep),
+                           key=lambda s: (s.last_use, s.serial)):
+            if self.budget - self.used >= need:
+                break
+            self._release(slab)
+        return self.budget - self.used >= need
+
+    def _need(self, key, per):
+        """Bytes a one-slot slab of `key` takes: the slot and the batch's
+        fixed part (roots, reduction buffers), measured at the shape's first
+        slab; until then one more slot's worth is assumed, or none when no
+        slab holds memory (the first slab then measures it)."""
+        return per + self._fixed.get(key, per if self.used > 0 else 0)
+
+    def _create(self, key, params, per, nwaiting, now):
+        # geometric growth: a shape's next slab holds as many slots as its
+        # slabs together (at least `first_slots` and the requests waiting for
+        # it), so a steady stream of one shape ends in a few large batches
+        have = sum(s.capacity for s in self.slabs.get(key, []))
+        want = max(nwaiting, self.first_slots, have)
+        fixed = self._need(key, per) - per
+        cap = int(max(1, min(self.max_slots, want, (self.budget - self.used - fixed) // per)))
+        slab = _Slab(self.dev, key, params, cap, per, self.created)
+        self._fixed[key] = max(0, slab.bytes - cap * per)
+        slab.last_use = now
+        self.created += 1
+        self.slabs.setdefault(key, []).append(slab)
+        self.used += slab.bytes
+        self.peak = max(self.peak, self.used)
+        return slab
+
+    def _drain_for(self, need):
+        """Mark slabs draining, least recently used first, until the budget
+        left plus the footprint of draining and idle slabs covers `need`."""
+        slabs = self._all_slabs()
+        covered = self.budget - self.used + sum(s.bytes for s in slabs if s.draining or s.idle)
+        for slab in sorted((s for s in slabs if not s.draining and not s.idle),
+                           key=lambda s: (s.last_use, s.serial)):
+            if covered >= need:
+                break
+            slab.draining = True
+            self.drained += 1
+            covered += slab.bytes
+
+    def _admit(self, waiting, now, rejected):
+        """One first-come-first-served admission pass; returns the requests
+        still waiting."""
+        # draining slabs that emptied are released at once
+        for slab in [s for s in self._all_slabs() if s.draining and s.idle]:
+            self._release(slab)
+        count = {}
+        for req in waiting:
+            k = self._shape(req)[0]
+            count[k] = count.get(k, 0) + 1
+        still = deque()
+        blocked = False  # the oldest request that needs memory is waiting for it
+        while waiting:
+            req = waiting.popleft()
+            key, params, per = self._shape(req)
+            slab = next((s for s in self.slabs.get(key, []) if s.free and not s.draining), None)
+            if slab is None and per + self._fixed.get(key, 0) > self.budget:
+                rejected.append(Result(req.id, float("nan"), float("nan"), req.arrival, now, False,
+                                       rejected=f"one slot needs {per} bytes, the budget is {int(self.budget)}"))
+                count[key] -= 1
+                continue
+            if slab is None and not blocked:
+                need = self._need(key, per)
+                if self.budget - self.used < need:
+                    self._release_idle(need + (min(count[key], self.max_slots) - 1) * per)
+                if self.budget - self.used < need:
+                    self._release_idle(need)
+                if self.budget - self.used >= need:
+                    slab = self._create(key, params, per, count[key], now)
+                elif not any(not s.draining for s in self.slabs.get(key, [])):
+                    # no slab of its shape will free a slot: claim memory
+                    self._drain_for(need)
+                    blocked = True
+                # else: a running slab of its shape frees a slot when one of
+                # its requests finishes; the request keeps its place in line
+            if slab is None:
+                still.append(req)
+                continue
+            slab.admit(req, now)
+            count[key] -= 1
+        return still
+
+    # -- the loop ---------------------------------------------------------------
     def run(self, requests, realtime=True, progress=None, progress_every=10.0):
         """Serve `requests` (sorted by arrival).  With realtime=True a request
         becomes visible at its arrival time; otherwise all are queued at once.
         `progress(done, total, seconds)` is called about every
         `progress_every` seconds."""
-        import torch
         reqs = sorted(requests, key=lambda r: r.arrival)
         pending = deque(reqs)
-        results = []
+        results, rejected = [], []
         t0 = time.perf_counter()
         last = t0
         if not realtime:
             for r in reqs:
                 r.arrival = 0.0
         waiting = deque()
-        while pending or waiting or any(p.running for p in self.pools.values()):
+        occupied = 0.0
+        while pending or waiting or any(not s.idle for s in self._all_slabs()):
             now = time.perf_counter() - t0
             while pending and pending[0].arrival <= now:
                 waiting.append(pending.popleft())
-            # admission: FIFO, a request waits only for its own pool's slots
-            still = deque()
-            while waiting:
-                req = waiting.popleft()
-                pool, _ = self._pool_for(req)
-                if pool is not None and pool.free:
-                    pool.admit(req, now)
-                else:
-                    still.append(req)
-            waiting = still
-            stepped = 0
-            for pool in self.pools.values():
-                stepped += pool.tick(self.tick_steps)
-            for pool in self.pools.values():
-                results.extend(pool.retire(t0, self.keep_fields))
+            if waiting:
+                waiting = self._admit(waiting, now, rejected)
+            running = any(not s.idle for s in self._all_slabs())
+            for slab in self._all_slabs():
+                slab.tick(self.tick_steps)
+            nres = len(results)
+            for slab in self._all_slabs():
+                results.extend(slab.retire(t0, self.keep_fields))
+            if running:
+                occupied += time.perf_counter() - t0 - now
+            if not running and len(results) == nres:
+                if waiting:
+                    # nothing runs, so nothing can free memory: admission must
+                    # have placed or rejected the oldest request
+                    raise RuntimeError("serve: waiting requests with nothing running (admission bug)")
+                wait = 0.001
+                if pending and not waiting:
+                    wait = min(max(0.0, pending[0].arrival - (time.perf_counter() - t0)), 0.01)
+                time.sleep(wait)
             if progress is not None and time.perf_counter() - last >= progress_every:
                 last = time.perf_counter()
                 progress(len(results), len(reqs), last - t0)
-            if stepped == 0 and pending and not waiting:
-                time.sleep(max(0.0, min(pending[0].arrival - (time.perf_counter() - t0), 0.01)))
-        torch.cuda.synchronize()
+        self.dev.synchronize()
         wall = time.perf_counter() - t0
         lat = np.array([r.latency for r in results]) if results else np.zeros(1)
-        return ServeStats(sorted(results, key=lambda r: r.id), wall, len(results) / wall,
-                          float(np.percentile(lat, 50)), float(np.percentile(lat, 99)), self.used,
-                          {str(k): p.capacity for k, p in self.pools.items()})
+        pools = {}
+        for s in self._all_slabs():
+            pools[str(s.key)] = pools.get(str(s.key), 0) + s.capacity
+        return ServeStats(sorted(results + rejected, key=lambda r: r.id), wall, len(results) / wall,
+                          float(np.percentile(lat, 50)), float(np.percentile(lat, 99)), self.peak, pools,
+                          self.created, self.released, len(rejected), self.drained, occupied / wall)
 
     def close(self):
-        for p in self.pools.values():
-            p.batch.close()
+        for s in self._all_slabs():
+            s.close()
+        self.slabs.clear()
+        self.used = 0

@@ -95,12 +95,32 @@ def test_pin_holds_mean_exactly():
     assert abs(sim.mean() - MEAN) < 1e-15
 
 
-def test_optimal_shift_secant():
-    g = -2.0
-    lo, hi = 0.2, 1.4
-    n = lambda x: g * x**2 + x**3 + math.log(x)
-    assert O.optimal_shift(lo, hi, g, 1e-6) == pytest.approx((n(hi) - n(lo)) / (hi - lo), rel=1e-15)
-    assert O.optimal_shift(-0.1, hi, g, 1e-6) == pytest.approx((n(hi) - n(1e-6)) / (hi - 1e-6), rel=1e-15)
+@pytest.mark.parametrize("lo,hi,g", [(0.2, 1.4, -2.0), (0.5, 0.6, 1.0), (0.05, 2.5, 0.5)])
+def test_optimal_shift_levels_the_shifted_nonlinearity(lo, hi, g):
+    """What the shift is for (logpfc_extv/simulation.py:332-373): with M =
+    optimal_shift(lo, hi), the shifted nonlinearity N(phi) - M phi, evaluated
+    by the step's own nonlinear_mu, takes equal values at the window's ends;
+    and M lies between N' at the two ends (mean value theorem, N convex or
+    not, the secant slope is attained inside)."""
+    M = O.optimal_shift(lo, hi, g, 1e-6)
+    p = O.Params(model="log", g=g, clamp_log=True, M=M)
+    v = O.nonlinear_mu(np.array([lo, hi]), p)
+    assert abs(v[1] - v[0]) <= 1e-13 * max(1.0, abs(v[0]))
+    xs = np.linspace(lo, hi, 20001)
+    dN = 1 / xs + 2 * g * xs + 3 * xs**2
+    assert dN.min() <= M <= dN.max()
+
+
+def test_optimal_shift_floors_a_non_positive_minimum():
+    """phi_min <= 0 is replaced by minPhi (extv :336-344): the shifted
+    nonlinearity with the log clamped at minPhi is level between minPhi and
+    phi_max."""
+    minPhi, hi, g = 1e-6, 1.4, -2.0
+    M = O.optimal_shift(-0.1, hi, g, minPhi)
+    assert M == O.optimal_shift(minPhi, hi, g, minPhi)
+    p = O.Params(model="log", g=g, clamp_log=True, M=M, minPhi=minPhi)
+    v = O.nonlinear_mu(np.array([minPhi, hi, -0.1]), p)
+    assert abs(v[1] - v[0]) <= 1e-12 * abs(v[0])
 
 
 def test_log_relaxes_monotonically_264():

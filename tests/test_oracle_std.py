@@ -5,6 +5,8 @@ Every assertion here comes from the reference's own tests
 mathematics that a plausible slip (sign of the Laplacian, a dropped 1/2, a
 transposed k-grid, a wrong normalisation) would break.
 """
+import math
+
 import numpy as np
 import pytest
 
@@ -164,6 +166,26 @@ def test_relaxes_monotonically_264():
         mass.append(sim.mean())
     assert (np.diff(fe) < 0).all()
     assert max(mass) - min(mass) < 1e-12
+
+
+@pytest.mark.parametrize("mx,my", [(3, 0), (0, 2), (4, 5)])
+def test_local_field_single_mode_closed_form(mx, my):
+    """Gaussian smoothing of one Fourier mode: local_field(cos(kx x) cos(ky y))
+    = exp(-k^2 w^2 / 2) cos(kx x) cos(ky y), k^2 = kx^2 + ky^2, w = 2 pi
+    (kernels.py:49-52 applied as in Get_local_n, stdpfc_cpu/simulation.py:250-261).
+    A w-for-w^2 or missing-half slip changes the factor by orders of magnitude
+    at these k (exp(-k^2 w^2/2) ~ 1e-3 .. 1e-30)."""
+    Lx, Ly = 64.0, 48.0
+    geom = O.build_geometry(64, 48, Lx, Ly)
+    kx, ky = 2 * np.pi * mx / Lx, 2 * np.pi * my / Ly
+    f = np.cos(kx * geom.X) * np.cos(ky * geom.Y)
+    sim = O.Simulation(O.Params(model="std", epsilon=-0.3, g=0.0, dt=DT), geom)
+    sim.set_field(f)
+    w = 2 * np.pi
+    expect = math.exp(-(kx**2 + ky**2) * w**2 / 2) * f
+    got = sim.local_field()
+    assert np.abs(got - expect).max() <= 1e-15 + 1e-12 * np.abs(expect).max()
+    assert np.abs(expect).max() > 1e-35  # the factor itself is resolved, not zero
 
 
 def test_geometry_contract():
