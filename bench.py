@@ -119,6 +119,7 @@ def run_gpu(precision, steps, warmup):
         },
         "device_bytes": pl.device_bytes(),
         "schedule": sched,
+        "stores": pl.stores(),
     }
 
 
@@ -302,8 +303,22 @@ def run_sweep(plain_fraction=8, steps=5000):
     relaxed = np.abs(plain.fe - plain.fe_prev) <= 1e-6 * np.abs(plain.fe)
     # coexistence densities at every r: FP32 batched against FP64 batched
     # (which matches the plain FP64 solve to rounding, rel64 above)
+    # the floors the still-relaxing instances' FP32 errors are read against:
+    # the FP64 solve's change under a one-ulp and an FP32-rounded start, and
+    # the FP32 path's own spread over starts moved by up to one FP32 ulp
+    moving = [i for i, ok in zip(sub, relaxed) if not ok]
+    floors = None
+    if moving:
+        cases = [(SW._model_kw(i, 0.5, 64, "etd1"), SW.initial_field(i)) for i in moving]
+        _, ic, ulp = SW.rounding_floor(cases, steps)
+        spread = SW.fp32_spread(cases, steps)
+        floors = {"fp64_one_ulp_start_max": float(ulp.max()), "fp64_fp32_rounded_start_max": float(ic.max()),
+                  "fp32_spread_over_perturbed_starts": [[float(v.min()), float(v.max())] for v in spread]}
     c32, c64 = SW.coexistence_by_r(inst, b32.fe), SW.coexistence_by_r(inst, b64.fe)
-    both = [r for r in c64 if c64[r].status == "ok" and c32[r].status == "ok"]
+    from pfc_hip.coexistence import liquid_crystal_pair_ok
+    tangents = [r for r in c64 if c64[r].status == "ok" and c32[r].status == "ok"]
+    both = [r for r in tangents if liquid_crystal_pair_ok(c64[r])]
+    spurious = [r for r in tangents if r not in both]
     dpsi = [max(abs(c32[r].phi_a - c64[r].phi_a), abs(c32[r].phi_b - c64[r].phi_b)) for r in both]
     return {
         "metric": f"sweep solves/s (256 stdPFC instances, 64^2..512^2, ETD1 dt=0.5, {steps} steps)",
@@ -320,8 +335,12 @@ def run_sweep(plain_fraction=8, steps=5000):
                                     "median": float(np.median(rel32[relaxed])) if relaxed.any() else None,
                                     "max": float(rel32[relaxed].max()) if relaxed.any() else None},
         "still_relaxing": {"n": int((~relaxed).sum()), "max_rel_fe_err": float(rel32[~relaxed].max())
-                           if (~relaxed).any() else None},
-        "coexistence": {"r_values_with_tangent": len(both), "r_values": len(c64),
+                           if (~relaxed).any() else None,
+                           "rel_fe_err": [float(v) for v in rel32[~relaxed]], "floors": floors},
+        "coexistence": {"r_values_with_tangent": len(tangents), "r_values_kept": len(both), "r_values": len(c64),
+                        "spurious": {f"{r:.3f}": [round(c64[r].phi_a, 6), round(c64[r].phi_b, 6)] for r in spurious},
+                        "spurious_rule": "phi_liquid >= phi_crystal - 1e-3 (reversed or degenerate: unconverged "
+                                         "crystal free energies)",
                         "max_abs_density_err_fp32_vs_fp64": max(dpsi) if dpsi else None,
                         "fp64": {f"{r:.3f}": [round(c64[r].phi_a, 6), round(c64[r].phi_b, 6)] for r in both}},
         "fp64_batched_rel_fe_err_max": float(rel64.max()),
@@ -366,7 +385,7 @@ def run_logpfc(steps=10000, n=2048):
         out[f"fp{prec}"] = {"value": n * n * (steps - 20) / wall / 1e6, "unit": "Mpts*steps/s",
                             "ms_per_step": 1e3 * wall / (steps - 20), "free_energy_start": fe0, "free_energy": fe,
                             "rel_above_liquid": (fe - liq) / abs(liq), "phi_min": lo, "phi_max": hi,
-                            "M": float(sim.State.M), "status": int(sim.status())}
+                            "M": float(sim.State.M), "status": int(sim.status()), "stores": sim._plan.stores()}
         sim.close()
     out["fp32_rel_fe_vs_fp64"] = abs(fes[32] - fes[64]) / abs(fes[64])
     return out
@@ -490,13 +509,14 @@ def main():
         "roofline": f64["roofline"],
         "step_roofline": f64["step_roofline"],
         "kernels_us": f64["kernels_us"],
+        "stores": f64["stores"],
         "step_graph": {"eager_ms_per_step": round(f64["eager_ms_per_step"], 5),
                        "captures_replays": f64["graph_captures_replays"]},
         "fp32": None if f32 is None else {
             "value": round(f32["value"], 1), "ms_per_step": round(f32["ms_per_step"], 5),
             "eager_ms_per_step": round(f32["eager_ms_per_step"], 5),
             "dtype": "f32 bulk, f64 tables, combine and reductions", "roofline": f32["roofline"],
-            "kernels_us": f32["kernels_us"],
+            "kernels_us": f32["kernels_us"], "stores": f32["stores"],
             "rel_fe_vs_fp64": abs(f32["free_energy"] - f64["free_energy"]) / abs(f64["free_energy"])},
         "cpu_baseline": cpu,
         "sweep": sweep,

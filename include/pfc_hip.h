@@ -172,6 +172,12 @@ int pfc_step_profile(pfc_plan* plan, long long nsteps, void* stream, double ms[3
  * holds the whole row spectrum in LDS for all the steps of a call).
  * (No reference counterpart.) */
 int pfc_plan_schedule(const pfc_plan* plan, int info[8]);
+/* Store kind of the fast row and column kernels' outputs (after the first
+ * step): 1 non-temporal, 0 ordinary, -1 when the two fast kernels do not both
+ * run.  The autotuner times a step loop with each and keeps the faster;
+ * PFC_HIP_STORES=nt|ordinary pins it.  The kind changes no result.
+ * (No reference counterpart.) */
+int pfc_plan_stores(const pfc_plan* plan, int* nt);
 /* Step-loop graph counters: out[0] = graphs captured, out[1] = graph replays.
  * A pfc_step / pfc_batch_step call of n >= 32 steps on the two-kernel path
  * replays the HIP graph of 32 steps of its launch configuration n / 32 times

@@ -214,6 +214,7 @@ std::vector<SchedInfo> schedules_for(int L) {
 
 struct FastShape {
   bool on = false;
+  bool nt = true;   // non-temporal output stores (else ordinary)
   int var = 0;      // schedule variant
   int P = 0;        // points per lane
   int tpl = 0;      // lanes per line
@@ -295,34 +296,17 @@ inline bool row_range_launch(int M, int var, dim3 g, dim3 b, size_t lds, hipStre
   return row_range_launch_f32(M, var, g, b, lds, st, S, Ny, lay, tw, sl, part);
 }
 
-template <typename T> struct FastLaunch;
-template <> struct FastLaunch<double> {
-  static void row(int M, const FastShape& f, int ny_inst, hipStream_t st, const V2<double>* S, V2<double>* T, int Ny,
+template <typename T> struct FastLaunch {
+  static void row(int M, const FastShape& f, int ny_inst, hipStream_t st, const V2<T>* S, V2<T>* Tout, int Ny,
                   const Lay& lay, const Nonlin& q, const V2<double>* tw, const Slots& sl, RangePart* rpart = nullptr) {
-    if (!row_fast_launch_f64(M, f.var, dim3(f.blocks, ny_inst), dim3(f.lines * f.tpl), f.lds, st, S, T, Ny, lay, q,
-                             tw, sl, rpart))
+    if (!row_fast_launch<T>(M, f.var, f.nt, dim3(f.blocks, ny_inst), dim3(f.lines * f.tpl), f.lds, st, S, Tout, Ny,
+                            lay, q, tw, sl, rpart))
       fail(PFC_EUNSUPPORTED, "no fast row schedule");
   }
-  static void col(int Ny, int mode, const FastShape& f, int ny_inst, hipStream_t st, V2<double>* S,
-                  const V2<double>* T, V2<double>* D, int Nc, const Lay& lay, const ColTables& tb, const V2<double>* tw,
-                  const Slots& sl) {
-    if (!col_fast_launch_f64(Ny, f.var, mode, dim3(f.blocks, ny_inst), dim3(f.lines * f.tpl), f.lds, st, S, T, D,
-                             Nc, lay, tb, tw, sl))
-      fail(PFC_EUNSUPPORTED, "no fast column schedule");
-  }
-};
-template <> struct FastLaunch<float> {
-  static void row(int M, const FastShape& f, int ny_inst, hipStream_t st, const V2<float>* S, V2<float>* T, int Ny,
-                  const Lay& lay, const Nonlin& q, const V2<double>* tw, const Slots& sl, RangePart* rpart = nullptr) {
-    if (!row_fast_launch_f32(M, f.var, dim3(f.blocks, ny_inst), dim3(f.lines * f.tpl), f.lds, st, S, T, Ny, lay, q,
-                             tw, sl, rpart))
-      fail(PFC_EUNSUPPORTED, "no fast row schedule");
-  }
-  static void col(int Ny, int mode, const FastShape& f, int ny_inst, hipStream_t st, V2<float>* S,
-                  const V2<float>* T, V2<float>* D, int Nc, const Lay& lay, const ColTables& tb, const V2<double>* tw,
-                  const Slots& sl) {
-    if (!col_fast_launch_f32(Ny, f.var, mode, dim3(f.blocks, ny_inst), dim3(f.lines * f.tpl), f.lds, st, S, T, D,
-                             Nc, lay, tb, tw, sl))
+  static void col(int Ny, int mode, const FastShape& f, int ny_inst, hipStream_t st, V2<T>* S, const V2<T>* Tin,
+                  V2<T>* D, int Nc, const Lay& lay, const ColTables& tb, const V2<double>* tw, const Slots& sl) {
+    if (!col_fast_launch<T>(Ny, f.var, mode, f.nt, dim3(f.blocks, ny_inst), dim3(f.lines * f.tpl), f.lds, st, S, Tin,
+                            D, Nc, lay, tb, tw, sl))
       fail(PFC_EUNSUPPORTED, "no fast column schedule");
   }
 };
@@ -496,6 +480,9 @@ void engine_init(Engine& e, const pfc_params* params, int cap, bool batched) {
         e.tuned = true;
       }
     }
+    // PFC_HIP_STORES=nt|ordinary pins the output store kind (else autotuned)
+    if (const char* sk = std::getenv("PFC_HIP_STORES"))
+      if (sk[0] == 'n' || sk[0] == 'o') e.frow.nt = e.fcol.nt = sk[0] == 'n';
   }
   const size_t bytes = (size_t)cap * e.nc * e.csize;
   const size_t tbytes = (size_t)cap * e.ntab * sizeof(double);  // ETD tables are FP64 in every precision
@@ -848,6 +835,58 @@ void autotune(Engine& e, hipStream_t st) {
       });
       e.fcol = pick;
       g_tune[key] = {pick.var, pick.lines};
+      hip_check(hipStreamSynchronize(st), "sync after tuning");
+    }
+  }
+  // Store kind of both kernels' outputs: non-temporal stores stream the
+  // arrays out while the kernel runs (the next kernel reads them from other
+  // XCDs), ordinary stores let L2 merge the partial 128-byte lines of tiles
+  // narrower than a line.  Which wins depends on the shape and the working
+  // set, and only shows across the kernel boundary, so the two are timed as
+  // a step loop (row, column[, row, column]) on scratch copies of the state.
+  const char* ntenv = std::getenv("PFC_HIP_STORES");  // "nt" or "ordinary" pins the kind
+  if (ntenv && (ntenv[0] == 'n' || ntenv[0] == 'o')) {
+    e.frow.nt = e.fcol.nt = ntenv[0] == 'n';
+  } else if (e.frow.on && e.fcol.on) {
+    const std::array<int, 7> key{dev, 2, M, Ny, e.p.precision, e.p.scheme, bucket(ny)};
+    auto it = g_tune.find(key);
+    if (it != g_tune.end()) {
+      e.frow.nt = e.fcol.nt = it->second.first != 0;
+    } else {
+      DevBuf S2, T2, D2;
+      S2.alloc(bytes);
+      T2.alloc(bytes);
+      D2.alloc(bytes);
+      const Nonlin q = nonlin_of(e);
+      ColTables tb = e.tables();
+      const bool rk2 = e.p.scheme == PFC_SCHEME_ETDRK2;
+      auto step = [&](bool nt) {
+        FastShape fr = e.frow, fc = e.fcol;
+        fr.nt = fc.nt = nt;
+        auto row = [&] {
+          FastLaunch<T>::row(M, fr, ny, st, (const V*)S2.p, (V*)T2.p, Ny, e.lay, q, (const V2<double>*)e.twN64.p, sl);
+        };
+        auto col = [&](int mode) {
+          FastLaunch<T>::col(Ny, mode, fc, ny, st, (V*)S2.p, (const V*)T2.p, (V*)D2.p, Nc, e.lay, tb,
+                             (const V2<double>*)e.twY64.p, sl);
+        };
+        row();
+        col(rk2 ? COL_RK2_STAGE1 : COL_ETD1);
+        if (rk2) {
+          row();
+          col(COL_RK2_STAGE2);
+        }
+      };
+      float t[2] = {1e30f, 1e30f};
+      for (int round = 0; round < 2; ++round)
+        for (int j = 0; j < 2; ++j) {
+          const int nt = round ? 1 - j : j;
+          hip_check(hipMemcpyAsync(S2.p, e.S.p, bytes, hipMemcpyDeviceToDevice, st), "copy");
+          t[nt] = std::min(t[nt], time_launches(st, ny > 8 ? 4 : 8, [&] { step(nt != 0); }));
+        }
+      const bool nt = t[1] <= t[0];
+      e.frow.nt = e.fcol.nt = nt;
+      g_tune[key] = {nt ? 1 : 0, 0};
       hip_check(hipStreamSynchronize(st), "sync after tuning");
     }
   }
@@ -1956,6 +1995,13 @@ int pfc_plan_schedule(const pfc_plan* pl, int info[8]) {
       info[4 * k + 2] = fs[k]->P;
       info[4 * k + 3] = fs[k]->lines;
     }
+  });
+}
+
+int pfc_plan_stores(const pfc_plan* pl, int* nt) {
+  return guarded([&] {
+    if (!pl || !nt) fail(PFC_EINVAL, "NULL argument");
+    *nt = (pl->e.resident || !(pl->e.frow.on && pl->e.fcol.on)) ? -1 : (pl->e.frow.nt ? 1 : 0);
   });
 }
 

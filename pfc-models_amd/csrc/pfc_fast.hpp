@@ -107,6 +107,16 @@ template <bool BLOCK> __device__ __forceinline__ void line_sync() {
 // write-back.  (Measured faster per step also for tiles narrower than a
 // 128-byte line, whose partial lines are then written separately.)
 template <typename V> __device__ __forceinline__ void store_stream(V* p, V v) { __builtin_nontemporal_store(v, p); }
+// The store of a row or column kernel's output: non-temporal (NT) or
+// ordinary.  Ordinary stores leave the lines in L2, which merges the partial
+// 128-byte lines of tiles narrower than a line before they reach memory; the
+// autotuner picks the kind per engine from a timed step loop.
+template <bool NT, typename V> __device__ __forceinline__ void store_out(V* p, V v) {
+  if constexpr (NT)
+    __builtin_nontemporal_store(v, p);
+  else
+    *p = v;
+}
 
 // LDS line index with one pad slot per 16 elements: Stockham scatters write
 // at strides of R (or R*Ns) elements, which without padding put every lane of
@@ -230,10 +240,12 @@ __device__ __forceinline__ void reg_fft(V2<T>* v, V2<T>* line, int lane, const V
 // M-point complex sequence whose inverse FFT interleaves the row's even and
 // odd real samples, and runs that FFT.  On return v[q] = (n[2k], n[2k+1]),
 // k = lane + TPL q, and wk, wM, wt hold the roots the forward half needs.
+// m0: the conserved mean, taken off the kx = 0 coefficient (the row's
+// average) so that the transform carries the fluctuation n - m0 only (0: none).
 template <typename T, class S>
 __device__ __forceinline__ void row_c2r(const V2<T>* __restrict__ Sg, const Lay& lay, int rowc, int lane, V2<T>* line,
                                         const V2<double>* __restrict__ twN, V2<T> (&v)[S::P], V2<T> (&wk)[S::P],
-                                        V2<T>& wM, V2<T> (&wt)[tw_count<S>()]) {
+                                        V2<T>& wM, V2<T> (&wt)[tw_count<S>()], double m0 = 0.0) {
   using V = V2<T>;
   constexpr int M = S::L, P = S::P, TPL = S::TPL;
   V x[P];
@@ -253,6 +265,7 @@ __device__ __forceinline__ void row_c2r(const V2<T>* __restrict__ Sg, const Lay&
   // keep every load above in flight together: no arithmetic is scheduled
   // ahead of the last one
   __builtin_amdgcn_sched_barrier(0);
+  if (lane == 0) x[0].x = T(double(x[0].x) - m0);
 #pragma unroll
   for (int qq = 0; qq < P; ++qq) line[lpad(lane + TPL * qq)] = x[qq];
   line[lpad(M)] = xM;  // every lane stores the same value: no branch for the load to sink into
@@ -280,7 +293,7 @@ __device__ __forceinline__ void row_c2r(const V2<T>* __restrict__ Sg, const Lay&
 // the values the nonlinearity sees, are also written to `rtile`: the range
 // check of the stabilised split folded into the ETD1 tables.  A compile-time
 // choice: the ordinary kernels keep exactly the code the group kernels share.
-template <typename T, class S, bool RANGE = false>
+template <typename T, class S, bool RANGE = false, bool NT = true>
 __device__ __forceinline__ void row_fast_tile(const V2<T>* __restrict__ Sg, V2<T>* __restrict__ Tout, int Ny,
                                               const Lay& lay, Nonlin q, const V2<double>* __restrict__ twN,
                                               const Slots& sl, int yi, int tile, unsigned char* smem_raw,
@@ -300,9 +313,15 @@ __device__ __forceinline__ void row_fast_tile(const V2<T>* __restrict__ Sg, V2<T
   // Loads are unconditional (rows past Ny read row Ny - 1; results are not
   // stored): a load under a branch makes the join wait for it, which would
   // serialise it with the root loads.
+  // FP32 bulk with a pinned mean m: the transforms carry n - m and the
+  // nonlinearity is taken about m (nonlin_mu_about), so FP32 round-off scales
+  // with the fluctuation, not with the mean (FP64 needs no such care)
+  constexpr bool kAbout = sizeof(T) == 4;
+  const bool about_m = kAbout && sl.pin;
+  const double m0 = about_m ? sl.pin[slot] : 0.0;
   V v[P], wk[P], wM;
   V wt[tw_count<S>()];
-  row_c2r<T, S>(Sg, lay, live ? row : Ny - 1, lane, line, twN, v, wk, wM, wt);
+  row_c2r<T, S>(Sg, lay, live ? row : Ny - 1, lane, line, twN, v, wk, wM, wt, m0);
   // the lane's min and max now; the block's after the stores, where the
   // reduction overlaps their drain
   double rmn = INFINITY, rmx = -INFINITY;
@@ -310,16 +329,21 @@ __device__ __forceinline__ void row_fast_tile(const V2<T>* __restrict__ Sg, V2<T
     if (live) {
 #pragma unroll
       for (int qq = 0; qq < P; ++qq) {
-        rmn = fmin(rmn, fmin((double)v[qq].x, (double)v[qq].y));
-        rmx = fmax(rmx, fmax((double)v[qq].x, (double)v[qq].y));
+        rmn = fmin(rmn, m0 + fmin((double)v[qq].x, (double)v[qq].y));
+        rmx = fmax(rmx, m0 + fmax((double)v[qq].x, (double)v[qq].y));
       }
     }
   }
   bool bad = false;
+  NonlinAbout about{};
+  if constexpr (kAbout) about = nonlin_about(q, m0);
 #pragma unroll
   for (int qq = 0; qq < P; ++qq) {
     bad |= !(isfinite(v[qq].x) && isfinite(v[qq].y));
-    v[qq] = V{nonlin_mu<T>(v[qq].x, q), nonlin_mu<T>(v[qq].y, q)};
+    if (about_m)
+      v[qq] = V{nonlin_mu_about<T>(v[qq].x, q, about), nonlin_mu_about<T>(v[qq].y, q, about)};
+    else
+      v[qq] = V{nonlin_mu<T>(v[qq].x, q), nonlin_mu<T>(v[qq].y, q)};
   }
   if (bad && live && sl.status) atomicOr(sl.status + slot, 1);
   reg_fft<T, S, false, !S::WAVE>(v, line, lane, wt);
@@ -334,9 +358,9 @@ __device__ __forceinline__ void row_fast_tile(const V2<T>* __restrict__ Sg, V2<T
       const int k = lane + TPL * qq;
       const V zk = v[qq];
       const V zm = line[lpad(k == 0 ? 0 : M - k)];
-      store_stream(Tout + lay.at(row, k), row_unpack_pair<T>(zk, zm, wk[qq]));
+      store_out<NT>(Tout + lay.at(row, k), row_unpack_pair<T>(zk, zm, wk[qq]));
     }
-    if (lane == 0) store_stream(Tout + lay.at(row, M), row_unpack_pair<T>(line[0], line[0], wM));
+    if (lane == 0) store_out<NT>(Tout + lay.at(row, M), row_unpack_pair<T>(line[0], line[0], wM));
   }
   if constexpr (RANGE) {
     block_minmax_waves(rmn, rmx);
@@ -345,7 +369,7 @@ __device__ __forceinline__ void row_fast_tile(const V2<T>* __restrict__ Sg, V2<T
 }
 
 // RANGE: rpart holds [instance][tile] range pairs (row_fast_tile)
-template <typename T, class S, bool RANGE>
+template <typename T, class S, bool RANGE, bool NT>
 __global__ void __launch_bounds__(256) k_row_fast(const V2<T>* __restrict__ Sg, V2<T>* __restrict__ Tout, int Ny,
                                                   Lay lay, Nonlin q, const V2<double>* __restrict__ twN,
                                                   Slots sl, RangePart* rpart) {
@@ -355,7 +379,7 @@ __global__ void __launch_bounds__(256) k_row_fast(const V2<T>* __restrict__ Sg, 
   const int ntile = gridDim.x;
   const int lin = xcd_remap(blockIdx.x + blockIdx.y * ntile, ntile * gridDim.y);
   const int yi = lin / ntile, tile = lin - yi * ntile;
-  row_fast_tile<T, S, RANGE>(Sg, Tout, Ny, lay, q, twN, sl, yi, tile, smem_raw,
+  row_fast_tile<T, S, RANGE, NT>(Sg, Tout, Ny, lay, q, twN, sl, yi, tile, smem_raw,
                              RANGE ? rpart + (size_t)yi * ntile + tile : nullptr);
 }
 
@@ -404,7 +428,7 @@ __global__ void __launch_bounds__(256) k_row_range(const V2<T>* __restrict__ Sg,
 // ---------------------------------------------------------------------------
 // Column tile `tile` (blockDim.x / TPL columns) of active instance `yi`;
 // every thread of the block calls it.
-template <typename T, class S, int MODE>
+template <typename T, class S, int MODE, bool NT = true>
 __device__ __forceinline__ void col_fast_tile(V2<T>* __restrict__ Sg, const V2<T>* __restrict__ Tin,
                                               V2<T>* __restrict__ D, int Nc, const Lay& lay, const ColTables& tb,
                                               const V2<double>* __restrict__ twY, const Slots& sl, int yi, int tile,
@@ -449,7 +473,21 @@ __device__ __forceinline__ void col_fast_tile(V2<T>* __restrict__ Sg, const V2<T
   tw_load<T, S>(wt, lane, twY, 1);
   // every load above is issued before the first transform starts
   __builtin_amdgcn_sched_barrier(0);
-  if constexpr (MODE != COL_RK2_STAGE2) reg_fft<T, S, false, true>(a, line, lane, wt);
+  // FP32 bulk with a pinned mean m: the kx = 0 column (the rows' averages)
+  // is transformed as S~ - m, its k = 0 coefficient pinned to 0 and m added
+  // back at the store, so the transforms carry the fluctuation only (the
+  // stage-2 carry D is already about m).  FP64 pins k = 0 to m directly.
+  constexpr bool kAbout = sizeof(T) == 4;
+  const double* pinp = sl.pin ? sl.pin + slot : nullptr;
+  const double pin = pinp ? *pinp : 0.0;
+  const double mcol = (kAbout && pinp && col == 0) ? pin : 0.0;
+  if constexpr (MODE != COL_RK2_STAGE2) {
+    if constexpr (kAbout) {
+#pragma unroll
+      for (int qq = 0; qq < P; ++qq) a[qq].x = T(double(a[qq].x) - mcol);
+    }
+    reg_fft<T, S, false, true>(a, line, lane, wt);
+  }
   // ETD tables (and the stage-2 carry) for this lane's ky, loaded now so they
   // arrive while the second forward transform runs
   const size_t tbase = (size_t)col * NY;         // carry, [Nc][Ny]
@@ -468,8 +506,6 @@ __device__ __forceinline__ void col_fast_tile(V2<T>* __restrict__ Sg, const V2<T
       if constexpr (MODE == COL_RK2_STAGE1) q2r[qq] = Q2[ti];
     }
   }
-  const double* pinp = sl.pin ? sl.pin + slot : nullptr;
-  const double pin = pinp ? *pinp : 0.0;
   __builtin_amdgcn_sched_barrier(0);  // table loads issued before the second transform
   reg_fft<T, S, false, true>(b, line, lane, wt);
 
@@ -487,7 +523,7 @@ __device__ __forceinline__ void col_fast_tile(V2<T>* __restrict__ Sg, const V2<T
         oy = e1r[qq] * double(a[qq].y) + q1r[qq] * by;
         if constexpr (MODE == COL_RK2_STAGE1) D[tbase + ky] = V{T(ox - q2r[qq] * bx), T(oy - q2r[qq] * by)};
       }
-      if (pinp && col == 0 && ky == 0) ox = pin, oy = 0.0;
+      if (pinp && col == 0 && ky == 0) ox = kAbout ? 0.0 : pin, oy = 0.0;  // FP32: the fluctuation's mean, 0
       b[qq] = V{T(ox), T(oy)};
     }
   }
@@ -499,12 +535,13 @@ __device__ __forceinline__ void col_fast_tile(V2<T>* __restrict__ Sg, const V2<T
     for (int qq = 0; qq < P; ++qq) {
       V xv = b[qq];
       if (realcol) xv.y = T(0);
-      store_stream(Sg + lay.at(lane + TPL * qq, col), xv);
+      if constexpr (kAbout) xv.x = T(double(xv.x) + mcol);
+      store_out<NT>(Sg + lay.at(lane + TPL * qq, col), xv);
     }
   }
 }
 
-template <typename T, class S, int MODE>
+template <typename T, class S, int MODE, bool NT>
 __global__ void __launch_bounds__(256) k_col_fast(V2<T>* __restrict__ Sg, const V2<T>* __restrict__ Tin,
                                                   V2<T>* __restrict__ D, int Nc, Lay lay, ColTables tb,
                                                   const V2<double>* __restrict__ twY, Slots sl) {
@@ -514,7 +551,7 @@ __global__ void __launch_bounds__(256) k_col_fast(V2<T>* __restrict__ Sg, const 
   const int ntile = gridDim.x;
   const int lin = xcd_remap(blockIdx.x + blockIdx.y * ntile, ntile * gridDim.y);
   const int yi = lin / ntile;
-  col_fast_tile<T, S, MODE>(Sg, Tin, D, Nc, lay, tb, twY, sl, yi, lin - yi * ntile, smem_raw);
+  col_fast_tile<T, S, MODE, NT>(Sg, Tin, D, Nc, lay, tb, twY, sl, yi, lin - yi * ntile, smem_raw);
 }
 
 // ---------------------------------------------------------------------------
@@ -527,9 +564,11 @@ __global__ void __launch_bounds__(256) k_col_fast(V2<T>* __restrict__ Sg, const 
   X(64, 0, 8, 8, 8)                   \
   X(64, 1, 4, 4, 4, 4)                \
   X(96, 0, 12, 4, 4, 2, 3)            \
+  X(96, 1, 12, 3, 4, 4, 2)            \
   X(128, 0, 8, 8, 8, 2)               \
   X(128, 1, 4, 4, 4, 4, 2)            \
   X(192, 0, 12, 4, 4, 4, 3)           \
+  X(192, 1, 12, 3, 4, 4, 4)           \
   X(256, 0, 8, 8, 8, 4)               \
   X(256, 1, 4, 4, 4, 4, 4)            \
   X(256, 2, 16, 16, 16)               \
@@ -551,22 +590,83 @@ struct SchedInfo {
 
 // Launchers, instantiated in the pfc_inst_*.hip translation units.  Each
 // returns false if (L, variant) has no schedule.
-bool row_fast_launch_f64(int M, int var, dim3 grid, dim3 block, size_t lds, hipStream_t st, const V2<double>* S,
-                         V2<double>* T, int Ny, const Lay& lay, const Nonlin& q, const V2<double>* tw,
-                         const Slots& sl, RangePart* rpart);
-bool row_fast_launch_f32(int M, int var, dim3 grid, dim3 block, size_t lds, hipStream_t st, const V2<float>* S,
-                         V2<float>* T, int Ny, const Lay& lay, const Nonlin& q, const V2<double>* tw,
-                         const Slots& sl, RangePart* rpart);
+// Per (bulk type, store kind) launchers, instantiated in the pfc_inst_*.hip
+// translation units (one per kind, precision and store kind, so the library
+// builds in parallel).  Each returns false if (L, variant) has no schedule.
+template <typename T, bool NT>
+bool row_fast_launch_impl(int M, int var, dim3 grid, dim3 block, size_t lds, hipStream_t st, const V2<T>* S,
+                          V2<T>* Tout, int Ny, const Lay& lay, const Nonlin& q, const V2<double>* tw, const Slots& sl,
+                          RangePart* rpart) {
+#define PFC_X(LEN, VAR, PP, ...)                                                                                 \
+  if (M == LEN && var == VAR) {                                                                                  \
+    if (rpart)                                                                                                   \
+      hipLaunchKernelGGL((k_row_fast<T, Sch<LEN, PP, __VA_ARGS__>, true, NT>), grid, block, lds, st, S, Tout, Ny, \
+                         lay, q, tw, sl, rpart);                                                                 \
+    else                                                                                                         \
+      hipLaunchKernelGGL((k_row_fast<T, Sch<LEN, PP, __VA_ARGS__>, false, NT>), grid, block, lds, st, S, Tout,    \
+                         Ny, lay, q, tw, sl, rpart);                                                             \
+    return true;                                                                                                 \
+  }
+  PFC_SCHEDULES(PFC_X)
+#undef PFC_X
+  return false;
+}
+
+template <typename T, bool NT>
+bool col_fast_launch_impl(int Ny, int var, int mode, dim3 grid, dim3 block, size_t lds, hipStream_t st, V2<T>* S,
+                          const V2<T>* Tin, V2<T>* D, int Nc, const Lay& lay, const ColTables& tb,
+                          const V2<double>* tw, const Slots& sl) {
+#define PFC_X(LEN, VAR, PP, ...)                                                                                 \
+  if (Ny == LEN && var == VAR) {                                                                                 \
+    using Sc = Sch<LEN, PP, __VA_ARGS__>;                                                                        \
+    if (mode == COL_ETD1)                                                                                        \
+      hipLaunchKernelGGL((k_col_fast<T, Sc, COL_ETD1, NT>), grid, block, lds, st, S, Tin, D, Nc, lay, tb, tw, sl); \
+    else if (mode == COL_RK2_STAGE1)                                                                             \
+      hipLaunchKernelGGL((k_col_fast<T, Sc, COL_RK2_STAGE1, NT>), grid, block, lds, st, S, Tin, D, Nc, lay, tb, tw, \
+                         sl);                                                                                    \
+    else                                                                                                         \
+      hipLaunchKernelGGL((k_col_fast<T, Sc, COL_RK2_STAGE2, NT>), grid, block, lds, st, S, Tin, D, Nc, lay, tb, tw, \
+                         sl);                                                                                    \
+    return true;                                                                                                 \
+  }
+  PFC_SCHEDULES(PFC_X)
+#undef PFC_X
+  return false;
+}
+
+#define PFC_LAUNCH_DECL(EXT, T, NT)                                                                              \
+  EXT template bool row_fast_launch_impl<T, NT>(int, int, dim3, dim3, size_t, hipStream_t, const V2<T>*, V2<T>*,  \
+                                                int, const Lay&, const Nonlin&, const V2<double>*, const Slots&,   \
+                                                RangePart*);                                                       \
+  EXT template bool col_fast_launch_impl<T, NT>(int, int, int, dim3, dim3, size_t, hipStream_t, V2<T>*,          \
+                                                const V2<T>*, V2<T>*, int, const Lay&, const ColTables&,           \
+                                                const V2<double>*, const Slots&);
+#ifndef PFC_INST_UNIT
+PFC_LAUNCH_DECL(extern, double, true)
+PFC_LAUNCH_DECL(extern, double, false)
+PFC_LAUNCH_DECL(extern, float, true)
+PFC_LAUNCH_DECL(extern, float, false)
+#endif
+
+template <typename T>
+inline bool row_fast_launch(int M, int var, bool nt, dim3 grid, dim3 block, size_t lds, hipStream_t st,
+                            const V2<T>* S, V2<T>* Tout, int Ny, const Lay& lay, const Nonlin& q,
+                            const V2<double>* tw, const Slots& sl, RangePart* rpart) {
+  return nt ? row_fast_launch_impl<T, true>(M, var, grid, block, lds, st, S, Tout, Ny, lay, q, tw, sl, rpart)
+            : row_fast_launch_impl<T, false>(M, var, grid, block, lds, st, S, Tout, Ny, lay, q, tw, sl, rpart);
+}
+template <typename T>
+inline bool col_fast_launch(int Ny, int var, int mode, bool nt, dim3 grid, dim3 block, size_t lds, hipStream_t st,
+                            V2<T>* S, const V2<T>* Tin, V2<T>* D, int Nc, const Lay& lay, const ColTables& tb,
+                            const V2<double>* tw, const Slots& sl) {
+  return nt ? col_fast_launch_impl<T, true>(Ny, var, mode, grid, block, lds, st, S, Tin, D, Nc, lay, tb, tw, sl)
+            : col_fast_launch_impl<T, false>(Ny, var, mode, grid, block, lds, st, S, Tin, D, Nc, lay, tb, tw, sl);
+}
+
 bool row_range_launch_f64(int M, int var, dim3 grid, dim3 block, size_t lds, hipStream_t st, const V2<double>* S,
                           int Ny, const Lay& lay, const V2<double>* tw, const Slots& sl, RangePart* part);
 bool row_range_launch_f32(int M, int var, dim3 grid, dim3 block, size_t lds, hipStream_t st, const V2<float>* S,
                           int Ny, const Lay& lay, const V2<double>* tw, const Slots& sl, RangePart* part);
-bool col_fast_launch_f64(int Ny, int var, int mode, dim3 grid, dim3 block, size_t lds, hipStream_t st, V2<double>* S,
-                         const V2<double>* T, V2<double>* D, int Nc, const Lay& lay, const ColTables& tb,
-                         const V2<double>* tw, const Slots& sl);
-bool col_fast_launch_f32(int Ny, int var, int mode, dim3 grid, dim3 block, size_t lds, hipStream_t st, V2<float>* S,
-                         const V2<float>* T, V2<float>* D, int Nc, const Lay& lay, const ColTables& tb,
-                         const V2<double>* tw, const Slots& sl);
 
 // ---------------------------------------------------------------------------
 // multi-shape step launches: batches of different grid sizes ("members")

@@ -47,6 +47,36 @@ __device__ __forceinline__ T nonlin_mu(T f, const Nonlin& q) {
   return lg + poly - T(q.M) * f;
 }
 
+// The nonlinearity about the conserved mean m: N(m + d) - N(m), the same
+// function up to a constant, which only the k = 0 mode sees and the update's
+// Gamma lap removes.  With d the fluctuation (small where the mean is large)
+// every term keeps d's relative precision: the polynomial in Horner form
+// about m, the log as log1p(d / m).  Coefficients (FP64, rounded once):
+// std  N(m + d) - N(m) = d (c1 + d (c2 + d)),  c1 = 2 g m + 3 m^2, c2 = g + 3 m;
+// log  the same polynomial + log1p(d / m) - M d, clamped like nonlin_mu.
+struct NonlinAbout {
+  double m, c1, c2, inv_m, clamp_lg;  // clamp_lg = ln(minPhi) - ln(m)
+};
+
+__device__ __forceinline__ NonlinAbout nonlin_about(const Nonlin& q, double m) {
+  NonlinAbout a;
+  a.m = m;
+  a.c1 = 2.0 * q.g * m + 3.0 * m * m;
+  a.c2 = q.g + 3.0 * m;
+  a.inv_m = m != 0.0 ? 1.0 / m : 0.0;
+  a.clamp_lg = (q.model != MODEL_STD && m > 0.0) ? q.logMin - log(m) : 0.0;
+  return a;
+}
+
+template <typename T>
+__device__ __forceinline__ T nonlin_mu_about(T d, const Nonlin& q, const NonlinAbout& a) {
+  const T poly = d * (T(a.c1) + d * (T(a.c2) + d));
+  if (q.model == MODEL_STD) return poly;
+  T lg = log1p(d * T(a.inv_m));
+  if (q.clamp && !(T(a.m) + d > T(q.minPhi))) lg = T(a.clamp_lg);
+  return lg + poly - T(q.M) * d;
+}
+
 // Instances stepped together by one launch.  blockIdx.y indexes the active
 // list (or is the slot itself when the list is null); a slot's arrays sit
 // `sstride` (state) / `tstride` (tables) elements from slot 0's.

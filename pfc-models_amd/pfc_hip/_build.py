@@ -15,6 +15,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(os.path.dirname(HERE), "csrc")
 OUT = os.path.join(HERE, "libpfc_hip.so")
 SOURCES = ["pfc_capi.hip", "pfc_inst_row64.hip", "pfc_inst_row32.hip", "pfc_inst_col64.hip", "pfc_inst_col32.hip",
+           "pfc_inst_row64o.hip", "pfc_inst_row32o.hip", "pfc_inst_col64o.hip", "pfc_inst_col32o.hip",
            "pfc_inst_group64.hip", "pfc_inst_group32.hip"]
 ARCH = os.environ.get("PFC_HIP_ARCH", "gfx950")
 

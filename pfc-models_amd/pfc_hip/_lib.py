@@ -63,6 +63,7 @@ _SIGNATURES = [
     ("pfc_step_profile", ctypes.c_int, [_P, ctypes.c_longlong, _P, ctypes.POINTER(ctypes.c_double),
                                         ctypes.POINTER(ctypes.c_longlong)]),
     ("pfc_plan_schedule", ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_int)]),
+    ("pfc_plan_stores", ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_int)]),
     ("pfc_plan_graph_stats", ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_longlong)]),
     ("pfc_set_auto_shift", ctypes.c_int, [_P, ctypes.c_int, ctypes.c_double, ctypes.c_double, _P]),
     ("pfc_get_shift", ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_double), _P]),

@@ -39,3 +39,13 @@ def std_sweep_coexistence(r, psibar, fe, crystal_margin=1e-6, min_points=4):
     lo = float(min(psibar.min(), -1.0))
     hi = float(max(psibar.max(), 1.0))
     return common_tangent(std_liquid(r, lo, hi), tri)
+
+
+def liquid_crystal_pair_ok(t, min_gap=1e-3):
+    """Whether a liquid-crystal tangent of the stdPFC sweep is a coexistence:
+    at psibar < 0 the uniform liquid is the lower density, so a pair with
+    phi_liquid >= phi_crystal - min_gap (reversed, or a degenerate sliver) is
+    spurious: it comes from crystal free energies that are not converged
+    relaxations (noise-seeded states still nucleating), not from the model's
+    phase diagram."""
+    return bool(t.ok and t.phi_b - t.phi_a >= min_gap)

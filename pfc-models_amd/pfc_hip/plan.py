@@ -221,6 +221,13 @@ class Plan:
         _lib.check(self._lib.pfc_plan_schedule(self._h, info))
         return {"row": tuple(info[0:4]), "col": tuple(info[4:8])}
 
+    def stores(self):
+        """Output store kind of the fast kernels: 'nt' (non-temporal),
+        'ordinary', or None when they do not both run."""
+        v = ctypes.c_int()
+        _lib.check(self._lib.pfc_plan_stores(self._h, ctypes.byref(v)))
+        return {1: "nt", 0: "ordinary"}.get(v.value)
+
     def graph_stats(self):
         """(step-loop graphs captured, graph replays) so far."""
         v = (ctypes.c_longlong * 2)()

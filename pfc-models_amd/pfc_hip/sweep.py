@@ -110,6 +110,60 @@ def run_plain(instances, steps, dt=0.5, precision=64, scheme="etd1", checkpoint=
     return SweepResult(fe, mean, wall, n / wall, bad, prev)
 
 
+def rounding_floor(cases, steps, seed=0):
+    """How far the FP64 solve itself moves under perturbations at the level of
+    rounding: for each (plan keyword arguments, initial field) in `cases`,
+    `steps` FP64 steps from the field, from the field rounded to FP32 (what an
+    FP32 state holds) and from the field with every value moved by one ulp
+    (random signs).  Returns (fe64, rel_fp32_ic, rel_ulp): the FP64 free
+    energies and the two relative free-energy changes.  A relaxation that
+    amplifies round-off (nucleation under way) shows it here: an FP32 result
+    cannot be held to a tighter bound than the FP32-rounded initial field
+    gives the FP64 solve."""
+    from .plan import Plan
+    rng = np.random.default_rng(seed)
+    fe = np.zeros(len(cases))
+    ic = np.zeros(len(cases))
+    ulp = np.zeros(len(cases))
+    for j, (kw, f0) in enumerate(cases):
+        f0 = np.asarray(f0, dtype=np.float64)
+        variants = (f0, f0.astype(np.float32).astype(np.float64),
+                    f0 + np.spacing(f0) * rng.choice([-1.0, 1.0], size=f0.shape))
+        out = []
+        for v in variants:
+            pl = Plan(**{**kw, "precision": 64}).set_field(v).step(steps)
+            out.append(pl.reduce()[0])
+            pl.close()
+        fe[j] = out[0]
+        ic[j] = abs(out[1] - out[0]) / abs(out[0])
+        ulp[j] = abs(out[2] - out[0]) / abs(out[0])
+    return fe, ic, ulp
+
+
+def fp32_spread(cases, steps, trials=3, seed=0):
+    """The FP32 path's own spread: for each (plan keyword arguments, initial
+    field), the relative free-energy error against FP64 of the FP32 solve from
+    the field and from `trials` copies moved by up to one FP32 ulp (random
+    -1/0/+1 per value).  Returns an (n, trials + 1) array.  Where a
+    relaxation amplifies round-off, this range is how precisely an FP32
+    result is defined at all."""
+    from .plan import Plan
+    rng = np.random.default_rng(seed)
+    out = np.zeros((len(cases), trials + 1))
+    for j, (kw, f0) in enumerate(cases):
+        f0 = np.asarray(f0, dtype=np.float64)
+        pl = Plan(**{**kw, "precision": 64}).set_field(f0).step(steps)
+        ref = pl.reduce()[0]
+        pl.close()
+        for t in range(trials + 1):
+            f = f0 if t == 0 else f0 + np.spacing(f0.astype(np.float32)).astype(np.float64) * rng.choice(
+                [-1.0, 0.0, 1.0], size=f0.shape)
+            pl = Plan(**{**kw, "precision": 32}).set_field(f).step(steps)
+            out[j, t] = abs(pl.reduce()[0] - ref) / abs(ref)
+            pl.close()
+    return out
+
+
 class BatchedSweep:
     """Instances grouped by grid size, one Batch per size.
 

@@ -117,3 +117,17 @@ def test_sweep_maxwell_lines_from_relaxed_states(tmp_path):
     ml.write_csv(str(path))
     got = list(csv.DictReader(open(path)))
     assert len(got) == len(ml.rows) and float(got[0]["mu"]) == ml.rows[0]["mu"]
+
+
+def test_reversed_and_degenerate_liquid_crystal_pairs_are_spurious():
+    """The sweep's hygiene rule: the liquid must be the lower density by at
+    least the gap; a reversed or sliver pair is not a coexistence."""
+    from pfc_hip.coexistence import liquid_crystal_pair_ok
+    from pfc_hip.phase_diagram.common_tangent import TangentResult
+    good = TangentResult("liquid", "crystal", "ok", phi_a=-0.51, phi_b=-0.39)
+    reversed_ = TangentResult("liquid", "crystal", "ok", phi_a=-0.4725, phi_b=-0.4769)
+    sliver = TangentResult("liquid", "crystal", "ok", phi_a=-0.5818, phi_b=-0.5812)
+    none = TangentResult("liquid", "crystal", "no_root")
+    assert liquid_crystal_pair_ok(good)
+    assert not liquid_crystal_pair_ok(reversed_) and not liquid_crystal_pair_ok(sliver)
+    assert not liquid_crystal_pair_ok(none)

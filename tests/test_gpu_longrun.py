@@ -108,9 +108,11 @@ def test_logpfc_2048_fp32_stabilised_10000_steps(hip_device):
     phi = sim.State.phi.get()
     assert np.isfinite(phi).all() and phi.min() > 0.0
     fe, mean, _, _ = sim._plan.reduce()
-    # the k = 0 coefficient is pinned every step to the FP32 rounding of the
-    # initial mean: bit-constant, within half an FP32 ulp of the FP64 mean
-    assert mean == mean20 and abs(mean - mean0) <= 0.5 * np.spacing(np.float32(mean0))
+    # the k = 0 coefficient is pinned every step to the FP64 mean; the field
+    # holds it up to the FP32 rounding of the stored rows (re-pinned every
+    # step, so it does not drift): well inside one FP32 ulp of the mean
+    ulp = float(np.spacing(np.float32(mean0)))
+    assert abs(mean - mean0) <= 0.05 * ulp and abs(mean - mean20) <= 0.05 * ulp, (mean, mean20, mean0)
     pl = Plan(model="log", Nx=n, Ny=n, Lx=L, Ly=L, dt=0.01, clamp_log=True).set_field(phi)
     fe64 = pl.free_energy()
     chk = O.Simulation(O.Params(model="log", dt=0.01, clamp_log=True), geom, fft="scipy",

@@ -49,3 +49,49 @@ def test_poisson_stream_latencies(hip_device):
     assert all(r.latency > 0 for r in stats.results)
     assert 0 < stats.p50_s <= stats.p99_s
     assert stats.solves_per_s > 0
+
+
+def test_serving_mix_at_its_sizes_under_a_binding_budget(hip_device, monkeypatch):
+    """The serving workload's shapes: std and log requests, sides 128 to 2048
+    (five 2048^2 requests), log-uniform step counts capped at 2000, FP32
+    bulk, arriving as a Poisson stream, under a 0.5 GB budget that cannot hold
+    every shape's slab at once (one 8-slot 2048^2 slab takes 0.4 GB; slabs
+    are drained and released on the way).
+    * every result is bit-identical to the same request run alone with a Plan
+      (autotuner off), fields included;
+    * one request per (side, model) against the plain FP64 solve: relative
+      free energy <= 1e-6;
+    * the server never reserved more than the budget."""
+    monkeypatch.setenv("PFC_HIP_TUNE", "0")
+    from pfc_hip.plan import Plan
+    from pfc_hip.serve import Server, poisson_stream, request_field, request_params
+    reqs = poisson_stream(n=20, rate=40.0, seed=6, steps=(500, 2000))
+    assert sum(q.N == 2048 for q in reqs) == 5 and {q.model for q in reqs} == {"std", "log"}
+    budget = 0.5e9
+    srv = Server(hbm_budget=budget, tick_steps=200, keep_fields=True)
+    print(f"serve mix: 20 requests, budget {budget:.1e}", flush=True)
+    stats = srv.run(reqs, realtime=True)
+    srv.close()
+    print(f"serve mix: wall {stats.wall_s:.2f} s, p50 {stats.p50_s:.2f} s, p99 {stats.p99_s:.2f} s, "
+          f"slabs {stats.slabs_created} created / {stats.slabs_released} released / {stats.slabs_drained} drained",
+          flush=True)
+    assert stats.rejected == 0 and len(stats.results) == 20
+    assert stats.peak_bytes <= budget
+    assert stats.slabs_released >= 1
+    checked = set()
+    for res, req in zip(stats.results, reqs):
+        assert res.id == req.id and not res.nonfinite
+        pl = Plan(**request_params(req)).set_field(request_field(req)).step(req.steps)
+        fe, mean, _, _ = pl.reduce()
+        assert res.fe == fe and res.mean == mean, req
+        assert np.array_equal(res.field, pl.get_field().cpu().numpy()), req
+        pl.close()
+        if (req.N, req.model) not in checked:
+            checked.add((req.N, req.model))
+            p64 = Plan(**request_params(req, precision=64)).set_field(request_field(req)).step(req.steps)
+            fe64 = p64.reduce()[0]
+            p64.close()
+            rel = abs(res.fe - fe64) / abs(fe64)
+            print(f"serve mix: request {req.id} N={req.N} {req.model} steps={req.steps} FP32 vs FP64 rel FE {rel:.2e}",
+                  flush=True)
+            assert rel <= 1e-6, (req, rel)

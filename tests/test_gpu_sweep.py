@@ -6,27 +6,61 @@ pytestmark = pytest.mark.gpu
 
 
 def test_batched_fp32_sweep_matches_plain_fp64(hip_device):
-    """A 4 x 4 corner of the (r, psibar) sweep over all six grid sizes, 300 ETD1
-    steps.  FP32-bulk batched free energies against the plain FP64 step:
-    median relative error <= 1e-6 and max <= 5e-6 (the instances above 1e-6
-    are mid-crystallisation, where the linear instability amplifies FP32
-    round-off in the noise-seeded pattern); means to 1e-6 absolute.  The FP64
-    batched run agrees with the plain FP64 run to 1e-12."""
+    """Every 8th instance of the 256-instance sweep (all six grid sizes),
+    5000 ETD1 steps.  FP32-bulk batched free energies against the plain FP64
+    step: <= 1e-6 relative for every instance whose FP64 free energy moved by
+    <= 1e-6 relative over its last 500 steps (relaxed), median <= 2e-7 over
+    all; instances still mid-nucleation (the linear instability amplifies
+    round-off there) <= 1e-5.  Means to 1e-7 absolute; FP64 batched equals
+    plain FP64 to 1e-12."""
     from pfc_hip import sweep as SW
-    inst = SW.sweep_instances(n_r=4, n_psi=4, r_range=(-0.5, -0.2), psi_range=(-0.45, -0.15))
-    assert len({i.N for i in inst}) >= 4
-    plain = SW.run_plain(inst, 300)
-    b32 = SW.run_batched(inst, 300, precision=32)
-    b64 = SW.run_batched(inst, 300, precision=64)
+    inst = SW.sweep_instances()[::8]
+    assert len({i.N for i in inst}) == 6
+    plain = SW.run_plain(inst, 5000, checkpoint=500)
+    b32 = SW.run_batched(inst, 5000, precision=32)
+    b64 = SW.run_batched(inst, 5000, precision=64)
     assert (plain.nonfinite == 0).all() and (b32.nonfinite == 0).all()
     rel32 = np.abs(b32.fe - plain.fe) / np.abs(plain.fe)
     rel64 = np.abs(b64.fe - plain.fe) / np.abs(plain.fe)
-    assert np.median(rel32) <= 1e-6, rel32
-    assert rel32.max() <= 5e-6, rel32
+    relaxed = np.abs(plain.fe - plain.fe_prev) <= 1e-6 * np.abs(plain.fe)
+    print(f"sweep FP32: relaxed {relaxed.sum()}/{len(inst)} max {rel32[relaxed].max():.2e}; "
+          f"still relaxing max {rel32[~relaxed].max() if (~relaxed).any() else 0:.2e}; median {np.median(rel32):.2e}",
+          flush=True)
+    assert relaxed.sum() >= 16
+    assert rel32[relaxed].max() <= 1e-6, rel32
+    assert np.median(rel32) <= 2e-7, rel32
+    assert rel32.max() <= 1e-5, rel32
     assert rel64.max() <= 1e-12, rel64
-    assert np.abs(b32.mean - plain.mean).max() <= 1e-6
-    psibar = np.array([i.psibar for i in inst])
-    assert np.abs(plain.mean - psibar).max() <= 0.01
+    assert np.abs(b32.mean - plain.mean).max() <= 1e-7
+
+
+@pytest.mark.parametrize("index", [40, 104])
+def test_fp64_sweep_instance_against_the_oracle_5000_steps(hip_device, index):
+    """Two 128^2 sweep instances still relaxing at 5000 steps (nucleation under
+    way, where round-off is amplified most): the plain FP64 GPU step against
+    the FP64 oracle at every 500 steps, relative free energy <= 1e-9, and the
+    same structure-factor peaks at the end."""
+    import os
+    from oracle import pfc_oracle as O
+    from pfc_hip import sweep as SW
+    from pfc_hip.plan import Plan
+    i = SW.sweep_instances()[index]
+    assert i.N == 128
+    geom = O.build_geometry(i.N, i.N, i.Lx, i.Ly)
+    ref = O.Simulation(O.Params(model="std", epsilon=i.r, g=0.0, dt=0.5, pin_mean=None), geom, fft="scipy",
+                       workers=min(16, os.cpu_count() or 1))
+    f0 = SW.initial_field(i)
+    ref.set_field(f0)
+    pl = Plan(**SW._model_kw(i, 0.5, 64, "etd1")).set_field(f0)
+    worst = 0.0
+    for _ in range(10):
+        ref.step(500)
+        pl.step(500)
+        fe = pl.reduce()[0]
+        worst = max(worst, abs(fe - ref.free_energy()) / abs(ref.free_energy()))
+    print(f"sweep instance {index}: FP64 GPU vs oracle max rel FE {worst:.2e}", flush=True)
+    assert worst <= 1e-9
+    assert O.structure_factor_peaks(pl.get_field().cpu().numpy(), 12) == O.structure_factor_peaks(ref.field, 12)
 
 
 def test_coexistence_densities_fp32_vs_fp64(hip_device):
