@@ -483,8 +483,10 @@ __device__ __forceinline__ void col_fast_tile(V2<T>* __restrict__ Sg, const V2<T
   const double mcol = (kAbout && pinp && col == 0) ? pin : 0.0;
   if constexpr (MODE != COL_RK2_STAGE2) {
     if constexpr (kAbout) {
+      if (mcol != 0.0) {  // the kx = 0 column only
 #pragma unroll
-      for (int qq = 0; qq < P; ++qq) a[qq].x = T(double(a[qq].x) - mcol);
+        for (int qq = 0; qq < P; ++qq) a[qq].x = T(double(a[qq].x) - mcol);
+      }
     }
     reg_fft<T, S, false, true>(a, line, lane, wt);
   }
@@ -535,7 +537,8 @@ __device__ __forceinline__ void col_fast_tile(V2<T>* __restrict__ Sg, const V2<T
     for (int qq = 0; qq < P; ++qq) {
       V xv = b[qq];
       if (realcol) xv.y = T(0);
-      if constexpr (kAbout) xv.x = T(double(xv.x) + mcol);
+      if constexpr (kAbout)
+        if (mcol != 0.0) xv.x = T(double(xv.x) + mcol);
       store_out<NT>(Sg + lay.at(lane + TPL * qq, col), xv);
     }
   }
