@@ -118,6 +118,41 @@ template <bool NT, typename V> __device__ __forceinline__ void store_out(V* p, V
     *p = v;
 }
 
+// Wavefront shuffles for the row kernel's pack and unpack.  Both pair the
+// element k = lane + TPL q of a line with M - k, which for a line inside one
+// wavefront lives in lane (TPL - lane) mod TPL of the same line, register
+// P - 1 - q (lane 0: register P - q, its own): a fixed lane permutation, done
+// with ds_bpermute pulls (no LDS store, no barrier) instead of a round trip
+// through the line's LDS.  Each pull moves one dword per lane.
+constexpr bool kRowShuffle = true;
+
+template <typename T> __device__ __forceinline__ V2<T> pull_lane(V2<T> x, int addr) {
+  constexpr int n = sizeof(V2<T>) / 4;
+  union {
+    V2<T> v;
+    int w[n];
+  } a, b;
+  a.v = x;
+#pragma unroll
+  for (int i = 0; i < n; ++i) b.w[i] = __builtin_amdgcn_ds_bpermute(addr, a.w[i]);
+  return b.v;
+}
+
+// m[q] = element M - (lane + TPL q) of the line (q >= 1 or lane > 0); for
+// lane 0, q = 0 the caller substitutes element M (or 0) itself.
+template <typename T, class S>
+__device__ __forceinline__ void mirror_pull(const V2<T> (&v)[S::P], V2<T> (&m)[S::P], int lane) {
+  constexpr int P = S::P, TPL = S::TPL;
+  const int wl = __lane_id();
+  const int src = wl - lane + ((TPL - lane) & (TPL - 1));
+  const int addr = src << 2;
+#pragma unroll
+  for (int qq = 0; qq < P; ++qq) {
+    const V2<T> far = pull_lane<T>(v[P - 1 - qq], addr);
+    m[qq] = (lane == 0) ? v[(P - qq) % P] : far;
+  }
+}
+
 // LDS line index with one pad slot per 16 elements: Stockham scatters write
 // at strides of R (or R*Ns) elements, which without padding put every lane of
 // a wavefront on the same bank.
@@ -266,15 +301,23 @@ __device__ __forceinline__ void row_c2r(const V2<T>* __restrict__ Sg, const Lay&
   // ahead of the last one
   __builtin_amdgcn_sched_barrier(0);
   if (lane == 0) x[0].x = T(double(x[0].x) - m0);
+  V xmr[P];
+  if constexpr (kRowShuffle && S::WAVE) {
+    mirror_pull<T, S>(x, xmr, lane);
+    if (lane == 0) xmr[0] = xM;
+  } else {
 #pragma unroll
-  for (int qq = 0; qq < P; ++qq) line[lpad(lane + TPL * qq)] = x[qq];
-  line[lpad(M)] = xM;  // every lane stores the same value: no branch for the load to sink into
-  line_sync<!S::WAVE>();
+    for (int qq = 0; qq < P; ++qq) line[lpad(lane + TPL * qq)] = x[qq];
+    line[lpad(M)] = xM;  // every lane stores the same value: no branch for the load to sink into
+    line_sync<!S::WAVE>();
+#pragma unroll
+    for (int qq = 0; qq < P; ++qq) xmr[qq] = line[lpad(M - (lane + TPL * qq))];
+  }
 #pragma unroll
   for (int qq = 0; qq < P; ++qq) {
     const int k = lane + TPL * qq;
     V xk = x[qq];
-    V xm = line[lpad(M - k)];
+    V xm = xmr[qq];
     if (k == 0) {
       xk.y = T(0);
       xm.y = T(0);
@@ -347,20 +390,30 @@ __device__ __forceinline__ void row_fast_tile(const V2<T>* __restrict__ Sg, V2<T
   }
   if (bad && live && sl.status) atomicOr(sl.status + slot, 1);
   reg_fft<T, S, false, !S::WAVE>(v, line, lane, wt);
-  line_sync<!S::WAVE>();
+  // unpack Y[k] from Z[k] (in registers) and Z[M-k] (pulled from its lane,
+  // or through LDS when the line spans wavefronts)
+  V zmr[P];
+  if constexpr (kRowShuffle && S::WAVE) {
+    mirror_pull<T, S>(v, zmr, lane);
+  } else {
+    line_sync<!S::WAVE>();
 #pragma unroll
-  for (int qq = 0; qq < P; ++qq) line[lpad(lane + TPL * qq)] = v[qq];
-  line_sync<!S::WAVE>();
+    for (int qq = 0; qq < P; ++qq) line[lpad(lane + TPL * qq)] = v[qq];
+    line_sync<!S::WAVE>();
+#pragma unroll
+    for (int qq = 0; qq < P; ++qq) {
+      const int k = lane + TPL * qq;
+      zmr[qq] = line[lpad(k == 0 ? 0 : M - k)];
+    }
+  }
   if (live) {
 #pragma unroll
     for (int qq = 0; qq < P; ++qq) {
-      // unpack Y[k] from Z[k] (in registers) and Z[M-k] (from LDS)
       const int k = lane + TPL * qq;
-      const V zk = v[qq];
-      const V zm = line[lpad(k == 0 ? 0 : M - k)];
-      store_out<NT>(Tout + lay.at(row, k), row_unpack_pair<T>(zk, zm, wk[qq]));
+      store_out<NT>(Tout + lay.at(row, k), row_unpack_pair<T>(v[qq], zmr[qq], wk[qq]));
     }
-    if (lane == 0) store_out<NT>(Tout + lay.at(row, M), row_unpack_pair<T>(line[0], line[0], wM));
+    // Z[0] is lane 0's register 0
+    if (lane == 0) store_out<NT>(Tout + lay.at(row, M), row_unpack_pair<T>(v[0], v[0], wM));
   }
   if constexpr (RANGE) {
     block_minmax_waves(rmn, rmx);
