@@ -411,6 +411,9 @@ struct Engine {
   // the row kernel writes the range partials itself)
   bool fold_shift() const { return auto_shift && p.scheme == PFC_SCHEME_ETD1 && !resident; }
   bool fused_range() const { return fold_shift() && frow.on && (frow.lines * frow.tpl) % 64 == 0; }
+  // the fused step's fast column kernel rebuilds a refitted slot's tables
+  // itself (no table launch between the decision and the column pass)
+  bool col_refit() const { return fused_range() && fcol.on; }
   Slots slots_active() const {
     return Slots{(const int*)active.p, (long long)nc, (long long)ntab, p.pin_mean ? (const double*)pin.p : nullptr,
                  (int*)status.p, (const double*)Ms.p};
@@ -609,9 +612,10 @@ void shift_check(Engine& e, hipStream_t st, bool row_partials = false) {
   const TabParams tp = tab_params(e);
   const SlotGrid sg{(const double*)e.kx2.p, (const double*)e.ky2.p, (const double*)e.dts.p};
   const int blocks = (int)std::min<size_t>((e.ntab + 255) / 256, 1024);
-  hipLaunchKernelGGL(k_tables<double>, dim3(blocks, ny), dim3(256), 0, st, (double*)e.E1.p, (double*)e.Q1.p,
-                     (double*)e.Q2.p, (long long)e.ntab, sg, tp, (const double*)e.eps.p, (const double*)e.Ms.p, 0,
-                     (const int*)e.active.p, (const int*)e.rebuildf.p);
+  if (!(row_partials && e.col_refit()))
+    hipLaunchKernelGGL(k_tables<double>, dim3(blocks, ny), dim3(256), 0, st, (double*)e.E1.p, (double*)e.Q1.p,
+                       (double*)e.Q2.p, (long long)e.ntab, sg, tp, (const double*)e.eps.p, (const double*)e.Ms.p, 0,
+                       (const int*)e.active.p, (const int*)e.rebuildf.p);
   if (e.has_ext()) {
     const long long cs = (long long)e.Nc * e.p.Ny;
     const int cb = (int)std::min<long long>((cs + 255) / 256, 1024);
@@ -629,8 +633,8 @@ struct Steps {
   static void row(Engine& e, hipStream_t st, const Slots& sl, int ny, RangePart* rpart = nullptr) {
     row(e, st, sl, ny, (const V*)e.S.p, (V*)e.Tb.p, rpart);
   }
-  static void col(Engine& e, int mode, hipStream_t st, const Slots& sl, int ny) {
-    col(e, mode, st, sl, ny, (V*)e.S.p, (const V*)e.Tb.p, (V*)e.D.p);
+  static void col(Engine& e, int mode, hipStream_t st, const Slots& sl, int ny, const int* refit = nullptr) {
+    col(e, mode, st, sl, ny, (V*)e.S.p, (const V*)e.Tb.p, (V*)e.D.p, refit);
   }
   static void row(Engine& e, hipStream_t st, const Slots& sl, int ny, const V* S, V* Tb,
                   RangePart* rpart = nullptr) {
@@ -644,8 +648,19 @@ struct Steps {
       hipLaunchKernelGGL((k_row_nonlin<T>), dim3(row_blocks(e.rg), ny), dim3(256), row_lds(e.rg, sizeof(V)), st, S,
                          Tb, e.rg, q, (const V*)e.twN.p, sr);
   }
-  static void col(Engine& e, int mode, hipStream_t st, const Slots& sl, int ny, V* S, const V* Tb, V* D) {
+  // refit (ETD1, fast column shape): the slots' refit flags; the kernel
+  // rebuilds a flagged slot's tables from the new shift before using them
+  static void col(Engine& e, int mode, hipStream_t st, const Slots& sl, int ny, V* S, const V* Tb, V* D,
+                  const int* refit = nullptr) {
     ColTables tb = e.tables();
+    if (refit) {
+      if (mode != COL_ETD1 || !e.fcol.on) fail(PFC_EUNSUPPORTED, "table rebuild in a column pass without it");
+      tb.refit = refit;
+      tb.eps = (const double*)e.eps.p;
+      tb.Ms = (const double*)e.Ms.p;
+      tb.sg = SlotGrid{(const double*)e.kx2.p, (const double*)e.ky2.p, (const double*)e.dts.p};
+      tb.tp = tab_params(e);
+    }
     if (e.fcol.on) {
       FastLaunch<T>::col(e.p.Ny, mode, e.fcol, ny, st, S, Tb, D, e.Nc, e.lay, tb, (const V2<double>*)e.twY64.p, sl);
       return;
@@ -710,10 +725,11 @@ struct Steps {
     for (long long s = 0; s < nsteps; ++s) {
       if (fused) {
         // row pass (N without the shift, and the field's range), decision and
-        // rebuild, then the column pass with the refitted E1
+        // rebuild (inside the column pass when its shape is a fast one), then
+        // the column pass with the refitted E1
         row(e, st, sl, ny, (RangePart*)e.rpart.p);
         shift_check<T>(e, st, true);
-        col(e, COL_ETD1, st, sl, ny);
+        col(e, COL_ETD1, st, sl, ny, e.col_refit() ? (const int*)e.rebuildf.p : nullptr);
         continue;
       }
       if (e.auto_shift) shift_check<T>(e, st);

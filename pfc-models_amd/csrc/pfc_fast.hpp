@@ -509,6 +509,33 @@ __device__ __forceinline__ void col_fast_tile(V2<T>* __restrict__ Sg, const V2<T
   // bounds, results discarded) so loads and barriers stay uniform
   const int col = c0 + (live ? wl : ncols - 1);
   V* line = lds + wl * LD;
+  // ETD1 under the device-managed shift: a slot refitted this step first gets
+  // its columns' table entries recomputed here (as k_tables computes them)
+  // and written in place, before anything else is live, in place of a
+  // separate table launch before this one; the load path below then reads
+  // them back (each lane reads entries it wrote, or the identical value
+  // written for the mirrored row)
+  if (MODE == COL_ETD1 && tb.refit && tb.refit[slot]) {
+    TabParams p = tb.tp;
+    p.epsilon = tb.eps[slot];
+    p.M = tb.Ms[slot];
+    p.dt = tb.sg.dt[slot];
+    const double kx2 = tb.sg.kx2[(size_t)slot * p.Nc + col];
+    const double* __restrict__ ky2 = tb.sg.ky2 + (size_t)slot * p.Ny;
+    const size_t hb = (size_t)col * tb.hy + (size_t)slot * sl.tstride;
+    double* E1w = const_cast<double*>(reinterpret_cast<const double*>(tb.E1)) + hb;
+    double* Q1w = const_cast<double*>(reinterpret_cast<const double*>(tb.Q1)) + hb;
+    double* Q2w = tb.Q2 ? const_cast<double*>(reinterpret_cast<const double*>(tb.Q2)) + hb : nullptr;
+    if (live)
+      for (int qq = 0; qq < P; ++qq) {
+        const int j = ky_half(lane + TPL * qq, NY);
+        const TabEntry t = tab_entry(-(kx2 + ky2[j]), p, Q2w != nullptr);
+        E1w[j] = t.e1;
+        Q1w[j] = t.q1;
+        if (Q2w) Q2w[j] = t.q2;
+      }
+    asm volatile("" ::: "memory");  // the table loads below read memory, not these values
+  }
   // ETD tables are FP64 in every precision; the combine runs in FP64 and is
   // rounded once to the bulk type
   const double* E1 = reinterpret_cast<const double*>(tb.E1) + (size_t)slot * sl.tstride;

@@ -261,6 +261,29 @@ enum { COL_ETD1 = 0, COL_RK2_STAGE1 = 1, COL_RK2_STAGE2 = 2 };
 __host__ __device__ __forceinline__ int ky_half(int ky, int Ny) { return ky <= Ny - ky ? ky : Ny - ky; }
 __host__ __device__ __forceinline__ int table_rows(int Ny) { return Ny / 2 + 1; }
 
+// E = exp(L dt), P = (E - 1)/L where |L dt| > smallZTol else dt (1 + z/2 + z^2/6)
+// (the ETD1 coefficients of kernels.py:55-71), L = Gamma lap (beta (q0^2 + lap)^2
+// + eps + M), lap = -(kx^2 + ky^2); P2 = dt (e^z - 1 - z)/z^2 (ETDRK2).
+// Stored column-major [Nc][Ny], pre-scaled for the row-spectrum state:
+// E1 = E/Ny, Q1 = P Gamma lap/(Nx Ny), Q2 = P2 Gamma lap/(Nx Ny); with `fold`
+// (ETD1 under the device-managed shift) E1 = (E - M P Gamma lap)/Ny, which
+// takes the split's -M phi term out of the row pass.
+struct TabParams {
+  double Gamma, beta, q0, epsilon, M, dt, smallZTol;
+  double inv_ny, inv_n;
+  int Nc, Ny;
+  int hy;    // stored rows per column: Ny/2 + 1 (see ky_half)
+  int fold;  // ETD1 with the shift folded into E1 (the row pass computes N without -M phi)
+};
+
+// Per-slot box and time step: slot s reads kx^2 at kx2 + s*Nc, ky^2 at
+// ky2 + s*Ny and its dt at dt[s].
+struct SlotGrid {
+  const double* kx2;
+  const double* ky2;
+  const double* dt;
+};
+
 struct ColTables {
   const void* E1;  // column-major [Nc][hy] per slot, FP64 in every precision
   const void* Q1;
@@ -270,6 +293,15 @@ struct ColTables {
   // [Ny][Nc] per slot (cstride apart), or null: no potential
   const V2<double>* C;
   long long cstride;
+  // ETD1 under the device-managed shift: the slots whose `refit` flag is set
+  // have a new M this step, and the fast column kernel recomputes its own
+  // columns' E1, Q1 (and Q2) from eps, Ms, sg and tp, uses them and writes
+  // them back in place; null: the tables are read as stored
+  const int* refit;
+  const double* eps;
+  const double* Ms;
+  SlotGrid sg;
+  TabParams tp;
 };
 
 constexpr int kColRegs = 16;  // A values held per thread between the two forward FFTs
@@ -886,29 +918,6 @@ __global__ void __launch_bounds__(256) k_col_filter(const V2<T>* __restrict__ S,
 // ---------------------------------------------------------------------------
 // ETD tables, built on the device in FP64 and stored in the bulk precision
 // ---------------------------------------------------------------------------
-// E = exp(L dt), P = (E - 1)/L where |L dt| > smallZTol else dt (1 + z/2 + z^2/6)
-// (the ETD1 coefficients of kernels.py:55-71), L = Gamma lap (beta (q0^2 + lap)^2
-// + eps + M), lap = -(kx^2 + ky^2); P2 = dt (e^z - 1 - z)/z^2 (ETDRK2).
-// Stored column-major [Nc][Ny], pre-scaled for the row-spectrum state:
-// E1 = E/Ny, Q1 = P Gamma lap/(Nx Ny), Q2 = P2 Gamma lap/(Nx Ny); with `fold`
-// (ETD1 under the device-managed shift) E1 = (E - M P Gamma lap)/Ny, which
-// takes the split's -M phi term out of the row pass.
-struct TabParams {
-  double Gamma, beta, q0, epsilon, M, dt, smallZTol;
-  double inv_ny, inv_n;
-  int Nc, Ny;
-  int hy;    // stored rows per column: Ny/2 + 1 (see ky_half)
-  int fold;  // ETD1 with the shift folded into E1 (the row pass computes N without -M phi)
-};
-
-// Per-slot box and time step: slot s reads kx^2 at kx2 + s*Nc, ky^2 at
-// ky2 + s*Ny and its dt at dt[s].
-struct SlotGrid {
-  const double* kx2;
-  const double* ky2;
-  const double* dt;
-};
-
 __device__ __forceinline__ double etd2_phi_dt(double z, double dt) {
   if (fabs(z) < 0.1) {
     double s = 0.0, t = 0.5;
@@ -939,6 +948,23 @@ __device__ __forceinline__ EtdCoef etd_coef(double lap, const TabParams& p) {
   return c;
 }
 
+// One stored table entry at lap = -(kx^2 + ky^2): E1, Q1 and (q2) Q2 as
+// described above; k_tables and the fast ETD1 column kernel's in-place
+// rebuild both compute their entries here.
+struct TabEntry {
+  double e1, q1, q2;
+};
+__device__ __forceinline__ TabEntry tab_entry(double lap, const TabParams& p, bool q2) {
+  const EtdCoef k = etd_coef(lap, p);
+  // folded shift: E phi + P Gamma lap (N - M phi) = (E - M P Gamma lap) phi + P Gamma lap N
+  const double e = p.fold ? k.E - p.M * k.P * (p.Gamma * lap) : k.E;
+  TabEntry t;
+  t.e1 = e * p.inv_ny;
+  t.q1 = k.P * (p.Gamma * lap) * p.inv_n;
+  t.q2 = q2 ? etd2_phi_dt(k.z, p.dt) * (p.Gamma * lap) * p.inv_n : 0.0;
+  return t;
+}
+
 // blockIdx.y = slot - slot0; per-slot epsilon, M, box and dt come from
 // eps[], Ms[] and the SlotGrid.
 template <typename T>
@@ -963,13 +989,10 @@ __global__ void __launch_bounds__(256) k_tables(T* __restrict__ E1, T* __restric
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
     const int c = (int)(i / p.hy);
     const int j = (int)(i - (size_t)c * p.hy);
-    const double lap = -(kx2[c] + ky2[j]);
-    const EtdCoef k = etd_coef(lap, p);
-    // folded shift: E phi + P Gamma lap (N - M phi) = (E - M P Gamma lap) phi + P Gamma lap N
-    const double e = p.fold ? k.E - p.M * k.P * (p.Gamma * lap) : k.E;
-    E1[i] = T(e * p.inv_ny);
-    Q1[i] = T(k.P * (p.Gamma * lap) * p.inv_n);
-    if (Q2) Q2[i] = T(etd2_phi_dt(k.z, p.dt) * (p.Gamma * lap) * p.inv_n);
+    const TabEntry t = tab_entry(-(kx2[c] + ky2[j]), p, Q2 != nullptr);
+    E1[i] = T(t.e1);
+    Q1[i] = T(t.q1);
+    if (Q2) Q2[i] = T(t.q2);
   }
 }
 
@@ -1280,7 +1303,8 @@ __global__ void __launch_bounds__(256) k_shift_update(const PartT* __restrict__ 
                                                       double* __restrict__ Ms, int* __restrict__ rebuild) {
   part += (size_t)blockIdx.x * pstride;
   double mn = INFINITY, mx = -INFINITY;
-  for (int b = threadIdx.x; b < nrow; b += blockDim.x) {
+#pragma unroll 8
+  for (int b = threadIdx.x; b < nrow; b += blockDim.x) {  // up to 8 loads per lane in flight at once
     mn = fmin(mn, part[b].mn);
     mx = fmax(mx, part[b].mx);
   }
