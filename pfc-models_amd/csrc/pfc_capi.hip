@@ -298,9 +298,10 @@ inline bool row_range_launch(int M, int var, dim3 g, dim3 b, size_t lds, hipStre
 
 template <typename T> struct FastLaunch {
   static void row(int M, const FastShape& f, int ny_inst, hipStream_t st, const V2<T>* S, V2<T>* Tout, int Ny,
-                  const Lay& lay, const Nonlin& q, const V2<double>* tw, const Slots& sl, RangePart* rpart = nullptr) {
+                  const Lay& lay, const Nonlin& q, const V2<double>* tw, const Slots& sl, RangePart* rpart = nullptr,
+                  const RangeCheck& chk = RangeCheck{}) {
     if (!row_fast_launch<T>(M, f.var, f.nt, dim3(f.blocks, ny_inst), dim3(f.lines * f.tpl), f.lds, st, S, Tout, Ny,
-                            lay, q, tw, sl, rpart))
+                            lay, q, tw, sl, rpart, chk))
       fail(PFC_EUNSUPPORTED, "no fast row schedule");
   }
   static void col(int Ny, int mode, const FastShape& f, int ny_inst, hipStream_t st, V2<T>* S, const V2<T>* Tin,
@@ -375,6 +376,10 @@ struct Engine {
   bool auto_shift = false;
   ShiftParams shp{};
   DevBuf shst, rebuildf;
+  // in-step decision (col_refit): [cap][2] breach flags and the parity of the
+  // next step's word
+  DevBuf brk;
+  int par = 0;
   DevBuf rpart;  // k_row_range: per-tile (min, max)
   DevBuf vhat, extc;  // external potential: rfft2(V) and its source term, [cap][Ny][Nc] complex FP64
   std::vector<char> h_ext;  // slots with a potential
@@ -385,7 +390,7 @@ struct Engine {
   int* host_status = nullptr;   // pinned [cap]
   // step-loop graphs: kGraphSteps steps of the row/column launches, captured
   // per launch configuration (see engine_step)
-  std::map<std::array<long long, 10>, hipGraphExec_t> graphs;
+  std::map<std::array<long long, 11>, hipGraphExec_t> graphs;
   hipStream_t cap_st = nullptr;  // private stream the graphs are captured on
   long long graph_replays = 0, graph_captures = 0;
   void drop_graphs() {
@@ -411,9 +416,12 @@ struct Engine {
   // the row kernel writes the range partials itself)
   bool fold_shift() const { return auto_shift && p.scheme == PFC_SCHEME_ETD1 && !resident; }
   bool fused_range() const { return fold_shift() && frow.on && (frow.lines * frow.tpl) % 64 == 0; }
-  // the fused step's fast column kernel rebuilds a refitted slot's tables
-  // itself (no table launch between the decision and the column pass)
-  bool col_refit() const { return fused_range() && fcol.on; }
+  // the fused step decides inside its two launches: the row kernel flags a
+  // slot whose tile left the fitted window, the fast column kernel refits it
+  // and rebuilds its own columns' tables (no decision or table launch)
+  bool col_refit() const {
+    return fused_range() && fcol.on && !has_ext() && (fcol.lines * fcol.tpl) % 64 == 0;
+  }
   Slots slots_active() const {
     return Slots{(const int*)active.p, (long long)nc, (long long)ntab, p.pin_mean ? (const double*)pin.p : nullptr,
                  (int*)status.p, (const double*)Ms.p};
@@ -612,10 +620,9 @@ void shift_check(Engine& e, hipStream_t st, bool row_partials = false) {
   const TabParams tp = tab_params(e);
   const SlotGrid sg{(const double*)e.kx2.p, (const double*)e.ky2.p, (const double*)e.dts.p};
   const int blocks = (int)std::min<size_t>((e.ntab + 255) / 256, 1024);
-  if (!(row_partials && e.col_refit()))
-    hipLaunchKernelGGL(k_tables<double>, dim3(blocks, ny), dim3(256), 0, st, (double*)e.E1.p, (double*)e.Q1.p,
-                       (double*)e.Q2.p, (long long)e.ntab, sg, tp, (const double*)e.eps.p, (const double*)e.Ms.p, 0,
-                       (const int*)e.active.p, (const int*)e.rebuildf.p);
+  hipLaunchKernelGGL(k_tables<double>, dim3(blocks, ny), dim3(256), 0, st, (double*)e.E1.p, (double*)e.Q1.p,
+                     (double*)e.Q2.p, (long long)e.ntab, sg, tp, (const double*)e.eps.p, (const double*)e.Ms.p, 0,
+                     (const int*)e.active.p, (const int*)e.rebuildf.p);
   if (e.has_ext()) {
     const long long cs = (long long)e.Nc * e.p.Ny;
     const int cb = (int)std::min<long long>((cs + 255) / 256, 1024);
@@ -630,34 +637,42 @@ void shift_check(Engine& e, hipStream_t st, bool row_partials = false) {
 template <typename T>
 struct Steps {
   using V = V2<T>;
-  static void row(Engine& e, hipStream_t st, const Slots& sl, int ny, RangePart* rpart = nullptr) {
-    row(e, st, sl, ny, (const V*)e.S.p, (V*)e.Tb.p, rpart);
+  static void row(Engine& e, hipStream_t st, const Slots& sl, int ny, RangePart* rpart = nullptr,
+                  const RangeCheck& chk = RangeCheck{}) {
+    row(e, st, sl, ny, (const V*)e.S.p, (V*)e.Tb.p, rpart, chk);
   }
-  static void col(Engine& e, int mode, hipStream_t st, const Slots& sl, int ny, const int* refit = nullptr) {
+  static void col(Engine& e, int mode, hipStream_t st, const Slots& sl, int ny, bool refit = false) {
     col(e, mode, st, sl, ny, (V*)e.S.p, (const V*)e.Tb.p, (V*)e.D.p, refit);
   }
+  // chk: the in-step decision's tile test (fused range shape only)
   static void row(Engine& e, hipStream_t st, const Slots& sl, int ny, const V* S, V* Tb,
-                  RangePart* rpart = nullptr) {
+                  RangePart* rpart = nullptr, const RangeCheck& chk = RangeCheck{}) {
     const Nonlin q = nonlin_of(e);
     Slots sr = sl;
     if (e.fold_shift()) sr.M = nullptr;  // the shift acts through E1
     if (e.frow.on)
       FastLaunch<T>::row(e.p.Nx / 2, e.frow, ny, st, S, Tb, e.p.Ny, e.lay, q, (const V2<double>*)e.twN64.p, sr,
-                         rpart);
+                         rpart, chk);
     else
       hipLaunchKernelGGL((k_row_nonlin<T>), dim3(row_blocks(e.rg), ny), dim3(256), row_lds(e.rg, sizeof(V)), st, S,
                          Tb, e.rg, q, (const V*)e.twN.p, sr);
   }
-  // refit (ETD1, fast column shape): the slots' refit flags; the kernel
-  // rebuilds a flagged slot's tables from the new shift before using them
+  // refit (col_refit engines, ETD1): the column kernel reads the step's
+  // breach flags and refits a flagged slot itself (shift, tables, fit)
   static void col(Engine& e, int mode, hipStream_t st, const Slots& sl, int ny, V* S, const V* Tb, V* D,
-                  const int* refit = nullptr) {
+                  bool refit = false) {
     ColTables tb = e.tables();
     if (refit) {
-      if (mode != COL_ETD1 || !e.fcol.on) fail(PFC_EUNSUPPORTED, "table rebuild in a column pass without it");
-      tb.refit = refit;
+      if (mode != COL_ETD1 || !e.col_refit()) fail(PFC_EUNSUPPORTED, "in-step refit on a path without it");
+      tb.brk = (const int*)e.brk.p;
+      tb.par = e.par;
+      tb.part = (const RangePart*)e.rpart.p;
+      tb.nrow = tb.pstride = e.frow.blocks;
+      tb.st = (ShiftState*)e.shst.p;
+      tb.Ms = (double*)e.Ms.p;
+      tb.g = e.shp.g;
+      tb.minPhi = e.shp.minPhi;
       tb.eps = (const double*)e.eps.p;
-      tb.Ms = (const double*)e.Ms.p;
       tb.sg = SlotGrid{(const double*)e.kx2.p, (const double*)e.ky2.p, (const double*)e.dts.p};
       tb.tp = tab_params(e);
     }
@@ -725,11 +740,19 @@ struct Steps {
     for (long long s = 0; s < nsteps; ++s) {
       if (fused) {
         // row pass (N without the shift, and the field's range), decision and
-        // rebuild (inside the column pass when its shape is a fast one), then
-        // the column pass with the refitted E1
-        row(e, st, sl, ny, (RangePart*)e.rpart.p);
-        shift_check<T>(e, st, true);
-        col(e, COL_ETD1, st, sl, ny, e.col_refit() ? (const int*)e.rebuildf.p : nullptr);
+        // rebuild, then the column pass with the refitted E1; with col_refit
+        // the decision and rebuild happen inside the two launches (breach
+        // flags of this step's parity)
+        if (e.col_refit()) {
+          const RangeCheck chk{(const ShiftState*)e.shst.p, (int*)e.brk.p, e.par, e.shp.floor_tol, e.shp.ceil_tol};
+          row(e, st, sl, ny, (RangePart*)e.rpart.p, chk);
+          col(e, COL_ETD1, st, sl, ny, true);
+          e.par ^= 1;
+        } else {
+          row(e, st, sl, ny, (RangePart*)e.rpart.p);
+          shift_check<T>(e, st, true);
+          col(e, COL_ETD1, st, sl, ny);
+        }
         continue;
       }
       if (e.auto_shift) shift_check<T>(e, st);
@@ -940,9 +963,12 @@ bool graphs_enabled() {
 // the cached graph of the engine's launch configuration; without one,
 // captured when `capture`, else null
 hipGraphExec_t step_graph(Engine& e, bool capture) {
-  const std::array<long long, 10> key{e.nact(),          e.frow.on, e.frow.var, e.frow.lines,
+  // par: the in-step decision's parity at the graph's first step (an even
+  // number of steps returns it there)
+  const std::array<long long, 11> key{e.nact(),          e.frow.on, e.frow.var, e.frow.lines,
                                       e.fcol.on,         e.fcol.var, e.fcol.lines,
-                                      (long long)(intptr_t)e.extc.p, (long long)(intptr_t)e.D.p, e.auto_shift};
+                                      (long long)(intptr_t)e.extc.p, (long long)(intptr_t)e.D.p, e.auto_shift,
+                                      e.par};
   auto it = e.graphs.find(key);
   if (it != e.graphs.end()) return it->second;
   if (!capture) return nullptr;
@@ -1107,6 +1133,7 @@ void reduce_slot(Engine& e, int slot, hipStream_t st, bool set_pin) {
 void reset_shift_fit(Engine& e, int slot, hipStream_t st) {
   if (!e.auto_shift) return;
   hip_check(hipMemsetAsync((ShiftState*)e.shst.p + slot, 0, sizeof(ShiftState), st), "memset");
+  hip_check(hipMemsetAsync((int*)e.brk.p + 2 * slot, 0, 2 * sizeof(int), st), "memset");
 }
 
 void engine_set_field(Engine& e, int slot, const void* field, int dtype, hipStream_t st) {
@@ -1266,11 +1293,13 @@ void engine_set_auto_shift(Engine& e, int on, double floor_tol, double ceil_tol,
   if (on && !e.shst.p) {
     e.shst.alloc((size_t)e.cap * sizeof(ShiftState));
     e.rebuildf.alloc((size_t)e.cap * sizeof(int));
+    e.brk.alloc((size_t)e.cap * 2 * sizeof(int));
     e.rpart.alloc((size_t)e.cap * e.p.Ny * sizeof(RangePart));  // at most Ny row tiles per instance
   }
   if (on) {
     hip_check(hipMemset(e.shst.p, 0, e.shst.n), "memset");
     hip_check(hipMemset(e.rebuildf.p, 0, e.rebuildf.n), "memset");
+    hip_check(hipMemset(e.brk.p, 0, e.brk.n), "memset");
   } else if (e.auto_shift) {
     // the host takes M back: its copy follows the device's
     hip_check(hipMemcpy(e.h_M.data(), e.Ms.p, (size_t)e.cap * sizeof(double), hipMemcpyDeviceToHost), "copy");
@@ -1404,10 +1433,11 @@ void engine_set_active(Engine& e, const int* slots, int n, hipStream_t st) {
     if (seen[s]++) fail(PFC_EINVAL, "duplicate slot in the active list");
   }
   e.h_active = list;
-  if (n) {
-    hip_check(hipMemcpyAsync(e.active.p, e.h_active.data(), n * sizeof(int), hipMemcpyHostToDevice, st), "copy");
-    hip_check(hipStreamSynchronize(st), "sync");
-  }
+  // the in-step decision's flags: between steps every word has been read, so
+  // clearing them all restarts a slot rejoining the list at either parity
+  if (e.brk.p) hip_check(hipMemsetAsync(e.brk.p, 0, e.brk.n, st), "memset");
+  if (n) hip_check(hipMemcpyAsync(e.active.p, e.h_active.data(), n * sizeof(int), hipMemcpyHostToDevice, st), "copy");
+  hip_check(hipStreamSynchronize(st), "sync");
 }
 
 // reductions of the active list into out[4 i ..] (host); synchronises
@@ -1521,7 +1551,7 @@ size_t engine_bytes(const Engine& e) {
   size_t b = 0;
   for (const DevBuf* d : {&e.S, &e.Tb, &e.D, &e.E1, &e.Q1, &e.Q2, &e.twN, &e.twY, &e.twN64, &e.twY64, &e.kx2, &e.ky2,
                           &e.dts, &e.part, &e.out, &e.pin, &e.status, &e.eps, &e.Ms, &e.active, &e.diag, &e.dgi, &e.dgo, &e.vhat,
-                          &e.extc, &e.cnt})
+                          &e.extc, &e.cnt, &e.shst, &e.rebuildf, &e.brk, &e.rpart})
     b += d->n;
   return b;
 }

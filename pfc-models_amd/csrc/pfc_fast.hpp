@@ -340,7 +340,7 @@ template <typename T, class S, bool RANGE = false, bool NT = true>
 __device__ __forceinline__ void row_fast_tile(const V2<T>* __restrict__ Sg, V2<T>* __restrict__ Tout, int Ny,
                                               const Lay& lay, Nonlin q, const V2<double>* __restrict__ twN,
                                               const Slots& sl, int yi, int tile, unsigned char* smem_raw,
-                                              RangePart* rtile = nullptr) {
+                                              RangePart* rtile = nullptr, RangeCheck chk = RangeCheck{}) {
   using V = V2<T>;
   constexpr int M = S::L, P = S::P, TPL = S::TPL, LD = row_ld(M);
   const int slot = slot_of(sl, yi);
@@ -417,7 +417,13 @@ __device__ __forceinline__ void row_fast_tile(const V2<T>* __restrict__ Sg, V2<T
   }
   if constexpr (RANGE) {
     block_minmax_waves(rmn, rmx);
-    if (threadIdx.x == 0) *rtile = RangePart{rmn, rmx};
+    if (threadIdx.x == 0) {
+      *rtile = RangePart{rmn, rmx};
+      if (chk.st) {
+        if (shift_breach(chk.st[slot], rmn, rmx, chk.floor_tol, chk.ceil_tol)) chk.brk[2 * slot + chk.par] = 1;
+        if (tile == 0) chk.brk[2 * slot + (chk.par ^ 1)] = 0;
+      }
+    }
   }
 }
 
@@ -425,7 +431,7 @@ __device__ __forceinline__ void row_fast_tile(const V2<T>* __restrict__ Sg, V2<T
 template <typename T, class S, bool RANGE, bool NT>
 __global__ void __launch_bounds__(256) k_row_fast(const V2<T>* __restrict__ Sg, V2<T>* __restrict__ Tout, int Ny,
                                                   Lay lay, Nonlin q, const V2<double>* __restrict__ twN,
-                                                  Slots sl, RangePart* rpart) {
+                                                  Slots sl, RangePart* rpart, RangeCheck chk) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   // XCD-aware order over (instance, row block): the blocks that share a row
   // group's 128-byte lines run on one XCD and meet in its L2
@@ -433,7 +439,7 @@ __global__ void __launch_bounds__(256) k_row_fast(const V2<T>* __restrict__ Sg, 
   const int lin = xcd_remap(blockIdx.x + blockIdx.y * ntile, ntile * gridDim.y);
   const int yi = lin / ntile, tile = lin - yi * ntile;
   row_fast_tile<T, S, RANGE, NT>(Sg, Tout, Ny, lay, q, twN, sl, yi, tile, smem_raw,
-                             RANGE ? rpart + (size_t)yi * ntile + tile : nullptr);
+                                 RANGE ? rpart + (size_t)yi * ntile + tile : nullptr, chk);
 }
 
 // ---------------------------------------------------------------------------
@@ -509,16 +515,25 @@ __device__ __forceinline__ void col_fast_tile(V2<T>* __restrict__ Sg, const V2<T
   // bounds, results discarded) so loads and barriers stay uniform
   const int col = c0 + (live ? wl : ncols - 1);
   V* line = lds + wl * LD;
-  // ETD1 under the device-managed shift: a slot refitted this step first gets
-  // its columns' table entries recomputed here (as k_tables computes them)
-  // and written in place, before anything else is live, in place of a
-  // separate table launch before this one; the load path below then reads
-  // them back (each lane reads entries it wrote, or the identical value
-  // written for the mirrored row)
-  if (MODE == COL_ETD1 && tb.refit && tb.refit[slot]) {
+  // ETD1 under the device-managed shift, decided in the step: where the row
+  // kernel flagged the slot, every block takes the field's range from all
+  // row tiles' pairs, refits M (the same value in every block) and first
+  // recomputes its columns' table entries with it (as k_tables computes
+  // them), before anything else is live; tile 0 stores the new fit.  The
+  // load path below then reads the entries back (each lane reads entries it
+  // wrote, or the identical value written for the mirrored row).
+  if (MODE == COL_ETD1 && tb.brk && tb.brk[2 * slot + tb.par]) {
+    const RangePart* __restrict__ pp = tb.part + (size_t)yi * tb.pstride;
+    double mn = INFINITY, mx = -INFINITY;
+    for (int b = threadIdx.x; b < tb.nrow; b += blockDim.x) {
+      mn = fmin(mn, pp[b].mn);
+      mx = fmax(mx, pp[b].mx);
+    }
+    block_minmax_all(mn, mx);  // whole wavefronts: the host checks the block size
+    double lo;
     TabParams p = tb.tp;
+    p.M = shift_fit(mn, mx, tb.g, tb.minPhi, lo);
     p.epsilon = tb.eps[slot];
-    p.M = tb.Ms[slot];
     p.dt = tb.sg.dt[slot];
     const double kx2 = tb.sg.kx2[(size_t)slot * p.Nc + col];
     const double* __restrict__ ky2 = tb.sg.ky2 + (size_t)slot * p.Ny;
@@ -534,6 +549,10 @@ __device__ __forceinline__ void col_fast_tile(V2<T>* __restrict__ Sg, const V2<T
         Q1w[j] = t.q1;
         if (Q2w) Q2w[j] = t.q2;
       }
+    if (tile == 0 && threadIdx.x == 0) {
+      tb.st[slot] = ShiftState{lo, mx, 1};
+      tb.Ms[slot] = p.M;
+    }
     asm volatile("" ::: "memory");  // the table loads below read memory, not these values
   }
   // ETD tables are FP64 in every precision; the combine runs in FP64 and is
@@ -679,15 +698,15 @@ struct SchedInfo {
 template <typename T, bool NT>
 bool row_fast_launch_impl(int M, int var, dim3 grid, dim3 block, size_t lds, hipStream_t st, const V2<T>* S,
                           V2<T>* Tout, int Ny, const Lay& lay, const Nonlin& q, const V2<double>* tw, const Slots& sl,
-                          RangePart* rpart) {
+                          RangePart* rpart, const RangeCheck& chk) {
 #define PFC_X(LEN, VAR, PP, ...)                                                                                 \
   if (M == LEN && var == VAR) {                                                                                  \
     if (rpart)                                                                                                   \
       hipLaunchKernelGGL((k_row_fast<T, Sch<LEN, PP, __VA_ARGS__>, true, NT>), grid, block, lds, st, S, Tout, Ny, \
-                         lay, q, tw, sl, rpart);                                                                 \
+                         lay, q, tw, sl, rpart, chk);                                                            \
     else                                                                                                         \
       hipLaunchKernelGGL((k_row_fast<T, Sch<LEN, PP, __VA_ARGS__>, false, NT>), grid, block, lds, st, S, Tout,    \
-                         Ny, lay, q, tw, sl, rpart);                                                             \
+                         Ny, lay, q, tw, sl, rpart, chk);                                                        \
     return true;                                                                                                 \
   }
   PFC_SCHEDULES(PFC_X)
@@ -720,7 +739,7 @@ bool col_fast_launch_impl(int Ny, int var, int mode, dim3 grid, dim3 block, size
 #define PFC_LAUNCH_DECL(EXT, T, NT)                                                                              \
   EXT template bool row_fast_launch_impl<T, NT>(int, int, dim3, dim3, size_t, hipStream_t, const V2<T>*, V2<T>*,  \
                                                 int, const Lay&, const Nonlin&, const V2<double>*, const Slots&,   \
-                                                RangePart*);                                                       \
+                                                RangePart*, const RangeCheck&);                                    \
   EXT template bool col_fast_launch_impl<T, NT>(int, int, int, dim3, dim3, size_t, hipStream_t, V2<T>*,          \
                                                 const V2<T>*, V2<T>*, int, const Lay&, const ColTables&,           \
                                                 const V2<double>*, const Slots&);
@@ -734,9 +753,9 @@ PFC_LAUNCH_DECL(extern, float, false)
 template <typename T>
 inline bool row_fast_launch(int M, int var, bool nt, dim3 grid, dim3 block, size_t lds, hipStream_t st,
                             const V2<T>* S, V2<T>* Tout, int Ny, const Lay& lay, const Nonlin& q,
-                            const V2<double>* tw, const Slots& sl, RangePart* rpart) {
-  return nt ? row_fast_launch_impl<T, true>(M, var, grid, block, lds, st, S, Tout, Ny, lay, q, tw, sl, rpart)
-            : row_fast_launch_impl<T, false>(M, var, grid, block, lds, st, S, Tout, Ny, lay, q, tw, sl, rpart);
+                            const V2<double>* tw, const Slots& sl, RangePart* rpart, const RangeCheck& chk) {
+  return nt ? row_fast_launch_impl<T, true>(M, var, grid, block, lds, st, S, Tout, Ny, lay, q, tw, sl, rpart, chk)
+            : row_fast_launch_impl<T, false>(M, var, grid, block, lds, st, S, Tout, Ny, lay, q, tw, sl, rpart, chk);
 }
 template <typename T>
 inline bool col_fast_launch(int Ny, int var, int mode, bool nt, dim3 grid, dim3 block, size_t lds, hipStream_t st,

@@ -23,6 +23,7 @@ bool row_range_launch_f64(int M, int var, dim3 grid, dim3 block, size_t lds, hip
 }
 
 template bool row_fast_launch_impl<double, true>(int, int, dim3, dim3, size_t, hipStream_t, const V2<double>*, V2<double>*, int,
-                                             const Lay&, const Nonlin&, const V2<double>*, const Slots&, RangePart*);
+                                             const Lay&, const Nonlin&, const V2<double>*, const Slots&, RangePart*,
+                                             const RangeCheck&);
 
 }  // namespace pfc

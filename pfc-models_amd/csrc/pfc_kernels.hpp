@@ -284,6 +284,9 @@ struct SlotGrid {
   const double* dt;
 };
 
+struct RangePart;
+struct ShiftState;
+
 struct ColTables {
   const void* E1;  // column-major [Nc][hy] per slot, FP64 in every precision
   const void* Q1;
@@ -293,13 +296,21 @@ struct ColTables {
   // [Ny][Nc] per slot (cstride apart), or null: no potential
   const V2<double>* C;
   long long cstride;
-  // ETD1 under the device-managed shift: the slots whose `refit` flag is set
-  // have a new M this step, and the fast column kernel recomputes its own
-  // columns' E1, Q1 (and Q2) from eps, Ms, sg and tp, uses them and writes
-  // them back in place; null: the tables are read as stored
-  const int* refit;
+  // ETD1 under the device-managed shift, decided inside the step: the row
+  // kernel raised brk[2 slot + par] where a row tile left the fitted window
+  // (or the slot is unfitted); then the fast column kernel takes the field's
+  // range from every row tile's pair (part: nrow pairs per active instance,
+  // pstride apart), refits M, recomputes its own columns' E1, Q1 (and Q2)
+  // from eps, sg and tp with it, uses them and writes them back in place, and
+  // tile 0 stores the new fit to st and Ms.  brk null: tables read as stored.
+  const int* brk;
+  int par;
+  const RangePart* part;
+  int nrow, pstride;
+  ShiftState* st;
+  double* Ms;
+  double g, minPhi;
   const double* eps;
-  const double* Ms;
   SlotGrid sg;
   TabParams tp;
 };
@@ -1269,31 +1280,66 @@ __device__ __forceinline__ void block_minmax_waves(double& mn, double& mx) {
     }
 }
 
+// Whether a field range [mn, mx] calls for a refit of the fit w: unfitted,
+// or the range left the fitted window by more than the tolerances.  Each
+// test is monotone in mn (mx), so the range of a whole field breaches exactly
+// when the range of one of its tiles does.
+__device__ __forceinline__ bool shift_breach(const ShiftState& w, double mn, double mx, double floor_tol,
+                                             double ceil_tol) {
+  if (!w.fitted) return true;
+  return (w.lo - mn) / w.lo > floor_tol || (mx - w.hi) / w.hi > ceil_tol;
+}
+
+// The shift fitted to [mn, mx]: the chord slope of the nonlinearity's
+// non-log part plus log between lo = max(mn, minPhi) and mx.
+__device__ __forceinline__ double shift_fit(double mn, double mx, double g, double minPhi, double& lo) {
+  lo = mn > 0.0 ? mn : minPhi;
+  const double n_hi = g * mx * mx + mx * mx * mx + log(mx);
+  const double n_lo = g * lo * lo + lo * lo * lo + log(lo);
+  return (n_hi - n_lo) / (mx - lo);
+}
+
 // the refit decision for one slot from its field's min and max (one thread)
 __device__ __forceinline__ void shift_refit(double mn, double mx, int slot, const ShiftParams& sp,
                                             ShiftState* __restrict__ st, double* __restrict__ Ms,
                                             int* __restrict__ rebuild) {
-  ShiftState w = st[slot];
-  bool refit = !w.fitted;
-  if (!refit) {
-    const bool floor_breached = (w.lo - mn) / w.lo > sp.floor_tol;
-    const bool ceil_breached = (mx - w.hi) / w.hi > sp.ceil_tol;
-    refit = floor_breached || ceil_breached;
-  }
+  const bool refit = shift_breach(st[slot], mn, mx, sp.floor_tol, sp.ceil_tol);
   if (refit) {
-    const double lo = mn > 0.0 ? mn : sp.minPhi;
-    const double n_hi = sp.g * mx * mx + mx * mx * mx + log(mx);
-    const double n_lo = sp.g * lo * lo + lo * lo * lo + log(lo);
-    Ms[slot] = (n_hi - n_lo) / (mx - lo);
+    double lo;
+    Ms[slot] = shift_fit(mn, mx, sp.g, sp.minPhi, lo);
     st[slot] = ShiftState{lo, mx, 1};
   }
   rebuild[slot] = refit ? 1 : 0;
 }
 
+// The row kernel's share of the in-step decision: every row tile tests its
+// own range against the slot's fit and raises brk[2 slot + par]; tile 0
+// clears the other word, which the next step (the other parity) raises.
+struct RangeCheck {
+  const ShiftState* st;  // null: no check (range pairs only)
+  int* brk;
+  int par;
+  double floor_tol, ceil_tol;
+};
+
 // per-tile min and max of the field (k_row_range)
 struct RangePart {
   double mn, mx;
 };
+
+// min and max over a block of whole wavefronts (at most four), returned to
+// every thread
+__device__ __forceinline__ void block_minmax_all(double& mn, double& mx) {
+  block_minmax_waves(mn, mx);
+  __shared__ double bmn, bmx;
+  if (threadIdx.x == 0) {
+    bmn = mn;
+    bmx = mx;
+  }
+  __syncthreads();
+  mn = bmn;
+  mx = bmx;
+}
 
 // PartT: Partial (k_fe_row<MINMAX>) or RangePart (k_row_range), `nrow` per
 // instance, `pstride` apart
