@@ -10,7 +10,10 @@ instance whose growing range forces refits, on the fast two-kernel path (step
 graphs included; the ETD1 step takes the shift through its E1 table and its
 row kernel reports the range), on the generic kernels and on the resident
 small-grid path.  Batch slots equal
-plans bit for bit; a new field refits at the next step.
+plans bit for bit; a new field refits at the next step.  On the fast ETD1
+path the decision happens inside the step's two launches (breach flags by
+step parity): graphs captured at either parity and batch slots rejoining the
+active list after a refit step as plans do.
 """
 import numpy as np
 import pytest
@@ -85,6 +88,66 @@ def test_batch_slots_equal_plans_and_new_field_refits(hip_device):
     assert np.isnan(lo) and np.isnan(hi)
     pl.step(1)
     assert np.isfinite(pl.shift()[1])
+
+
+def _log_kw(nx, ny):
+    return dict(model="log", Nx=nx, Ny=ny, Lx=nx * np.pi / 4, Ly=ny * np.pi / 4, beta=0.5, q0=Q0, g=-2.0, dt=0.1,
+                clamp_log=True, pin_mean=True)
+
+
+def test_in_step_decision_graphs_at_either_parity(hip_device):
+    """The fast ETD1 step decides refits inside its two launches, the row
+    tiles raising breach flags that alternate with the step's parity.  Step
+    graphs captured after an odd number of steps (a 5-step call, then 300 and
+    33) give the same fields and fits, bit for bit, as one 338-step call
+    (graphs captured at even parity), through the refits of a crystallising
+    field."""
+    from pfc_hip.plan import Plan
+    kw = dict(_log_kw(96, 64), epsilon=-0.9)
+    f0 = F.uniform_noise((64, 96), 0.55, 0.05, seed=5)
+    a = Plan(**kw).set_field(f0).set_auto_shift(True)
+    b = Plan(**kw).set_field(f0).set_auto_shift(True)
+    a.step(5)
+    M5 = a.shift()[0]
+    a.step(300)
+    a.step(33)
+    b.step(338)
+    assert a.shift()[0] != M5  # refitted inside the graphs
+    assert np.array_equal(a.get_field().cpu().numpy(), b.get_field().cpu().numpy())
+    assert a.shift() == b.shift()
+
+
+def test_batch_slot_rejoining_after_a_refit_matches_its_plan(hip_device):
+    """A batch slot that refits at its last step before leaving the active
+    list, sits out one step (so it rejoins at the parity of that refit's
+    flag) and is stepped on: its field and fit equal a plan stepped the same
+    number of steps, bit for bit (no stale flag, no spurious refit)."""
+    from pfc_hip.plan import Batch, Plan
+    kw = _log_kw(64, 48)
+    fields = [F.uniform_noise((48, 64), 0.55 + 0.02 * i, 0.05, seed=20 + i) for i in range(3)]
+    eps = [-0.9, -0.8, -0.95]
+    probe = Plan(epsilon=eps[1], **kw).set_field(fields[1]).set_auto_shift(True)
+    Ms = []
+    for _ in range(120):
+        probe.step(1)
+        Ms.append(probe.shift()[0])
+    # k steps end with a refit, and step k + 1 keeps the fit
+    k = next(i + 1 for i in range(2, len(Ms) - 1) if Ms[i] != Ms[i - 1] and Ms[i + 1] == Ms[i])
+    b = Batch(3, **kw)
+    for i in range(3):
+        b.set_slot(i, eps[i])
+        b.set_field(i, fields[i])
+    b.set_auto_shift(True)
+    b.step(k)
+    b.set_active([0, 2])
+    b.step(1)
+    b.set_active([0, 1, 2])
+    b.step(300)
+    for i, n in ((0, k + 301), (1, k + 300), (2, k + 301)):
+        pl = Plan(epsilon=eps[i], **kw).set_field(fields[i]).set_auto_shift(True)
+        pl.step(n)
+        assert np.array_equal(b.get_field(i).cpu().numpy(), pl.get_field().cpu().numpy()), i
+        assert b.shift(i) == pl.shift(), i
 
 
 def test_device_shift_is_a_logpfc_option(hip_device):
