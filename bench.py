@@ -286,13 +286,16 @@ def run_sweep(plain_fraction=8, steps=5000):
     # untimed 10-step runs of each form first: the process loads the sweep
     # sizes' kernel code and the autotuner caches their schedules; the timed
     # runs still include allocation, tables, fields, graph capture and results
-    for prec, grouped in ((32, False), (64, False), (32, True)):
+    for prec, grouped in ((32, False), (64, False), (32, True), (32, "flow")):
         SW.run_batched(inst, 10, precision=prec, grouped=grouped)
     b32 = SW.run_batched(inst, steps, precision=32)
     b64 = SW.run_batched(inst, steps, precision=64)
     # the same sweep through one Group (a row and a column launch per
     # register class per step over every size's tiles)
     g32 = SW.run_batched(inst, steps, precision=32, grouped=True)
+    # and through the Group's persistent form (one launch per register class
+    # per 256 steps over the tiles of every size, per-instance dependencies)
+    f32 = SW.run_batched(inst, steps, precision=32, grouped="flow")
     sub = inst[::plain_fraction]
     plain = SW.run_plain(sub, steps, checkpoint=min(500, steps))
     idx = [i.index for i in sub]
@@ -326,6 +329,8 @@ def run_sweep(plain_fraction=8, steps=5000):
         "batched_fp64_solves_per_s": round(b64.solves_per_s, 2),
         "plain_fp64_solves_per_s": round(plain.solves_per_s, 2),
         "grouped_fp32_solves_per_s": round(g32.solves_per_s, 2),
+        "persistent_fp32_solves_per_s": round(f32.solves_per_s, 2),
+        "persistent_nonfinite_or_aborted": int(np.count_nonzero(f32.nonfinite)),
         "speedup_fp32_vs_plain_fp64": round(b32.solves_per_s / plain.solves_per_s, 2),
         "plain_sample": f"every {plain_fraction}th instance ({len(sub)} solves)",
         "timing": "whole job (allocation, tables, fields, graph capture, stepping, results) after an untimed "

@@ -52,6 +52,7 @@ extern "C" {
 #define PFC_SCHEME_ETDRK2 2
 
 #define PFC_STATUS_NONFINITE 1 /* a non-finite field value was seen by a step */
+#define PFC_STATUS_ABORTED 2   /* a persistent group step stopped early (pfc_group_set_flow): the field is undefined */
 
 typedef struct pfc_plan pfc_plan;
 
@@ -283,6 +284,27 @@ int pfc_group_destroy(pfc_group* group);
 int pfc_group_step(pfc_group* group, long long nsteps, void* stream);
 int pfc_group_graph_stats(const pfc_group* group, long long out[2]);
 int pfc_group_chunks(const pfc_group* group, int* nchunks);
+/* Persistent form of pfc_group_step.  steps_per_launch > 0: each register
+ * class's members advance in one launch per `steps_per_launch` steps
+ * (k_flow), whose blocks take the tiles of a work list in order -- for each
+ * step, each phase (rows, columns; ETDRK2: rows, stage 1, rows, stage 2),
+ * each member (largest first), each active instance, each tile -- and start
+ * a tile once its instance has finished every tile of the phases before it
+ * (per-instance counters), so instances' passes overlap with no launch
+ * boundary or device-wide drain between passes or steps; the two classes'
+ * launches run side by side.  Tiles run the same transforms as the
+ * per-pass launches, so results are bit-identical to them.  Progress does
+ * not depend on how many blocks are resident.  A dependency wait that
+ * exceeds its bound (~1 s; not expected) stops the launch and flags every
+ * member slot PFC_STATUS_ABORTED.  Needs each member's Nx/2 and Ny in one
+ * register class (PFC_EUNSUPPORTED otherwise).  0: the per-pass launches.
+ * pfc_group_flow_stats: out[0] launches so far, out[1..2] blocks per launch
+ * of classes 0 and 1 (resident capacity), out[3] steps per launch.
+ * (No reference counterpart: the reference relaxes one Simulation at a time,
+ * phase_diagram/q0star.py:409-477; this is the persistent uneven-size form of
+ * the sweep.) */
+int pfc_group_set_flow(pfc_group* group, int steps_per_launch);
+int pfc_group_flow_stats(const pfc_group* group, long long out[4]);
 
 /* Batched 2D real transforms over (batch, Ny, Nx) device arrays.
  * pfc_rfft2:  out[b] = rfft2(in[b])             (unnormalised)

@@ -1603,12 +1603,34 @@ struct pfc_group {
   hipStream_t cap_st = nullptr, cap_st2 = nullptr;
   hipEvent_t fork = nullptr, join = nullptr;
   long long captures = 0, replays = 0;
+  // Persistent form (pfc_group_set_flow): each register class's members step
+  // in one k_flow launch per `flow_steps` steps (0: the per-pass launches
+  // above).  Per class: the members' row and column descriptors and the
+  // work list's segment and counter offsets (fdesc, fidx), the ticket, the
+  // per-instance counters and the abort flag (fcnt); the two classes run on
+  // the caller's stream and flow_st2 side by side.
+  int flow_steps = 0;
+  struct FlowClass {
+    int nmem = 0, per_step = 0, ninst = 0, blocks = 0;
+    size_t lds = 0;
+    size_t desc_off = 0, idx_off = 0, cnt_off = 0;  // bytes into fdesc; ints into fidx; words into fcnt
+    size_t abort_off = 0;                             // word of the class's abort flag in fcnt
+  };
+  FlowClass flow[kGroupClasses];
+  DevBuf fdesc, fidx, fcnt;
+  std::vector<long long> flow_built;
+  hipStream_t flow_st2 = nullptr;
+  hipEvent_t flow_fork = nullptr, flow_join = nullptr;
+  long long flow_launches = 0;
   ~pfc_group() {
     for (auto& kv : graphs) (void)hipGraphExecDestroy(kv.second);
     if (fork) (void)hipEventDestroy(fork);
     if (join) (void)hipEventDestroy(join);
     if (cap_st) (void)hipStreamDestroy(cap_st);
     if (cap_st2) (void)hipStreamDestroy(cap_st2);
+    if (flow_fork) (void)hipEventDestroy(flow_fork);
+    if (flow_join) (void)hipEventDestroy(flow_join);
+    if (flow_st2) (void)hipStreamDestroy(flow_st2);
   }
 };
 
@@ -1774,6 +1796,168 @@ void group_capture_steps(pfc_group& g) {
   }
 }
 
+// ---- persistent form: k_flow -------------------------------------------------
+// every member's row and column lengths in one register class (a member's
+// chain of passes then lives in one launch)
+void flow_check(const pfc_group& g) {
+  for (const pfc_batch* b : g.mem) {
+    int P = 0, rc = 0, cc = 0;
+    group_case(b->e.p.Nx / 2, &P, &rc);
+    group_case(b->e.p.Ny, &P, &cc);
+    if (rc != cc)
+      fail(PFC_EUNSUPPORTED, "persistent group step: a member's row (Nx/2) and column (Ny) lengths are in different "
+                             "register classes");
+  }
+}
+
+template <typename T>
+void flow_build(pfc_group& g, hipStream_t st) {
+  const size_t n = g.mem.size();
+  const int nph = g.scheme == PFC_SCHEME_ETD1 ? 2 : 4;
+  const bool rk2 = nph == 4;
+  int dev = 0, ncu = 0;
+  hip_check(hipGetDevice(&dev), "hipGetDevice");
+  hip_check(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev), "device attribute");
+  std::vector<GroupMember<T>> desc(kGroupClasses * 2 * n);
+  std::vector<int> idx(kGroupClasses * ((size_t)4 * n + 1 + n + 1));
+  size_t cnt_words = 0;
+  for (int c = 0; c < kGroupClasses; ++c) {
+    pfc_group::FlowClass& F = g.flow[c];
+    F = pfc_group::FlowClass{};
+    F.desc_off = (size_t)c * 2 * n * sizeof(GroupMember<T>);
+    F.idx_off = (size_t)c * ((size_t)4 * n + 1 + n + 1);
+    std::vector<int> mems;
+    for (int i : g.order) {
+      const Engine& e = g.mem[i]->e;
+      int P = 0, rc = 0;
+      group_case(e.p.Nx / 2, &P, &rc);
+      if (rc == c && e.nact() > 0) mems.push_back(i);
+    }
+    F.nmem = (int)mems.size();
+    if (F.nmem == 0) continue;
+    GroupMember<T>* rows = desc.data() + (size_t)c * 2 * n;
+    GroupMember<T>* cols = rows + n;
+    int* seg = idx.data() + F.idx_off;
+    int* cnt0 = seg + nph * F.nmem + 1;
+    cnt0[0] = 1;  // word 0 is the ticket
+    for (int k = 0; k < F.nmem; ++k) {
+      Engine& e = g.mem[mems[k]]->e;
+      for (int pass = 0; pass < 2; ++pass) {
+        const int len = pass == 0 ? e.p.Nx / 2 : e.p.Ny;
+        int P = 0;
+        GroupMember<T> m{};
+        m.S = (V2<T>*)e.S.p;
+        m.Tb = (V2<T>*)e.Tb.p;
+        m.D = (V2<T>*)e.D.p;
+        m.tw = (const V2<double>*)(pass == 0 ? e.twN64.p : e.twY64.p);
+        m.lay = e.lay;
+        m.q = nonlin_of(e);
+        m.sl = e.slots_active();
+        m.tb = e.tables();
+        m.Ny = e.p.Ny;
+        m.Nc = e.Nc;
+        m.kcase = group_case(len, &P);
+        const int per = kGroupThreads / (len / P);  // rows (columns) per block
+        m.ntile = pass == 0 ? (e.p.Ny + per - 1) / per : (e.Nc + per - 1) / per;
+        m.blk0 = 0;
+        F.lds = std::max(F.lds, (size_t)per * (pass == 0 ? row_ld(len) : col_ld(len)) * sizeof(V2<T>));
+        (pass == 0 ? rows : cols)[k] = m;
+      }
+      cnt0[k + 1] = cnt0[k] + e.nact();
+    }
+    int at = 0;
+    for (int p = 0; p < nph; ++p)
+      for (int k = 0; k < F.nmem; ++k) {
+        seg[p * F.nmem + k] = at;
+        const Engine& e = g.mem[mems[k]]->e;
+        const long long add = (long long)e.nact() * ((p & 1) == 0 ? rows[k].ntile : cols[k].ntile);
+        if (at + add > (1LL << 30)) fail(PFC_EUNSUPPORTED, "persistent group step: too many tiles per step");
+        at += (int)add;
+      }
+    seg[nph * F.nmem] = at;
+    F.per_step = at;
+    F.ninst = cnt0[F.nmem] - 1;
+    F.cnt_off = cnt_words;
+    cnt_words += (size_t)(1 + F.ninst) * kFlowCntStride;
+    const int occ = g.precision == 64 ? flow_occupancy_f64(c, rk2, F.lds) : flow_occupancy_f32(c, rk2, F.lds);
+    if (occ < 1) fail(PFC_EUNSUPPORTED, "persistent group step: the kernel does not fit a CU");
+    F.blocks = occ * ncu;
+  }
+  // launches still reading the old plan finish first
+  hip_check(hipStreamSynchronize(st), "sync before flow rebuild");
+  if (g.fdesc.n < desc.size() * sizeof(GroupMember<T>)) g.fdesc.alloc(desc.size() * sizeof(GroupMember<T>));
+  hip_check(hipMemcpy(g.fdesc.p, desc.data(), desc.size() * sizeof(GroupMember<T>), hipMemcpyHostToDevice), "copy");
+  g.fidx.upload(idx);
+  // counters, then one abort flag per class on its own 256-byte line
+  for (int c = 0; c < kGroupClasses; ++c) g.flow[c].abort_off = cnt_words + (size_t)c * kFlowCntStride;
+  const size_t words = cnt_words + (size_t)kGroupClasses * kFlowCntStride;
+  // all of it zero: the new layout's abort flags may sit where the old
+  // layout kept counters
+  if (g.fcnt.n < words * sizeof(unsigned)) g.fcnt.alloc(words * sizeof(unsigned));
+  hip_check(hipMemset(g.fcnt.p, 0, g.fcnt.n), "memset");
+  g.flow_built = group_counts(g);
+}
+
+template <typename T>
+void flow_launch(pfc_group& g, int c, long long steps, hipStream_t st) {
+  const pfc_group::FlowClass& F = g.flow[c];
+  const size_t n = g.mem.size();
+  const int nph = g.scheme == PFC_SCHEME_ETD1 ? 2 : 4;
+  if ((unsigned long long)steps * (unsigned long long)F.per_step >= (1ULL << 31))
+    fail(PFC_EUNSUPPORTED, "persistent group step: too many tiles per launch");
+  FlowPlan<T> f{};
+  const GroupMember<T>* base = (const GroupMember<T>*)((const char*)g.fdesc.p + F.desc_off);
+  f.rows = base;
+  f.cols = base + n;
+  f.seg = (const int*)g.fidx.p + F.idx_off;
+  f.cnt0 = f.seg + nph * F.nmem + 1;
+  unsigned* words = (unsigned*)g.fcnt.p;
+  f.cnt = words + F.cnt_off;
+  f.ticket = f.cnt;
+  f.abort = (int*)(words + F.abort_off);
+  f.nmem = F.nmem;
+  f.nph = nph;
+  f.per_step = F.per_step;
+  f.total = (unsigned)(steps * F.per_step);
+  f.nwords = (1 + F.ninst) * kFlowCntStride;
+  const int blocks = (int)std::max(1LL, std::min<long long>(F.blocks, (long long)f.total));
+  if constexpr (sizeof(T) == 8)
+    flow_launch_f64(c, nph == 4, blocks, F.lds, st, f);
+  else
+    flow_launch_f32(c, nph == 4, blocks, F.lds, st, f);
+  ++g.flow_launches;
+}
+
+// nsteps steps in launches of at most flow_steps steps per class; the two
+// classes' launch chains are independent and run side by side (class 1 on
+// flow_st2, forked from and joined back to `st`)
+template <typename T>
+void flow_step(pfc_group& g, long long nsteps, hipStream_t st) {
+  const std::vector<long long> counts = group_counts(g);
+  if (counts != g.flow_built) flow_build<T>(g, st);
+  if (nsteps == 0) return;
+  const bool two = g.flow[0].nmem > 0 && g.flow[1].nmem > 0;
+  hipStream_t st1 = st;
+  if (two) {
+    if (!g.flow_st2) hip_check(hipStreamCreateWithFlags(&g.flow_st2, hipStreamNonBlocking), "hipStreamCreate");
+    if (!g.flow_fork) hip_check(hipEventCreateWithFlags(&g.flow_fork, hipEventDisableTiming), "hipEventCreate");
+    if (!g.flow_join) hip_check(hipEventCreateWithFlags(&g.flow_join, hipEventDisableTiming), "hipEventCreate");
+    hip_check(hipEventRecord(g.flow_fork, st), "hipEventRecord");
+    hip_check(hipStreamWaitEvent(g.flow_st2, g.flow_fork, 0), "hipStreamWaitEvent");
+    st1 = g.flow_st2;
+  }
+  for (long long done = 0; done < nsteps; done += g.flow_steps) {
+    const long long k = std::min<long long>(g.flow_steps, nsteps - done);
+    if (g.flow[0].nmem) flow_launch<T>(g, 0, k, st);
+    if (g.flow[1].nmem) flow_launch<T>(g, 1, k, st1);
+  }
+  if (two) {
+    hip_check(hipEventRecord(g.flow_join, g.flow_st2), "hipEventRecord");
+    hip_check(hipStreamWaitEvent(st, g.flow_join, 0), "hipStreamWaitEvent");
+  }
+  hip_check(hipGetLastError(), "flow launch");
+}
+
 template <typename T>
 void group_step(pfc_group& g, long long nsteps, hipStream_t st) {
   if (nsteps < 0) fail(PFC_EINVAL, "nsteps < 0");
@@ -1781,6 +1965,10 @@ void group_step(pfc_group& g, long long nsteps, hipStream_t st) {
     group_check_member(b->e);
     for (int s : b->e.h_active)
       if (!b->e.field_set[s]) fail(PFC_EINVAL, "a stepped slot has no field (pfc_batch_set_field)");
+  }
+  if (g.flow_steps > 0) {
+    flow_step<T>(g, nsteps, st);
+    return;
   }
   const std::vector<long long> counts = group_counts(g);
   if (counts != g.built) group_build<T>(g, st);
@@ -2250,6 +2438,26 @@ int pfc_group_step(pfc_group* g, long long nsteps, void* stream) {
       group_step<double>(*g, nsteps, st);
     else
       group_step<float>(*g, nsteps, st);
+  });
+}
+
+int pfc_group_set_flow(pfc_group* g, int steps_per_launch) {
+  return guarded([&] {
+    if (!g) fail(PFC_EINVAL, "NULL group");
+    if (steps_per_launch < 0) fail(PFC_EINVAL, "steps_per_launch < 0");
+    if (steps_per_launch > 0) flow_check(*g);
+    g->flow_steps = steps_per_launch;
+    g->flow_built.clear();  // the next step builds the plan
+  });
+}
+
+int pfc_group_flow_stats(const pfc_group* g, long long out[4]) {
+  return guarded([&] {
+    if (!g || !out) fail(PFC_EINVAL, "NULL argument");
+    out[0] = g->flow_launches;
+    out[1] = g->flow[0].blocks;
+    out[2] = g->flow[1].blocks;
+    out[3] = g->flow_steps;
   });
 }
 

@@ -118,6 +118,40 @@ template <bool NT, typename V> __device__ __forceinline__ void store_out(V* p, V
     *p = v;
 }
 
+// State accesses (S~, T, the ETDRK2 carry) of a tile.  MEM 0: ordinary loads
+// and store_out<NT>.  MEM 1 (the persistent group step, k_flow, where one
+// launch's tiles hand their results to each other): device-scope relaxed
+// atomic loads and stores (sc1), which read from and write through to the
+// device's coherence point, so a tile that waits for another tile's counter
+// sees its stores without an L2 write-back or invalidate.  FP64 complex
+// values move as two FP64 atomics (an element-wise bit cast of the vector
+// miscompiles here), FP32 ones as one 64-bit atomic.
+template <int MEM, typename T> __device__ __forceinline__ V2<T> ld_state(const V2<T>* p) {
+  if constexpr (MEM == 0) {
+    return *p;
+  } else if constexpr (sizeof(T) == 8) {
+    const T* q = reinterpret_cast<const T*>(p);
+    const T a = __hip_atomic_load(q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const T b = __hip_atomic_load(q + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return V2<T>{a, b};
+  } else {
+    const unsigned long long* q = reinterpret_cast<const unsigned long long*>(p);
+    return __builtin_bit_cast(V2<T>, __hip_atomic_load(q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+  }
+}
+template <int MEM, bool NT, typename T> __device__ __forceinline__ void st_state(V2<T>* p, V2<T> v) {
+  if constexpr (MEM == 0) {
+    store_out<NT>(p, v);
+  } else if constexpr (sizeof(T) == 8) {
+    T* q = reinterpret_cast<T*>(p);
+    __hip_atomic_store(q, v.x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(q + 1, v.y, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  } else {
+    __hip_atomic_store(reinterpret_cast<unsigned long long*>(p), __builtin_bit_cast(unsigned long long, v),
+                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+
 // Wavefront shuffles for the row kernel's pack and unpack.  Both pair the
 // element k = lane + TPL q of a line with M - k, which for a line inside one
 // wavefront lives in lane (TPL - lane) mod TPL of the same line, register
@@ -277,7 +311,7 @@ __device__ __forceinline__ void reg_fft(V2<T>* v, V2<T>* line, int lane, const V
 // k = lane + TPL q, and wk, wM, wt hold the roots the forward half needs.
 // m0: the conserved mean, taken off the kx = 0 coefficient (the row's
 // average) so that the transform carries the fluctuation n - m0 only (0: none).
-template <typename T, class S>
+template <typename T, class S, int MEM = 0>
 __device__ __forceinline__ void row_c2r(const V2<T>* __restrict__ Sg, const Lay& lay, int rowc, int lane, V2<T>* line,
                                         const V2<double>* __restrict__ twN, V2<T> (&v)[S::P], V2<T> (&wk)[S::P],
                                         V2<T>& wM, V2<T> (&wt)[tw_count<S>()], double m0 = 0.0) {
@@ -285,8 +319,8 @@ __device__ __forceinline__ void row_c2r(const V2<T>* __restrict__ Sg, const Lay&
   constexpr int M = S::L, P = S::P, TPL = S::TPL;
   V x[P];
 #pragma unroll
-  for (int qq = 0; qq < P; ++qq) x[qq] = Sg[lay.at(rowc, lane + TPL * qq)];
-  const V xM = Sg[lay.at(rowc, M)];
+  for (int qq = 0; qq < P; ++qq) x[qq] = ld_state<MEM, T>(Sg + lay.at(rowc, lane + TPL * qq));
+  const V xM = ld_state<MEM, T>(Sg + lay.at(rowc, M));
   // roots: w_Nx^k for the (un)packing of k = lane + TPL q, as w_Nx^lane times
   // the wave-uniform w_Nx^(TPL q) (scalar loads), in FP64; and the FFT roots
   const V2<double> wl0 = twN[lane];
@@ -336,10 +370,10 @@ __device__ __forceinline__ void row_c2r(const V2<T>* __restrict__ Sg, const Lay&
 // the values the nonlinearity sees, are also written to `rtile`: the range
 // check of the stabilised split folded into the ETD1 tables.  A compile-time
 // choice: the ordinary kernels keep exactly the code the group kernels share.
-template <typename T, class S, bool RANGE = false, bool NT = true>
+template <typename T, class S, bool RANGE = false, bool NT = true, int MEM = 0>
 __device__ __forceinline__ void row_fast_tile(const V2<T>* __restrict__ Sg, V2<T>* __restrict__ Tout, int Ny,
                                               const Lay& lay, Nonlin q, const V2<double>* __restrict__ twN,
-                                              const Slots& sl, int yi, int tile, unsigned char* smem_raw,
+                                              const Slots& sl, int yi, int tile, unsigned char* smem_raw, int tid,
                                               RangePart* rtile = nullptr, RangeCheck chk = RangeCheck{}) {
   using V = V2<T>;
   constexpr int M = S::L, P = S::P, TPL = S::TPL, LD = row_ld(M);
@@ -347,8 +381,8 @@ __device__ __forceinline__ void row_fast_tile(const V2<T>* __restrict__ Sg, V2<T
   Sg += (size_t)slot * sl.sstride;
   Tout += (size_t)slot * sl.sstride;
   if (sl.M) q.M = sl.M[slot];
-  const int wl = threadIdx.x / TPL;
-  const int lane = threadIdx.x - wl * TPL;
+  const int wl = tid / TPL;
+  const int lane = tid - wl * TPL;
   const int row = tile * (blockDim.x / TPL) + wl;
   const bool live = row < Ny;
   V* line = reinterpret_cast<V*>(smem_raw) + wl * LD;
@@ -364,7 +398,7 @@ __device__ __forceinline__ void row_fast_tile(const V2<T>* __restrict__ Sg, V2<T
   const double m0 = about_m ? sl.pin[slot] : 0.0;
   V v[P], wk[P], wM;
   V wt[tw_count<S>()];
-  row_c2r<T, S>(Sg, lay, live ? row : Ny - 1, lane, line, twN, v, wk, wM, wt, m0);
+  row_c2r<T, S, MEM>(Sg, lay, live ? row : Ny - 1, lane, line, twN, v, wk, wM, wt, m0);
   // the lane's min and max now; the block's after the stores, where the
   // reduction overlaps their drain
   // (in T, then once in FP64 about m0: m0 + x is monotone in x, so this is
@@ -417,14 +451,14 @@ __device__ __forceinline__ void row_fast_tile(const V2<T>* __restrict__ Sg, V2<T
 #pragma unroll
     for (int qq = 0; qq < P; ++qq) {
       const int k = lane + TPL * qq;
-      store_out<NT>(Tout + lay.at(row, k), row_unpack_pair<T>(v[qq], zmr[qq], wk[qq]));
+      st_state<MEM, NT, T>(Tout + lay.at(row, k), row_unpack_pair<T>(v[qq], zmr[qq], wk[qq]));
     }
     // Z[0] is lane 0's register 0
-    if (lane == 0) store_out<NT>(Tout + lay.at(row, M), row_unpack_pair<T>(v[0], v[0], wM));
+    if (lane == 0) st_state<MEM, NT, T>(Tout + lay.at(row, M), row_unpack_pair<T>(v[0], v[0], wM));
   }
   if constexpr (RANGE) {
     block_minmax_waves(rmn, rmx);
-    if (threadIdx.x == 0) {
+    if (tid == 0) {
       *rtile = RangePart{rmn, rmx};
       if (chk.st) {
         if (shift_breach(wfit, rmn, rmx, chk.floor_tol, chk.ceil_tol)) chk.brk[2 * slot + chk.par] = 1;
@@ -445,7 +479,7 @@ __global__ void __launch_bounds__(256) k_row_fast(const V2<T>* __restrict__ Sg, 
   const int ntile = gridDim.x;
   const int lin = xcd_remap(blockIdx.x + blockIdx.y * ntile, ntile * gridDim.y);
   const int yi = lin / ntile, tile = lin - yi * ntile;
-  row_fast_tile<T, S, RANGE, NT>(Sg, Tout, Ny, lay, q, twN, sl, yi, tile, smem_raw,
+  row_fast_tile<T, S, RANGE, NT>(Sg, Tout, Ny, lay, q, twN, sl, yi, tile, smem_raw, threadIdx.x,
                                  RANGE ? rpart + (size_t)yi * ntile + tile : nullptr, chk);
 }
 
@@ -494,11 +528,11 @@ __global__ void __launch_bounds__(256) k_row_range(const V2<T>* __restrict__ Sg,
 // ---------------------------------------------------------------------------
 // Column tile `tile` (blockDim.x / TPL columns) of active instance `yi`;
 // every thread of the block calls it.
-template <typename T, class S, int MODE, bool NT = true>
+template <typename T, class S, int MODE, bool NT = true, int MEM = 0>
 __device__ __forceinline__ void col_fast_tile(V2<T>* __restrict__ Sg, const V2<T>* __restrict__ Tin,
                                               V2<T>* __restrict__ D, int Nc, const Lay& lay, const ColTables& tb,
                                               const V2<double>* __restrict__ twY, const Slots& sl, int yi, int tile,
-                                              unsigned char* smem_raw) {
+                                              unsigned char* smem_raw, int tid) {
   using V = V2<T>;
   constexpr int NY = S::L, P = S::P, TPL = S::TPL, LD = col_ld(NY);
   V* lds = reinterpret_cast<V*>(smem_raw);
@@ -509,8 +543,8 @@ __device__ __forceinline__ void col_fast_tile(V2<T>* __restrict__ Sg, const V2<T
   // transforms' strided order: no LDS transpose on the way in or out.
   const int C = blockDim.x / TPL;  // columns per block, a power of two
   const int lgC = __ffs(C) - 1;
-  const int wl = threadIdx.x & (C - 1);
-  const int lane = threadIdx.x >> lgC;
+  const int wl = tid & (C - 1);
+  const int lane = tid >> lgC;
   const int slot = slot_of(sl, yi);
   const int c0 = tile * C;
   Sg += (size_t)slot * sl.sstride;
@@ -532,7 +566,7 @@ __device__ __forceinline__ void col_fast_tile(V2<T>* __restrict__ Sg, const V2<T
   if (MODE == COL_ETD1 && tb.brk && tb.brk[2 * slot + tb.par]) {
     const RangePart* __restrict__ pp = tb.part + (size_t)yi * tb.pstride;
     double mn = INFINITY, mx = -INFINITY;
-    for (int b = threadIdx.x; b < tb.nrow; b += blockDim.x) {
+    for (int b = tid; b < tb.nrow; b += blockDim.x) {
       mn = fmin(mn, pp[b].mn);
       mx = fmax(mx, pp[b].mx);
     }
@@ -556,7 +590,7 @@ __device__ __forceinline__ void col_fast_tile(V2<T>* __restrict__ Sg, const V2<T
         Q1w[j] = t.q1;
         if (Q2w) Q2w[j] = t.q2;
       }
-    if (tile == 0 && threadIdx.x == 0) {
+    if (tile == 0 && tid == 0) {
       tb.st[slot] = ShiftState{lo, mx, 1};
       tb.Ms[slot] = p.M;
     }
@@ -572,8 +606,8 @@ __device__ __forceinline__ void col_fast_tile(V2<T>* __restrict__ Sg, const V2<T
 #pragma unroll
   for (int qq = 0; qq < P; ++qq) {
     const size_t off = lay.at(lane + TPL * qq, col);
-    if constexpr (MODE != COL_RK2_STAGE2) a[qq] = Sg[off];
-    b[qq] = Tin[off];  // in flight while the first transform runs
+    if constexpr (MODE != COL_RK2_STAGE2) a[qq] = ld_state<MEM, T>(Sg + off);
+    b[qq] = ld_state<MEM, T>(Tin + off);  // in flight while the first transform runs
   }
   V wt[tw_count<S>()];
   tw_load<T, S>(wt, lane, twY, 1);
@@ -607,7 +641,7 @@ __device__ __forceinline__ void col_fast_tile(V2<T>* __restrict__ Sg, const V2<T
     const size_t ti = hbase + ky_half(lane + TPL * qq, NY);
     if constexpr (MODE == COL_RK2_STAGE2) {
       q2r[qq] = Q2[ti];
-      dr[qq] = D[tbase + lane + TPL * qq];
+      dr[qq] = ld_state<MEM, T>(D + tbase + lane + TPL * qq);
     } else {
       e1r[qq] = E1[ti];
       q1r[qq] = Q1[ti];
@@ -629,7 +663,7 @@ __device__ __forceinline__ void col_fast_tile(V2<T>* __restrict__ Sg, const V2<T
       } else {
         ox = e1r[qq] * double(a[qq].x) + q1r[qq] * bx;
         oy = e1r[qq] * double(a[qq].y) + q1r[qq] * by;
-        if constexpr (MODE == COL_RK2_STAGE1) D[tbase + ky] = V{T(ox - q2r[qq] * bx), T(oy - q2r[qq] * by)};
+        if constexpr (MODE == COL_RK2_STAGE1) st_state<MEM, false, T>(D + tbase + ky, V{T(ox - q2r[qq] * bx), T(oy - q2r[qq] * by)});
       }
       if (pinp && col == 0 && ky == 0) ox = kAbout ? 0.0 : pin, oy = 0.0;  // FP32: the fluctuation's mean, 0
       b[qq] = V{T(ox), T(oy)};
@@ -645,7 +679,7 @@ __device__ __forceinline__ void col_fast_tile(V2<T>* __restrict__ Sg, const V2<T
       if (realcol) xv.y = T(0);
       if constexpr (kAbout)
         if (mcol != 0.0) xv.x = T(double(xv.x) + mcol);
-      store_out<NT>(Sg + lay.at(lane + TPL * qq, col), xv);
+      st_state<MEM, NT, T>(Sg + lay.at(lane + TPL * qq, col), xv);
     }
   }
 }
@@ -660,7 +694,7 @@ __global__ void __launch_bounds__(256) k_col_fast(V2<T>* __restrict__ Sg, const 
   const int ntile = gridDim.x;
   const int lin = xcd_remap(blockIdx.x + blockIdx.y * ntile, ntile * gridDim.y);
   const int yi = lin / ntile;
-  col_fast_tile<T, S, MODE, NT>(Sg, Tin, D, Nc, lay, tb, twY, sl, yi, lin - yi * ntile, smem_raw);
+  col_fast_tile<T, S, MODE, NT>(Sg, Tin, D, Nc, lay, tb, twY, sl, yi, lin - yi * ntile, smem_raw, threadIdx.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -851,7 +885,8 @@ __global__ void __launch_bounds__(kGroupThreads) k_row_group(const GroupMember<T
 #define PFC_X(CASE, CLS, LEN, PP, ...)                                                                           \
   case CASE:                                                                                                     \
     if constexpr (CLS == CLASS)                                                                                  \
-      row_fast_tile<T, Sch<LEN, PP, __VA_ARGS__>>(g.S, g.Tb, g.Ny, g.lay, g.q, g.tw, g.sl, yi, tile, smem_raw);  \
+      row_fast_tile<T, Sch<LEN, PP, __VA_ARGS__>>(g.S, g.Tb, g.Ny, g.lay, g.q, g.tw, g.sl, yi, tile, smem_raw,    \
+                                                  threadIdx.x);                                                  \
     break;
     PFC_GROUP_SCHEDULES(PFC_X)
 #undef PFC_X
@@ -875,7 +910,7 @@ __global__ void __launch_bounds__(kGroupThreads) k_col_group(const GroupMember<T
   case CASE:                                                                                                    \
     if constexpr (CLS == CLASS)                                                                                 \
       col_fast_tile<T, Sch<LEN, PP, __VA_ARGS__>, MODE>(g.S, g.Tb, g.D, g.Nc, g.lay, g.tb, g.tw, g.sl, yi, tile, \
-                                                        smem_raw);                                              \
+                                                        smem_raw, threadIdx.x);                                 \
     break;
     PFC_GROUP_SCHEDULES(PFC_X)
 #undef PFC_X
@@ -890,5 +925,221 @@ void group_launch_f64(int kind, int cls, int blocks, size_t lds, hipStream_t st,
                       int nmem);
 void group_launch_f32(int kind, int cls, int blocks, size_t lds, hipStream_t st, const GroupMember<float>* mem,
                       int nmem);
+
+// ---------------------------------------------------------------------------
+// persistent multi-shape step ("flow"): one launch advances the members of
+// one register class by several steps.  Its work list is the sequence
+//   for each step, for each phase (row pass, column pass; ETDRK2: row,
+//   column stage 1, row, column stage 2), for each member (largest first),
+//   for each active instance, for each of the phase's tiles
+// and blocks take positions in it ("tickets") from a counter, in order.  A
+// tile first waits until its instance has completed every tile of the phases
+// before its own (a per-instance counter), all of them earlier tickets, then
+// runs the same tile function as the per-pass launches, with device-coherent
+// state accesses (st_state / ld_state, MEM 1), and counts itself done.  Each
+// instance thus follows its own chain of passes while other instances' tiles
+// keep the device busy: no launch boundary and no device-wide drain between
+// passes or steps.
+// Progress needs no co-resident blocks: the unfinished ticket with the
+// lowest number depends only on finished tickets, and the block holding it
+// runs it (the ticket a block prefetches is above the one it runs), so the
+// launch completes with any number of blocks resident, alongside other
+// kernels.  A wait that exceeds its bound (never expected) raises `abort`,
+// which stops every block; k_flow_mark then flags the members' slots
+// PFC_STATUS_ABORTED.
+// ---------------------------------------------------------------------------
+constexpr int kFlowCntStride = 64;      // counters 256 bytes apart
+constexpr int kFlowSpinLimit = 1 << 21;  // polls of one wait (~1 s) before the launch aborts
+constexpr int kStatusAborted = 2;        // PFC_STATUS_ABORTED
+
+template <typename T> struct FlowPlan {
+  const GroupMember<T>* rows;  // [nmem] row-pass descriptors
+  const GroupMember<T>* cols;  // [nmem] column-pass descriptors
+  const int* seg;              // [nph * nmem + 1]: first ticket of (phase, member) within a step
+  const int* cnt0;             // [nmem + 1]: first instance counter of each member
+  unsigned* ticket;            // next ticket (cnt[0]), zero at launch
+  unsigned* cnt;               // per-instance completed tiles, kFlowCntStride apart, zero at launch
+  int* abort;                  // a wait exceeded its bound (cleared by k_flow_mark)
+  int nmem, nph, per_step;
+  unsigned total;              // tickets of the launch: steps * per_step
+  int nwords;                  // words from cnt[0] that k_flow_reset clears
+};
+
+// before a flow launch: the ticket and the instance counters to zero (a fill
+// kernel, not a memset node, so the sequence may sit in a captured graph)
+template <typename U>
+__global__ void k_flow_reset(U* cnt, int nwords) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nwords; i += gridDim.x * blockDim.x) cnt[i] = U(0);
+}
+
+// thread 0: wait until *c >= need; false if the launch aborted
+__device__ __forceinline__ bool flow_wait(const unsigned* c, unsigned need, int* abort) {
+  for (int i = 0;; ++i) {
+    if (__hip_atomic_load(c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= need) return true;
+    if ((i & 31) == 31 && __hip_atomic_load(abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return false;
+    if (i >= kFlowSpinLimit) {
+      __hip_atomic_store(abort, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      return false;
+    }
+    __builtin_amdgcn_s_sleep(2);
+  }
+}
+
+template <typename T, int CLASS>
+__device__ __forceinline__ void flow_row(const GroupMember<T>& g, int yi, int tile, unsigned char* smem, int tid) {
+  switch (g.kcase) {
+#define PFC_X(CASE, CLS, LEN, PP, ...)                                                                           \
+  case CASE:                                                                                                     \
+    if constexpr (CLS == CLASS)                                                                                  \
+      row_fast_tile<T, Sch<LEN, PP, __VA_ARGS__>, false, true, 1>(g.S, g.Tb, g.Ny, g.lay, g.q, g.tw, g.sl, yi,    \
+                                                                  tile, smem, tid);                              \
+    break;
+    PFC_GROUP_SCHEDULES(PFC_X)
+#undef PFC_X
+    default:
+      break;
+  }
+}
+
+template <typename T, int CLASS, int MODE>
+__device__ __forceinline__ void flow_col(const GroupMember<T>& g, int yi, int tile, unsigned char* smem, int tid) {
+  switch (g.kcase) {
+#define PFC_X(CASE, CLS, LEN, PP, ...)                                                                           \
+  case CASE:                                                                                                     \
+    if constexpr (CLS == CLASS)                                                                                  \
+      col_fast_tile<T, Sch<LEN, PP, __VA_ARGS__>, MODE, true, 1>(g.S, g.Tb, g.D, g.Nc, g.lay, g.tb, g.tw, g.sl,   \
+                                                                 yi, tile, smem, tid);                           \
+    break;
+    PFC_GROUP_SCHEDULES(PFC_X)
+#undef PFC_X
+    default:
+      break;
+  }
+}
+
+// Waves per SIMD a flow launch is compiled for: the register budget of the
+// per-pass kernels of the same class (class 0: 8 points per lane, class 1:
+// 12), so the persistent kernel keeps their occupancy.
+template <typename T, int CLASS> constexpr int flow_waves() {
+  return sizeof(T) == 4 ? (CLASS == 0 ? 3 : 2) : (CLASS == 0 ? 2 : 1);
+}
+
+// A ticket's place in the work list (every field block-uniform).
+struct FlowTicket {
+  unsigned s;     // step within the launch
+  int p, m;       // phase, member
+  int yi, tile;   // active instance, tile
+  unsigned need;  // the instance's counter value that makes the tile ready
+  unsigned* c;    // the instance's counter
+};
+
+// Integer quotients are computed in vector registers; readfirstlane returns
+// them to scalar ones, so the member's descriptor is read with scalar loads.
+template <typename T> __device__ __forceinline__ FlowTicket flow_decode(const FlowPlan<T>& f, unsigned t) {
+  FlowTicket k;
+  k.s = __builtin_amdgcn_readfirstlane(t / (unsigned)f.per_step);
+  const int r = (int)(t - k.s * (unsigned)f.per_step);
+  int gs = 0;
+  while (f.seg[gs + 1] <= r) ++gs;  // seg[nph * nmem] = per_step > r
+  k.p = __builtin_amdgcn_readfirstlane(gs / f.nmem);
+  k.m = gs - k.p * f.nmem;
+  const int nr = f.rows[k.m].ntile, nc = f.cols[k.m].ntile;
+  const int ntile = (k.p & 1) == 0 ? nr : nc;
+  const int local = r - f.seg[gs];
+  k.yi = __builtin_amdgcn_readfirstlane(local / ntile);
+  k.tile = local - k.yi * ntile;
+  // the instance's tiles of every earlier phase: s whole steps, then the
+  // phases of this step before p
+  const unsigned per_inst = (unsigned)(f.nph / 2) * (unsigned)(nr + nc);
+  k.need = k.s * per_inst + (unsigned)(k.p >> 1) * (unsigned)(nr + nc) + (unsigned)(k.p & 1) * (unsigned)nr;
+  k.c = f.cnt + (size_t)(f.cnt0[k.m] + k.yi) * kFlowCntStride;
+  return k;
+}
+
+// Thread 0 keeps tickets two deep: while the block runs ticket t it knows
+// the next one (tn), has the one after in flight (atomicAdd, waited for only
+// at the tile's end) and has tn's counter polled, so a ready tile starts
+// without a round trip to memory.
+template <typename T, int CLASS, bool RK2>
+__global__ void __launch_bounds__(kGroupThreads, (flow_waves<T, CLASS>())) k_flow(FlowPlan<T> f) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  __shared__ unsigned s_tk[2];
+  __shared__ int s_go;
+  if (threadIdx.x == 0) {
+    s_tk[0] = atomicAdd(f.ticket, 1u);
+    s_tk[1] = atomicAdd(f.ticket, 1u);
+  }
+  __syncthreads();
+  // tickets are block-uniform: readfirstlane keeps them (and the descriptors
+  // found from them) in scalar registers
+  unsigned t = __builtin_amdgcn_readfirstlane(s_tk[0]);
+  unsigned tn = __builtin_amdgcn_readfirstlane(s_tk[1]);
+  __syncthreads();  // every thread has read s_tk before thread 0 writes it again
+  unsigned early = 0;  // thread 0: t's counter as polled during the previous tile
+  while (t < f.total) {
+    const FlowTicket k = flow_decode(f, t);
+    unsigned nxt = 0, e2 = 0;  // thread 0: the ticket after tn, tn's counter (both in flight)
+    if (threadIdx.x == 0) {
+      const bool ok = k.need == 0 || early >= k.need || flow_wait(k.c, k.need, f.abort);
+      s_go = ok;
+      if (ok) {
+        nxt = atomicAdd(f.ticket, 1u);
+        if (tn < f.total) e2 = __hip_atomic_load(flow_decode(f, tn).c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
+    }
+    __syncthreads();
+    if (!s_go) break;
+    // the thread index, opaque to the compiler in each iteration: otherwise
+    // it hoists every schedule's lane arithmetic out of the loop and keeps
+    // all of it live at once (past 256 VGPRs)
+    int tid = threadIdx.x;
+    asm volatile("" : "+v"(tid));
+    // the descriptor stays in memory: fields load where the tile uses them
+    // (a copy would hold the whole struct, ~100 dwords, in registers)
+    if ((k.p & 1) == 0) {
+      flow_row<T, CLASS>(f.rows[k.m], k.yi, k.tile, smem_raw, tid);
+    } else {
+      if constexpr (RK2) {
+        if (k.p == 1)
+          flow_col<T, CLASS, COL_RK2_STAGE1>(f.cols[k.m], k.yi, k.tile, smem_raw, tid);
+        else
+          flow_col<T, CLASS, COL_RK2_STAGE2>(f.cols[k.m], k.yi, k.tile, smem_raw, tid);
+      } else {
+        flow_col<T, CLASS, COL_ETD1>(f.cols[k.m], k.yi, k.tile, smem_raw, tid);
+      }
+    }
+    // every thread's stores have completed (written through) before the
+    // tile counts as done
+    __builtin_amdgcn_s_waitcnt(0);
+    if (threadIdx.x == 0) s_tk[0] = nxt;
+    __syncthreads();
+    if (threadIdx.x == 0) __hip_atomic_fetch_add(k.c, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    t = tn;
+    tn = __builtin_amdgcn_readfirstlane(s_tk[0]);
+    early = e2;
+  }
+}
+
+// After a flow launch: when it aborted, flag every member's active slots
+// and clear the flag for the next launch.  One block.
+template <typename T>
+__global__ void k_flow_mark(const GroupMember<T>* __restrict__ rows, const int* __restrict__ cnt0, int nmem,
+                            int* abort) {
+  if (*abort == 0) return;
+  for (int m = 0; m < nmem; ++m) {
+    const Slots& sl = rows[m].sl;
+    for (int yi = threadIdx.x; yi < cnt0[m + 1] - cnt0[m]; yi += blockDim.x)
+      if (sl.status) atomicOr(sl.status + slot_of(sl, yi), kStatusAborted);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) *abort = 0;
+}
+
+// Launchers (pfc_inst_group*.hip): a flow launch of class `cls` (rk2: the
+// ETDRK2 phases), and its resident blocks per CU at `lds` bytes of LDS.
+void flow_launch_f64(int cls, bool rk2, int blocks, size_t lds, hipStream_t st, const FlowPlan<double>& f);
+void flow_launch_f32(int cls, bool rk2, int blocks, size_t lds, hipStream_t st, const FlowPlan<float>& f);
+int flow_occupancy_f64(int cls, bool rk2, size_t lds);
+int flow_occupancy_f32(int cls, bool rk2, size_t lds);
 
 }  // namespace pfc

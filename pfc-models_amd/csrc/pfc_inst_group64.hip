@@ -1,4 +1,4 @@
-// Multi-shape step kernels (k_row_group, k_col_group), double bulk.
+// Multi-shape step kernels (k_row_group, k_col_group, the persistent k_flow), double bulk.
 #include "pfc_fast.hpp"
 
 namespace pfc {
@@ -27,6 +27,32 @@ void group_launch_f64(int kind, int cls, int blocks, size_t lds, hipStream_t st,
     launch_class<0>(kind, grid, block, lds, st, mem, nmem);
   else
     launch_class<1>(kind, grid, block, lds, st, mem, nmem);
+}
+
+template <int CLS, bool RK2>
+static void flow_one(int blocks, size_t lds, hipStream_t st, const FlowPlan<double>& f) {
+  hipLaunchKernelGGL(k_flow_reset<unsigned>, dim3((f.nwords + 255) / 256), dim3(256), 0, st, f.cnt, f.nwords);
+  hipLaunchKernelGGL((k_flow<double, CLS, RK2>), dim3(blocks), dim3(kGroupThreads), lds, st, f);
+  hipLaunchKernelGGL((k_flow_mark<double>), dim3(1), dim3(256), 0, st, f.rows, f.cnt0, f.nmem, f.abort);
+}
+
+void flow_launch_f64(int cls, bool rk2, int blocks, size_t lds, hipStream_t st, const FlowPlan<double>& f) {
+  if (cls == 0)
+    rk2 ? flow_one<0, true>(blocks, lds, st, f) : flow_one<0, false>(blocks, lds, st, f);
+  else
+    rk2 ? flow_one<1, true>(blocks, lds, st, f) : flow_one<1, false>(blocks, lds, st, f);
+}
+
+int flow_occupancy_f64(int cls, bool rk2, size_t lds) {
+  int n = 0;
+  hipError_t e;
+  if (cls == 0)
+    e = rk2 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_flow<double, 0, true>, kGroupThreads, lds)
+            : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_flow<double, 0, false>, kGroupThreads, lds);
+  else
+    e = rk2 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_flow<double, 1, true>, kGroupThreads, lds)
+            : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_flow<double, 1, false>, kGroupThreads, lds);
+  return e == hipSuccess ? n : 0;
 }
 
 }  // namespace pfc

@@ -77,6 +77,8 @@ _SIGNATURES = [
     ("pfc_group_step", ctypes.c_int, [_P, ctypes.c_longlong, _P]),
     ("pfc_group_graph_stats", ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_longlong)]),
     ("pfc_group_chunks", ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_int)]),
+    ("pfc_group_set_flow", ctypes.c_int, [_P, ctypes.c_int]),
+    ("pfc_group_flow_stats", ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_longlong)]),
     ("pfc_batch_graph_stats", ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_longlong)]),
     ("pfc_plan_bytes", ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_longlong)]),
     ("pfc_batch_create", ctypes.c_int, [ctypes.POINTER(PfcParams), ctypes.c_int, ctypes.POINTER(_P)]),

@@ -409,7 +409,7 @@ class Group:
     Ny in {32, 48, 64, 96, 128, 192, 256, 384, 512}; they stay usable on
     their own (set fields, reduce, read).  The group keeps its members alive."""
 
-    def __init__(self, batches):
+    def __init__(self, batches, flow=0):
         self._lib = _lib.load()
         self.torch = _torch()
         self.batches = list(batches)
@@ -417,6 +417,21 @@ class Group:
         h = ctypes.c_void_p()
         _lib.check(self._lib.pfc_group_create(arr, len(self.batches), ctypes.byref(h)))
         self._h = h
+        if flow:
+            self.set_flow(flow)
+
+    def set_flow(self, steps_per_launch):
+        """Persistent form (pfc_group_set_flow): each register class steps in
+        one launch per `steps_per_launch` steps over a work list of tiles with
+        per-instance dependencies; 0 returns to the per-pass launches."""
+        _lib.check(self._lib.pfc_group_set_flow(self._h, int(steps_per_launch)))
+        return self
+
+    def flow_stats(self):
+        """(launches so far, blocks per class-0 launch, blocks per class-1 launch, steps per launch)."""
+        v = (ctypes.c_longlong * 4)()
+        _lib.check(self._lib.pfc_group_flow_stats(self._h, v))
+        return tuple(int(x) for x in v)
 
     def step(self, nsteps=1):
         _lib.check(self._lib.pfc_group_step(self._h, int(nsteps), current_stream_handle()))

@@ -172,9 +172,12 @@ class BatchedSweep:
     overlap on the device.  grouped=True advances every size through one
     Group instead: per step one row and one column launch per register class
     over every size's tiles (variant-0 schedules); on the 256-instance sweep
-    this measured slower, so it is not the default."""
+    this measured slower, so it is not the default.  grouped="flow" steps the
+    Group in its persistent form: one launch per register class per
+    `flow_steps` steps over a work list of every size's tiles, each instance
+    following its own chain of passes (pfc_group_set_flow)."""
 
-    def __init__(self, instances, dt=0.5, precision=32, scheme="etd1", grouped=False):
+    def __init__(self, instances, dt=0.5, precision=32, scheme="etd1", grouped=False, flow_steps=256):
         import torch
         from .plan import Batch
         self.instances = list(instances)
@@ -205,7 +208,8 @@ class BatchedSweep:
         self.group = None
         if grouped:
             from .plan import Group
-            self.group = Group([self.batches[n] for n in sorted(self.batches, reverse=True)])
+            self.group = Group([self.batches[n] for n in sorted(self.batches, reverse=True)],
+                               flow=flow_steps if grouped == "flow" else 0)
 
     def step(self, steps):
         import torch
@@ -238,13 +242,13 @@ class BatchedSweep:
             b.close()
 
 
-def run_batched(instances, steps, dt=0.5, precision=32, scheme="etd1", chunk=None, grouped=False):
+def run_batched(instances, steps, dt=0.5, precision=32, scheme="etd1", chunk=None, grouped=False, flow_steps=256):
     """All instances at once; the wall time covers setup (allocation, tables,
     upload, kernel tuning), stepping and the reductions, as run_plain's does."""
     import torch
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    sw = BatchedSweep(instances, dt=dt, precision=precision, scheme=scheme, grouped=grouped)
+    sw = BatchedSweep(instances, dt=dt, precision=precision, scheme=scheme, grouped=grouped, flow_steps=flow_steps)
     done = 0
     chunk = chunk or steps
     while done < steps:
