@@ -1614,10 +1614,11 @@ struct pfc_group {
     int nmem = 0, per_step = 0, ninst = 0, blocks = 0;
     size_t lds = 0;
     size_t desc_off = 0, idx_off = 0, cnt_off = 0;  // bytes into fdesc; ints into fidx; words into fcnt
+    size_t tab_off = 0;                               // entries into ftab
     size_t abort_off = 0;                             // word of the class's abort flag in fcnt
   };
   FlowClass flow[kGroupClasses];
-  DevBuf fdesc, fidx, fcnt;
+  DevBuf fdesc, fidx, fcnt, ftab;
   std::vector<long long> flow_built;
   hipStream_t flow_st2 = nullptr;
   hipEvent_t flow_fork = nullptr, flow_join = nullptr;
@@ -1820,6 +1821,7 @@ void flow_build(pfc_group& g, hipStream_t st) {
   hip_check(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev), "device attribute");
   std::vector<GroupMember<T>> desc(kGroupClasses * 2 * n);
   std::vector<int> idx(kGroupClasses * ((size_t)4 * n + 1 + n + 1));
+  std::vector<FlowEntry> tab;
   size_t cnt_words = 0;
   for (int c = 0; c < kGroupClasses; ++c) {
     pfc_group::FlowClass& F = g.flow[c];
@@ -1876,6 +1878,26 @@ void flow_build(pfc_group& g, hipStream_t st) {
       }
     seg[nph * F.nmem] = at;
     F.per_step = at;
+    // the step's tickets in work-list order
+    F.tab_off = tab.size();
+    for (int p = 0; p < nph; ++p)
+      for (int k = 0; k < F.nmem; ++k) {
+        const int nr = rows[k].ntile, nc = cols[k].ntile;
+        const int ntile = (p & 1) == 0 ? nr : nc;
+        for (int yi = 0; yi < cnt0[k + 1] - cnt0[k]; ++yi)
+          for (int t = 0; t < ntile; ++t) {
+            FlowEntry en{};
+            en.p = p;
+            en.m = k;
+            en.yi = yi;
+            en.tile = t;
+            en.cidx = cnt0[k] + yi;
+            en.pre = (unsigned)((p >> 1) * (nr + nc) + (p & 1) * nr);
+            en.per_inst = (unsigned)(nph / 2 * (nr + nc));
+            tab.push_back(en);
+          }
+      }
+    if ((long long)(tab.size() - F.tab_off) != at) fail(PFC_EUNSUPPORTED, "flow work list size mismatch");
     F.ninst = cnt0[F.nmem] - 1;
     F.cnt_off = cnt_words;
     cnt_words += (size_t)(1 + F.ninst) * kFlowCntStride;
@@ -1888,6 +1910,7 @@ void flow_build(pfc_group& g, hipStream_t st) {
   if (g.fdesc.n < desc.size() * sizeof(GroupMember<T>)) g.fdesc.alloc(desc.size() * sizeof(GroupMember<T>));
   hip_check(hipMemcpy(g.fdesc.p, desc.data(), desc.size() * sizeof(GroupMember<T>), hipMemcpyHostToDevice), "copy");
   g.fidx.upload(idx);
+  g.ftab.upload(tab);
   // counters, then one abort flag per class on its own 256-byte line
   for (int c = 0; c < kGroupClasses; ++c) g.flow[c].abort_off = cnt_words + (size_t)c * kFlowCntStride;
   const size_t words = cnt_words + (size_t)kGroupClasses * kFlowCntStride;
@@ -1909,8 +1932,8 @@ void flow_launch(pfc_group& g, int c, long long steps, hipStream_t st) {
   const GroupMember<T>* base = (const GroupMember<T>*)((const char*)g.fdesc.p + F.desc_off);
   f.rows = base;
   f.cols = base + n;
-  f.seg = (const int*)g.fidx.p + F.idx_off;
-  f.cnt0 = f.seg + nph * F.nmem + 1;
+  f.cnt0 = (const int*)g.fidx.p + F.idx_off + nph * F.nmem + 1;
+  f.tab = (const FlowEntry*)g.ftab.p + F.tab_off;
   unsigned* words = (unsigned*)g.fcnt.p;
   f.cnt = words + F.cnt_off;
   f.ticket = f.cnt;

@@ -952,11 +952,21 @@ constexpr int kFlowCntStride = 64;      // counters 256 bytes apart
 constexpr int kFlowSpinLimit = 1 << 21;  // polls of one wait (~1 s) before the launch aborts
 constexpr int kStatusAborted = 2;        // PFC_STATUS_ABORTED
 
+// A ticket's place within a step, precomputed on the host (32 bytes, one
+// scalar load): phase, member, active instance, tile, the instance's counter
+// index, the counter value its step's earlier phases add up to, and the
+// instance's tiles per step.
+struct FlowEntry {
+  int p, m, yi, tile, cidx;
+  unsigned pre, per_inst;
+  int pad;
+};
+
 template <typename T> struct FlowPlan {
   const GroupMember<T>* rows;  // [nmem] row-pass descriptors
   const GroupMember<T>* cols;  // [nmem] column-pass descriptors
-  const int* seg;              // [nph * nmem + 1]: first ticket of (phase, member) within a step
   const int* cnt0;             // [nmem + 1]: first instance counter of each member
+  const FlowEntry* tab;        // [per_step]: each ticket's place in a step (one scalar load to decode)
   unsigned* ticket;            // next ticket (cnt[0]), zero at launch
   unsigned* cnt;               // per-instance completed tiles, kFlowCntStride apart, zero at launch
   int* abort;                  // a wait exceeded its bound (cleared by k_flow_mark)
@@ -1033,33 +1043,28 @@ struct FlowTicket {
   unsigned* c;    // the instance's counter
 };
 
-// Integer quotients are computed in vector registers; readfirstlane returns
-// them to scalar ones, so the member's descriptor is read with scalar loads.
+// The step's quotient is computed in vector registers; readfirstlane returns
+// it to a scalar one, so the entry (and the member's descriptor found from
+// it) is read with scalar loads.
 template <typename T> __device__ __forceinline__ FlowTicket flow_decode(const FlowPlan<T>& f, unsigned t) {
   FlowTicket k;
   k.s = __builtin_amdgcn_readfirstlane(t / (unsigned)f.per_step);
-  const int r = (int)(t - k.s * (unsigned)f.per_step);
-  int gs = 0;
-  while (f.seg[gs + 1] <= r) ++gs;  // seg[nph * nmem] = per_step > r
-  k.p = __builtin_amdgcn_readfirstlane(gs / f.nmem);
-  k.m = gs - k.p * f.nmem;
-  const int nr = f.rows[k.m].ntile, nc = f.cols[k.m].ntile;
-  const int ntile = (k.p & 1) == 0 ? nr : nc;
-  const int local = r - f.seg[gs];
-  k.yi = __builtin_amdgcn_readfirstlane(local / ntile);
-  k.tile = local - k.yi * ntile;
+  const FlowEntry e = f.tab[t - k.s * (unsigned)f.per_step];
+  k.p = e.p;
+  k.m = e.m;
+  k.yi = e.yi;
+  k.tile = e.tile;
   // the instance's tiles of every earlier phase: s whole steps, then the
   // phases of this step before p
-  const unsigned per_inst = (unsigned)(f.nph / 2) * (unsigned)(nr + nc);
-  k.need = k.s * per_inst + (unsigned)(k.p >> 1) * (unsigned)(nr + nc) + (unsigned)(k.p & 1) * (unsigned)nr;
-  k.c = f.cnt + (size_t)(f.cnt0[k.m] + k.yi) * kFlowCntStride;
+  k.need = k.s * e.per_inst + e.pre;
+  k.c = f.cnt + (size_t)e.cidx * kFlowCntStride;
   return k;
 }
 
 // Thread 0 keeps tickets two deep: while the block runs ticket t it knows
-// the next one (tn), has the one after in flight (atomicAdd, waited for only
-// at the tile's end) and has tn's counter polled, so a ready tile starts
-// without a round trip to memory.
+// the next one (tn) and has the one after in flight (atomicAdd, waited for
+// only at the tile's end); tn's counter is polled under the tile's final
+// store drain, so a ready tile starts without a round trip to memory.
 template <typename T, int CLASS, bool RK2>
 __global__ void __launch_bounds__(kGroupThreads, (flow_waves<T, CLASS>())) k_flow(FlowPlan<T> f) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -1078,14 +1083,11 @@ __global__ void __launch_bounds__(kGroupThreads, (flow_waves<T, CLASS>())) k_flo
   unsigned early = 0;  // thread 0: t's counter as polled during the previous tile
   while (t < f.total) {
     const FlowTicket k = flow_decode(f, t);
-    unsigned nxt = 0, e2 = 0;  // thread 0: the ticket after tn, tn's counter (both in flight)
+    unsigned nxt = 0;  // thread 0: the ticket after tn (in flight)
     if (threadIdx.x == 0) {
       const bool ok = k.need == 0 || early >= k.need || flow_wait(k.c, k.need, f.abort);
       s_go = ok;
-      if (ok) {
-        nxt = atomicAdd(f.ticket, 1u);
-        if (tn < f.total) e2 = __hip_atomic_load(flow_decode(f, tn).c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      }
+      if (ok) nxt = atomicAdd(f.ticket, 1u);
     }
     __syncthreads();
     if (!s_go) break;
@@ -1108,6 +1110,10 @@ __global__ void __launch_bounds__(kGroupThreads, (flow_waves<T, CLASS>())) k_flo
         flow_col<T, CLASS, COL_ETD1>(f.cols[k.m], k.yi, k.tile, smem_raw, tid);
       }
     }
+    // tn's counter, polled now so its latency hides under the drain below
+    unsigned e2 = 0;
+    if (threadIdx.x == 0 && tn < f.total)
+      e2 = __hip_atomic_load(flow_decode(f, tn).c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     // every thread's stores have completed (written through) before the
     // tile counts as done
     __builtin_amdgcn_s_waitcnt(0);
