@@ -166,7 +166,8 @@ int pfc_diagnose(pfc_plan* plan, const int* ky, const int* kx, int nent, double*
  * instrumentation lives outside the model code, README.md:59-60.) */
 int pfc_step_profile(pfc_plan* plan, long long nsteps, void* stream, double ms[3], long long calls[3]);
 /* Kernel schedule in use (after the first step has run the autotuner):
- * info[0..3] = row kernel  {compile-time schedule in use, variant, points per lane, rows per block},
+ * info[0..3] = row kernel  {compile-time schedule in use, variant (+16: the FP32 form held to three
+ *               waves per SIMD), points per lane, rows per block},
  * info[4..7] = column kernel {same, columns per block}.  A zero in info[0] or
  * info[4] means the generic mixed-radix kernel runs; -1 in both means the
  * resident stepper runs (small ETD1 instances: one workgroup per instance

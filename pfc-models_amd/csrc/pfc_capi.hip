@@ -215,7 +215,10 @@ std::vector<SchedInfo> schedules_for(int L) {
 struct FastShape {
   bool on = false;
   bool nt = true;   // non-temporal output stores (else ordinary)
+  bool cap = false; // the kernel form held to kCapWaves waves per SIMD (FP32, <= 8 points per lane)
   int var = 0;      // schedule variant
+  // the launchers' variant argument: the variant, with kCapFlag for the capped form
+  int code() const { return var | (cap ? kCapFlag : 0); }
   int P = 0;        // points per lane
   int tpl = 0;      // lanes per line
   int lines = 0;    // lines per block
@@ -253,30 +256,35 @@ FastShape col_shape(int Nx, int Ny, size_t csize, const SchedInfo& si, int lines
 
 // Candidate (variant, lines) shapes; the default is variant 0 with enough
 // blocks to cover the 256 CUs.
+// FP32 schedules of up to 8 points per lane also come held to kCapWaves
+// waves per SIMD (see kCapWaves): both forms are candidates.
+void add_shape(std::vector<FastShape>& out, FastShape f, size_t csize) {
+  if (!f.on) return;
+  out.push_back(f);
+  if (csize == sizeof(V2<float>) && f.P <= 8) {
+    f.cap = true;
+    out.push_back(f);
+  }
+}
+
 std::vector<FastShape> row_candidates(int Nx, int Ny, size_t csize) {
   std::vector<FastShape> out;
   for (const SchedInfo& si : schedules_for(Nx / 2))
-    for (int w = 1; w <= 256; w *= 2) {
-      FastShape f = row_shape(Nx, Ny, csize, si, w);
-      if (f.on) out.push_back(f);
-    }
+    for (int w = 1; w <= 256; w *= 2) add_shape(out, row_shape(Nx, Ny, csize, si, w), csize);
   return out;
 }
 
 std::vector<FastShape> col_candidates(int Nx, int Ny, size_t csize) {
   std::vector<FastShape> out;
   for (const SchedInfo& si : schedules_for(Ny))
-    for (int c = 1; c <= 256; c *= 2) {
-      FastShape f = col_shape(Nx, Ny, csize, si, c);
-      if (f.on) out.push_back(f);
-    }
+    for (int c = 1; c <= 256; c *= 2) add_shape(out, col_shape(Nx, Ny, csize, si, c), csize);
   return out;
 }
 
 FastShape pick_default(const std::vector<FastShape>& cands, int target_blocks) {
   FastShape best;
   for (const FastShape& f : cands) {
-    if (f.var != 0) continue;
+    if (f.var != 0 || f.cap) continue;
     // most lines per block that still gives >= target_blocks blocks, else the fewest lines
     if (!best.on) {
       best = f;
@@ -300,13 +308,13 @@ template <typename T> struct FastLaunch {
   static void row(int M, const FastShape& f, int ny_inst, hipStream_t st, const V2<T>* S, V2<T>* Tout, int Ny,
                   const Lay& lay, const Nonlin& q, const V2<double>* tw, const Slots& sl, RangePart* rpart = nullptr,
                   const RangeCheck& chk = RangeCheck{}) {
-    if (!row_fast_launch<T>(M, f.var, f.nt, dim3(f.blocks, ny_inst), dim3(f.lines * f.tpl), f.lds, st, S, Tout, Ny,
+    if (!row_fast_launch<T>(M, f.code(), f.nt, dim3(f.blocks, ny_inst), dim3(f.lines * f.tpl), f.lds, st, S, Tout, Ny,
                             lay, q, tw, sl, rpart, chk))
       fail(PFC_EUNSUPPORTED, "no fast row schedule");
   }
   static void col(int Ny, int mode, const FastShape& f, int ny_inst, hipStream_t st, V2<T>* S, const V2<T>* Tin,
                   V2<T>* D, int Nc, const Lay& lay, const ColTables& tb, const V2<double>* tw, const Slots& sl) {
-    if (!col_fast_launch<T>(Ny, f.var, mode, f.nt, dim3(f.blocks, ny_inst), dim3(f.lines * f.tpl), f.lds, st, S, Tin,
+    if (!col_fast_launch<T>(Ny, f.code(), mode, f.nt, dim3(f.blocks, ny_inst), dim3(f.lines * f.tpl), f.lds, st, S, Tin,
                             D, Nc, lay, tb, tw, sl))
       fail(PFC_EUNSUPPORTED, "no fast column schedule");
   }
@@ -477,15 +485,16 @@ void engine_init(Engine& e, const pfc_params* params, int cap, bool batched) {
     e.frow = pick_default(rc, 512);
     e.fcol = pick_default(cc, 256);
     // PFC_HIP_SCHEDULE="rowvar:rowlines:colvar:collines" pins a schedule
-    // (tests and tuning studies); the autotuner is then skipped.
+    // (tests and tuning studies; a variant + 16 is its capped form); the
+    // autotuner is then skipped.
     if (const char* fx = std::getenv("PFC_HIP_SCHEDULE")) {
       int v[4] = {-1, -1, -1, -1};
       if (std::sscanf(fx, "%d:%d:%d:%d", &v[0], &v[1], &v[2], &v[3]) == 4) {
         bool okr = false, okc = false;
         for (const FastShape& f : rc)
-          if (f.var == v[0] && f.lines == v[1]) e.frow = f, okr = true;
+          if (f.code() == v[0] && f.lines == v[1]) e.frow = f, okr = true;
         for (const FastShape& f : cc)
-          if (f.var == v[2] && f.lines == v[3]) e.fcol = f, okc = true;
+          if (f.code() == v[2] && f.lines == v[3]) e.fcol = f, okc = true;
         if (!okr && e.frow.on) fail(PFC_EINVAL, "PFC_HIP_SCHEDULE: no such row schedule");
         if (!okc && e.fcol.on) fail(PFC_EINVAL, "PFC_HIP_SCHEDULE: no such column schedule");
         e.tuned = true;
@@ -832,7 +841,7 @@ void autotune(Engine& e, hipStream_t st) {
   std::lock_guard<std::mutex> lk(g_tune_mu);
   auto apply = [](FastShape& cur, const std::vector<FastShape>& cands, std::pair<int, int> vl) {
     for (const FastShape& f : cands)
-      if (f.var == vl.first && f.lines == vl.second) cur = f;
+      if (f.code() == vl.first && f.lines == vl.second) cur = f;
   };
   // scratch arrays for every slot index the launches may touch
   const size_t bytes = (size_t)e.cap * e.nc * sizeof(V);
@@ -850,7 +859,7 @@ void autotune(Engine& e, hipStream_t st) {
         FastLaunch<T>::row(M, f, ny, st, (const V*)e.S.p, (V*)T2.p, Ny, e.lay, q, (const V2<double>*)e.twN64.p, sl);
       });
       e.frow = pick;
-      g_tune[key] = {pick.var, pick.lines};
+      g_tune[key] = {pick.code(), pick.lines};
     }
   }
   if (e.fcol.on) {
@@ -873,7 +882,7 @@ void autotune(Engine& e, hipStream_t st) {
                            (const V2<double>*)e.twY64.p, sl);
       });
       e.fcol = pick;
-      g_tune[key] = {pick.var, pick.lines};
+      g_tune[key] = {pick.code(), pick.lines};
       hip_check(hipStreamSynchronize(st), "sync after tuning");
     }
   }
@@ -965,8 +974,8 @@ bool graphs_enabled() {
 hipGraphExec_t step_graph(Engine& e, bool capture) {
   // par: the in-step decision's parity at the graph's first step (an even
   // number of steps returns it there)
-  const std::array<long long, 11> key{e.nact(),          e.frow.on, e.frow.var, e.frow.lines,
-                                      e.fcol.on,         e.fcol.var, e.fcol.lines,
+  const std::array<long long, 11> key{e.nact(),          e.frow.on, e.frow.code(), e.frow.lines,
+                                      e.fcol.on,         e.fcol.code(), e.fcol.lines,
                                       (long long)(intptr_t)e.extc.p, (long long)(intptr_t)e.D.p, e.auto_shift,
                                       e.par};
   auto it = e.graphs.find(key);
@@ -2248,7 +2257,7 @@ int pfc_plan_schedule(const pfc_plan* pl, int info[8]) {
     const FastShape* fs[2] = {&pl->e.frow, &pl->e.fcol};
     for (int k = 0; k < 2; ++k) {
       info[4 * k + 0] = pl->e.resident ? -1 : fs[k]->on ? 1 : 0;
-      info[4 * k + 1] = fs[k]->var;
+      info[4 * k + 1] = fs[k]->code();
       info[4 * k + 2] = fs[k]->P;
       info[4 * k + 3] = fs[k]->lines;
     }

@@ -469,8 +469,17 @@ __device__ __forceinline__ void row_fast_tile(const V2<T>* __restrict__ Sg, V2<T
 }
 
 // RANGE: rpart holds [instance][tile] range pairs (row_fast_tile)
-template <typename T, class S, bool RANGE, bool NT>
-__global__ void __launch_bounds__(256) k_row_fast(const V2<T>* __restrict__ Sg, V2<T>* __restrict__ Tout, int Ny,
+// WAVES > 0: the kernel is compiled for that many waves per SIMD (its
+// registers held to 512 / WAVES); 0 leaves the compiler's choice.  FP32
+// schedules of up to 8 points per lane come in both forms, the autotuner
+// picks (a 2048-point FP32 column at its own 171 VGPRs fits two one-column
+// blocks per CU, so its 1025 columns take three rounds; held to 168 they
+// take two, with a few spilled registers).
+constexpr int kCapWaves = 3;
+constexpr int kCapFlag = 16;  // in a launcher's `var`: the WAVES = kCapWaves form
+
+template <typename T, class S, bool RANGE, bool NT, int WAVES = 0>
+__global__ void __launch_bounds__(256, (WAVES > 0 ? WAVES : 1)) k_row_fast(const V2<T>* __restrict__ Sg, V2<T>* __restrict__ Tout, int Ny,
                                                   Lay lay, Nonlin q, const V2<double>* __restrict__ twN,
                                                   Slots sl, RangePart* rpart, RangeCheck chk) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -684,8 +693,8 @@ __device__ __forceinline__ void col_fast_tile(V2<T>* __restrict__ Sg, const V2<T
   }
 }
 
-template <typename T, class S, int MODE, bool NT>
-__global__ void __launch_bounds__(256) k_col_fast(V2<T>* __restrict__ Sg, const V2<T>* __restrict__ Tin,
+template <typename T, class S, int MODE, bool NT, int WAVES = 0>
+__global__ void __launch_bounds__(256, (WAVES > 0 ? WAVES : 1)) k_col_fast(V2<T>* __restrict__ Sg, const V2<T>* __restrict__ Tin,
                                                   V2<T>* __restrict__ D, int Nc, Lay lay, ColTables tb,
                                                   const V2<double>* __restrict__ twY, Slots sl) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -741,13 +750,25 @@ bool row_fast_launch_impl(int M, int var, dim3 grid, dim3 block, size_t lds, hip
                           V2<T>* Tout, int Ny, const Lay& lay, const Nonlin& q, const V2<double>* tw, const Slots& sl,
                           RangePart* rpart, const RangeCheck& chk) {
 #define PFC_X(LEN, VAR, PP, ...)                                                                                 \
-  if (M == LEN && var == VAR) {                                                                                  \
+  if (M == LEN && (var & (kCapFlag - 1)) == VAR) {                                                               \
+    using Sc = Sch<LEN, PP, __VA_ARGS__>;                                                                        \
+    if constexpr (sizeof(T) == 4 && PP <= 8) {                                                                   \
+      if (var & kCapFlag) {                                                                                      \
+        if (rpart)                                                                                               \
+          hipLaunchKernelGGL((k_row_fast<T, Sc, true, NT, kCapWaves>), grid, block, lds, st, S, Tout, Ny, lay, q, \
+                             tw, sl, rpart, chk);                                                                \
+        else                                                                                                     \
+          hipLaunchKernelGGL((k_row_fast<T, Sc, false, NT, kCapWaves>), grid, block, lds, st, S, Tout, Ny, lay,   \
+                             q, tw, sl, rpart, chk);                                                             \
+        return true;                                                                                             \
+      }                                                                                                          \
+    }                                                                                                            \
     if (rpart)                                                                                                   \
-      hipLaunchKernelGGL((k_row_fast<T, Sch<LEN, PP, __VA_ARGS__>, true, NT>), grid, block, lds, st, S, Tout, Ny, \
-                         lay, q, tw, sl, rpart, chk);                                                            \
+      hipLaunchKernelGGL((k_row_fast<T, Sc, true, NT>), grid, block, lds, st, S, Tout, Ny, lay, q, tw, sl, rpart, \
+                         chk);                                                                                   \
     else                                                                                                         \
-      hipLaunchKernelGGL((k_row_fast<T, Sch<LEN, PP, __VA_ARGS__>, false, NT>), grid, block, lds, st, S, Tout,    \
-                         Ny, lay, q, tw, sl, rpart, chk);                                                        \
+      hipLaunchKernelGGL((k_row_fast<T, Sc, false, NT>), grid, block, lds, st, S, Tout, Ny, lay, q, tw, sl,      \
+                         rpart, chk);                                                                            \
     return true;                                                                                                 \
   }
   PFC_SCHEDULES(PFC_X)
@@ -760,8 +781,22 @@ bool col_fast_launch_impl(int Ny, int var, int mode, dim3 grid, dim3 block, size
                           const V2<T>* Tin, V2<T>* D, int Nc, const Lay& lay, const ColTables& tb,
                           const V2<double>* tw, const Slots& sl) {
 #define PFC_X(LEN, VAR, PP, ...)                                                                                 \
-  if (Ny == LEN && var == VAR) {                                                                                 \
+  if (Ny == LEN && (var & (kCapFlag - 1)) == VAR) {                                                              \
     using Sc = Sch<LEN, PP, __VA_ARGS__>;                                                                        \
+    if constexpr (sizeof(T) == 4 && PP <= 8) {                                                                   \
+      if (var & kCapFlag) {                                                                                      \
+        if (mode == COL_ETD1)                                                                                    \
+          hipLaunchKernelGGL((k_col_fast<T, Sc, COL_ETD1, NT, kCapWaves>), grid, block, lds, st, S, Tin, D, Nc,   \
+                             lay, tb, tw, sl);                                                                   \
+        else if (mode == COL_RK2_STAGE1)                                                                         \
+          hipLaunchKernelGGL((k_col_fast<T, Sc, COL_RK2_STAGE1, NT, kCapWaves>), grid, block, lds, st, S, Tin, D, \
+                             Nc, lay, tb, tw, sl);                                                               \
+        else                                                                                                     \
+          hipLaunchKernelGGL((k_col_fast<T, Sc, COL_RK2_STAGE2, NT, kCapWaves>), grid, block, lds, st, S, Tin, D, \
+                             Nc, lay, tb, tw, sl);                                                               \
+        return true;                                                                                             \
+      }                                                                                                          \
+    }                                                                                                            \
     if (mode == COL_ETD1)                                                                                        \
       hipLaunchKernelGGL((k_col_fast<T, Sc, COL_ETD1, NT>), grid, block, lds, st, S, Tin, D, Nc, lay, tb, tw, sl); \
     else if (mode == COL_RK2_STAGE1)                                                                             \

@@ -216,7 +216,8 @@ class Plan:
         return v.value
 
     def schedule(self):
-        """Kernel schedule in use: {'row': (fast, variant, P, lines), 'col': (...)}."""
+        """Kernel schedule in use: {'row': (fast, variant, P, lines), 'col': (...)}; variant + 16 is
+        the FP32 kernel form held to three waves per SIMD."""
         info = (ctypes.c_int * 8)()
         _lib.check(self._lib.pfc_plan_schedule(self._h, info))
         return {"row": tuple(info[0:4]), "col": tuple(info[4:8])}

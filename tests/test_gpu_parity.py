@@ -261,3 +261,27 @@ def test_every_schedule_matches_generic(hip_device, shape, precision, monkeypatc
         assert sched["col"] == (1, cv, dict(SCHEDULES[ny])[cv], cl)
         d = np.abs(pl.get_field().cpu().numpy() - ref).max()
         assert d <= tol, (rv, rl, cv, cl, d)
+
+
+@pytest.mark.parametrize("shape", [(1024, 1024), (256, 512)])
+def test_capped_kernel_forms_are_bitwise_equal(hip_device, shape, monkeypatch):
+    """The FP32 kernel forms held to three waves per SIMD (variant + 16, an
+    autotuner candidate for schedules of up to 8 points per lane) compute the
+    same ETDRK2 step as their uncapped twins, bit for bit: only the register
+    allocation differs."""
+    from pfc_hip.plan import Plan
+    ny, nx = shape
+    psi0 = F.uniform_noise(shape, -0.3, 0.05, seed=9)
+    kw = dict(Nx=nx, Ny=ny, Lx=nx * np.pi / 4, Ly=ny * np.pi / 4, epsilon=-0.3, g=0.2, dt=0.5, scheme="etdrk2",
+              precision=32)
+    lpad = lambda i: i + (i >> 4)
+    rows = [s for s in _shapes(nx // 2, lambda w: w * (lpad(nx // 2) + 2), 8) if dict(SCHEDULES[nx // 2])[s[0]] <= 8]
+    cols = [s for s in _shapes(ny, lambda w: w * (lpad(ny - 1) + 2), 8) if dict(SCHEDULES[ny])[s[0]] <= 8]
+    for (rv, rl), (cv, cl) in [(r, cols[0]) for r in rows] + [(rows[0], c) for c in cols[1:]]:
+        fields = []
+        for cap in (0, 16):
+            monkeypatch.setenv("PFC_HIP_SCHEDULE", f"{rv + cap}:{rl}:{cv + cap}:{cl}")
+            pl = Plan(**kw).set_field(psi0).step(6)
+            assert pl.schedule()["row"][1] == rv + cap and pl.schedule()["col"][1] == cv + cap
+            fields.append(pl.get_field().cpu().numpy())
+        assert np.array_equal(fields[0], fields[1]), (rv, rl, cv, cl)
