@@ -396,6 +396,43 @@ def run_logpfc(steps=10000, n=2048):
     return out
 
 
+def run_sweep_forms(steps=2000):
+    """The sweep's three forms on subsets of the 256 instances (FP32 ETD1,
+    whole job as in the sweep leg, us per sweep step): per-size Batches on
+    their own streams (autotuned, and with the autotuner off, i.e. the
+    variant-0 schedules the group forms run), the per-pass Group, and the
+    Group's persistent form (one launch per register class per `steps`
+    steps).  It separates what the persistent form's rate owes to the
+    launch structure from what it owes to the per-size schedule choice."""
+    from pfc_hip import sweep as SW
+    inst = SW.sweep_instances()
+    subsets = {"all": inst, "class0": [i for i in inst if i.N in (64, 128, 256, 512)],
+               "class1": [i for i in inst if i.N in (192, 384)], "512": [i for i in inst if i.N == 512],
+               "256": [i for i in inst if i.N == 256]}
+    forms = [("per_size_tuned", False, None), ("per_size_variant0", False, "0"), ("per_pass_group", True, None),
+             ("persistent", "flow", None)]
+    out = {"unit": "us per sweep step", "steps": steps}
+    for name, sub in subsets.items():
+        row = {"instances": len(sub)}
+        for form, grouped, tune in forms:
+            old = os.environ.get("PFC_HIP_TUNE")
+            if tune is not None:
+                os.environ["PFC_HIP_TUNE"] = tune
+            try:
+                SW.run_batched(sub, 10, precision=32, grouped=grouped, flow_steps=steps)
+                r = SW.run_batched(sub, steps, precision=32, grouped=grouped, flow_steps=steps)
+            finally:
+                if tune is not None:
+                    if old is None:
+                        os.environ.pop("PFC_HIP_TUNE", None)
+                    else:
+                        os.environ["PFC_HIP_TUNE"] = old
+            row[form] = round(r.wall_s / steps * 1e6, 1)
+        out[name] = row
+        print(f"sweep forms {name}: {row}", file=sys.stderr, flush=True)
+    return out
+
+
 def run_phase_diagram(n_dens=12, serial_every=1):
     """q0* over a density list with the phase-diagram driver: logPFC triangle,
     beta 0.5, g -2, eps -0.9 on the 28 x 24 cell with the driver tests'
@@ -463,6 +500,9 @@ def main():
     ap.add_argument("--sweep", action="store_true",
                     help="only run the 256-instance sweep leg and print its JSON line")
     ap.add_argument("--sweep-steps", type=int, default=5000, help="steps per sweep instance (short profiling runs)")
+    ap.add_argument("--sweep-forms", action="store_true",
+                    help="only time the sweep's per-size, per-pass group and persistent forms on subsets "
+                         "(2000 steps) and print their JSON line")
     ap.add_argument("--no-logpfc", action="store_true", help="skip the logPFC 2048^2 leg")
     ap.add_argument("--logpfc", action="store_true",
                     help="only run the logPFC 2048^2 stabilised leg and print its JSON line")
@@ -481,6 +521,9 @@ def main():
         return
     if a.sweep:
         print(json.dumps({"sweep": run_sweep(steps=a.sweep_steps)}), flush=True)
+        return
+    if a.sweep_forms:
+        print(json.dumps({"sweep_forms": run_sweep_forms()}), flush=True)
         return
     if a.logpfc:
         print(json.dumps({"logpfc": run_logpfc(steps=a.logpfc_steps)}), flush=True)
