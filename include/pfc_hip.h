@@ -296,8 +296,8 @@ int pfc_group_chunks(const pfc_group* group, int* nchunks);
  * launches run side by side.  Tiles run the same transforms as the
  * per-pass launches, so results are bit-identical to them.  Progress does
  * not depend on how many blocks are resident.  A dependency wait that
- * exceeds its bound (~1 s; not expected) stops the launch and flags every
- * member slot PFC_STATUS_ABORTED.  Needs each member's Nx/2 and Ny in one
+ * exceeds its bound (~1 s; not expected; PFC_HIP_FLOW_SPIN sets the number
+ * of polls) stops the launch and flags every member slot PFC_STATUS_ABORTED.  Needs each member's Nx/2 and Ny in one
  * register class (PFC_EUNSUPPORTED otherwise).  0: the per-pass launches.
  * pfc_group_flow_stats: out[0] launches so far, out[1..2] blocks per launch
  * of classes 0 and 1 (resident capacity), out[3] steps per launch.

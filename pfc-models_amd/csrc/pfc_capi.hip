@@ -1952,6 +1952,10 @@ void flow_launch(pfc_group& g, int c, long long steps, hipStream_t st) {
   f.per_step = F.per_step;
   f.total = (unsigned)(steps * F.per_step);
   f.nwords = (1 + F.ninst) * kFlowCntStride;
+  // PFC_HIP_FLOW_SPIN: polls of one dependency wait before the launch aborts
+  // (tests force the abort path with 0; the default bounds a wait to ~1 s)
+  const char* spin = std::getenv("PFC_HIP_FLOW_SPIN");
+  f.spin_limit = spin ? std::max(0, std::atoi(spin)) : kFlowSpinLimit;
   const int blocks = (int)std::max(1LL, std::min<long long>(F.blocks, (long long)f.total));
   if constexpr (sizeof(T) == 8)
     flow_launch_f64(c, nph == 4, blocks, F.lds, st, f);

@@ -984,7 +984,7 @@ void group_launch_f32(int kind, int cls, int blocks, size_t lds, hipStream_t st,
 // PFC_STATUS_ABORTED.
 // ---------------------------------------------------------------------------
 constexpr int kFlowCntStride = 64;      // counters 256 bytes apart
-constexpr int kFlowSpinLimit = 1 << 21;  // polls of one wait (~1 s) before the launch aborts
+constexpr int kFlowSpinLimit = 1 << 21;  // default polls of one wait (~1 s) before the launch aborts
 constexpr int kStatusAborted = 2;        // PFC_STATUS_ABORTED
 
 // A ticket's place within a step, precomputed on the host (32 bytes, one
@@ -1008,6 +1008,7 @@ template <typename T> struct FlowPlan {
   int nmem, nph, per_step;
   unsigned total;              // tickets of the launch: steps * per_step
   int nwords;                  // words from cnt[0] that k_flow_reset clears
+  int spin_limit;              // polls of one wait before the launch aborts
 };
 
 // before a flow launch: the ticket and the instance counters to zero (a fill
@@ -1018,11 +1019,11 @@ __global__ void k_flow_reset(U* cnt, int nwords) {
 }
 
 // thread 0: wait until *c >= need; false if the launch aborted
-__device__ __forceinline__ bool flow_wait(const unsigned* c, unsigned need, int* abort) {
+__device__ __forceinline__ bool flow_wait(const unsigned* c, unsigned need, int* abort, int limit) {
   for (int i = 0;; ++i) {
     if (__hip_atomic_load(c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= need) return true;
     if ((i & 31) == 31 && __hip_atomic_load(abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return false;
-    if (i >= kFlowSpinLimit) {
+    if (i >= limit) {
       __hip_atomic_store(abort, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       return false;
     }
@@ -1120,7 +1121,7 @@ __global__ void __launch_bounds__(kGroupThreads, (flow_waves<T, CLASS>())) k_flo
     const FlowTicket k = flow_decode(f, t);
     unsigned nxt = 0;  // thread 0: the ticket after tn (in flight)
     if (threadIdx.x == 0) {
-      const bool ok = k.need == 0 || early >= k.need || flow_wait(k.c, k.need, f.abort);
+      const bool ok = k.need == 0 || early >= k.need || flow_wait(k.c, k.need, f.abort, f.spin_limit);
       s_go = ok;
       if (ok) nxt = atomicAdd(f.ticket, 1u);
     }

@@ -104,3 +104,28 @@ def test_flow_refuses_mixed_class_member(hip_device):
     with pytest.raises(_lib.PfcError):
         g.set_flow(8)
     g.step(2)  # the per-pass form still runs it
+
+
+def test_flow_abort_flags_every_slot(hip_device, monkeypatch):
+    # a dependency wait bounded at zero polls: the first tile that finds its
+    # instance's previous phase unfinished stops the launch, every block
+    # exits, and every member slot reads PFC_STATUS_ABORTED; the next
+    # launch, with the default bound, starts clean
+    from pfc_hip.plan import Group
+    monkeypatch.setenv("PFC_HIP_TUNE", "0")
+    monkeypatch.setenv("PFC_HIP_FLOW_SPIN", "0")
+    members = [_batch(512, 512, 4, 32, "etd1", 1), _batch(256, 256, 3, 32, "etd1", 9)]
+    grp = Group(members, flow=50)
+    grp.step(50)
+    for b in members:
+        assert all(int(f) & 2 for f in b.status()), b.status()
+    monkeypatch.delenv("PFC_HIP_FLOW_SPIN")
+    alone = [_batch(512, 512, 4, 32, "etd1", 1), _batch(256, 256, 3, 32, "etd1", 9)]
+    for i, (nx, ny, n, seed) in enumerate([(512, 512, 4, 1), (256, 256, 3, 9)]):
+        for s in range(n):
+            members[i].set_field(s, F.uniform_noise((ny, nx), -0.3 + 0.05 * s, 0.05, seed=seed + s))
+    grp.step(20)
+    for a in alone:
+        a.step(20)
+    _same(members, alone, "after abort")
+    assert not any(b.status().any() for b in members)
