@@ -275,7 +275,8 @@ int pfc_slot_bytes(const pfc_params* params, long long* bytes);
  * chunk, then of the next, so a chunk's working set can stay in the 256 MB
  * Infinity Cache across its steps; chunks are independent, so the chunking
  * changes no result.  pfc_group_chunks gives the number of chunks of the
- * last call.
+ * last call.  The launches' output stores are ordinary ones
+ * (PFC_HIP_GROUP_STORES=nt: non-temporal).
  * (No reference counterpart: the reference relaxes one Simulation
  * at a time, phase_diagram/q0star.py:409-477; this is the uneven-size sweep
  * form, one launch per kernel kind per step over a work list of FFT tiles.) */

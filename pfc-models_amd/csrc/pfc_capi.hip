@@ -1593,6 +1593,7 @@ struct pfc_group {
   std::vector<pfc_batch*> mem;  // not owned
   std::vector<int> order;       // members by decreasing instance size (launch order)
   int precision = 0, scheme = 0;
+  bool nt = false;              // output stores of the per-pass launches: non-temporal (else ordinary)
   DevBuf desc;                  // GroupMember<T>[chunks][2 passes][2 classes][members]
   std::vector<long long> built; // active counts the descriptors hold
   // Members are stepped in chunks whose arrays and tables fit `budget` bytes
@@ -1740,16 +1741,12 @@ void group_build(pfc_group& g, hipStream_t st) {
 }
 
 template <typename T>
-void group_launch(int kind, int cls, int blocks, size_t lds, hipStream_t st, const GroupMember<T>* mem, int nmem);
-template <>
-void group_launch<double>(int kind, int cls, int blocks, size_t lds, hipStream_t st, const GroupMember<double>* mem,
-                          int nmem) {
-  group_launch_f64(kind, cls, blocks, lds, st, mem, nmem);
-}
-template <>
-void group_launch<float>(int kind, int cls, int blocks, size_t lds, hipStream_t st, const GroupMember<float>* mem,
-                         int nmem) {
-  group_launch_f32(kind, cls, blocks, lds, st, mem, nmem);
+void group_launch(int kind, int cls, bool nt, int blocks, size_t lds, hipStream_t st, const GroupMember<T>* mem,
+                  int nmem) {
+  if constexpr (sizeof(T) == 8)
+    group_launch_f64(kind, cls, nt, blocks, lds, st, mem, nmem);
+  else
+    group_launch_f32(kind, cls, nt, blocks, lds, st, mem, nmem);
 }
 
 // one pass (kind 0: rows, else the column mode kind - 1) of chunk `k` over
@@ -1760,7 +1757,7 @@ void group_pass(pfc_group& g, size_t k, int kind, hipStream_t st, int cls = -1) 
   const GroupLaunch* L = kind == 0 ? g.chunks[k].rows : g.chunks[k].cols;
   for (int c = 0; c < kGroupClasses; ++c)
     if (L[c].nmem && (cls < 0 || cls == c))
-      group_launch<T>(kind, c, L[c].blocks, L[c].lds, st, base + L[c].off, L[c].nmem);
+      group_launch<T>(kind, c, g.nt, L[c].blocks, L[c].lds, st, base + L[c].off, L[c].nmem);
 }
 
 template <typename T>
@@ -2457,6 +2454,11 @@ int pfc_group_create(pfc_batch* const* batches, int n, pfc_group** out) {
     // the cached working set)
     const char* cm = std::getenv("PFC_HIP_GROUP_CACHE_MB");
     g->budget = (size_t)((cm ? std::max(0.0, std::atof(cm)) : 0.0) * 1024 * 1024);
+    // output stores of the per-pass launches: ordinary by default (on the
+    // 256-instance sweep's sizes they measured 3-25 % faster than streaming
+    // ones); PFC_HIP_GROUP_STORES=nt streams them
+    const char* gs = std::getenv("PFC_HIP_GROUP_STORES");
+    g->nt = gs && gs[0] == 'n';
     g->built.assign(n, -1);
     *out = g.release();
   });

@@ -905,7 +905,7 @@ template <typename T> struct GroupMember {
   int kcase, ntile, blk0;
 };
 
-template <typename T, int CLASS>
+template <typename T, int CLASS, bool NT = true>
 __global__ void __launch_bounds__(kGroupThreads) k_row_group(const GroupMember<T>* __restrict__ mem, int nmem) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   // members are laid out in block order; XCD-aware order as in k_row_fast
@@ -920,8 +920,8 @@ __global__ void __launch_bounds__(kGroupThreads) k_row_group(const GroupMember<T
 #define PFC_X(CASE, CLS, LEN, PP, ...)                                                                           \
   case CASE:                                                                                                     \
     if constexpr (CLS == CLASS)                                                                                  \
-      row_fast_tile<T, Sch<LEN, PP, __VA_ARGS__>>(g.S, g.Tb, g.Ny, g.lay, g.q, g.tw, g.sl, yi, tile, smem_raw,    \
-                                                  threadIdx.x);                                                  \
+      row_fast_tile<T, Sch<LEN, PP, __VA_ARGS__>, false, NT>(g.S, g.Tb, g.Ny, g.lay, g.q, g.tw, g.sl, yi, tile,   \
+                                                             smem_raw, threadIdx.x);                             \
     break;
     PFC_GROUP_SCHEDULES(PFC_X)
 #undef PFC_X
@@ -930,7 +930,7 @@ __global__ void __launch_bounds__(kGroupThreads) k_row_group(const GroupMember<T
   }
 }
 
-template <typename T, int MODE, int CLASS>
+template <typename T, int MODE, int CLASS, bool NT = true>
 __global__ void __launch_bounds__(kGroupThreads) k_col_group(const GroupMember<T>* __restrict__ mem, int nmem) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int lin = xcd_remap(blockIdx.x, gridDim.x);
@@ -944,8 +944,8 @@ __global__ void __launch_bounds__(kGroupThreads) k_col_group(const GroupMember<T
 #define PFC_X(CASE, CLS, LEN, PP, ...)                                                                          \
   case CASE:                                                                                                    \
     if constexpr (CLS == CLASS)                                                                                 \
-      col_fast_tile<T, Sch<LEN, PP, __VA_ARGS__>, MODE>(g.S, g.Tb, g.D, g.Nc, g.lay, g.tb, g.tw, g.sl, yi, tile, \
-                                                        smem_raw, threadIdx.x);                                 \
+      col_fast_tile<T, Sch<LEN, PP, __VA_ARGS__>, MODE, NT>(g.S, g.Tb, g.D, g.Nc, g.lay, g.tb, g.tw, g.sl, yi,   \
+                                                            tile, smem_raw, threadIdx.x);                       \
     break;
     PFC_GROUP_SCHEDULES(PFC_X)
 #undef PFC_X
@@ -955,11 +955,12 @@ __global__ void __launch_bounds__(kGroupThreads) k_col_group(const GroupMember<T
 }
 
 // Launchers (pfc_inst_group*.hip): kind 0 = row pass, else the column pass of
-// mode kind - 1 (COL_ETD1, COL_RK2_STAGE1, COL_RK2_STAGE2); cls = register class.
-void group_launch_f64(int kind, int cls, int blocks, size_t lds, hipStream_t st, const GroupMember<double>* mem,
-                      int nmem);
-void group_launch_f32(int kind, int cls, int blocks, size_t lds, hipStream_t st, const GroupMember<float>* mem,
-                      int nmem);
+// mode kind - 1 (COL_ETD1, COL_RK2_STAGE1, COL_RK2_STAGE2); cls = register
+// class; nt: non-temporal (else ordinary) output stores.
+void group_launch_f64(int kind, int cls, bool nt, int blocks, size_t lds, hipStream_t st,
+                      const GroupMember<double>* mem, int nmem);
+void group_launch_f32(int kind, int cls, bool nt, int blocks, size_t lds, hipStream_t st,
+                      const GroupMember<float>* mem, int nmem);
 
 // ---------------------------------------------------------------------------
 // persistent multi-shape step ("flow"): one launch advances the members of

@@ -7,26 +7,28 @@ namespace pfc {
 PFC_GROUP_SCHEDULES(PFC_X)
 #undef PFC_X
 
-template <int CLS>
+template <int CLS, bool NT>
 static void launch_class(int kind, dim3 grid, dim3 block, size_t lds, hipStream_t st, const GroupMember<float>* mem,
                          int nmem) {
   if (kind == 0)
-    hipLaunchKernelGGL((k_row_group<float, CLS>), grid, block, lds, st, mem, nmem);
+    hipLaunchKernelGGL((k_row_group<float, CLS, NT>), grid, block, lds, st, mem, nmem);
   else if (kind == 1 + COL_ETD1)
-    hipLaunchKernelGGL((k_col_group<float, COL_ETD1, CLS>), grid, block, lds, st, mem, nmem);
+    hipLaunchKernelGGL((k_col_group<float, COL_ETD1, CLS, NT>), grid, block, lds, st, mem, nmem);
   else if (kind == 1 + COL_RK2_STAGE1)
-    hipLaunchKernelGGL((k_col_group<float, COL_RK2_STAGE1, CLS>), grid, block, lds, st, mem, nmem);
+    hipLaunchKernelGGL((k_col_group<float, COL_RK2_STAGE1, CLS, NT>), grid, block, lds, st, mem, nmem);
   else
-    hipLaunchKernelGGL((k_col_group<float, COL_RK2_STAGE2, CLS>), grid, block, lds, st, mem, nmem);
+    hipLaunchKernelGGL((k_col_group<float, COL_RK2_STAGE2, CLS, NT>), grid, block, lds, st, mem, nmem);
 }
 
-void group_launch_f32(int kind, int cls, int blocks, size_t lds, hipStream_t st, const GroupMember<float>* mem,
-                      int nmem) {
+void group_launch_f32(int kind, int cls, bool nt, int blocks, size_t lds, hipStream_t st,
+                      const GroupMember<float>* mem, int nmem) {
   const dim3 grid(blocks), block(kGroupThreads);
   if (cls == 0)
-    launch_class<0>(kind, grid, block, lds, st, mem, nmem);
+    nt ? launch_class<0, true>(kind, grid, block, lds, st, mem, nmem)
+       : launch_class<0, false>(kind, grid, block, lds, st, mem, nmem);
   else
-    launch_class<1>(kind, grid, block, lds, st, mem, nmem);
+    nt ? launch_class<1, true>(kind, grid, block, lds, st, mem, nmem)
+       : launch_class<1, false>(kind, grid, block, lds, st, mem, nmem);
 }
 
 template <int CLS, bool RK2>
