@@ -537,42 +537,54 @@ __global__ void __launch_bounds__(256) k_row_range(const V2<T>* __restrict__ Sg,
 // ---------------------------------------------------------------------------
 // Column tile `tile` (blockDim.x / TPL columns) of active instance `yi`;
 // every thread of the block calls it.
-template <typename T, class S, int MODE, bool NT = true, int MEM = 0>
-__device__ __forceinline__ void col_fast_tile(V2<T>* __restrict__ Sg, const V2<T>* __restrict__ Tin,
-                                              V2<T>* __restrict__ D, int Nc, const Lay& lay, const ColTables& tb,
-                                              const V2<double>* __restrict__ twY, const Slots& sl, int yi, int tile,
-                                              unsigned char* smem_raw, int tid) {
-  using V = V2<T>;
-  constexpr int NY = S::L, P = S::P, TPL = S::TPL, LD = col_ld(NY);
-  V* lds = reinterpret_cast<V*>(smem_raw);
-  // Thread t is transform lane t / C of column t % C of the tile, holding rows
-  // lane + TPL q.  The lanes of one load or store then cover C columns of
-  // consecutive rows (whole 128-byte lines when C is a multiple of the
-  // layout's tile width), and the loaded registers are already in the
-  // transforms' strided order: no LDS transpose on the way in or out.
-  const int C = blockDim.x / TPL;  // columns per block, a power of two
+// Where a block's threads sit in column tile `tile` of active instance `yi`.
+// Thread t is transform lane t / C of column t % C of the tile, holding rows
+// lane + TPL q.  The lanes of one load or store then cover C columns of
+// consecutive rows (whole 128-byte lines when C is a multiple of the
+// layout's tile width), and the loaded registers are already in the
+// transforms' strided order: no LDS transpose on the way in or out.  Lanes
+// past the last column of a partial tile shadow that column (in bounds,
+// results discarded) so loads and barriers stay uniform.
+template <typename T> struct ColPos {
+  V2<T>* Sg;          // the instance's S~ (and the output)
+  const V2<T>* Tin;   // its T
+  V2<T>* D;           // its ETDRK2 carry, or null
+  int slot, col, lane, wl;
+  bool live;
+};
+
+template <typename T, class S>
+__device__ __forceinline__ ColPos<T> col_pos(V2<T>* __restrict__ Sg, const V2<T>* __restrict__ Tin,
+                                             V2<T>* __restrict__ D, int Nc, const Slots& sl, int yi, int tile,
+                                             int tid) {
+  ColPos<T> c;
+  const int C = blockDim.x / S::TPL;  // columns per block, a power of two
   const int lgC = __ffs(C) - 1;
-  const int wl = tid & (C - 1);
-  const int lane = tid >> lgC;
-  const int slot = slot_of(sl, yi);
+  c.wl = tid & (C - 1);
+  c.lane = tid >> lgC;
+  c.slot = slot_of(sl, yi);
   const int c0 = tile * C;
-  Sg += (size_t)slot * sl.sstride;
-  Tin += (size_t)slot * sl.sstride;
-  if (D) D += (size_t)slot * sl.sstride;
+  c.Sg = Sg + (size_t)c.slot * sl.sstride;
+  c.Tin = Tin + (size_t)c.slot * sl.sstride;
+  c.D = D ? D + (size_t)c.slot * sl.sstride : nullptr;
   const int ncols = min(C, Nc - c0);
-  const bool live = wl < ncols;
-  // lanes past the last column of a partial tile shadow that column (in
-  // bounds, results discarded) so loads and barriers stay uniform
-  const int col = c0 + (live ? wl : ncols - 1);
-  V* line = lds + wl * LD;
-  // ETD1 under the device-managed shift, decided in the step: where the row
-  // kernel flagged the slot, every block takes the field's range from all
-  // row tiles' pairs, refits M (the same value in every block) and first
-  // recomputes its columns' table entries with it (as k_tables computes
-  // them), before anything else is live; tile 0 stores the new fit.  The
-  // load path below then reads the entries back (each lane reads entries it
-  // wrote, or the identical value written for the mirrored row).
-  if (MODE == COL_ETD1 && tb.brk && tb.brk[2 * slot + tb.par]) {
+  c.live = c.wl < ncols;
+  c.col = c0 + (c.live ? c.wl : ncols - 1);
+  return c;
+}
+
+// ETD1 under the device-managed shift, decided in the step: where the row
+// kernel flagged the slot, every block takes the field's range from all row
+// tiles' pairs, refits M (the same value in every block) and first
+// recomputes its columns' table entries with it (as k_tables computes them),
+// before anything reads them; tile 0 stores the new fit.  The body then
+// reads the entries back (each lane reads entries it wrote, or the identical
+// value written for the mirrored row).
+template <typename T, class S, int MODE>
+__device__ __forceinline__ void col_refit(const ColPos<T>& c, const ColTables& tb, const Slots& sl, int yi,
+                                          int tile, int tid) {
+  constexpr int NY = S::L, P = S::P, TPL = S::TPL;
+  if (MODE == COL_ETD1 && tb.brk && tb.brk[2 * c.slot + tb.par]) {
     const RangePart* __restrict__ pp = tb.part + (size_t)yi * tb.pstride;
     double mn = INFINITY, mx = -INFINITY;
     for (int b = tid; b < tb.nrow; b += blockDim.x) {
@@ -583,51 +595,62 @@ __device__ __forceinline__ void col_fast_tile(V2<T>* __restrict__ Sg, const V2<T
     double lo;
     TabParams p = tb.tp;
     p.M = shift_fit(mn, mx, tb.g, tb.minPhi, lo);
-    p.epsilon = tb.eps[slot];
-    p.dt = tb.sg.dt[slot];
-    const double kx2 = tb.sg.kx2[(size_t)slot * p.Nc + col];
-    const double* __restrict__ ky2 = tb.sg.ky2 + (size_t)slot * p.Ny;
-    const size_t hb = (size_t)col * tb.hy + (size_t)slot * sl.tstride;
+    p.epsilon = tb.eps[c.slot];
+    p.dt = tb.sg.dt[c.slot];
+    const double kx2 = tb.sg.kx2[(size_t)c.slot * p.Nc + c.col];
+    const double* __restrict__ ky2 = tb.sg.ky2 + (size_t)c.slot * p.Ny;
+    const size_t hb = (size_t)c.col * tb.hy + (size_t)c.slot * sl.tstride;
     double* E1w = const_cast<double*>(reinterpret_cast<const double*>(tb.E1)) + hb;
     double* Q1w = const_cast<double*>(reinterpret_cast<const double*>(tb.Q1)) + hb;
     double* Q2w = tb.Q2 ? const_cast<double*>(reinterpret_cast<const double*>(tb.Q2)) + hb : nullptr;
-    if (live)
+    if (c.live)
       for (int qq = 0; qq < P; ++qq) {
-        const int j = ky_half(lane + TPL * qq, NY);
+        const int j = ky_half(c.lane + TPL * qq, NY);
         const TabEntry t = tab_entry(-(kx2 + ky2[j]), p, Q2w != nullptr);
         E1w[j] = t.e1;
         Q1w[j] = t.q1;
         if (Q2w) Q2w[j] = t.q2;
       }
     if (tile == 0 && tid == 0) {
-      tb.st[slot] = ShiftState{lo, mx, 1};
-      tb.Ms[slot] = p.M;
+      tb.st[c.slot] = ShiftState{lo, mx, 1};
+      tb.Ms[c.slot] = p.M;
     }
-    asm volatile("" ::: "memory");  // the table loads below read memory, not these values
+    asm volatile("" ::: "memory");  // the table loads that follow read memory, not these values
   }
-  // ETD tables are FP64 in every precision; the combine runs in FP64 and is
-  // rounded once to the bulk type
-  const double* E1 = reinterpret_cast<const double*>(tb.E1) + (size_t)slot * sl.tstride;
-  const double* Q1 = reinterpret_cast<const double*>(tb.Q1) + (size_t)slot * sl.tstride;
-  const double* Q2 = reinterpret_cast<const double*>(tb.Q2) + (size_t)slot * sl.tstride;
+}
 
-  V a[P], b[P];
+// The tile's column of S~ (not in the second ETDRK2 stage) and of T.
+template <typename T, class S, int MODE, int MEM>
+__device__ __forceinline__ void col_load_ab(V2<T> (&a)[S::P], V2<T> (&b)[S::P], const ColPos<T>& c,
+                                            const Lay& lay) {
 #pragma unroll
-  for (int qq = 0; qq < P; ++qq) {
-    const size_t off = lay.at(lane + TPL * qq, col);
-    if constexpr (MODE != COL_RK2_STAGE2) a[qq] = ld_state<MEM, T>(Sg + off);
-    b[qq] = ld_state<MEM, T>(Tin + off);  // in flight while the first transform runs
+  for (int qq = 0; qq < S::P; ++qq) {
+    const size_t off = lay.at(c.lane + S::TPL * qq, c.col);
+    if constexpr (MODE != COL_RK2_STAGE2) a[qq] = ld_state<MEM, T>(c.Sg + off);
+    b[qq] = ld_state<MEM, T>(c.Tin + off);
   }
-  V wt[tw_count<S>()];
-  tw_load<T, S>(wt, lane, twY, 1);
-  // every load above is issued before the first transform starts
-  __builtin_amdgcn_sched_barrier(0);
+}
+
+// Everything after the loads: forward column FFTs of S~ and T, the ETD
+// combine in FP64 (tables FP64 in every precision, rounded once to the bulk
+// type), the inverse column FFT and the store of S~.  `wt`: the lane's roots.
+template <typename T, class S, int MODE, bool NT, int MEM>
+__device__ __forceinline__ void col_body(V2<T> (&a)[S::P], V2<T> (&b)[S::P], const V2<T>* wt, const ColPos<T>& c,
+                                         int Nc, const Lay& lay, const ColTables& tb, const Slots& sl,
+                                         unsigned char* smem_raw) {
+  using V = V2<T>;
+  constexpr int NY = S::L, P = S::P, TPL = S::TPL, LD = col_ld(NY);
+  V* line = reinterpret_cast<V*>(smem_raw) + c.wl * LD;
+  const int lane = c.lane, col = c.col;
+  const double* E1 = reinterpret_cast<const double*>(tb.E1) + (size_t)c.slot * sl.tstride;
+  const double* Q1 = reinterpret_cast<const double*>(tb.Q1) + (size_t)c.slot * sl.tstride;
+  const double* Q2 = reinterpret_cast<const double*>(tb.Q2) + (size_t)c.slot * sl.tstride;
   // FP32 bulk with a pinned mean m: the kx = 0 column (the rows' averages)
   // is transformed as S~ - m, its k = 0 coefficient pinned to 0 and m added
   // back at the store, so the transforms carry the fluctuation only (the
   // stage-2 carry D is already about m).  FP64 pins k = 0 to m directly.
   constexpr bool kAbout = sizeof(T) == 4;
-  const double* pinp = sl.pin ? sl.pin + slot : nullptr;
+  const double* pinp = sl.pin ? sl.pin + c.slot : nullptr;
   const double pin = pinp ? *pinp : 0.0;
   const double mcol = (kAbout && pinp && col == 0) ? pin : 0.0;
   if constexpr (MODE != COL_RK2_STAGE2) {
@@ -650,7 +673,7 @@ __device__ __forceinline__ void col_fast_tile(V2<T>* __restrict__ Sg, const V2<T
     const size_t ti = hbase + ky_half(lane + TPL * qq, NY);
     if constexpr (MODE == COL_RK2_STAGE2) {
       q2r[qq] = Q2[ti];
-      dr[qq] = ld_state<MEM, T>(D + tbase + lane + TPL * qq);
+      dr[qq] = ld_state<MEM, T>(c.D + tbase + lane + TPL * qq);
     } else {
       e1r[qq] = E1[ti];
       q1r[qq] = Q1[ti];
@@ -660,7 +683,7 @@ __device__ __forceinline__ void col_fast_tile(V2<T>* __restrict__ Sg, const V2<T
   __builtin_amdgcn_sched_barrier(0);  // table loads issued before the second transform
   reg_fft<T, S, false, true>(b, line, lane, wt);
 
-  if (live) {
+  if (c.live) {
 #pragma unroll
     for (int qq = 0; qq < P; ++qq) {
       const int ky = lane + TPL * qq;
@@ -672,14 +695,15 @@ __device__ __forceinline__ void col_fast_tile(V2<T>* __restrict__ Sg, const V2<T
       } else {
         ox = e1r[qq] * double(a[qq].x) + q1r[qq] * bx;
         oy = e1r[qq] * double(a[qq].y) + q1r[qq] * by;
-        if constexpr (MODE == COL_RK2_STAGE1) st_state<MEM, false, T>(D + tbase + ky, V{T(ox - q2r[qq] * bx), T(oy - q2r[qq] * by)});
+        if constexpr (MODE == COL_RK2_STAGE1)
+          st_state<MEM, false, T>(c.D + tbase + ky, V{T(ox - q2r[qq] * bx), T(oy - q2r[qq] * by)});
       }
       if (pinp && col == 0 && ky == 0) ox = kAbout ? 0.0 : pin, oy = 0.0;  // FP32: the fluctuation's mean, 0
       b[qq] = V{T(ox), T(oy)};
     }
   }
   reg_fft<T, S, true, true>(b, line, lane, wt);
-  if (live) {
+  if (c.live) {
     // real projection of the kx = 0 and kx = Nx/2 columns (see k_col_etd)
     const bool realcol = (col == 0 || col == Nc - 1);
 #pragma unroll
@@ -688,9 +712,28 @@ __device__ __forceinline__ void col_fast_tile(V2<T>* __restrict__ Sg, const V2<T
       if (realcol) xv.y = T(0);
       if constexpr (kAbout)
         if (mcol != 0.0) xv.x = T(double(xv.x) + mcol);
-      st_state<MEM, NT, T>(Sg + lay.at(lane + TPL * qq, col), xv);
+      st_state<MEM, NT, T>(c.Sg + lay.at(lane + TPL * qq, col), xv);
     }
   }
+}
+
+// Column tile `tile` (blockDim.x / TPL columns) of active instance `yi`:
+// forward column FFTs of S~ and T, ETD combine in registers, inverse column
+// FFT, S~ written in place.  Every thread of the block calls it.
+template <typename T, class S, int MODE, bool NT = true, int MEM = 0>
+__device__ __forceinline__ void col_fast_tile(V2<T>* __restrict__ Sg, const V2<T>* __restrict__ Tin,
+                                              V2<T>* __restrict__ D, int Nc, const Lay& lay, const ColTables& tb,
+                                              const V2<double>* __restrict__ twY, const Slots& sl, int yi, int tile,
+                                              unsigned char* smem_raw, int tid) {
+  const ColPos<T> c = col_pos<T, S>(Sg, Tin, D, Nc, sl, yi, tile, tid);
+  col_refit<T, S, MODE>(c, tb, sl, yi, tile, tid);
+  V2<T> a[S::P], b[S::P];
+  col_load_ab<T, S, MODE, MEM>(a, b, c, lay);  // in flight while the first transform runs
+  V2<T> wt[tw_count<S>()];
+  tw_load<T, S>(wt, c.lane, twY, 1);
+  // every load above is issued before the first transform starts
+  __builtin_amdgcn_sched_barrier(0);
+  col_body<T, S, MODE, NT, MEM>(a, b, wt, c, Nc, lay, tb, sl, smem_raw);
 }
 
 template <typename T, class S, int MODE, bool NT, int WAVES = 0>
