@@ -1615,10 +1615,10 @@ struct pfc_group {
   long long captures = 0, replays = 0;
   // Persistent form (pfc_group_set_flow): each register class's members step
   // in one k_flow launch per `flow_steps` steps (0: the per-pass launches
-  // above).  Per class: the members' row and column descriptors and the
-  // work list's segment and counter offsets (fdesc, fidx), the ticket, the
-  // per-instance counters and the abort flag (fcnt); the two classes run on
-  // the caller's stream and flow_st2 side by side.
+  // above).  Per class: the members' row and column descriptors (fdesc),
+  // each member's first instance counter (fidx), the step's ticket entries
+  // (ftab), the ticket, the per-instance counters and the abort flag (fcnt);
+  // the two classes run on the caller's stream and flow_st2 side by side.
   int flow_steps = 0;
   struct FlowClass {
     int nmem = 0, per_step = 0, ninst = 0, blocks = 0;
@@ -1826,14 +1826,14 @@ void flow_build(pfc_group& g, hipStream_t st) {
   hip_check(hipGetDevice(&dev), "hipGetDevice");
   hip_check(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev), "device attribute");
   std::vector<GroupMember<T>> desc(kGroupClasses * 2 * n);
-  std::vector<int> idx(kGroupClasses * ((size_t)4 * n + 1 + n + 1));
+  std::vector<int> idx(kGroupClasses * (n + 1));
   std::vector<FlowEntry> tab;
   size_t cnt_words = 0;
   for (int c = 0; c < kGroupClasses; ++c) {
     pfc_group::FlowClass& F = g.flow[c];
     F = pfc_group::FlowClass{};
     F.desc_off = (size_t)c * 2 * n * sizeof(GroupMember<T>);
-    F.idx_off = (size_t)c * ((size_t)4 * n + 1 + n + 1);
+    F.idx_off = (size_t)c * (n + 1);
     std::vector<int> mems;
     for (int i : g.order) {
       const Engine& e = g.mem[i]->e;
@@ -1845,8 +1845,7 @@ void flow_build(pfc_group& g, hipStream_t st) {
     if (F.nmem == 0) continue;
     GroupMember<T>* rows = desc.data() + (size_t)c * 2 * n;
     GroupMember<T>* cols = rows + n;
-    int* seg = idx.data() + F.idx_off;
-    int* cnt0 = seg + nph * F.nmem + 1;
+    int* cnt0 = idx.data() + F.idx_off;
     cnt0[0] = 1;  // word 0 is the ticket
     for (int k = 0; k < F.nmem; ++k) {
       Engine& e = g.mem[mems[k]]->e;
@@ -1873,17 +1872,12 @@ void flow_build(pfc_group& g, hipStream_t st) {
       }
       cnt0[k + 1] = cnt0[k] + e.nact();
     }
-    int at = 0;
+    long long at = 0;  // tickets per step
     for (int p = 0; p < nph; ++p)
-      for (int k = 0; k < F.nmem; ++k) {
-        seg[p * F.nmem + k] = at;
-        const Engine& e = g.mem[mems[k]]->e;
-        const long long add = (long long)e.nact() * ((p & 1) == 0 ? rows[k].ntile : cols[k].ntile);
-        if (at + add > (1LL << 30)) fail(PFC_EUNSUPPORTED, "persistent group step: too many tiles per step");
-        at += (int)add;
-      }
-    seg[nph * F.nmem] = at;
-    F.per_step = at;
+      for (int k = 0; k < F.nmem; ++k)
+        at += (long long)g.mem[mems[k]]->e.nact() * ((p & 1) == 0 ? rows[k].ntile : cols[k].ntile);
+    if (at > (1LL << 30)) fail(PFC_EUNSUPPORTED, "persistent group step: too many tiles per step");
+    F.per_step = (int)at;
     // the step's tickets in work-list order
     F.tab_off = tab.size();
     for (int p = 0; p < nph; ++p)
@@ -1938,7 +1932,7 @@ void flow_launch(pfc_group& g, int c, long long steps, hipStream_t st) {
   const GroupMember<T>* base = (const GroupMember<T>*)((const char*)g.fdesc.p + F.desc_off);
   f.rows = base;
   f.cols = base + n;
-  f.cnt0 = (const int*)g.fidx.p + F.idx_off + nph * F.nmem + 1;
+  f.cnt0 = (const int*)g.fidx.p + F.idx_off;
   f.tab = (const FlowEntry*)g.ftab.p + F.tab_off;
   unsigned* words = (unsigned*)g.fcnt.p;
   f.cnt = words + F.cnt_off;
