@@ -119,6 +119,8 @@ def run_gpu(precision, steps, warmup):
         },
         "device_bytes": pl.device_bytes(),
         "schedule": sched,
+        # the same kernels under PFC_HIP_SCHEDULE64 / PFC_HIP_SCHEDULE32
+        "schedule_pin": f"{sched['row'][1]}:{sched['row'][3]}:{sched['col'][1]}:{sched['col'][3]}",
         "stores": pl.stores(),
     }
 
@@ -558,13 +560,14 @@ def main():
         "step_roofline": f64["step_roofline"],
         "kernels_us": f64["kernels_us"],
         "stores": f64["stores"],
+        "schedule_pin": f64["schedule_pin"],
         "step_graph": {"eager_ms_per_step": round(f64["eager_ms_per_step"], 5),
                        "captures_replays": f64["graph_captures_replays"]},
         "fp32": None if f32 is None else {
             "value": round(f32["value"], 1), "ms_per_step": round(f32["ms_per_step"], 5),
             "eager_ms_per_step": round(f32["eager_ms_per_step"], 5),
             "dtype": "f32 bulk, f64 tables, combine and reductions", "roofline": f32["roofline"],
-            "kernels_us": f32["kernels_us"], "stores": f32["stores"],
+            "kernels_us": f32["kernels_us"], "stores": f32["stores"], "schedule_pin": f32["schedule_pin"],
             "rel_fe_vs_fp64": abs(f32["free_energy"] - f64["free_energy"]) / abs(f64["free_energy"])},
         "cpu_baseline": cpu,
         "sweep": sweep,

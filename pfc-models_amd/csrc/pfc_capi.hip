@@ -485,9 +485,12 @@ void engine_init(Engine& e, const pfc_params* params, int cap, bool batched) {
     e.frow = pick_default(rc, 512);
     e.fcol = pick_default(cc, 256);
     // PFC_HIP_SCHEDULE="rowvar:rowlines:colvar:collines" pins a schedule
-    // (tests and tuning studies; a variant + 16 is its capped form); the
-    // autotuner is then skipped.
-    if (const char* fx = std::getenv("PFC_HIP_SCHEDULE")) {
+    // (tests, tuning studies and profiles; a variant + 16 is its capped
+    // form); PFC_HIP_SCHEDULE64 / PFC_HIP_SCHEDULE32 pin one precision's
+    // plans and take precedence.  The autotuner is then skipped.
+    const char* fx = std::getenv(e.f64() ? "PFC_HIP_SCHEDULE64" : "PFC_HIP_SCHEDULE32");
+    if (!fx) fx = std::getenv("PFC_HIP_SCHEDULE");
+    if (fx) {
       int v[4] = {-1, -1, -1, -1};
       if (std::sscanf(fx, "%d:%d:%d:%d", &v[0], &v[1], &v[2], &v[3]) == 4) {
         bool okr = false, okc = false;
