@@ -171,7 +171,9 @@ int pfc_step_profile(pfc_plan* plan, long long nsteps, void* stream, double ms[3
  * info[4..7] = column kernel {same, columns per block}.  A zero in info[0] or
  * info[4] means the generic mixed-radix kernel runs; -1 in both means the
  * resident stepper runs (small ETD1 instances: one workgroup per instance
- * holds the whole row spectrum in LDS for all the steps of a call).
+ * holds the whole row spectrum in LDS for all the steps of a call), -2 its
+ * dense form (cells of at most (Nx/2 + 1) Ny = 512 points, Nx, Ny <= 32;
+ * PFC_HIP_RESIDENT_DENSE=0 keeps the FFT form).
  * (No reference counterpart.) */
 int pfc_plan_schedule(const pfc_plan* plan, int info[8]);
 /* Store kind of the fast row and column kernels' outputs (after the first

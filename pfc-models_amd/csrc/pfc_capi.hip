@@ -373,6 +373,7 @@ struct Engine {
   FastShape frow{}, fcol{};
   ResGeom res{};
   bool resident = false;  // small ETD1 instances: k_resident_etd1, all steps in LDS
+  bool dense = false;     // ... the smallest of them: k_resident_dense
   bool tuned = false, batched = false;
   DevBuf S, Tb, D, E1, Q1, Q2;       // [cap][...]
   DevBuf twN, twY, twN64, twY64, kx2, ky2, dts;  // kx2 [cap][Nc], ky2 [cap][Ny], dts [cap]
@@ -459,6 +460,15 @@ bool resident_fits(const pfc_params& p) {
   const size_t lds = resident_lds(p.Nx, p.Ny, p.precision == 64 ? sizeof(V2<double>) : sizeof(V2<float>));
   return p.scheme == PFC_SCHEME_ETD1 && lds <= std::min(kResidentLds, (size_t)lds_limit());
 }
+// Cells of at most (Nx/2 + 1) Ny = 512 points with Ny <= 32 and Nx <= 32 take
+// the dense resident stepper (three LDS phases per step instead of the FFT
+// stepper's dozen).  PFC_HIP_RESIDENT_DENSE=0 keeps the FFT stepper.
+bool dense_ok(const pfc_params& p) {
+  const char* env = std::getenv("PFC_HIP_RESIDENT_DENSE");
+  if (env && std::string(env) == "0") return false;
+  const size_t lds = dense_lds_bytes(p.Nx, p.Ny, p.precision == 64 ? sizeof(V2<double>) : sizeof(V2<float>));
+  return dense_fits(p.Nx, p.Ny) && lds <= std::min(kResidentLds, (size_t)lds_limit());
+}
 
 void engine_init(Engine& e, const pfc_params* params, int cap, bool batched) {
   validate(params);
@@ -479,6 +489,7 @@ void engine_init(Engine& e, const pfc_params* params, int cap, bool batched) {
   e.rg.lay = e.cg.lay = e.rg64.lay = e.cg64.lay = e.lay;
   e.res = ResGeom{Nx, Nx / 2, e.Nc, Ny, table_rows(Ny), e.lay, e.rg.plM, e.cg.plY};
   e.resident = resident_fits(*params);
+  e.dense = e.resident && dense_ok(*params);
   if (!e.resident && !std::getenv("PFC_HIP_GENERIC")) {
     auto rc = row_candidates(Nx, Ny, e.csize);
     auto cc = col_candidates(Nx, Ny, e.csize);
@@ -709,6 +720,33 @@ struct Steps {
                        const long long* counts = nullptr) {
     constexpr long long kResidentChunk = 4096;
     ColTables tb = e.tables();
+    if (e.dense) {
+      const int nsp = e.Nc * e.p.Ny;
+      const size_t lds = dense_lds_bytes(e.p.Nx, e.p.Ny, sizeof(V));
+      // the smallest compiled row and angle bounds that hold this cell
+      const int rows = e.p.Ny <= 16 ? 16 : e.p.Ny <= 24 ? 24 : 32;
+      const int angles = e.p.Nx / 2 - 1 <= 7 ? 7 : 15;
+      for (long long done = 0; done < nsteps; done += kResidentChunk) {
+        const long long n = std::min(kResidentChunk, nsteps - done);
+#define PFC_DENSE(NY, MH)                                                                                 \
+  hipLaunchKernelGGL((k_resident_dense<T, NY, MH>), dim3(ny), dim3((nsp + 63) / 64 * 64), lds, st, S, e.res, \
+                     nonlin_of(e), tb, sl, n, counts, done)
+        if (rows == 16 && angles == 7)
+          PFC_DENSE(16, 7);
+        else if (rows == 16)
+          PFC_DENSE(16, 15);
+        else if (rows == 24 && angles == 7)
+          PFC_DENSE(24, 7);
+        else if (rows == 24)
+          PFC_DENSE(24, 15);
+        else if (angles == 7)
+          PFC_DENSE(32, 7);
+        else
+          PFC_DENSE(32, 15);
+#undef PFC_DENSE
+      }
+      return;
+    }
     const size_t lds = resident_lds(e.p.Nx, e.p.Ny, sizeof(V));
     static const int threads = [] {
       const char* v = std::getenv("PFC_HIP_RESIDENT_THREADS");
@@ -2254,7 +2292,7 @@ int pfc_plan_schedule(const pfc_plan* pl, int info[8]) {
     if (!pl || !info) fail(PFC_EINVAL, "NULL argument");
     const FastShape* fs[2] = {&pl->e.frow, &pl->e.fcol};
     for (int k = 0; k < 2; ++k) {
-      info[4 * k + 0] = pl->e.resident ? -1 : fs[k]->on ? 1 : 0;
+      info[4 * k + 0] = pl->e.dense ? -2 : pl->e.resident ? -1 : fs[k]->on ? 1 : 0;
       info[4 * k + 1] = fs[k]->code();
       info[4 * k + 2] = fs[k]->P;
       info[4 * k + 3] = fs[k]->lines;

@@ -647,6 +647,219 @@ __global__ void __launch_bounds__(BS) k_resident_etd1(V2<T>* __restrict__ S, Res
 }
 
 // ---------------------------------------------------------------------------
+// dense resident stepper: the smallest cells, three LDS phases per step
+// ---------------------------------------------------------------------------
+// For cells of a few hundred points the FFT stepper above is a chain of 12
+// short LDS phases (pack, two row passes each way, nonlinearity, unpack, two
+// column passes each way, combine), several of them with fewer work items
+// than one wave (a radix-7 pass of a 28-point row has 48).  Here one thread
+// owns one (row y, index j) pair, j = 0..Nx/2, and a step is three phases
+// with every thread busy:
+//   1. the two real-space values at x = j and x = Nx - j of row y from the
+//      half spectrum (cosine and sine sums over kx, sharing one set of
+//      angles 2 pi kx j / Nx), and their nonlinearity;
+//   2. T~(y, j) = sum_x N(y, x) exp(-2 pi i j x / Nx), pairing x with Nx - x
+//      (the same angles);
+//   3. the whole column step at once: the column transform, the ETD1
+//      combine and the inverse column transform form, per column j, the map
+//      S~'(., j) = A_j S~(., j) + B_j T~(., j) with A_j, B_j the circulant
+//      matrices of the inverse transforms of E1(j, .) and Q1(j, .).  E1 and Q1
+//      depend on ky only through ky^2, so these are real and symmetric:
+//      A_j[y][y'] = a_j((y - y') mod Ny), a_j(d) = sum_ky E1(j, ky) cos(2 pi ky d / Ny).
+// The angles of phases 1-2 and the thread's row of A_j and B_j are set up
+// once per launch and held in registers.  The k = 0 pin drops the ky = 0
+// term from column 0's A and B and adds the pinned value to every row of
+// that column (the unnormalised inverse transform of a lone k = 0 value);
+// the external-potential source adds its column-inverse transform, also set
+// up once.  In exact arithmetic this is the FFT stepper's step; the results
+// differ from it by rounding only.  LDS arrays are column-major ([j][y]) so
+// that a wave's lanes (consecutive y) touch consecutive words.
+//
+// NY and MH bound Ny and Nx/2 - 1 at compile time: the sums run over NY rows
+// and MH angles with zero coefficients past Ny and Nx/2 - 1 and their loads
+// clamped to a valid element, so every loop body is branch-free and the
+// compiler can issue a phase's LDS loads ahead of its FMAs.
+constexpr int kDenseThreads = 512;  // (Nx/2 + 1) Ny work items at most
+constexpr int kDenseNyMax = 32;
+constexpr int kDenseMhMax = 15;
+
+__host__ __device__ inline bool dense_fits(int Nx, int Ny) {
+  return Ny <= kDenseNyMax && Nx / 2 - 1 <= kDenseMhMax && (Nx / 2 + 1) * Ny <= kDenseThreads;
+}
+
+// LDS bytes of the dense stepper: the circulant coefficients and the trig
+// tables (FP64), two state buffers and T~ (complex), the nonlinearity (real).
+__host__ __device__ inline size_t dense_lds_bytes(int Nx, int Ny, size_t csize) {
+  const size_t nsp = (size_t)(Nx / 2 + 1) * Ny;
+  return (2 * nsp + 2 * (size_t)(Nx + Ny)) * sizeof(double) + 3 * nsp * csize + (size_t)Nx * Ny * (csize / 2);
+}
+
+template <typename T, int NY, int MH>
+__global__ void __launch_bounds__(kDenseThreads) k_resident_dense(V2<T>* __restrict__ S, ResGeom g, Nonlin q,
+                                                                  ColTables tb, Slots sl, long long nsteps,
+                                                                  const long long* __restrict__ counts,
+                                                                  long long done) {
+  if (counts) nsteps = max(0LL, min(nsteps, counts[blockIdx.x] - done));
+  if (nsteps == 0) return;
+  using V = V2<T>;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  const int Nx = g.Nx, M = g.M, Nc = g.Nc, Ny = g.Ny, nsp = Nc * Ny;
+  double* ca = reinterpret_cast<double*>(smem_raw);  // [Nc][Ny] a_j(d)
+  double* cb = ca + nsp;                             // [Nc][Ny] b_j(d)
+  double* cx = cb + nsp;                             // cos, sin (2 pi m / Nx), m < Nx
+  double* sx = cx + Nx;
+  double* cy = sx + Nx;                              // cos, sin (2 pi m / Ny), m < Ny
+  double* sy = cy + Ny;
+  V* Sa = reinterpret_cast<V*>(sy + Ny);             // state [j][y], two buffers
+  V* Sb = Sa + nsp;
+  V* Tt = Sb + nsp;                                  // T~ [j][y]
+  T* Nb = reinterpret_cast<T*>(Tt + nsp);            // N [x][y]
+  const int tid = threadIdx.x;
+  const bool live = tid < nsp;
+  const int j = live ? tid / Ny : 0;
+  const int y = live ? tid - j * Ny : 0;
+  const int slot = slot_of(sl, blockIdx.x);
+  S += (size_t)slot * sl.sstride;
+  if (sl.M) q.M = sl.M[slot];
+  const double* E1 = reinterpret_cast<const double*>(tb.E1) + (size_t)slot * sl.tstride;
+  const double* Q1 = reinterpret_cast<const double*>(tb.Q1) + (size_t)slot * sl.tstride;
+  const bool pinned = sl.pin != nullptr;
+  const double pin = pinned ? sl.pin[slot] : 0.0;
+
+  // set-up: trig tables, the state (column-major, kx = 0 and Nx/2 columns
+  // projected on the reals)
+  for (int m = tid; m < Nx; m += blockDim.x) {
+    cx[m] = cospi(2.0 * m / Nx);
+    sx[m] = sinpi(2.0 * m / Nx);
+  }
+  for (int m = tid; m < Ny; m += blockDim.x) {
+    cy[m] = cospi(2.0 * m / Ny);
+    sy[m] = sinpi(2.0 * m / Ny);
+  }
+  for (int e = tid; e < nsp; e += blockDim.x) {
+    const int yy = e / Nc;
+    const int k = e - yy * Nc;
+    V v = S[g.lay.at(yy, k)];
+    if (k == 0 || k == M) v.y = T(0);
+    Sa[k * Ny + yy] = v;
+  }
+  __syncthreads();
+  // circulant coefficients of every column
+  for (int e = tid; e < nsp; e += blockDim.x) {
+    const int k = e / Ny;
+    const int d = e - k * Ny;
+    double a = 0.0, b = 0.0;
+    for (int ky = 0; ky < Ny; ++ky) {
+      if (pinned && k == 0 && ky == 0) continue;
+      const int ti = k * g.hy + ky_half(ky, Ny);
+      const double c = cy[(ky * d) % Ny];
+      a = fma(E1[ti], c, a);
+      b = fma(Q1[ti], c, b);
+    }
+    ca[e] = a;
+    cb[e] = b;
+  }
+  // this thread's angles 2 pi k j / Nx, k = 1..MH (zero past Nx/2 - 1)
+  T cj[MH], sj[MH];
+#pragma unroll
+  for (int k = 0; k < MH; ++k) {
+    const int m = ((k + 1) * j) % Nx;
+    cj[k] = k + 1 < M ? T(cx[m]) : T(0);
+    sj[k] = k + 1 < M ? T(sx[m]) : T(0);
+  }
+  // the external-potential source's column-inverse transform at (y, j)
+  double srcx = 0.0, srcy = 0.0;
+  if (tb.C && live) {
+    const V2<double>* C = tb.C + (size_t)slot * tb.cstride;
+    for (int ky = 0; ky < Ny; ++ky) {
+      if (pinned && j == 0 && ky == 0) continue;
+      const V2<double> c = C[ky * Nc + j];
+      const int m = (ky * y) % Ny;
+      srcx += c.x * cy[m] - c.y * sy[m];
+      srcy += c.x * sy[m] + c.y * cy[m];
+    }
+  }
+  if (pinned && j == 0) srcx += pin;
+  if (j == 0 || j == M) srcy = 0.0;
+  __syncthreads();
+  // this thread's rows of A_j and B_j: r[y'] = coefficient at (y - y') mod Ny
+  // (zero past Ny)
+  double ra[NY], rb[NY];
+#pragma unroll
+  for (int yp = 0; yp < NY; ++yp) {
+    int d = y - yp;
+    d += d < 0 ? Ny : 0;
+    const bool in = live && yp < Ny;
+    ra[yp] = in ? ca[j * Ny + d] : 0.0;
+    rb[yp] = in ? cb[j * Ny + d] : 0.0;
+  }
+  const T sgn = (j & 1) ? T(-1) : T(1);  // (-1)^j
+  const bool edge = j == 0 || j == M;
+  bool bad = false;
+  V* X = Sa;
+  V* Y = Sb;
+  for (long long step = 0; step < nsteps; ++step) {
+    // phase 1: n(y, j) and n(y, Nx - j), then the nonlinearity
+    if (live) {
+      const T base = X[y].x + sgn * X[M * Ny + y].x;
+      T P = T(0), Qs = T(0);
+#pragma unroll
+      for (int k = 0; k < MH; ++k) {
+        const int kk = k + 1 < M ? k + 1 : 0;
+        const V s = X[kk * Ny + y];
+        P = fma(s.x, cj[k], P);
+        Qs = fma(s.y, sj[k], Qs);
+      }
+      const T n0 = base + T(2) * (P - Qs);
+      const T n1 = base + T(2) * (P + Qs);
+      bad |= !(isfinite(n0) && isfinite(n1));
+      Nb[j * Ny + y] = nonlin_mu<T>(n0, q);
+      if (!edge) Nb[(Nx - j) * Ny + y] = nonlin_mu<T>(n1, q);
+    }
+    __syncthreads();
+    // phase 2: T~(y, j)
+    if (live) {
+      T re = Nb[y] + sgn * Nb[M * Ny + y], im = T(0);
+#pragma unroll
+      for (int k = 0; k < MH; ++k) {
+        const bool in = k + 1 < M;
+        const T a = Nb[(in ? k + 1 : 0) * Ny + y], b = Nb[(in ? Nx - k - 1 : 0) * Ny + y];
+        re = fma(a + b, cj[k], re);
+        im = fma(b - a, sj[k], im);
+      }
+      Tt[j * Ny + y] = V{re, edge ? T(0) : im};
+    }
+    __syncthreads();
+    // phase 3: the column step as one circulant map per column
+    if (live) {
+      double ox = srcx, oy = srcy;
+      const V* sc = X + j * Ny;
+      const V* tc = Tt + j * Ny;
+#pragma unroll
+      for (int yp = 0; yp < NY; ++yp) {
+        const int yy = yp < Ny ? yp : 0;
+        const V s = sc[yy], t = tc[yy];
+        ox = fma(ra[yp], double(s.x), ox);
+        oy = fma(ra[yp], double(s.y), oy);
+        ox = fma(rb[yp], double(t.x), ox);
+        oy = fma(rb[yp], double(t.y), oy);
+      }
+      Y[j * Ny + y] = V{T(ox), edge ? T(0) : T(oy)};
+    }
+    __syncthreads();
+    V* t = X;
+    X = Y;
+    Y = t;
+  }
+  for (int e = tid; e < nsp; e += blockDim.x) {
+    const int yy = e / Nc;
+    const int k = e - yy * Nc;
+    S[g.lay.at(yy, k)] = X[k * Ny + yy];
+  }
+  if (bad && sl.status) atomicOr(sl.status + slot, 1);
+}
+
+// ---------------------------------------------------------------------------
 // spectral diagnostics of the phase-diagram driver (one workgroup per slot)
 // ---------------------------------------------------------------------------
 // From the row spectrum S~ alone: A = FFT_y of each column in FP64 in LDS,

@@ -217,7 +217,8 @@ class Plan:
 
     def schedule(self):
         """Kernel schedule in use: {'row': (fast, variant, P, lines), 'col': (...)}; variant + 16 is
-        the FP32 kernel form held to three waves per SIMD."""
+        the FP32 kernel form held to three waves per SIMD; fast = -1 / -2: the resident stepper
+        in its FFT / dense form."""
         info = (ctypes.c_int * 8)()
         _lib.check(self._lib.pfc_plan_schedule(self._h, info))
         return {"row": tuple(info[0:4]), "col": tuple(info[4:8])}
