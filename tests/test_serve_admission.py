@@ -19,6 +19,9 @@ class _Event:
     def synchronize(self):
         pass
 
+    def query(self):
+        return True
+
 
 class _Batch:
     def __init__(self, dev, capacity, per):
