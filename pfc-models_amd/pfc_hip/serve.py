@@ -14,12 +14,14 @@ launches overlap on the device.
 Tick length and backpressure.  A slab keeps at most two ticks queued on its
 stream; the loop skips a slab whose two ticks are still running instead of
 waiting for it, so the slabs of small grids advance at their own pace rather
-than at the pace of the slowest slab.  Ticks are `tick_steps` long for grids
+than at the pace of the slowest slab.  A slab with finished requests queues
+no new tick until they are retired, and retires them once its stream has
+drained, so the read-back does not hold the loop either.  Ticks are `tick_steps` long for grids
 of 1024^2 points and more and longer for smaller ones (by the ratio of the
 areas, at most 4x), so that small requests need fewer ticks.  (Measured on
-the 2000-request stream: p50 0.24-0.33 s against 0.41-0.42 s with a blocking
-wait on every slab's third tick and equal ticks, p99 unchanged within the
-runs' spread of 4.8-6.1 s; longer ticks for the small grids, 16x, gave p99
+the 2000-request stream, this loop: p50 0.24-0.25 s, p99 4.5-4.8 s; with a
+blocking wait on every slab's third tick, equal ticks and read-back on the
+spot: p50 0.41-0.42 s, p99 5.3-5.5 s; with small-grid ticks 16x longer: p99
 7.3-7.6 s.)
 
 Memory admission.  The HBM budget is spent slab by slab: a slab of k slots
@@ -203,6 +205,8 @@ class _Slab:
         if len(self.inflight) >= 2:
             return 0
         k = min(tick_steps, min(rem for _, rem in self.running.values()))
+        if k == 0:  # finished requests wait for retirement: let the stream drain
+            return 0
         with self.dev.using(self.stream):
             if self.dirty:
                 self.batch.set_active(sorted(self.running))
@@ -217,6 +221,12 @@ class _Slab:
         """Read back and free every finished request."""
         done = [s for s, (_, rem) in self.running.items() if rem == 0]
         if not done:
+            return []
+        # read back once the slab's queued ticks have run (no new tick is
+        # queued meanwhile), so the reductions do not hold the loop
+        while self.inflight and self.inflight[0].query():
+            self.inflight.popleft()
+        if self.inflight:
             return []
         out = []
         with self.dev.using(self.stream):
