@@ -48,6 +48,7 @@ import time
 import numpy as np
 
 from . import q0star as Q
+from ..models._base import advance_time
 
 
 class _Point:
@@ -125,8 +126,7 @@ def _block(m, ad, r, sb, p, j, f, field, n, dt, stepT):
     starting state; the outcome goes back into the slot).  Returns the
     block's change of f."""
     t_before, f_before, before = p.t, p.f, p.state
-    for _ in range(n):  # t accumulates step by step, as the serial run does
-        p.t += dt
+    p.t = advance_time(p.t, dt, n)  # step by step, to the bit, as the serial run does
     p.steps += n
     df = f - f_before
     if not (np.isnan(f) or df > r.fTol * abs(f_before)):

@@ -60,6 +60,10 @@ CASES = [
     dict(model="std", Nx=16, Ny=10, epsilon=-0.3, g=0.0, mean=-0.2, dt=0.3, precision=64),
     dict(model="log", Nx=28, Ny=24, epsilon=-0.9, g=-2.0, mean=0.63, dt=0.1, precision=32),
     dict(model="std", Nx=40, Ny=36, epsilon=-0.25, g=0.0, mean=-0.25, dt=0.5, precision=32),
+    # odd and small lengths (radix-3 and radix-5 transforms; a 4 x 2 cell)
+    dict(model="std", Nx=16, Ny=9, epsilon=-0.3, g=0.5, mean=-0.2, dt=0.3, precision=64),
+    dict(model="log", Nx=12, Ny=25, epsilon=-0.9, g=-2.0, mean=0.63, dt=0.1, precision=64),
+    dict(model="std", Nx=4, Ny=2, epsilon=-0.3, g=0.0, mean=-0.2, dt=0.3, precision=64),
 ]
 
 
