@@ -252,7 +252,8 @@ int pfc_batch_run_tracked(pfc_batch* batch, long long nsteps, long long every, d
 /* Read and clear every slot's PFC_STATUS_* flags: flags[capacity]. */
 int pfc_batch_get_status(pfc_batch* batch, int* flags, void* stream);
 int pfc_batch_bytes(const pfc_batch* batch, long long* bytes);
-/* Device bytes one slot of these parameters needs (state + tables); for admission control. */
+/* Device bytes one slot of these parameters needs (state + tables, FP32
+ * engines' FP32 table copies included); for admission control. */
 int pfc_slot_bytes(const pfc_params* params, long long* bytes);
 
 /* ---- groups: batches of different grid sizes advanced by shared launches ----

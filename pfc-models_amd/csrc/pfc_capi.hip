@@ -376,6 +376,7 @@ struct Engine {
   bool dense = false;     // ... the smallest of them: k_resident_dense
   bool tuned = false, batched = false;
   DevBuf S, Tb, D, E1, Q1, Q2;       // [cap][...]
+  DevBuf E1f, Q1f, Q2f;              // FP32 engines: the tables' FP32 copies (ColTables::E1f)
   DevBuf twN, twY, twN64, twY64, kx2, ky2, dts;  // kx2 [cap][Nc], ky2 [cap][Ny], dts [cap]
   DevBuf part, out, pin, status, eps, Ms, active, diag, dgi, dgo;  // dgi/dgo: diagnostics entries / results
   DevBuf cnt;         // per-slot step counts (pfc_batch_step_counts)
@@ -416,7 +417,12 @@ struct Engine {
   int nact() const { return (int)h_active.size(); }
   bool has_ext() const { return extc.p != nullptr; }
   ColTables tables() const {
-    return ColTables{E1.p, Q1.p, Q2.p, table_rows(p.Ny), (const V2<double>*)extc.p, (long long)Nc * p.Ny};
+    ColTables t{E1.p, Q1.p, Q2.p, table_rows(p.Ny), (const V2<double>*)extc.p, (long long)Nc * p.Ny};
+    t.E1f = (float*)E1f.p;
+    t.Q1f = (float*)Q1f.p;
+    t.Q2f = (float*)Q2f.p;
+    t.tp.inv_ny = 1.0 / p.Ny;  // E1 = (1 + E1f) / Ny (the refit path sets the whole of tp)
+    return t;
   }
   int pstride() const { return row_blocks(rg64) + col_blocks(cg64); }
   // ETD1 under the device-managed shift folds -M phi into the E1 table: the
@@ -526,6 +532,12 @@ void engine_init(Engine& e, const pfc_params* params, int cap, bool batched) {
   e.E1.alloc(tbytes);
   e.Q1.alloc(tbytes);
   if (params->scheme == PFC_SCHEME_ETDRK2) e.Q2.alloc(tbytes);
+  if (!e.f64()) {  // FP32 copies for the FP32 fast column kernel
+    const size_t fbytes = (size_t)cap * e.ntab * sizeof(float);
+    e.E1f.alloc(fbytes);
+    e.Q1f.alloc(fbytes);
+    if (params->scheme == PFC_SCHEME_ETDRK2) e.Q2f.alloc(fbytes);
+  }
   // every slot starts in the shared box with the shared dt
   const std::vector<double> kx2 = k_squared(rfft_k(Nx, params->Lx / Nx)), ky2 = k_squared(fft_k(Ny, params->Ly / Ny));
   std::vector<double> kx2s, ky2s;
@@ -583,7 +595,8 @@ void engine_tables(Engine& e, int slot0, int n, hipStream_t st) {
   const SlotGrid sg{(const double*)e.kx2.p, (const double*)e.ky2.p, (const double*)e.dts.p};
   hipLaunchKernelGGL(k_tables<double>, dim3(blocks, n), dim3(256), 0, st, (double*)e.E1.p, (double*)e.Q1.p,
                      (double*)e.Q2.p, (long long)e.ntab, sg, tp, (const double*)e.eps.p, (const double*)e.Ms.p,
-                     slot0);
+                     slot0, (const int*)nullptr, (const int*)nullptr, (float*)e.E1f.p, (float*)e.Q1f.p,
+                     (float*)e.Q2f.p);
   if (e.has_ext()) {
     const long long cs = (long long)e.Nc * e.p.Ny;
     const int cb = (int)std::min<long long>((cs + 255) / 256, 1024);
@@ -645,7 +658,8 @@ void shift_check(Engine& e, hipStream_t st, bool row_partials = false) {
   const int blocks = (int)std::min<size_t>((e.ntab + 255) / 256, 1024);
   hipLaunchKernelGGL(k_tables<double>, dim3(blocks, ny), dim3(256), 0, st, (double*)e.E1.p, (double*)e.Q1.p,
                      (double*)e.Q2.p, (long long)e.ntab, sg, tp, (const double*)e.eps.p, (const double*)e.Ms.p, 0,
-                     (const int*)e.active.p, (const int*)e.rebuildf.p);
+                     (const int*)e.active.p, (const int*)e.rebuildf.p, (float*)e.E1f.p, (float*)e.Q1f.p,
+                     (float*)e.Q2f.p);
   if (e.has_ext()) {
     const long long cs = (long long)e.Nc * e.p.Ny;
     const int cb = (int)std::min<long long>((cs + 255) / 256, 1024);
@@ -1599,7 +1613,7 @@ void launch_filter(Engine& e, int slot, int mode, void* out0, void* out1, int dt
 
 size_t engine_bytes(const Engine& e) {
   size_t b = 0;
-  for (const DevBuf* d : {&e.S, &e.Tb, &e.D, &e.E1, &e.Q1, &e.Q2, &e.twN, &e.twY, &e.twN64, &e.twY64, &e.kx2, &e.ky2,
+  for (const DevBuf* d : {&e.S, &e.Tb, &e.D, &e.E1, &e.Q1, &e.Q2, &e.E1f, &e.Q1f, &e.Q2f, &e.twN, &e.twY, &e.twN64, &e.twY64, &e.kx2, &e.ky2,
                           &e.dts, &e.part, &e.out, &e.pin, &e.status, &e.eps, &e.Ms, &e.active, &e.diag, &e.dgi, &e.dgo, &e.vhat,
                           &e.extc, &e.cnt, &e.shst, &e.rebuildf, &e.brk, &e.rpart})
     b += d->n;
@@ -2558,7 +2572,8 @@ int pfc_slot_bytes(const pfc_params* params, long long* bytes) {
     const size_t cs = params->precision == 64 ? 16 : 8, ts = sizeof(double);
     const bool rk2 = params->scheme == PFC_SCHEME_ETDRK2;
     const size_t grid = (size_t)(params->Nx / 2 + 1) + params->Ny + 1;  // the slot's k-grids and dt
-    *bytes = (long long)(nc * cs * (rk2 ? 3 : 2) + nt * ts * (rk2 ? 3 : 2) + grid * ts);
+    const size_t tf = params->precision == 64 ? 0 : sizeof(float);      // FP32 engines' table copies
+    *bytes = (long long)(nc * cs * (rk2 ? 3 : 2) + nt * (ts + tf) * (rk2 ? 3 : 2) + grid * ts);
   });
 }
 

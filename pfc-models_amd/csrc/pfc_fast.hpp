@@ -14,6 +14,8 @@
 // When a line fits in one wavefront (TPL <= 64, TPL | 64) its LDS exchanges
 // need only wave-level ordering; otherwise they use block barriers.
 #pragma once
+#include <type_traits>
+
 #include "pfc_kernels.hpp"
 
 namespace pfc {
@@ -603,6 +605,10 @@ __device__ __forceinline__ void col_refit(const ColPos<T>& c, const ColTables& t
     double* E1w = const_cast<double*>(reinterpret_cast<const double*>(tb.E1)) + hb;
     double* Q1w = const_cast<double*>(reinterpret_cast<const double*>(tb.Q1)) + hb;
     double* Q2w = tb.Q2 ? const_cast<double*>(reinterpret_cast<const double*>(tb.Q2)) + hb : nullptr;
+    // the FP32 copies (FP32 engines) at the same places
+    float* E1fw = tb.E1f ? tb.E1f + hb : nullptr;
+    float* Q1fw = tb.E1f ? tb.Q1f + hb : nullptr;
+    float* Q2fw = tb.E1f && tb.Q2f ? tb.Q2f + hb : nullptr;
     if (c.live)
       for (int qq = 0; qq < P; ++qq) {
         const int j = ky_half(c.lane + TPL * qq, NY);
@@ -610,6 +616,11 @@ __device__ __forceinline__ void col_refit(const ColPos<T>& c, const ColTables& t
         E1w[j] = t.e1;
         Q1w[j] = t.q1;
         if (Q2w) Q2w[j] = t.q2;
+        if (E1fw) {
+          E1fw[j] = float(t.em1);
+          Q1fw[j] = float(t.q1);
+          if (Q2fw) Q2fw[j] = float(t.q2);
+        }
       }
     if (tile == 0 && tid == 0) {
       tb.st[c.slot] = ShiftState{lo, mx, 1};
@@ -666,22 +677,33 @@ __device__ __forceinline__ void col_body(V2<T> (&a)[S::P], V2<T> (&b)[S::P], con
   // arrive while the second forward transform runs
   const size_t tbase = (size_t)col * NY;         // carry, [Nc][Ny]
   const size_t hbase = (size_t)col * tb.hy;      // tables, [Nc][Ny/2 + 1]
-  double e1r[P], q1r[P], q2r[P];
+  // FP32 bulk reads the tables' FP32 copies (tb.E1f, ...; the host launches
+  // no FP32 column kernel without them): half the bytes, widened to FP64 at
+  // the combine, E1 as (1 + E1f) / Ny
+  constexpr bool kF32Tab = sizeof(T) == 4;
+  using TT = std::conditional_t<kF32Tab, float, double>;
+  const size_t soff = (size_t)c.slot * sl.tstride;
+  const TT* E1t = kF32Tab ? reinterpret_cast<const TT*>(tb.E1f + soff) : reinterpret_cast<const TT*>(E1);
+  const TT* Q1t = kF32Tab ? reinterpret_cast<const TT*>(tb.Q1f + soff) : reinterpret_cast<const TT*>(Q1);
+  const TT* Q2t = kF32Tab ? reinterpret_cast<const TT*>(tb.Q2f + soff) : reinterpret_cast<const TT*>(Q2);
+  TT e1r[P], q1r[P], q2r[P];
   V dr[P];
 #pragma unroll
   for (int qq = 0; qq < P; ++qq) {
     const size_t ti = hbase + ky_half(lane + TPL * qq, NY);
     if constexpr (MODE == COL_RK2_STAGE2) {
-      q2r[qq] = Q2[ti];
+      q2r[qq] = Q2t[ti];
       dr[qq] = ld_state<MEM, T>(c.D + tbase + lane + TPL * qq);
     } else {
-      e1r[qq] = E1[ti];
-      q1r[qq] = Q1[ti];
-      if constexpr (MODE == COL_RK2_STAGE1) q2r[qq] = Q2[ti];
+      e1r[qq] = E1t[ti];
+      q1r[qq] = Q1t[ti];
+      if constexpr (MODE == COL_RK2_STAGE1) q2r[qq] = Q2t[ti];
     }
   }
   __builtin_amdgcn_sched_barrier(0);  // table loads issued before the second transform
   reg_fft<T, S, false, true>(b, line, lane, wt);
+  const double inv_ny = tb.tp.inv_ny;
+  auto e1_of = [&](TT v) { return kF32Tab ? (1.0 + double(v)) * inv_ny : double(v); };
 
   if (c.live) {
 #pragma unroll
@@ -690,13 +712,17 @@ __device__ __forceinline__ void col_body(V2<T> (&a)[S::P], V2<T> (&b)[S::P], con
       const double bx = b[qq].x, by = b[qq].y;
       double ox, oy;
       if constexpr (MODE == COL_RK2_STAGE2) {
-        ox = double(dr[qq].x) + q2r[qq] * bx;
-        oy = double(dr[qq].y) + q2r[qq] * by;
+        const double q2 = double(q2r[qq]);
+        ox = double(dr[qq].x) + q2 * bx;
+        oy = double(dr[qq].y) + q2 * by;
       } else {
-        ox = e1r[qq] * double(a[qq].x) + q1r[qq] * bx;
-        oy = e1r[qq] * double(a[qq].y) + q1r[qq] * by;
-        if constexpr (MODE == COL_RK2_STAGE1)
-          st_state<MEM, false, T>(c.D + tbase + ky, V{T(ox - q2r[qq] * bx), T(oy - q2r[qq] * by)});
+        const double e1 = e1_of(e1r[qq]), q1 = double(q1r[qq]);
+        ox = e1 * double(a[qq].x) + q1 * bx;
+        oy = e1 * double(a[qq].y) + q1 * by;
+        if constexpr (MODE == COL_RK2_STAGE1) {
+          const double q2 = double(q2r[qq]);
+          st_state<MEM, false, T>(c.D + tbase + ky, V{T(ox - q2 * bx), T(oy - q2 * by)});
+        }
       }
       if (pinp && col == 0 && ky == 0) ox = kAbout ? 0.0 : pin, oy = 0.0;  // FP32: the fluctuation's mean, 0
       b[qq] = V{T(ox), T(oy)};

@@ -318,6 +318,16 @@ struct ColTables {
   const double* eps;
   SlotGrid sg;
   TabParams tp;
+  // FP32 copies of the tables for FP32 engines' fast column kernel (null in
+  // FP64 engines): E1f = Ny E1 - 1, so that E1 = (1 + E1f) / Ny in FP64 is
+  // within an FP32 rounding of E - 1 (absolute error <= 6e-8 |E - 1| / Ny,
+  // where a rounded E1 would be off by up to 6e-8 E / Ny: for the slow modes
+  // E is ~1 and E - 1 tiny), Q1f = Q1, Q2f = Q2 rounded once.  Half the
+  // bytes of the FP64 tables, which the kernel otherwise reads for every
+  // point of the spectrum.
+  float* E1f = nullptr;
+  float* Q1f = nullptr;
+  float* Q2f = nullptr;
 };
 
 constexpr int kColRegs = 16;  // A values held per thread between the two forward FFTs
@@ -1182,6 +1192,7 @@ __device__ __forceinline__ EtdCoef etd_coef(double lap, const TabParams& p) {
 // rebuild both compute their entries here.
 struct TabEntry {
   double e1, q1, q2;
+  double em1;  // Ny e1 - 1 (the E1f form)
 };
 __device__ __forceinline__ TabEntry tab_entry(double lap, const TabParams& p, bool q2) {
   const EtdCoef k = etd_coef(lap, p);
@@ -1189,6 +1200,7 @@ __device__ __forceinline__ TabEntry tab_entry(double lap, const TabParams& p, bo
   const double e = p.fold ? k.E - p.M * k.P * (p.Gamma * lap) : k.E;
   TabEntry t;
   t.e1 = e * p.inv_ny;
+  t.em1 = e - 1.0;
   t.q1 = k.P * (p.Gamma * lap) * p.inv_n;
   t.q2 = q2 ? etd2_phi_dt(k.z, p.dt) * (p.Gamma * lap) * p.inv_n : 0.0;
   return t;
@@ -1201,7 +1213,8 @@ __global__ void __launch_bounds__(256) k_tables(T* __restrict__ E1, T* __restric
                                                 long long tstride, SlotGrid sg, TabParams p,
                                                 const double* __restrict__ eps, const double* __restrict__ Ms,
                                                 int slot0, const int* __restrict__ list = nullptr,
-                                                const int* __restrict__ rebuild = nullptr) {
+                                                const int* __restrict__ rebuild = nullptr, float* __restrict__ E1f = nullptr,
+                                                float* __restrict__ Q1f = nullptr, float* __restrict__ Q2f = nullptr) {
   // slots slot0 + blockIdx.y, or list[blockIdx.y]; with `rebuild`, only the
   // slots whose flag is set
   const int slot = list ? list[blockIdx.y] : slot0 + (int)blockIdx.y;
@@ -1214,6 +1227,11 @@ __global__ void __launch_bounds__(256) k_tables(T* __restrict__ E1, T* __restric
   E1 += (size_t)slot * tstride;
   Q1 += (size_t)slot * tstride;
   if (Q2) Q2 += (size_t)slot * tstride;
+  if (E1f) {
+    E1f += (size_t)slot * tstride;
+    Q1f += (size_t)slot * tstride;
+    if (Q2f) Q2f += (size_t)slot * tstride;
+  }
   const size_t n = (size_t)p.Nc * p.hy;
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
     const int c = (int)(i / p.hy);
@@ -1222,6 +1240,11 @@ __global__ void __launch_bounds__(256) k_tables(T* __restrict__ E1, T* __restric
     E1[i] = T(t.e1);
     Q1[i] = T(t.q1);
     if (Q2) Q2[i] = T(t.q2);
+    if (E1f) {
+      E1f[i] = float(t.em1);
+      Q1f[i] = float(t.q1);
+      if (Q2f) Q2f[i] = float(t.q2);
+    }
   }
 }
 

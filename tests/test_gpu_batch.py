@@ -134,3 +134,9 @@ def test_slot_bytes_and_mixed_sizes(hip_device):
     assert per == nc * 16 * 3 + nt * 8 * 3 + grid * 8  # state (S, T, D) + FP64 tables (E1, Q1, Q2) + grid
     b = Batch(3, **_kw())
     assert b.device_bytes() >= 3 * per
+    # FP32 bulk: the same with the FP32 copies of the three tables counted,
+    # and the batch holds at least three such slots
+    per32 = slot_bytes(**_kw(precision=32))
+    st32 = per32 - nt * 12 * 3 - grid * 8
+    assert st32 > 0 and st32 % (8 * 3) == 0  # state (S, T, D) in FP32 complex
+    assert Batch(3, **_kw(precision=32)).device_bytes() >= 3 * per32
