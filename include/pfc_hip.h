@@ -66,8 +66,9 @@ typedef struct pfc_params {
   double minPhi;    /* logPFC clamp floor (used when clamp_log) */
   double smallZTol; /* ETD1 Taylor switch |L dt| <= smallZTol (kernels.py:55-71) */
   double M;         /* logPFC stabilising shift (0 = none) */
-  int precision;    /* 64: FP64 bulk; 32: FP32 bulk (transforms, state) with FP64 ETD tables and
-                       combine, FP64 reductions */
+  int precision;    /* 64: FP64 bulk; 32: FP32 bulk (transforms, state) with FP64 ETD tables
+                       (the fast column kernel reads their FP32 copies, Ny E1 - 1 and Q1,
+                       widened to FP64), FP64 combine, FP64 reductions */
   int scheme;       /* PFC_SCHEME_* */
   int clamp_log;    /* logPFC: ln(phi) -> ln(minPhi) where phi <= minPhi */
   int pin_mean;     /* pin the k=0 mode of every update to the mean captured by pfc_set_field */

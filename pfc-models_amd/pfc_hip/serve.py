@@ -25,7 +25,7 @@ spot: p50 0.41-0.42 s, p99 5.3-5.5 s; with small-grid ticks 16x longer: p99
 7.3-7.6 s.)
 
 Memory admission.  The HBM budget is spent slab by slab: a slab of k slots
-reserves k times the per-slot footprint (state and FP64 tables, as the
+reserves k times the per-slot footprint (state and ETD tables, as the
 library reports it) when it is created and returns it when it is released.
 Admission is first come, first served:
 
