@@ -217,3 +217,23 @@ def test_extv_chemical_potential_returns_mu_only(hip_device):
     mu = sim.CalculateChemicalPotential()
     assert isinstance(mu, np.ndarray) and mu.shape == (48, 64)
     assert np.array_equal(mu, sim.State.mu)
+
+
+def test_fp32_device_shift_follows_the_host_check(hip_device):
+    # FP32 bulk on the fast ETD1 path: the column kernel's in-step refit
+    # writes the FP32 table copies the FP32 combine reads (as k_tables does
+    # for the host-driven rebuild); the two runs refit at the same steps to
+    # close fits and stay within FP32 rounding of each other
+    dev, host = _sim(96, 64, True, precision=32), _sim(96, 64, "host", precision=32)
+    assert dev._device_shift and host._host_shift
+    fits = set()
+    for n in [5] * 40 + [300] * 4:
+        dev.run(n)
+        host.run(n)
+        Md, Mh = dev.State.M, host.State.M
+        fits.add(round(Mh, 6))
+        assert abs(Md - Mh) <= 1e-5 * abs(Mh), (Md, Mh)
+        err = float(np.abs(dev.State.phi.get() - host.State.phi.get()).max())
+        assert err <= 1e-4, err
+    assert len(fits) >= 3
+    assert dev.status() == 0
