@@ -64,6 +64,10 @@ CASES = [
     dict(model="std", Nx=16, Ny=9, epsilon=-0.3, g=0.5, mean=-0.2, dt=0.3, precision=64),
     dict(model="log", Nx=12, Ny=25, epsilon=-0.9, g=-2.0, mean=0.63, dt=0.1, precision=64),
     dict(model="std", Nx=4, Ny=2, epsilon=-0.3, g=0.0, mean=-0.2, dt=0.3, precision=64),
+    # odd Nx/2 with Ny at the dense form's compiled row bounds (16 and 32
+    # rows; 15 angles)
+    dict(model="std", Nx=30, Ny=16, epsilon=-0.3, g=0.5, mean=-0.2, dt=0.3, precision=64),
+    dict(model="log", Nx=26, Ny=32, epsilon=-0.9, g=-2.0, mean=0.63, dt=0.1, precision=64),
 ]
 
 
